@@ -1,0 +1,459 @@
+"""CPU oracle for pyprop's tensor-potential Krylov propagation path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.  It wraps the C
+restatement in pyprop_oracle.c (built by the Makefile beside it) and adds the
+operator assembly the reference performs in Python:
+
+* potential list, per-potential projection and consolidation
+  (pyprop/tensorpotential/BasisPropagator.py:41-114, TensorPotential.py:128-163,
+  TensorPotentialGenerator.py:72-256);
+* the hydrogen / SAE evaluators of examples/tensor/helium_stabilization_sae/
+  potential.cpp:7-35,112-314 and the time function LaserFunctionVelocity
+  (example.py:220-230);
+* the 1s start state by inverse iteration on the l=0 block (the reference
+  offers shift-invert Arnoldi, pyprop/solver/PiramSolver.py:36-69; for a
+  single block this is the same fixed point).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+ST_IDENT, ST_DIAG, ST_SIMP, ST_HERM, ST_BANDNH, ST_BAND = range(6)
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liborc.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_bspline_create.restype = P
+        L.orc_bspline_create.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_bspline_destroy.argtypes = [P]
+        L.orc_bspline_count.argtypes = [P]
+        L.orc_bspline_grid_size.argtypes = [P]
+        L.orc_bspline_grid.argtypes = [P, dp, dp]
+        L.orc_bspline_value.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.orc_bspline_value.restype = ctypes.c_double
+        L.orc_bspline_integral.argtypes = [P, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+        L.orc_bspline_project_bandnh.argtypes = [P, dp, dp, ctypes.c_int, dp]
+        L.orc_bspline_overlap_upper.argtypes = [P, dp]
+        L.orc_zpbtrf_upper.argtypes = [ctypes.c_int, ctypes.c_int, dp]
+        L.orc_band_apply.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, ctypes.c_int, dp, dp]
+        L.orc_angular_tables.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp]
+        L.orc_gauss_legendre.argtypes = [ctypes.c_int, dp, dp]
+        L.orc_tp_multiply.argtypes = [ctypes.c_int, ip, ip, dp, ctypes.c_double, ip, dp, dp,
+                                      ctypes.POINTER(ip)]
+        L.orc_ham_create.restype = P
+        L.orc_ham_create.argtypes = [ctypes.c_int, ip]
+        L.orc_ham_set_overlap.argtypes = [P, ctypes.c_int, ctypes.c_int, dp]
+        L.orc_ham_add_term.argtypes = [P, ip, ip, dp, ctypes.POINTER(ip)]
+        L.orc_ham_set_scaling.argtypes = [P, dp]
+        L.orc_ham_destroy.argtypes = [P]
+        L.orc_ham_balanced.argtypes = [P, dp, dp]
+        L.orc_ham_potentials.argtypes = [P, dp, dp]
+        L.orc_ham_inner.argtypes = [P, dp, dp, dp]
+        L.orc_pade_exp.argtypes = [ctypes.c_int, dp, dp, ctypes.c_int]
+        L.orc_scaling_pade_exp.argtypes = [ctypes.c_int, dp, dp]
+        L.orc_pamp_create.restype = P
+        L.orc_pamp_create.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_int]
+        L.orc_pamp_destroy.argtypes = [P]
+        L.orc_pamp_step.argtypes = [P, P, dp, ctypes.c_double, ctypes.c_double]
+        L.orc_pamp_error_estimate.argtypes = [P]
+        L.orc_pamp_error_estimate.restype = ctypes.c_double
+        L.orc_propagate_pamp.argtypes = [P, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int, ctypes.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+# --------------------------------------------------------------------------
+# breakpoints (pyprop/bspline/breakpointsequences.py:31-41,6-28,72-93)
+# --------------------------------------------------------------------------
+def linear_breakpoints(rmin, rmax, n):
+    return np.array([rmin + (rmax - rmin) * i / float(n - 1) for i in range(n)])
+
+
+def exponential_breakpoints(rmin, rmax, n, gamma):
+    h = rmax - rmin
+    return np.array([rmin + h * (np.exp(gamma * float(i - 1) / float(n - 1)) - 1) / (np.exp(gamma) - 1)
+                     for i in range(1, n + 1)])
+
+
+class BSpline:
+    """B-spline basis of bspline.cpp restated in C; 'zero' continuity by default."""
+
+    def __init__(self, breakpoints, order, continuity="zero", quad_add=0):
+        bp = np.ascontiguousarray(breakpoints, dtype=np.float64)
+        ends = {"zero": (1, 1), "vanilla": (0, 0)}[continuity]
+        self.order = order
+        self.breakpoints = bp
+        self.h = lib().orc_bspline_create(_dp(bp), len(bp), order, ends[0], ends[1], quad_add)
+        self.N = lib().orc_bspline_count(self.h)
+        G = lib().orc_bspline_grid_size(self.h)
+        self.x = np.zeros(G)
+        self.w = np.zeros(G)
+        lib().orc_bspline_grid(self.h, _dp(self.x), _dp(self.w))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.orc_bspline_destroy(self.h)
+            self.h = None
+
+    def value(self, x, i, deriv=0):
+        return lib().orc_bspline_value(self.h, float(x), int(i), int(deriv))
+
+    def integral(self, f, d, i, j):
+        f = np.asarray(f, dtype=np.complex128)
+        re = np.ascontiguousarray(f.real)
+        im = np.ascontiguousarray(f.imag)
+        out = np.zeros(2)
+        lib().orc_bspline_integral(self.h, _dp(re), _dp(im), d, i, j, _dp(out))
+        return out[0] + 1j * out[1]
+
+    def project_bandnh(self, f, d):
+        f = np.broadcast_to(np.asarray(f, dtype=np.complex128), self.x.shape)
+        re = np.ascontiguousarray(f.real)
+        im = np.ascontiguousarray(f.imag)
+        bw = 2 * (self.order - 1) + 1
+        out = np.zeros(self.N * bw, dtype=np.complex128)
+        lib().orc_bspline_project_bandnh(self.h, _dp(re), _dp(im), d, _dp(out.view(np.float64)))
+        return out
+
+    def overlap_upper(self):
+        kd = self.order - 1
+        out = np.zeros(self.N * (kd + 1), dtype=np.complex128)
+        lib().orc_bspline_overlap_upper(self.h, _dp(out.view(np.float64)))
+        return out
+
+
+def gauss_legendre(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().orc_gauss_legendre(n, _dp(x), _dp(w))
+    return x, w
+
+
+def zpbtrf_upper(ab, N, kd):
+    a = np.array(ab, dtype=np.complex128)
+    info = lib().orc_zpbtrf_upper(N, kd, _dp(a.view(np.float64)))
+    return a, info
+
+
+OP_SOLVE_N, OP_SOLVE_C, OP_MUL_N = 0, 1, 2
+
+
+def band_apply(op, rank, x, U, kd):
+    x = np.array(x, dtype=np.complex128, order="C")
+    shape = np.array(x.shape, dtype=np.int32)
+    U = np.ascontiguousarray(U, dtype=np.complex128)
+    lib().orc_band_apply(op, rank, x.ndim, _ip(shape), kd, _dp(U.view(np.float64)), _dp(x.view(np.float64)))
+    return x
+
+
+def angular_tables(lmax, m=0):
+    nth = lmax + 1
+    L = lmax + 1 - abs(m)
+    x = np.zeros(nth)
+    w = np.zeros(nth)
+    P = np.zeros((nth, L))
+    D = np.zeros((nth, L))
+    lib().orc_angular_tables(lmax, m, _dp(x), _dp(w), _dp(P), _dp(D))
+    return x, w, P, D
+
+
+def dipole_pairs(lmax, m=0):
+    """GeometryInfoReducedSphHarmSelectionRule.GetBasisPairs (ReducedSphericalHarmonic.py:20-37)."""
+    count = lmax - abs(m) + 1
+    pairs = []
+    for i in range(count):
+        if i > 0:
+            pairs.append((i, i - 1))
+        if i < count - 1:
+            pairs.append((i, i + 1))
+    return np.array(pairs, dtype=np.int32).reshape(-1, 2)
+
+
+def tp_multiply(storage, pot, scaling, src, dst, pairs=None):
+    R = src.ndim
+    st = np.array(storage, dtype=np.int32)
+    ps = np.array(pot.shape, dtype=np.int32)
+    sh = np.array(src.shape, dtype=np.int32)
+    pot = np.ascontiguousarray(pot, dtype=np.complex128)
+    src = np.ascontiguousarray(src, dtype=np.complex128)
+    keep = []
+    arr = (ctypes.POINTER(ctypes.c_int) * R)()
+    for r in range(R):
+        if pairs is not None and pairs[r] is not None:
+            p = np.ascontiguousarray(pairs[r], dtype=np.int32)
+            keep.append(p)
+            arr[r] = _ip(p)
+        else:
+            arr[r] = ctypes.POINTER(ctypes.c_int)()
+    assert dst.flags.c_contiguous and dst.dtype == np.complex128
+    rc = lib().orc_tp_multiply(R, _ip(st), _ip(ps), _dp(pot.view(np.float64)), float(scaling), _ip(sh),
+                               _dp(src.view(np.float64)), _dp(dst.view(np.float64)), arr)
+    if rc:
+        raise RuntimeError("orc_tp_multiply failed %d" % rc)
+    return dst
+
+
+# --------------------------------------------------------------------------
+# time function (examples/tensor/helium_stabilization_sae/example.py:220-230)
+# --------------------------------------------------------------------------
+def laser_velocity(t, amplitude, frequency, pulse_duration, phase=0.0):
+    if 0 <= t < pulse_duration:
+        f = amplitude
+        f *= np.sin(t * np.pi / pulse_duration) ** 2
+        f *= -np.cos(t * frequency + phase)
+        return f
+    return 0.0
+
+
+def step_times(nsteps, dt, t0=0.0):
+    """PropagatedTime sequence of Problem.AdvanceStep (Problem.py:158-173)."""
+    ts = []
+    t = t0
+    for _ in range(nsteps):
+        ts.append(t)
+        t += dt
+    return ts
+
+
+# --------------------------------------------------------------------------
+# hydrogen operator in the SAE example's form, potential list
+#   RadialKineticEnergy, AngularKineticEnergy, SAEPotential,
+#   LaserPotentialVelocity1..3  (config_paper.ini:41,79-140 + stabilization.py:44)
+# --------------------------------------------------------------------------
+class Hydrogen:
+    def __init__(self, rmin=0.0, rmax=100.0, nbp=100, order=7, lmax=10, m=0, z=1.0,
+                 a=(0, 0, 0, 0, 0, 0), soft=0.0, mass=1.0, charge=-1.0,
+                 frequency=0.057, cycles=6, E0=0.05, phase=0.0, quad_add=0,
+                 bpstype="linear", gamma=1.0):
+        if bpstype == "linear":
+            bp = linear_breakpoints(rmin, rmax, nbp)
+        else:
+            bp = exponential_breakpoints(rmin, rmax, nbp, gamma)
+        self.bs = BSpline(bp, order, "zero", quad_add)
+        self.N = N = self.bs.N
+        self.k = order
+        self.kd = order - 1
+        self.bw = 2 * (order - 1) + 1
+        self.lmax, self.m = lmax, m
+        self.L = L = lmax + 1 - abs(m)
+        self.frequency, self.cycles = frequency, cycles
+        self.pulse_duration = 2 * np.pi / frequency * cycles
+        self.amplitude = E0 / frequency
+        self.phase = phase
+        r = self.bs.x
+        ls = np.arange(abs(m), lmax + 1)
+        # ---- Diag x BandNH, time independent (consolidated in list order)
+        kin = self.bs.project_bandnh(-1.0 / (2 * mass) * np.ones_like(r), 2)
+        r2 = self.bs.project_bandnh(1.0 / (r * r), 0)
+        a1, a2, a3, a4, a5, a6 = a
+        sae = self.bs.project_bandnh(-(z + a1 * np.exp(-a2 * r) + a3 * r * np.exp(-a4 * r)
+                                       + a5 * np.exp(-a6 * r)) / np.sqrt(r * r + soft * soft), 0)
+        H0 = np.zeros((L, N * self.bw), dtype=np.complex128)
+        for li, l in enumerate(ls):
+            H0[li] += kin
+        for li, l in enumerate(ls):
+            H0[li] += l * (l + 1.0) / (2 * mass) * r2
+        for li, l in enumerate(ls):
+            H0[li] += sae
+        self.H0 = H0
+        # ---- Simp x BandNH, time dependent (LaserFunctionVelocity)
+        x, w, P, D = angular_tables(lmax, m)
+        pairs = dipole_pairs(lmax, m)
+        self.pairs = pairs
+        cos_ang = np.array([np.sum(w * P[:, a_] * x * P[:, b_]) for a_, b_ in pairs])
+        der_ang = np.array([np.sum(w * P[:, a_] * D[:, b_]) for a_, b_ in pairs])
+        self.cos_ang, self.der_ang = cos_ang, der_ang
+        d1 = self.bs.project_bandnh(np.ones_like(r), 1)
+        r1 = self.bs.project_bandnh(1.0 / r, 0)
+        Hi = np.zeros((len(pairs), N * self.bw), dtype=np.complex128)
+        Hi += (charge * 1j) * cos_ang[:, None] * d1[None, :]           # Velocity1: charge*i*cos, d/dr
+        Hi += (-charge * 1j) * der_ang[:, None] * r1[None, :]          # Velocity2: -charge*i/r, angular deriv
+        Hi += (-charge * 1j) * cos_ang[:, None] * r1[None, :]          # Velocity3: -charge*i*cos/r
+        self.Hint = Hi
+        # ---- overlap + Cholesky
+        self.S = self.bs.overlap_upper()
+        self.U, info = zpbtrf_upper(self.S, N, self.kd)
+        if info:
+            raise RuntimeError("overlap not positive definite (%d)" % info)
+        self.shape = (L, N)
+
+    def scaling(self, t):
+        return laser_velocity(t, self.amplitude, self.frequency, self.pulse_duration, self.phase)
+
+    def ham(self):
+        return Ham(self.shape, [(ST_DIAG, ST_BANDNH, self.H0, None),
+                                (ST_SIMP, ST_BANDNH, self.Hint, self.pairs)],
+                   {1: (self.U, self.kd)})
+
+    # dense helpers (for small checks)
+    def band_to_dense(self, band):
+        N, s, bw = self.N, self.kd, self.bw
+        M = np.zeros((N, N), dtype=np.complex128)
+        for j in range(N):
+            for i in range(max(0, j - s), min(N, j + s + 1)):
+                M[i, j] = band[j * bw + s + i - j]
+        return M
+
+    def overlap_dense(self):
+        N, kd = self.N, self.kd
+        S = np.zeros((N, N), dtype=np.complex128)
+        for j in range(N):
+            for i in range(max(0, j - kd), j + 1):
+                S[i, j] = self.S[j * (kd + 1) + kd + i - j]
+                S[j, i] = np.conj(S[i, j])
+        return S
+
+    def ground_state(self, l=0, shift=-0.5 + 1e-3, iterations=5):
+        """1s start state: inverse iteration on the l block, S-normalised."""
+        A = self.band_to_dense(self.H0[l - abs(self.m)])
+        S = self.overlap_dense()
+        x = np.ones(self.N, dtype=np.complex128)
+        M = A - shift * S
+        for _ in range(iterations):
+            x = np.linalg.solve(M, S @ x)
+            x /= np.sqrt(np.real(np.conj(x) @ S @ x))
+        i = np.argmax(np.abs(x))
+        x *= np.abs(x[i]) / x[i]
+        psi = np.zeros(self.shape, dtype=np.complex128)
+        psi[l - abs(self.m)] = x
+        return psi
+
+    def bound_states(self, l, emax=0.0):
+        """Per-l generalized eigenproblem (analysis.py:92-140), S-normalised."""
+        import scipy.linalg
+        A = self.band_to_dense(self.H0[l - abs(self.m)])
+        S = self.overlap_dense()
+        E, V = scipy.linalg.eigh(A, S)
+        sel = E < emax
+        return E[sel], V[:, sel]
+
+    def populations(self, psi, emax=0.0):
+        """P_bound = sum_l sum_{E<0} |c^H S psi_l|^2 (analysis.py:344-364)."""
+        S = self.overlap_dense()
+        tot = 0.0
+        per = []
+        for li in range(self.L):
+            E, V = self.bound_states(li + abs(self.m), emax)
+            c = V.conj().T @ (S @ psi[li])
+            p = np.abs(c) ** 2
+            per.append(p)
+            tot += p.sum()
+        return tot, per
+
+    def norm(self, psi):
+        S = self.overlap_dense()
+        return float(np.sqrt(np.real(np.sum(np.conj(psi) * (psi @ S.T)))))
+
+
+class Ham:
+    """Balanced Hamiltonian U^-H (sum_t s_t V_t) U^-1 held by the C oracle."""
+
+    def __init__(self, shape, terms, overlaps):
+        self.shape = tuple(shape)
+        sh = np.array(shape, dtype=np.int32)
+        self.h = lib().orc_ham_create(len(shape), _ip(sh))
+        self.nterms = 0
+        self._keep = []
+        for rank, (U, kd) in overlaps.items():
+            U = np.ascontiguousarray(U, dtype=np.complex128)
+            lib().orc_ham_set_overlap(self.h, rank, kd, _dp(U.view(np.float64)))
+        for t in terms:
+            st = t[:-2]
+            pot, pairs = t[-2], t[-1]
+            R = len(shape)
+            st = np.array(st, dtype=np.int32)
+            ps = np.array(pot.shape, dtype=np.int32)
+            arr = (ctypes.POINTER(ctypes.c_int) * R)()
+            for r in range(R):
+                arr[r] = ctypes.POINTER(ctypes.c_int)()
+            if pairs is not None:
+                pr = pairs if isinstance(pairs, (list, tuple)) else [pairs] + [None] * (R - 1)
+                for r in range(R):
+                    if pr[r] is not None:
+                        p = np.ascontiguousarray(pr[r], dtype=np.int32)
+                        self._keep.append(p)
+                        arr[r] = _ip(p)
+            pot = np.ascontiguousarray(pot, dtype=np.complex128)
+            lib().orc_ham_add_term(self.h, _ip(st), _ip(ps), _dp(pot.view(np.float64)), arr)
+            self.nterms += 1
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.orc_ham_destroy(self.h)
+            self.h = None
+
+    def set_scaling(self, s):
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        lib().orc_ham_set_scaling(self.h, _dp(s))
+
+    def balanced(self, x):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros_like(x)
+        lib().orc_ham_balanced(self.h, _dp(x.view(np.float64)), _dp(y.view(np.float64)))
+        return y
+
+    def potentials(self, x):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros_like(x)
+        lib().orc_ham_potentials(self.h, _dp(x.view(np.float64)), _dp(y.view(np.float64)))
+        return y
+
+    def inner(self, a, b):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        out = np.zeros(2)
+        lib().orc_ham_inner(self.h, _dp(a.view(np.float64)), _dp(b.view(np.float64)), _dp(out))
+        return out[0] + 1j * out[1]
+
+    def propagate_pamp(self, psi, nsteps, dt, scalings, m, reorth=False):
+        psi = np.array(psi, dtype=np.complex128, order="C")
+        sc = np.ascontiguousarray(np.asarray(scalings, dtype=np.float64).reshape(nsteps, self.nterms))
+        rc = lib().orc_propagate_pamp(self.h, _dp(psi.view(np.float64)), nsteps, float(dt), _dp(sc),
+                                      int(m), int(bool(reorth)))
+        if rc:
+            raise RuntimeError("oracle pAMP failed (%d)" % rc)
+        return psi
+
+
+def pade_exp(A, order=6):
+    A = np.asfortranarray(A, dtype=np.complex128)
+    m = A.shape[0]
+    out = np.zeros((m, m), dtype=np.complex128, order="F")
+    info = lib().orc_pade_exp(m, _dp(A.reshape(-1, order="F").view(np.float64)),
+                              _dp(out.reshape(-1, order="F").view(np.float64)), order)
+    return out, info
+
+
+def scaling_pade_exp(A):
+    A = np.array(A, dtype=np.complex128, order="F")
+    m = A.shape[0]
+    a = A.reshape(-1, order="F").copy()
+    out = np.zeros(m * m, dtype=np.complex128)
+    info = lib().orc_scaling_pade_exp(m, _dp(a.view(np.float64)), _dp(out.view(np.float64)))
+    return out.reshape((m, m), order="F"), info

@@ -1,0 +1,80 @@
+"""ctypes binding of libpyprop_amd.so, the C-ABI of the HIP path.
+
+The library is built in-tree by ``pyprop_amd.build`` (hipcc, gfx950).  Every
+entry point returns an int status: 0 ok, <0 argument / unsupported / HIP
+errors, >0 numerical failures; :func:`check` turns them into exceptions the
+way the reference turns C++ ``runtime_error`` into Python ``RuntimeError``.
+There is no CPU fallback: if the library is missing, importing the compute
+path raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpyprop_amd.so")
+
+PP_OK, PP_EARG, PP_EUNSUP, PP_EHIP = 0, -1, -2, -3
+
+# storage ids (tensorpotentialmultiply_generator.py:1336)
+STORAGE_IDS = {"Ident": 0, "Diag": 1, "Simp": 2, "Herm": 3, "BandNH": 4, "Band": 5}
+
+# overlap operations
+OV_SOLVE_SQRT, OV_SOLVE_SQRT_CONJ, OV_MUL_SQRT, OV_MUL_SQRT_CONJ, OV_SOLVE, OV_MUL = range(6)
+
+EXPORTS = [
+    "pp_band_cholesky_upper", "pp_overlap_create", "pp_overlap_factor", "pp_overlap_destroy",
+    "pp_overlap_apply", "pp_tp_multiply", "pp_pamp_create", "pp_pamp_add_term",
+    "pp_pamp_propagate", "pp_pamp_diagnostics", "pp_pamp_destroy",
+]
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def check(rc, what):
+    if rc == PP_OK:
+        return
+    names = {PP_EARG: "invalid argument", PP_EUNSUP: "unsupported on the HIP path", PP_EHIP: "HIP runtime error"}
+    raise NativeError("%s failed: %s (status %d)" % (what, names.get(rc, "numerical failure"), rc))
+
+
+def lib():
+    """Load libpyprop_amd.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NativeError("libpyprop_amd.so not built; run pyprop_amd.build.build() (hipcc, gfx950)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    I = ctypes.c_int
+    D = ctypes.c_double
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.pp_band_cholesky_upper.argtypes = [I, I, dp]
+    L.pp_overlap_create.argtypes = [I, I, dp, ctypes.POINTER(P), ip]
+    L.pp_overlap_factor.argtypes = [P, dp]
+    L.pp_overlap_destroy.argtypes = [P]
+    L.pp_overlap_apply.argtypes = [P, I, I, I, ip, P, P]
+    L.pp_tp_multiply.argtypes = [I, ip, ip, P, D, ip, P, P, ctypes.POINTER(ip), P]
+    L.pp_pamp_create.argtypes = [I, I, I, I, P, ctypes.POINTER(P)]
+    L.pp_pamp_add_term.argtypes = [P, I, I, ip, I, I, P]
+    L.pp_pamp_propagate.argtypes = [P, P, I, D, D, dp, P]
+    L.pp_pamp_diagnostics.argtypes = [P, I, dp, P]
+    L.pp_pamp_destroy.argtypes = [P]
+    for name in EXPORTS:
+        getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def dptr(a):
+    """ctypes double* of a contiguous numpy array."""
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def iptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
