@@ -1,0 +1,92 @@
+/*
+ * pyprop_amd.h -- C ABI of the MI355X tensor-potential Krylov path.
+ *
+ * Every entry point returns an int status and never throws:
+ *   0  ok
+ *  -1  invalid argument (null pointer, shape / storage mismatch)
+ *  -2  combination not implemented on this path
+ *  -3  HIP runtime error
+ *  >0  numerical failure (band Cholesky: LAPACK info; see each function)
+ * Device pointers are complex128 (interleaved re, im; the layout of
+ * std::complex<double> / blitz::Array<cplx, R>), C-ordered with the largest
+ * stride first.  `stream` is a hipStream_t (NULL = default stream).
+ * Handles are not re-entrant: one stream per handle.
+ */
+#ifndef PYPROP_AMD_H
+#define PYPROP_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pp_overlap pp_overlap;
+typedef struct pp_pamp pp_pamp;
+typedef void *pp_stream;  /* hipStream_t */
+
+/* storage ids (core/tensorpotential/tensorpotentialmultiply_generator.py:1336) */
+enum { PP_STORAGE_IDENT = 0, PP_STORAGE_DIAG = 1, PP_STORAGE_SIMP = 2,
+       PP_STORAGE_HERM = 3, PP_STORAGE_BANDNH = 4, PP_STORAGE_BAND = 5 };
+
+/* overlap operations (core/representation/combinedrepresentation.cpp:186-238) */
+enum { PP_OV_SOLVE_SQRT = 0,       /* x := U^-1 x   SolveSqrtOverlap(false)    */
+       PP_OV_SOLVE_SQRT_CONJ = 1,  /* x := U^-H x   SolveSqrtOverlap(true)     */
+       PP_OV_MUL_SQRT = 2,         /* x := U x      MultiplySqrtOverlap(false) */
+       PP_OV_MUL_SQRT_CONJ = 3,    /* x := U^H x    MultiplySqrtOverlap(true)  */
+       PP_OV_SOLVE = 4,            /* x := S^-1 x   SolveOverlap               */
+       PP_OV_MUL = 5 };            /* x := S x      MultiplyOverlap            */
+
+/* zpbtrf 'U' on host band storage ab[j*(kd+1) + kd+i-j] (complex interleaved).
+ * Replaces blitz::linalg::LAPACK::CalculateCholeskyFactorizationPositiveDefiniteBanded
+ * (core/utility/blitzlapack.h:843-856) as called by OverlapMatrix::Setup
+ * (core/representation/overlapmatrix.cpp:35-37).  Returns LAPACK info. */
+int pp_band_cholesky_upper(int N, int kd, double *ab);
+
+/* OverlapMatrix::Setup (overlapmatrix.cpp:9-104): S in upper band storage on
+ * the host, factorised and uploaded with both solve plans.  *info = zpbtrf
+ * info (the reference leaves it unchecked; here info>0 returns 1).
+ * Supported: kd in {4,5,6,8} (B-spline orders 5,6,7,9), N <= 2048. */
+int pp_overlap_create(int N, int kd, const double *S_upper, pp_overlap **out, int *info);
+/* Copy the real Cholesky factor U (N*(kd+1) doubles, upper band storage) to the host. */
+int pp_overlap_factor(const pp_overlap *ov, double *U_upper_host);
+int pp_overlap_destroy(pp_overlap *ov);
+/* Apply `op` along rank `rank` of a rank-R tensor x (device), every fiber of
+ * MapToRank3 (core/utility/blitztricks.h:63-107).  Replaces
+ * OverlapMatrix::{Solve,Multiply}{Sqrt,}OverlapTensor (overlapmatrix.cpp:174-229). */
+int pp_overlap_apply(const pp_overlap *ov, int op, int rank, int R, const int *shape, void *x,
+                     pp_stream stream);
+
+/* core.TensorPotentialMultiply_<ids>(potential, scaling, source, dest, pairs...)
+ * (generator wrapper tensorpotentialmultiply_generator.py:1428-1521):
+ * dest += scaling * (x_r V_r) source.  storage[R] ids, potShape[R] extents of
+ * the potential (device), shape[R] extents of source/dest (device),
+ * pairs[r] host int32 (potShape[r] x 2) for Simp/Herm ranks, else NULL.
+ * R = 1..3. */
+int pp_tp_multiply(int R, const int *storage, const int *potShape, const void *potential,
+                   double scaling, const int *shape, const void *source, void *dest,
+                   const int *const *pairs, pp_stream stream);
+
+/* pAMP propagator (core.krylov_PampWrapper_R, core/krylov/pamp/pampwrapper.h:38-40,
+ * pampwrapper.cpp:51-108, pamp.h:613-669) with the balanced-overlap matvec of
+ * BasisPropagator.MultiplyHamiltonianBalancedOverlap (BasisPropagator.py:131-159)
+ * fused in.  State: (L angular, N B-spline) complex128; m = krylov_basis_size;
+ * reorth = krylov_double_orthogonalization. */
+int pp_pamp_create(int L, int N, int m, int reorth, const pp_overlap *ov, pp_pamp **out);
+/* One consolidated tensor potential: angular storage (Ident/Diag/Simp/Herm,
+ * P0 compressed indices, host pairs for Simp/Herm) x radial BandNH of total
+ * band width bw; `potential` device (P0, N*bw) in the reference layout. */
+int pp_pamp_add_term(pp_pamp *p, int angular_storage, int P0, const int *pairs, int radial_storage,
+                     int bw, const void *potential);
+/* nsteps consecutive PamPropagator.AdvanceStep(t_n, dt) (PamPropagator.py:50-59)
+ * in one launch; scal (host, nsteps x nterms) = each term's time factor at t_n. */
+int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re, double dt_im,
+                      const double *scal, pp_stream stream);
+/* Per step of the last launch (host, nsteps x 4): |last Krylov coefficient|
+ * (pamp.h:666; -1 if the Pade denominator was singular, pamp.h:544-550),
+ * basis size used (< m after happy breakdown), |U psi|, re-orthogonalisation passes. */
+int pp_pamp_diagnostics(pp_pamp *p, int nsteps, double *diag, pp_stream stream);
+int pp_pamp_destroy(pp_pamp *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -140,15 +140,20 @@ class BSplineBasis:
                 ok = (r >= 0) & (r <= k - 3)
                 return np.where(ok, pp[np.arange(G), np.clip(r, 0, k - 3)], 0.0)
 
+            nk = len(t)
+
+            def T(idx):
+                return t[np.clip(idx, 0, nk - 1)]
+
             def safe(num, den):
                 return np.where(den > 0, num / np.where(den > 0, den, 1.0), 0.0)
 
             for r in range(k):
                 i = mu - k + 1 + r
-                B1 = Bkm2(i) * safe(k - 2.0, t[i + k - 2] - t[i])
-                B2 = Bkm2(i + 1) * safe(k - 2.0, t[i + k - 1] - t[i + 1])
-                B3 = Bkm2(i + 2) * safe(k - 2.0, t[i + k] - t[i + 2])
-                d2[:, r] = safe(k - 1.0, t[i + k - 1] - t[i]) * (B1 - B2) - safe(k - 1.0, t[i + k] - t[i + 1]) * (B2 - B3)
+                B1 = Bkm2(i) * safe(k - 2.0, T(i + k - 2) - T(i))
+                B2 = Bkm2(i + 1) * safe(k - 2.0, T(i + k - 1) - T(i + 1))
+                B3 = Bkm2(i + 2) * safe(k - 2.0, T(i + k) - T(i + 2))
+                d2[:, r] = safe(k - 1.0, T(i + k - 1) - T(i)) * (B1 - B2) - safe(k - 1.0, T(i + k) - T(i + 1)) * (B2 - B3)
         self.D2 = d2
 
     def table(self, d):

@@ -297,138 +297,162 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
     for (int i = tid; i < N; i += PP_THREADS) A.y[(size_t)a * N + i] = X[i];
     grid.sync();
 
+    // Every workgroup executes the same number of grid barriers whatever the
+    // data: breakdown and re-orthogonalisation only predicate work, so a
+    // disagreement between workgroups could never leave one waiting.
     double beta0 = 0.0, beta = 0.0;
     int mEff = m;
+    bool broken = false;
     int reorth_passes = 0;
     for (int j = 0; j < m; j++) {
       // ---------------- phase A: beta_j, v_j = w/beta, w' = U^-H H U^-1 v_j, <v_k, w'>
+      const int ncomp = 2 * (j + 1);
       grid_partial_sum(A.part, A.pstr, NRM, 1, L, hred, gscr);
       beta = sqrt(hred[0]);
       if (j == 0) beta0 = beta;
-      else {
-        if (fabs(beta) < tol) { mEff = j; break; }   // happy breakdown (pamp.h:305-312)
-        if (a == 0 && tid == 0) A.Hm[j + (j - 1) * m] = zmk(beta, 0.0);
+      else if (!broken) {
+        if (fabs(beta) < tol) { broken = true; mEff = j; }   // happy breakdown (pamp.h:305-312)
+        else if (a == 0 && tid == 0) A.Hm[j + (j - 1) * m] = zmk(beta, 0.0);
       }
-      const double inv = 1.0 / beta;
-      for (int i = tid; i < N; i += PP_THREADS) {
-        zd v = zscale(W[i], inv);
-        W[i] = v;
-        Va[(size_t)j * N + i] = v;
-      }
-      for (int i = tid; i < padN; i += PP_THREADS) X[i] = zmk(0.0, 0.0);
-      __syncthreads();
-      for (int t = 0; t < A.nterms; t++) {
-        const double st = A.scal[(size_t)step * A.nterms + t] * inv;
-        const pp_kterm T = A.term[t];
-        const int e0 = T.rowptr[a], e1 = T.rowptr[a + 1];
-        for (int e = e0; e < e1; e++) {
-          const int col = T.ent[3 * e], pidx = T.ent[3 * e + 1], cj = T.ent[3 * e + 2];
-          for (int i = tid; i < N; i += PP_THREADS) Yn[i] = A.y[(size_t)col * N + i];
-          __syncthreads();
-          const zd *Vp = T.pot + (size_t)pidx * N * A.bw;
-          for (int i = tid; i < N; i += PP_THREADS) {
-            zd acc = zmk(0.0, 0.0);
-            int d0 = -A.s, d1 = A.s;
-            if (i + d0 < 0) d0 = -i;
-            if (i + d1 > N - 1) d1 = N - 1 - i;
-            for (int d = d0; d <= d1; d++) {
-              zd v = Vp[(size_t)(i + d) * A.bw + A.s - d];
-              if (cj) zfmac(acc, v, Yn[i + d]); else zfma(acc, v, Yn[i + d]);
-            }
-            zd x = X[i];
-            x.x = fma(st, acc.x, x.x);
-            x.y = fma(st, acc.y, x.y);
-            X[i] = x;
-          }
-          __syncthreads();
+      if (!broken) {
+        const double inv = 1.0 / beta;
+        for (int i = tid; i < N; i += PP_THREADS) {
+          zd v = zscale(W[i], inv);
+          W[i] = v;
+          Va[(size_t)j * N + i] = v;
         }
-      }
-      if (tid < 64) pp_band_solve_wave<KD, MAXB>(ov.solveC, X, true, N);
-      __syncthreads();
-      // w' -> W ; partial <v_k, w'> for k <= j ; |w'|^2 for re-orthogonalisation
-      for (int i = tid; i < N; i += PP_THREADS) W[i] = X[i];
-      __syncthreads();
-      const int ncomp = 2 * (j + 1);
-      block_sums(ncomp + (A.reorth ? 1 : 0), [&](int c) {
-        double s = 0.0;
-        if (c < ncomp) {
-          const zd *vk = Va + (size_t)(c >> 1) * N;
-          if ((c & 1) == 0) { for (int i = tid; i < N; i += PP_THREADS) { zd v = vk[i], w = W[i]; s += v.x * w.x + v.y * w.y; } }
-          else { for (int i = tid; i < N; i += PP_THREADS) { zd v = vk[i], w = W[i]; s += v.x * w.y - v.y * w.x; } }
-        } else {
-          for (int i = tid; i < N; i += PP_THREADS) s += zabs2(W[i]);
-        }
-        return s; }, wsum, hred);
-      for (int c = tid; c < ncomp; c += PP_THREADS) mypart[c] = hred[c];
-      if (A.reorth && tid == 0) mypart[ONRM] = hred[ncomp];
-      grid.sync();
-
-      // ---------------- phase B: h = sum_a partials ; w' -= V h ; |w'|^2 ; y = U^-1 w'
-      grid_partial_sum(A.part, A.pstr, 0, ncomp, L, hred, gscr);
-      if (a == 0 && tid <= j) A.Hm[tid + j * m] = zmk(hred[2 * tid], hred[2 * tid + 1]);
-      if (j == m - 1) break;   // the last residual is not used (pamp.h:419-426)
-      for (int i = tid; i < N; i += PP_THREADS) {
-        zd w = W[i];
-        for (int k = 0; k <= j; k++) {
-          zd h = zmk(hred[2 * k], hred[2 * k + 1]);
-          zd v = Va[(size_t)k * N + i];
-          w = zsub(w, zmul(v, h));
-        }
-        W[i] = w;
-      }
-      __syncthreads();
-      if (A.reorth) {
-        // PerformOrthogonalization (pamp.h:353-400): repeat CGS while the
-        // residual shrank below 0.717 of the previous norm, at most 3 times.
-        for (int c = tid; c < ncomp; c += PP_THREADS) hacc[c] = hred[c];
-        grid_partial_sum(A.part, A.pstr, ONRM, 1, L, hred, gscr);
-        double orig = sqrt(hred[0]);
+        for (int i = tid; i < padN; i += PP_THREADS) X[i] = zmk(0.0, 0.0);
         __syncthreads();
-        int cnt = 0;
-        while (true) {
-          block_sums(1, [&](int) { double s = 0.0; for (int i = tid; i < N; i += PP_THREADS) s += zabs2(W[i]); return s; }, wsum, hred);
-          if (tid == 0) mypart[NRM] = hred[0];
-          grid.sync();
-          grid_partial_sum(A.part, A.pstr, NRM, 1, L, hred, gscr);
-          double res = sqrt(hred[0]);
-          __syncthreads();
-          if (!(fabs(res) < 0.717 * fabs(orig))) break;
-          cnt++;
-          if (cnt > 3) break;
-          block_sums(ncomp, [&](int c) {
-            double s = 0.0;
+        for (int t = 0; t < A.nterms; t++) {
+          const double st = A.scal[(size_t)step * A.nterms + t] * inv;
+          const pp_kterm T = A.term[t];
+          const int e0 = T.rowptr[a], e1 = T.rowptr[a + 1];
+          for (int e = e0; e < e1; e++) {
+            const int col = T.ent[3 * e], pidx = T.ent[3 * e + 1], cj = T.ent[3 * e + 2];
+            for (int i = tid; i < N; i += PP_THREADS) Yn[i] = A.y[(size_t)col * N + i];
+            __syncthreads();
+            const zd *Vp = T.pot + (size_t)pidx * N * A.bw;
+            for (int i = tid; i < N; i += PP_THREADS) {
+              zd acc = zmk(0.0, 0.0);
+              int d0 = -A.s, d1 = A.s;
+              if (i + d0 < 0) d0 = -i;
+              if (i + d1 > N - 1) d1 = N - 1 - i;
+              for (int d = d0; d <= d1; d++) {
+                zd v = Vp[(size_t)(i + d) * A.bw + A.s - d];
+                if (cj) zfmac(acc, v, Yn[i + d]); else zfma(acc, v, Yn[i + d]);
+              }
+              zd x = X[i];
+              x.x = fma(st, acc.x, x.x);
+              x.y = fma(st, acc.y, x.y);
+              X[i] = x;
+            }
+            __syncthreads();
+          }
+        }
+        if (tid < 64) pp_band_solve_wave<KD, MAXB>(ov.solveC, X, true, N);
+        __syncthreads();
+        // w' -> W ; partial <v_k, w'> for k <= j ; |w'|^2 for re-orthogonalisation
+        for (int i = tid; i < N; i += PP_THREADS) W[i] = X[i];
+        __syncthreads();
+        block_sums(ncomp + (A.reorth ? 1 : 0), [&](int c) {
+          double s = 0.0;
+          if (c < ncomp) {
             const zd *vk = Va + (size_t)(c >> 1) * N;
             if ((c & 1) == 0) { for (int i = tid; i < N; i += PP_THREADS) { zd v = vk[i], w = W[i]; s += v.x * w.x + v.y * w.y; } }
             else { for (int i = tid; i < N; i += PP_THREADS) { zd v = vk[i], w = W[i]; s += v.x * w.y - v.y * w.x; } }
-            return s; }, wsum, hred);
-          for (int c = tid; c < ncomp; c += PP_THREADS) mypart[c] = hred[c];
-          grid.sync();
-          grid_partial_sum(A.part, A.pstr, 0, ncomp, L, hred, gscr);
-          for (int i = tid; i < N; i += PP_THREADS) {
-            zd w = W[i];
-            for (int k = 0; k <= j; k++) w = zsub(w, zmul(Va[(size_t)k * N + i], zmk(hred[2 * k], hred[2 * k + 1])));
-            W[i] = w;
+          } else {
+            for (int i = tid; i < N; i += PP_THREADS) s += zabs2(W[i]);
           }
+          return s; }, wsum, hred);
+        for (int c = tid; c < ncomp; c += PP_THREADS) mypart[c] = hred[c];
+        if (A.reorth && tid == 0) mypart[ONRM] = hred[ncomp];
+      }
+      grid.sync();
+
+      // ---------------- phase B: h = sum_a partials ; w' -= V h ; |w'|^2 ; y = U^-1 w'
+      if (!broken) {
+        grid_partial_sum(A.part, A.pstr, 0, ncomp, L, hred, gscr);
+        if (a == 0 && tid <= j) A.Hm[tid + j * m] = zmk(hred[2 * tid], hred[2 * tid + 1]);
+      }
+      if (j == m - 1) break;   // the last residual is not used (pamp.h:419-426)
+      if (!broken) {
+        for (int i = tid; i < N; i += PP_THREADS) {
+          zd w = W[i];
+          for (int k = 0; k <= j; k++) {
+            zd h = zmk(hred[2 * k], hred[2 * k + 1]);
+            zd v = Va[(size_t)k * N + i];
+            w = zsub(w, zmul(v, h));
+          }
+          W[i] = w;
+        }
+        __syncthreads();
+      }
+      if (A.reorth) {
+        // PerformOrthogonalization (pamp.h:353-400): repeat CGS while the
+        // residual shrank below 0.717 of the previous norm, at most 3 times.
+        double orig = 0.0;
+        bool stop = broken;
+        if (!broken) {
+          for (int c = tid; c < ncomp; c += PP_THREADS) hacc[c] = hred[c];
+          grid_partial_sum(A.part, A.pstr, ONRM, 1, L, hred, gscr);
+          orig = sqrt(hred[0]);
           __syncthreads();
-          for (int c = tid; c < ncomp; c += PP_THREADS) hacc[c] += hred[c];
-          __syncthreads();
-          if (a == 0 && tid <= j) A.Hm[tid + j * m] = zmk(hacc[2 * tid], hacc[2 * tid + 1]);
-          orig = res;
-          reorth_passes++;
-          __syncthreads();
+        }
+        for (int pass = 0; pass < 4; pass++) {
+          if (!stop) {
+            block_sums(1, [&](int) { double s = 0.0; for (int i = tid; i < N; i += PP_THREADS) s += zabs2(W[i]); return s; }, wsum, hred);
+            if (tid == 0) mypart[NRM] = hred[0];
+          }
+          grid.sync();
+          bool doit = false;
+          double res = 0.0;
+          if (!stop) {
+            grid_partial_sum(A.part, A.pstr, NRM, 1, L, hred, gscr);
+            res = sqrt(hred[0]);
+            __syncthreads();
+            if (fabs(res) < 0.717 * fabs(orig) && pass < 3) doit = true;
+            else stop = true;
+          }
+          if (doit) {
+            block_sums(ncomp, [&](int c) {
+              double s = 0.0;
+              const zd *vk = Va + (size_t)(c >> 1) * N;
+              if ((c & 1) == 0) { for (int i = tid; i < N; i += PP_THREADS) { zd v = vk[i], w = W[i]; s += v.x * w.x + v.y * w.y; } }
+              else { for (int i = tid; i < N; i += PP_THREADS) { zd v = vk[i], w = W[i]; s += v.x * w.y - v.y * w.x; } }
+              return s; }, wsum, hred);
+            for (int c = tid; c < ncomp; c += PP_THREADS) mypart[c] = hred[c];
+          }
+          grid.sync();
+          if (doit) {
+            grid_partial_sum(A.part, A.pstr, 0, ncomp, L, hred, gscr);
+            for (int i = tid; i < N; i += PP_THREADS) {
+              zd w = W[i];
+              for (int k = 0; k <= j; k++) w = zsub(w, zmul(Va[(size_t)k * N + i], zmk(hred[2 * k], hred[2 * k + 1])));
+              W[i] = w;
+            }
+            __syncthreads();
+            for (int c = tid; c < ncomp; c += PP_THREADS) hacc[c] += hred[c];
+            __syncthreads();
+            if (a == 0 && tid <= j) A.Hm[tid + j * m] = zmk(hacc[2 * tid], hacc[2 * tid + 1]);
+            orig = res;
+            reorth_passes++;
+            __syncthreads();
+          }
         }
         grid.sync();   // everyone has read NRM before it is rewritten below
       }
-      block_sums(1, [&](int) {
-        double s = 0.0;
-        for (int i = tid; i < N; i += PP_THREADS) s += zabs2(W[i]);
-        return s; }, wsum, hred);
-      if (tid == 0) mypart[NRM] = hred[0];
-      for (int i = tid; i < padN; i += PP_THREADS) X[i] = (i < N) ? W[i] : zmk(0.0, 0.0);
-      __syncthreads();
-      if (tid < 64) pp_band_solve_wave<KD, MAXB>(ov.solveN, X, false, N);
-      __syncthreads();
-      for (int i = tid; i < N; i += PP_THREADS) A.y[(size_t)a * N + i] = X[i];
+      if (!broken) {
+        block_sums(1, [&](int) {
+          double s = 0.0;
+          for (int i = tid; i < N; i += PP_THREADS) s += zabs2(W[i]);
+          return s; }, wsum, hred);
+        if (tid == 0) mypart[NRM] = hred[0];
+        for (int i = tid; i < padN; i += PP_THREADS) X[i] = (i < N) ? W[i] : zmk(0.0, 0.0);
+        __syncthreads();
+        if (tid < 64) pp_band_solve_wave<KD, MAXB>(ov.solveN, X, false, N);
+        __syncthreads();
+        for (int i = tid; i < N; i += PP_THREADS) A.y[(size_t)a * N + i] = X[i];
+      }
       grid.sync();
     }
 

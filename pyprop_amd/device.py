@@ -1,0 +1,146 @@
+"""Device-side objects over the C-ABI: overlap factor, tensor multiply and the
+persistent pAMP engine.  Device buffers are torch tensors on ``cuda:0``; torch
+is used only for allocation and the current stream.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as nat
+
+
+def torch():
+    import torch as _t
+    return _t
+
+
+def device():
+    t = torch()
+    if not t.cuda.is_available():
+        raise nat.NativeError("no GPU visible: the pyprop_amd compute path runs on MI355X only")
+    return t.device("cuda:0")
+
+
+def stream_ptr():
+    return ctypes.c_void_p(torch().cuda.current_stream().cuda_stream)
+
+
+def to_device(a):
+    """numpy -> contiguous complex128/float64/int32 torch tensor on the GPU."""
+    t = torch()
+    return t.from_numpy(np.ascontiguousarray(a)).to(device())
+
+
+def ptr(tensor):
+    if not tensor.is_contiguous():
+        raise ValueError("device buffers must be C-contiguous (MapToRank3 requires it)")
+    return ctypes.c_void_p(tensor.data_ptr())
+
+
+class Overlap:
+    """Banded overlap S = U^H U of one B-spline rank, factorised on creation
+    (OverlapMatrix::Setup, overlapmatrix.cpp:9-104)."""
+
+    def __init__(self, S_upper, N, kd):
+        L = nat.lib()
+        S = np.ascontiguousarray(S_upper, dtype=np.complex128)
+        if S.size != N * (kd + 1):
+            raise ValueError("overlap band has %d entries, expected %d" % (S.size, N * (kd + 1)))
+        h = ctypes.c_void_p()
+        info = ctypes.c_int(0)
+        rc = L.pp_overlap_create(N, kd, nat.dptr(S.view(np.float64)), ctypes.byref(h), ctypes.byref(info))
+        if rc > 0:
+            raise RuntimeError("overlap matrix is not positive definite (zpbtrf info=%d)" % info.value)
+        nat.check(rc, "pp_overlap_create")
+        self.h = h
+        self.N, self.kd = N, kd
+
+    def factor(self):
+        U = np.zeros(self.N * (self.kd + 1))
+        nat.check(nat.lib().pp_overlap_factor(self.h, nat.dptr(U)), "pp_overlap_factor")
+        return U
+
+    def apply(self, op, rank, x):
+        shape = np.array(x.shape, dtype=np.int32)
+        nat.check(nat.lib().pp_overlap_apply(self.h, op, rank, x.dim(), nat.iptr(shape), ptr(x), stream_ptr()),
+                  "pp_overlap_apply")
+        return x
+
+    def __del__(self):
+        if getattr(self, "h", None) and nat._lib is not None:
+            nat._lib.pp_overlap_destroy(self.h)
+            self.h = None
+
+
+def tp_multiply(storage_ids, potential, scaling, src, dst, pairs=None):
+    """core.TensorPotentialMultiply_<ids>(potential, scaling, src, dst, *pairs): dst += s * V src."""
+    R = src.dim()
+    if len(storage_ids) != R or potential.dim() != R or dst.shape != src.shape:
+        raise ValueError("rank mismatch in tensor multiply")
+    st = np.array([nat.STORAGE_IDS[s] if isinstance(s, str) else int(s) for s in storage_ids], dtype=np.int32)
+    ps = np.array(potential.shape, dtype=np.int32)
+    sh = np.array(src.shape, dtype=np.int32)
+    keep = []
+    arr = (ctypes.POINTER(ctypes.c_int) * R)()
+    for r in range(R):
+        p = None if pairs is None else pairs[r]
+        if p is not None:
+            p = np.ascontiguousarray(p, dtype=np.int32)
+            keep.append(p)
+            arr[r] = nat.iptr(p)
+        else:
+            arr[r] = ctypes.POINTER(ctypes.c_int)()
+    rc = nat.lib().pp_tp_multiply(R, nat.iptr(st), nat.iptr(ps), ptr(potential), float(scaling), nat.iptr(sh),
+                                  ptr(src), ptr(dst), arr, stream_ptr())
+    nat.check(rc, "pp_tp_multiply")
+    return dst
+
+
+class PampEngine:
+    """Persistent-kernel pAMP propagator for rank-2 (angular x B-spline) states.
+
+    Terms: (angular storage id, angular pairs or None, device potential of shape
+    (P0, N*bw)) with radial BandNH storage.  ``propagate`` runs nsteps whole
+    PamPropagator.AdvanceStep's (PamPropagator.py:50-59) in one launch."""
+
+    def __init__(self, L, N, m, overlap, reorth=False):
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().pp_pamp_create(L, N, m, int(bool(reorth)), overlap.h, ctypes.byref(h)), "pp_pamp_create")
+        self.h = h
+        self.L, self.N, self.m = L, N, m
+        self.overlap = overlap
+        self.terms = []
+
+    def add_term(self, angular_storage, potential, pairs=None, radial_storage="BandNH"):
+        P0, width = potential.shape
+        bw = width // self.N
+        if bw * self.N != width:
+            raise ValueError("radial band width does not divide the potential extent")
+        aid = nat.STORAGE_IDS[angular_storage] if isinstance(angular_storage, str) else int(angular_storage)
+        rid = nat.STORAGE_IDS[radial_storage] if isinstance(radial_storage, str) else int(radial_storage)
+        pp = ctypes.POINTER(ctypes.c_int)()
+        keep = None
+        if pairs is not None:
+            keep = np.ascontiguousarray(pairs, dtype=np.int32)
+            pp = nat.iptr(keep)
+        nat.check(nat.lib().pp_pamp_add_term(self.h, aid, P0, pp, rid, bw, ptr(potential)), "pp_pamp_add_term")
+        self.terms.append((potential, keep))
+
+    def propagate(self, psi, nsteps, dt, scalings):
+        dt = complex(dt)
+        sc = np.ascontiguousarray(np.asarray(scalings, dtype=np.float64).reshape(nsteps, len(self.terms)))
+        nat.check(nat.lib().pp_pamp_propagate(self.h, ptr(psi), int(nsteps), dt.real, dt.imag, nat.dptr(sc),
+                                              stream_ptr()), "pp_pamp_propagate")
+        self._last = nsteps
+        return psi
+
+    def diagnostics(self, nsteps=None):
+        n = self._last if nsteps is None else nsteps
+        d = np.zeros((n, 4))
+        nat.check(nat.lib().pp_pamp_diagnostics(self.h, n, nat.dptr(d), stream_ptr()), "pp_pamp_diagnostics")
+        return d
+
+    def __del__(self):
+        if getattr(self, "h", None) and nat._lib is not None:
+            nat._lib.pp_pamp_destroy(self.h)
+            self.h = None

@@ -1,0 +1,129 @@
+"""Propagators: PropagatorBase and the Krylov PamPropagator.
+
+PropagatorBase follows pyprop/propagator/Propagator.py:5-80.  PamPropagator
+keeps the reference's config keys and its balanced-overlap stepping
+(pyprop/propagator/PamPropagator.py:2-83, core/krylov/pamp/pampwrapper.cpp:
+51-108): psi~ = U psi, pAMP on U^-H H U^-1, psi = U^-1 psi~.  The Arnoldi loop,
+the matvec callback and the Pade exponential all run inside one persistent HIP
+launch; the time functions are sampled on the host at t_n for each step
+(H held at the step start, Problem.py:166, TensorPotential.py:98-100).
+"""
+import numpy as np
+
+from . import device as dev
+
+
+class PropagatorBase:
+    def __init__(self, psi):
+        self.psi = psi
+        self.PotentialList = []
+        self.RenormalizeActive = False
+
+    def ApplyConfig(self, config):
+        from .problem import CreatePotential
+        names = getattr(config.Propagation, "potential_evaluation", [])
+        self.PotentialList = [CreatePotential(config, name, self.psi) for name in names]
+
+    def ApplyConfigSection(self, configSection):
+        self.RenormalizeActive = getattr(configSection, "renormalization", False)
+
+    def SetupStep(self, dt):
+        self.SetupPotential(dt)
+
+    def RestartPropagation(self, timestep, startTime, propagationTime):
+        pass
+
+    def AdvanceStep(self, t, dt):
+        raise NotImplementedError("AdvanceStep must be implemented by the propagator")
+
+    def MultiplyHamiltonian(self, srcPsi, destPsi, t, dt):
+        raise NotImplementedError("MultiplyHamiltonian should be implemented by inheriting class %s" % self.__class__)
+
+    def SetupPotential(self, dt):
+        for potential in self.PotentialList:
+            potential.SetupStep(dt)
+
+    def MultiplyPotential(self, srcPsi, destPsi, t, dt):
+        for potential in self.PotentialList:
+            potential.MultiplyPotential(srcPsi, destPsi, t, dt)
+
+    def GetPotentialList(self):
+        return self.PotentialList
+
+
+class PamPropagator(PropagatorBase):
+    """Fixed-m Arnoldi/Pade exponential propagator (pamp.h:613-669)."""
+
+    def __init__(self, psi):
+        PropagatorBase.__init__(self, psi)
+        self.Rank = psi.GetRank()
+        self.BasisSize = 20
+        self.ExponentiationMethod = 0
+        self.DoubleOrthogonalization = False
+        self.Engine = None
+
+    def ApplyConfig(self, config):
+        PropagatorBase.ApplyConfig(self, config)
+        self.BasePropagator = config.Propagation.base_propagator(self.psi)
+        config.Apply(self.BasePropagator)
+        self._apply_krylov(config.Propagation)
+
+    def ApplyConfigSection(self, configSection):
+        PropagatorBase.ApplyConfigSection(self, configSection)
+        configSection.Apply(self.BasePropagator)
+        self._apply_krylov(configSection)
+
+    def _apply_krylov(self, section):
+        # pampwrapper.cpp:51-70
+        self.BasisSize = int(section.krylov_basis_size)
+        if hasattr(section, "krylov_exponentiation_method"):
+            self.ExponentiationMethod = int(section.krylov_exponentiation_method)
+        if hasattr(section, "krylov_double_orthogonalization"):
+            self.DoubleOrthogonalization = bool(section.krylov_double_orthogonalization)
+
+    def SetupStep(self, dt):
+        self.BasePropagator.SetupStep(dt)
+        if self.ExponentiationMethod != 0:
+            raise NotImplementedError("krylov_exponentiation_method=1 (eigenvector) is not on the HIP path")
+        repr = self.psi.GetRepresentation()
+        self.IsOrthogonalBasis = all(repr.IsOrthogonalBasis(i) for i in range(self.Rank))
+        self.UseBalancedOverlap = hasattr(self.BasePropagator, "MultiplyHamiltonianBalancedOverlap")
+        self.Engine = self._build_engine()
+
+    def _build_engine(self):
+        """Map the consolidated tensor potentials onto the persistent engine:
+        rank 2, angular rank 0 (orthogonal), B-spline rank 1 with BandNH storage."""
+        repr = self.psi.GetRepresentation()
+        pots = self.BasePropagator.PotentialList
+        if self.Rank != 2 or repr.IsOrthogonalBasis(1) is not False or not repr.IsOrthogonalBasis(0):
+            raise NotImplementedError("the pAMP HIP engine handles (angular, B-spline) rank-2 states")
+        L, N = repr.GetFullShape()
+        overlap = repr.GetRepresentation(1).GetOverlap()
+        eng = dev.PampEngine(L, N, self.BasisSize, overlap, reorth=self.DoubleOrthogonalization)
+        for pot in pots:
+            ids = pot.StorageIds()
+            if ids[1] != "BandNH" or ids[0] not in ("Ident", "Diag", "Simp", "Herm"):
+                raise NotImplementedError("storage %s is not on the pAMP HIP engine" % "_".join(ids))
+            pairs = pot.GeometryList[0].GetBasisPairs() if ids[0] in ("Simp", "Herm") else None
+            eng.add_term(ids[0], pot.DeviceData(), pairs=pairs)
+        self._pots = pots
+        return eng
+
+    def TimeScalings(self, times):
+        return np.array([[p.GetTimeScaling(t) for p in self._pots] for t in times], dtype=np.float64)
+
+    def MultiplyHamiltonian(self, srcPsi, destPsi, t, dt):
+        self.BasePropagator.MultiplyHamiltonian(srcPsi, destPsi, t, dt)
+
+    def AdvanceStep(self, t, dt):
+        self.AdvanceSteps([t], dt)
+
+    def AdvanceSteps(self, times, dt):
+        """Consecutive AdvanceStep(t_n, dt) for each t_n in ``times`` in one launch."""
+        sc = self.TimeScalings(times)
+        self.Engine.propagate(self.psi.GetData(), len(times), dt, sc)
+
+    def GetDiagnostics(self):
+        """Per step of the last launch: |last Krylov coefficient| (pamp.h:666),
+        basis size used, |U psi|, re-orthogonalisation passes."""
+        return self.Engine.diagnostics()

@@ -1,0 +1,240 @@
+"""Pin the CPU oracle against library routines, closed forms and brute force.
+
+The reference ships no golden vectors for this path, so each oracle function
+is checked against something independent: numpy/scipy routines, exact
+integrals, closed-form angular couplings, the hydrogen spectrum, dense
+Kronecker assembly and dense matrix exponentials.
+"""
+import numpy as np
+import pytest
+import scipy.interpolate
+import scipy.linalg
+
+from oracle import oracle as O
+
+
+@pytest.fixture(scope="module")
+def small():
+    return O.Hydrogen(rmax=30.0, nbp=20, order=7, lmax=3, frequency=0.3, cycles=4, E0=0.05)
+
+
+def test_gauss_legendre_vs_numpy():
+    for n in [1, 2, 5, 7, 11, 64]:
+        x, w = O.gauss_legendre(n)
+        xr, wr = np.polynomial.legendre.leggauss(n)
+        assert np.abs(x - xr).max() < 1e-14
+        assert np.abs(w - wr).max() < 1e-14
+
+
+def test_bspline_values_vs_scipy():
+    bp = O.linear_breakpoints(0.0, 10.0, 12)
+    bs = O.BSpline(bp, 5)
+    # 'zero' continuity leaves k-1 end knots; scipy needs k, so pad one knot
+    # per end: B_i of the oracle is scipy's B_{i+1} on the padded vector.
+    t = np.concatenate([[0.0] * 5, bp[1:-1], [10.0] * 5])
+    k = 5
+    xs = np.linspace(0.01, 9.99, 37)
+    for i in range(bs.N):
+        c = np.zeros(bs.N + 2)
+        c[i + 1] = 1.0
+        spl = scipy.interpolate.BSpline(t, c, k - 1)
+        for d in (0, 1, 2):
+            ref = spl.derivative(d)(xs) if d else spl(xs)
+            got = np.array([bs.value(x, i, d) for x in xs])
+            assert np.abs(got - ref).max() < 1e-11 * max(1.0, np.abs(ref).max())
+
+
+def test_overlap_exact_and_cholesky_vs_scipy(small):
+    # S_ij = int B_i B_j is a polynomial integral, exact with k Gauss points per
+    # interval; compare against scipy's B-spline antiderivative of the product.
+    bs = small.bs
+    N, kd = small.N, small.kd
+    S = small.overlap_dense()
+    t = np.concatenate([[bs.breakpoints[0]] * small.k, bs.breakpoints[1:-1], [bs.breakpoints[-1]] * small.k])
+    xq, wq = np.polynomial.legendre.leggauss(12)
+    for i in range(0, N, 5):
+        for j in range(i, min(N, i + kd + 1)):
+            ci = np.zeros(N + 2); ci[i + 1] = 1
+            cj = np.zeros(N + 2); cj[j + 1] = 1
+            bi = scipy.interpolate.BSpline(t, ci, small.k - 1)
+            bj = scipy.interpolate.BSpline(t, cj, small.k - 1)
+            val = 0.0
+            for a, b in zip(bs.breakpoints[:-1], bs.breakpoints[1:]):
+                x = (b - a) / 2 * xq + (a + b) / 2
+                val += np.sum((b - a) / 2 * wq * bi(x) * bj(x))
+            assert abs(S[i, j].real - val) < 1e-13
+    # Cholesky against scipy (upper, S = U^H U)
+    Ub = np.zeros((kd + 1, N))
+    for j in range(N):
+        for i in range(max(0, j - kd), j + 1):
+            Ub[kd + i - j, j] = S[i, j].real
+    ref = scipy.linalg.cholesky_banded(Ub, lower=False)
+    for j in range(N):
+        for i in range(max(0, j - kd), j + 1):
+            assert abs(small.U[j * (kd + 1) + kd + i - j] - ref[kd + i - j, j]) < 1e-13
+
+
+def test_band_solves_vs_dense(small):
+    N, kd = small.N, small.kd
+    Ud = np.zeros((N, N))
+    for j in range(N):
+        for i in range(max(0, j - kd), j + 1):
+            Ud[i, j] = small.U[j * (kd + 1) + kd + i - j].real
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((3, N)) + 1j * rng.standard_normal((3, N))
+    got = O.band_apply(O.OP_SOLVE_N, 1, x, small.U, kd)
+    ref = scipy.linalg.solve_triangular(Ud, x.T, lower=False).T
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    got = O.band_apply(O.OP_SOLVE_C, 1, x, small.U, kd)
+    ref = scipy.linalg.solve_triangular(Ud.T, x.T, lower=True).T
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    got = O.band_apply(O.OP_MUL_N, 1, x, small.U, kd)
+    assert np.abs(got - (Ud @ x.T).T).max() < 1e-13
+    # along rank 0 of a (N, 2) tensor
+    y = rng.standard_normal((N, 2)) + 0j
+    got = O.band_apply(O.OP_SOLVE_N, 0, y, small.U, kd)
+    assert np.abs(got - scipy.linalg.solve_triangular(Ud, y)).max() < 1e-12 * np.abs(got).max()
+
+
+def test_angular_couplings_closed_form(small):
+    # <l|cos|l+1> = (l+1)/sqrt((2l+1)(2l+3)) (laserhelper.h E(l,0));
+    # <l|sin d/dtheta|l'> = l' <l|cos|l'> - l' sqrt((2l'+1)/(2l'-1)) delta_{l,l'-1}
+    for (a, b), c, d in zip(small.pairs, small.cos_ang, small.der_ang):
+        l = min(a, b)
+        E = (l + 1) / np.sqrt((2 * l + 1) * (2 * l + 3))
+        assert abs(c - E) < 1e-14
+        if a == b + 1:      # row l+1, col l: l * <l+1|cos|l>
+            assert abs(d - b * E) < 1e-13
+        else:               # row l, col l+1
+            lp = b
+            assert abs(d - (lp * E - lp * np.sqrt((2 * lp + 1.0) / (2 * lp - 1.0)))) < 1e-13
+
+
+def test_hydrogen_spectrum():
+    h = O.Hydrogen(rmax=60.0, nbp=80, order=7, lmax=2)
+    for l in range(3):
+        E, _ = h.bound_states(l)
+        for n in range(l + 1, 4):   # n <= 3 is converged at rmax = 60
+            assert abs(E[n - l - 1] + 0.5 / n ** 2) < 1e-7
+
+
+def _dense_from_terms(h, s):
+    """Brute-force Kronecker assembly of H(t) = sum_l e_l e_l^T x A_l + s sum_p e_r e_c^T x B_p."""
+    L, N = h.shape
+    H = np.zeros((L * N, L * N), dtype=np.complex128)
+    for l in range(L):
+        H[l * N:(l + 1) * N, l * N:(l + 1) * N] += h.band_to_dense(h.H0[l])
+    for p, (r, c) in enumerate(h.pairs):
+        H[r * N:(r + 1) * N, c * N:(c + 1) * N] += s * h.band_to_dense(h.Hint[p])
+    return H
+
+
+def test_tensor_multiply_vs_kronecker(small):
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(small.shape) + 1j * rng.standard_normal(small.shape)
+    y = np.zeros_like(x)
+    O.tp_multiply((O.ST_DIAG, O.ST_BANDNH), small.H0, 1.0, x, y)
+    O.tp_multiply((O.ST_SIMP, O.ST_BANDNH), small.Hint, 0.3, x, y, [small.pairs, None])
+    ref = (_dense_from_terms(small, 0.3) @ x.ravel()).reshape(small.shape)
+    assert np.abs(y - ref).max() < 1e-12 * np.abs(ref).max()
+    # Hermitian storage reproduces the mirrored matrix with conjugation
+    V = rng.standard_normal((3, small.N * small.bw)) + 1j * rng.standard_normal((3, small.N * small.bw))
+    pairs = np.array([[0, 1], [1, 2], [0, 3]], dtype=np.int32)
+    y = np.zeros_like(x)
+    O.tp_multiply((O.ST_HERM, O.ST_BANDNH), V, 1.0, x, y, [pairs, None])
+    ref = np.zeros_like(x)
+    for p, (r, c) in enumerate(pairs):
+        B = small.band_to_dense(V[p])
+        ref[r] += B @ x[c]
+        ref[c] += B.conj() @ x[r]
+    assert np.abs(y - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_balanced_operator_is_hermitian_field_free(small):
+    ham = small.ham()
+    ham.set_scaling([1.0, 0.0])
+    n = small.L * small.N
+    M = np.zeros((n, n), dtype=np.complex128)
+    for k in range(n):
+        e = np.zeros(n, dtype=np.complex128)
+        e[k] = 1
+        M[:, k] = ham.balanced(e.reshape(small.shape)).ravel()
+    assert np.abs(M - M.conj().T).max() < 1e-10 * np.abs(M).max()
+    # U^-H H U^-1 reproduces the generalised eigenvalues of (H0, S)
+    E = np.sort(np.linalg.eigvalsh(0.5 * (M + M.conj().T)))
+    Eref = np.sort(np.concatenate([scipy.linalg.eigh(small.band_to_dense(small.H0[l]), small.overlap_dense(),
+                                                     eigvals_only=True) for l in range(small.L)]))
+    assert np.abs(E[:5] - Eref[:5]).max() < 1e-9
+
+
+def test_pade_vs_expm_and_testpade_matrix():
+    # pampwrapper.h:84-100 TestPade input; closed form by eigendecomposition
+    A = np.array([[-49.0, 24.0], [24.0, 31.0]], dtype=np.complex128)
+    w, V = np.linalg.eigh(A.real)
+    for M in (A / 64, 1j * A):
+        ref = V @ np.diag(np.exp(w * (M[0, 0] / A[0, 0]))) @ V.T
+        E, info = O.scaling_pade_exp(M)
+        assert info == 0
+        # Pade(6) after scaling to max|A_ij| <= 1 is accurate to a few 1e-12 here
+        assert np.abs(E - ref).max() < 5e-12 * max(1, np.abs(ref).max())
+    rng = np.random.default_rng(5)
+    H = rng.standard_normal((12, 12)) + 1j * rng.standard_normal((12, 12))
+    H = 0.5 * (H + H.conj().T)
+    A = -1j * 0.05 * H
+    E, info = O.scaling_pade_exp(A)
+    assert np.abs(E - scipy.linalg.expm(A)).max() < 1e-13
+
+
+def test_pamp_step_vs_dense_expm(small):
+    ham = small.ham()
+    s = 0.7
+    ham.set_scaling([1.0, s])
+    n = small.L * small.N
+    M = np.zeros((n, n), dtype=np.complex128)
+    for k in range(n):
+        e = np.zeros(n, dtype=np.complex128)
+        e[k] = 1
+        M[:, k] = ham.balanced(e.reshape(small.shape)).ravel()
+    psi0 = small.ground_state()
+    dt = 0.01
+    ref_t = scipy.linalg.expm(-1j * dt * M) @ band_mul_u(small, psi0).ravel()
+    ref = O.band_apply(O.OP_SOLVE_N, 1, ref_t.reshape(small.shape), small.U, small.kd)
+    got = ham.propagate_pamp(psi0, 1, dt, [[1.0, s]], 20)
+    assert np.abs(got - ref).max() < 1e-12
+
+
+def band_mul_u(h, x):
+    return O.band_apply(O.OP_MUL_N, 1, x, h.U, h.kd)
+
+
+def test_pamp_norm_conservation(small):
+    ham = small.ham()
+    psi0 = small.ground_state()
+    ts = O.step_times(40, 0.05)
+    sc = [[1.0, small.scaling(t)] for t in ts]
+    psi = ham.propagate_pamp(psi0, 40, 0.05, sc, 16)
+    assert abs(small.norm(psi) - 1.0) < 1e-12
+
+
+def test_field_free_phase():
+    h = O.Hydrogen(rmax=40.0, nbp=50, order=7, lmax=2)
+    psi0 = h.ground_state()
+    E, V = h.bound_states(0)
+    ham = h.ham()
+    T = 1.0
+    psi = ham.propagate_pamp(psi0, 20, T / 20, [[1.0, 0.0]] * 20, 16)
+    S = h.overlap_dense()
+    ov = np.conj(psi0[0]) @ S @ psi[0]
+    assert abs(ov - np.exp(-1j * E[0] * T)) < 1e-9
+
+
+def test_planted_mistake_is_caught(small):
+    """Flipping the transposed-pair sign of the d/dr term breaks the
+    anti-hermiticity the velocity coupling must have."""
+    d1 = small.bs.project_bandnh(np.ones_like(small.bs.x), 1)
+    D = small.band_to_dense(d1)
+    assert np.abs(D + D.T).max() < 1e-12          # <B_i|d/dr|B_j> is antisymmetric
+    bad = D.copy()
+    iu = np.tril_indices(small.N, -1)
+    bad[iu] = -bad[iu]                              # drop the (-1)^d factor
+    assert np.abs(bad + bad.T).max() > 1e-3
