@@ -76,6 +76,11 @@ def lib():
         L.orc_pamp_error_estimate.argtypes = [P]
         L.orc_pamp_error_estimate.restype = ctypes.c_double
         L.orc_propagate_pamp.argtypes = [P, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int, ctypes.c_int]
+        L.orc_zgpadm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, dp, dp, ip]
+        L.orc_zgexpv.argtypes = [P, ctypes.c_int, ctypes.c_double, dp, dp, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_int, ip]
+        L.orc_propagate_expokit.argtypes = [P, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_double, ip]
         _LIB = L
     return _LIB
 
@@ -431,6 +436,25 @@ class Ham:
         lib().orc_ham_inner(self.h, _dp(a.view(np.float64)), _dp(b.view(np.float64)), _dp(out))
         return out[0] + 1j * out[1]
 
+    def zgexpv(self, v, t, m, tol=0.0, anorm=1e6, imag_time=False):
+        """w = exp(t A) v, A = -i U^-H H U^-1 (zgexpv.cpp:32-586); returns (w, iflag, stats)."""
+        v = np.ascontiguousarray(v, dtype=np.complex128)
+        w = np.zeros_like(v)
+        st = np.zeros(5, dtype=np.int32)
+        rc = lib().orc_zgexpv(self.h, int(m), float(t), _dp(v.view(np.float64)), _dp(w.view(np.float64)),
+                              float(tol), float(anorm), int(bool(imag_time)), _ip(st))
+        return w, rc, st
+
+    def propagate_expokit(self, psi, nsteps, dt, scalings, m, tol=0.0, anorm=1e6):
+        psi = np.array(psi, dtype=np.complex128, order="C")
+        sc = np.ascontiguousarray(np.asarray(scalings, dtype=np.float64).reshape(nsteps, self.nterms))
+        st = np.zeros(5, dtype=np.int32)
+        rc = lib().orc_propagate_expokit(self.h, _dp(psi.view(np.float64)), nsteps, float(dt), _dp(sc), int(m),
+                                         float(tol), float(anorm), _ip(st))
+        if rc:
+            raise RuntimeError("Expokit returned nonzero value %d" % rc)
+        return psi, st
+
     def propagate_pamp(self, psi, nsteps, dt, scalings, m, reorth=False):
         psi = np.array(psi, dtype=np.complex128, order="C")
         sc = np.ascontiguousarray(np.asarray(scalings, dtype=np.float64).reshape(nsteps, self.nterms))
@@ -439,6 +463,19 @@ class Ham:
         if rc:
             raise RuntimeError("oracle pAMP failed (%d)" % rc)
         return psi
+
+
+def zgpadm(A, t, ideg=6):
+    """Expokit zgpadm (zgpadm.cpp:28-285): exp(t A), returns (E, ns)."""
+    A = np.array(A, dtype=np.complex128)
+    m = A.shape[0]
+    a = A.reshape(-1, order="F").copy()
+    out = np.zeros(m * m, dtype=np.complex128)
+    ns = ctypes.c_int(0)
+    rc = lib().orc_zgpadm(ideg, m, float(t), _dp(a.view(np.float64)), _dp(out.view(np.float64)), ctypes.byref(ns))
+    if rc:
+        raise RuntimeError("zgpadm failed %d" % rc)
+    return out.reshape((m, m), order="F"), ns.value
 
 
 def pade_exp(A, order=6):

@@ -872,3 +872,211 @@ ORC_API int orc_propagate_pamp(orc_ham *h, double *psid, int nsteps, double dt,
     orc_pamp_destroy(p);
     return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* Expokit (Sidje, ACM TOMS 24(1):130-156, 1998) as translated by f2c  */
+/* in core/krylov/expokit/f2c/zgpadm.cpp:28-285 and zgexpv.cpp:32-586. */
+/* ------------------------------------------------------------------ */
+
+/* zgpadm: exp(t H) for an m x m complex matrix (column-major, ldh = m), */
+/* irreducible Pade(ideg) with ns = max(0, int(log2 ||tH||_inf) + 2).    */
+ORC_API int orc_zgpadm(int ideg, int m, double t, const double *Hd, double *outd, int *ns_out)
+{
+    const zc *H = (const zc *)Hd;
+    zc *out = (zc *)outd;
+    size_t sz = sizeof(zc) * m * m;
+    double hnorm = 0.0;
+    for (int i = 0; i < m; i++) {
+        double rs = 0.0;
+        for (int j = 0; j < m; j++) rs += cabs(H[i + j * m]);
+        if (rs > hnorm) hnorm = rs;
+    }
+    hnorm = fabs(t * hnorm);
+    if (hnorm == 0.0) return -1;                     /* "null H" stop */
+    int ns = (int)(log(hnorm) / log(2.0)) + 2;
+    if (ns < 0) ns = 0;
+    double scale = t / pow(2.0, ns);
+    double scale2 = scale * scale;
+    double coef[32];
+    coef[0] = 1.0;
+    for (int k = 1; k <= ideg; k++)
+        coef[k] = coef[k - 1] * (double)(ideg + 1 - k) / (double)(k * (2 * ideg + 1 - k));
+    zc *H2 = malloc(sz), *P = calloc(m * m, sizeof(zc)), *Q = calloc(m * m, sizeof(zc)), *F = malloc(sz);
+    int *piv = malloc(sizeof(int) * m);
+    zmm(m, H, H, H2);
+    for (int e = 0; e < m * m; e++) H2[e] *= scale2;
+    for (int j = 0; j < m; j++) { P[j + j * m] = coef[ideg - 1]; Q[j + j * m] = coef[ideg]; }
+    int iodd = 1;
+    for (int k = ideg - 1; k > 0; k--) {
+        zc *U = iodd ? Q : P;
+        zmm(m, U, H2, F);
+        for (int j = 0; j < m; j++) F[j + j * m] += coef[k - 1];
+        memcpy(U, F, sz);
+        iodd = 1 - iodd;
+    }
+    if (iodd) { zmm(m, Q, H, F); for (int e = 0; e < m * m; e++) Q[e] = scale * F[e]; }
+    else { zmm(m, P, H, F); for (int e = 0; e < m * m; e++) P[e] = scale * F[e]; }
+    for (int e = 0; e < m * m; e++) Q[e] -= P[e];
+    int info = zlu_solve(m, Q, P, m, piv);
+    if (info) { free(H2); free(P); free(Q); free(F); free(piv); return info; }
+    for (int e = 0; e < m * m; e++) P[e] *= 2.0;
+    for (int j = 0; j < m; j++) P[j + j * m] += 1.0;
+    if (ns == 0 && iodd) for (int e = 0; e < m * m; e++) P[e] = -P[e];
+    else for (int k = 0; k < ns; k++) { zmm(m, P, P, F); memcpy(P, F, sz); }
+    memcpy(out, P, sz);
+    if (ns_out) *ns_out = ns;
+    free(H2); free(P); free(Q); free(F); free(piv);
+    return 0;
+}
+
+/* f2c helpers: i_dnnt rounds half away from zero, d_int truncates. */
+static double x_roundstep(double t)
+{
+    double sqr1 = sqrt(0.1);
+    double e = log10(t) - sqr1;
+    int ie = (int)(e >= 0 ? floor(e + 0.5) : -floor(0.5 - e)) - 1;
+    double p1 = pow(10.0, ie);
+    return trunc(t / p1 + 0.55) * p1;
+}
+
+typedef struct { orc_ham *h; int imag_time; zc *tmp; } orc_exp_ctx;
+
+/* MultiplyHamiltonian (krylovcommon.h:15-78): out = -i H^ in (real time), -H^ in (imaginary) */
+static void exp_matvec(orc_exp_ctx *c, const zc *in, zc *out)
+{
+    orc_ham_balanced(c->h, (const double *)in, (double *)out);
+    long n = c->h->n;
+    if (!c->imag_time) for (long i = 0; i < n; i++) out[i] *= -I;
+    else for (long i = 0; i < n; i++) out[i] = -out[i];
+}
+
+static double dznrm2(long n, const zc *x) { return sqrt(creal(zdotc(n, x, x))); }
+
+/* zgexpv: w = exp(t A) v with A the balanced operator times -i (or -1).   */
+/* Returns iflag (0 ok, 1 max 500 steps); stats[0..4] = nmult, nexph,     */
+/* nstep, nreject, mbrkdwn.                                                */
+ORC_API int orc_zgexpv(orc_ham *h, int m, double t, const double *vd, double *wd, double tol, double anorm,
+                       int imag_time, int *stats)
+{
+    long n = h->n;
+    const zc *v = (const zc *)vd;
+    zc *w = (zc *)wd;
+    int mh = m + 2;
+    zc *V = calloc((size_t)n * (m + 1) + n, sizeof(zc));
+    zc *Hm = calloc((size_t)mh * mh, sizeof(zc));
+    zc *E = malloc(sizeof(zc) * mh * mh);
+    orc_exp_ctx ctx = { h, imag_time, NULL };
+    int k1 = 2, mbrkdwn = m, nmult = 0, nreject = 0, nexph = 0, nstep = 0, iflag = 0;
+    double t_out = fabs(t), t_now = 0.0, t_new = 0.0, avnorm = 0.0;
+    double eps = DBL_EPSILON;   /* the 4/3 loop of zgexpv.cpp:277-283 gives 2^-52 */
+    if (tol <= eps) tol = sqrt(eps);
+    double rndoff = eps * anorm, break_tol = 1e-7;
+    double sgn = t >= 0 ? 1.0 : -1.0;
+    memcpy(w, v, sizeof(zc) * n);
+    double beta = dznrm2(n, w);
+    double sqr1 = sqrt(0.1);
+    double xm = 1.0 / m;
+    double p2 = tol * pow((m + 1) / 2.72, m + 1) * sqrt((m + 1) * 6.28);
+    t_new = (1.0 / anorm) * pow(p2 / (4.0 * beta * anorm), xm);
+    (void)sqr1;
+    t_new = x_roundstep(t_new);
+    double err_loc = 0.0, t_step = 0.0;
+    while (t_now < t_out) {
+        nstep++;
+        t_step = fmin(t_out - t_now, t_new);
+        double p1 = 1.0 / beta;
+        for (long i = 0; i < n; i++) V[i] = p1 * w[i];
+        memset(Hm, 0, sizeof(zc) * mh * mh);
+        k1 = 2; mbrkdwn = m;
+        int j;
+        for (j = 1; j <= m; j++) {
+            nmult++;
+            zc *vj = V + (size_t)(j - 1) * n, *vj1 = V + (size_t)j * n;
+            exp_matvec(&ctx, vj, vj1);
+            for (int i = 1; i <= j; i++) {
+                zc hij = zdotc(n, V + (size_t)(i - 1) * n, vj1);
+                for (long q = 0; q < n; q++) vj1[q] -= hij * V[(size_t)(i - 1) * n + q];
+                Hm[(i - 1) + (j - 1) * mh] = hij;
+            }
+            double hj1j = dznrm2(n, vj1);
+            if (hj1j <= break_tol) {
+                k1 = 0; mbrkdwn = j; t_step = t_out - t_now;
+                break;
+            }
+            Hm[j + (j - 1) * mh] = hj1j;
+            for (long q = 0; q < n; q++) vj1[q] *= 1.0 / hj1j;
+        }
+        if (k1 != 0) {
+            nmult++;
+            exp_matvec(&ctx, V + (size_t)m * n, V + (size_t)(m + 1) * n);
+            avnorm = dznrm2(n, V + (size_t)(m + 1) * n);
+        }
+        Hm[(m + 1) + m * mh] = 1.0;
+        int ireject = 0;
+        double xmloc = 1.0 / m;
+        for (;;) {
+            nexph++;
+            int mx = mbrkdwn + k1;
+            /* zgpadm on the leading mx x mx block of the (mh x mh) H */
+            zc *Hs = malloc(sizeof(zc) * mx * mx);
+            for (int c = 0; c < mx; c++) for (int r = 0; r < mx; r++) Hs[r + c * mx] = Hm[r + c * mh];
+            int ns;
+            int info = orc_zgpadm(6, mx, sgn * t_step, (const double *)Hs, (double *)Hs, &ns);
+            for (int c = 0; c < mx; c++) for (int r = 0; r < mx; r++) E[r + c * mh] = Hs[r + c * mx];
+            free(Hs);
+            if (info) { iflag = -9; goto done; }
+            if (k1 == 0) err_loc = tol;
+            else {
+                double q1 = cabs(E[m]) * beta;
+                double q2 = cabs(E[m + 1]) * beta * avnorm;
+                if (q1 > 10.0 * q2) { err_loc = q2; xmloc = 1.0 / m; }
+                else if (q1 > q2) { err_loc = q1 * q2 / (q1 - q2); xmloc = 1.0 / m; }
+                else { err_loc = q1; xmloc = 1.0 / (m - 1); }
+            }
+            if (k1 != 0 && err_loc > 1.2 * t_step * tol) {
+                t_step = 0.9 * t_step * pow(t_step * tol / err_loc, xmloc);
+                t_step = x_roundstep(t_step);
+                ireject++; nreject++;
+                continue;
+            }
+            break;
+        }
+        int mx = mbrkdwn + (k1 - 1 > 0 ? k1 - 1 : 0);
+        for (long i = 0; i < n; i++) {
+            zc s = 0.0;
+            for (int k = 0; k < mx; k++) s += V[(size_t)k * n + i] * E[k];
+            w[i] = beta * s;
+        }
+        beta = dznrm2(n, w);
+        t_new = 0.9 * t_step * pow(t_step * tol / err_loc, xmloc);
+        t_new = x_roundstep(t_new);
+        err_loc = fmax(err_loc, rndoff);
+        t_now += t_step;
+        if (nstep >= 500 && t_now < t_out) { iflag = 1; break; }
+    }
+done:
+    if (stats) { stats[0] = nmult; stats[1] = nexph; stats[2] = nstep; stats[3] = nreject; stats[4] = mbrkdwn; }
+    free(V); free(Hm); free(E);
+    return iflag;
+}
+
+/* ExpokitPropagator::AdvanceStep (expokitpropagator.cpp:36-103) inside the */
+/* balanced-overlap wrapping of KrylovPropagator.py:41-45.  The time passed  */
+/* to zgexpv is |dt| (the reference's line 61 computes dt^2 for real dt).    */
+ORC_API int orc_propagate_expokit(orc_ham *h, double *psid, int nsteps, double dt, const double *scalings,
+                                  int m, double tol, double anorm, int *stats)
+{
+    zc *w = malloc(sizeof(zc) * h->n);
+    int rc = 0;
+    for (int s = 0; s < nsteps && rc == 0; s++) {
+        orc_ham_set_scaling(h, scalings + (size_t)s * h->nterms);
+        for (int r = 0; r < h->R; r++)
+            if (h->U[r]) orc_band_apply(OP_MUL_N, r, h->R, h->shape, h->kd[r], (const double *)h->U[r], psid);
+        rc = orc_zgexpv(h, m, fabs(dt), psid, (double *)w, tol, anorm, 0, stats);
+        memcpy(psid, w, sizeof(zc) * h->n);
+        for (int r = 0; r < h->R; r++)
+            if (h->U[r]) orc_band_apply(OP_SOLVE_N, r, h->R, h->shape, h->kd[r], (const double *)h->U[r], psid);
+    }
+    free(w);
+    return rc;
+}

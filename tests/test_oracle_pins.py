@@ -238,3 +238,39 @@ def test_planted_mistake_is_caught(small):
     iu = np.tril_indices(small.N, -1)
     bad[iu] = -bad[iu]                              # drop the (-1)^d factor
     assert np.abs(bad + bad.T).max() > 1e-3
+
+
+def test_zgpadm_vs_expm():
+    rng = np.random.default_rng(7)
+    for m, t in [(5, 0.3), (12, 2.0), (22, 0.01)]:
+        A = rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))
+        E, ns = O.zgpadm(A, t)
+        ref = scipy.linalg.expm(t * A)
+        assert np.abs(E - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_zgexpv_vs_dense_expm(small):
+    ham = small.ham()
+    ham.set_scaling([1.0, 0.4])
+    n = small.L * small.N
+    M = np.zeros((n, n), dtype=np.complex128)
+    for k in range(n):
+        e = np.zeros(n, dtype=np.complex128)
+        e[k] = 1
+        M[:, k] = ham.balanced(e.reshape(small.shape)).ravel()
+    v = small.ground_state()
+    for t in [0.01, 0.2]:
+        w, rc, st = ham.zgexpv(v, t, 20, tol=0.0, anorm=np.abs(M).sum(axis=1).max())
+        assert rc == 0
+        ref = (scipy.linalg.expm(-1j * t * M) @ v.ravel()).reshape(small.shape)
+        assert np.abs(w - ref).max() < 1e-9, (t, np.abs(w - ref).max(), st)
+
+
+def test_expokit_propagation_agrees_with_pamp(small):
+    ham = small.ham()
+    psi0 = small.ground_state()
+    ts = O.step_times(10, 0.02, t0=3.0)
+    sc = [[1.0, small.scaling(t)] for t in ts]
+    a = ham.propagate_pamp(psi0, 10, 0.02, sc, 20)
+    b, st = ham.propagate_expokit(psi0, 10, 0.02, sc, 20, tol=0.0, anorm=1e6)
+    assert np.abs(a - b).max() < 1e-9
