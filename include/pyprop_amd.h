@@ -84,6 +84,11 @@ int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re, double dt
  * (pamp.h:666; -1 if the Pade denominator was singular, pamp.h:544-550),
  * basis size used (< m after happy breakdown), |U psi|, re-orthogonalisation passes. */
 int pp_pamp_diagnostics(pp_pamp *p, int nsteps, double *diag, pp_stream stream);
+/* Profiling hook (no reference counterpart; the reference keeps per-phase
+ * Timers in pAMP, pamp.h:41,125,295-349): record wall-clock stamps (100 MHz)
+ * of workgroup 0 around every grid barrier of step 0 of later launches. */
+int pp_pamp_set_timing(pp_pamp *p, int enable);
+int pp_pamp_timings(pp_pamp *p, long long *out, int max, int *count);
 int pp_pamp_destroy(pp_pamp *p);
 
 #ifdef __cplusplus

@@ -51,6 +51,7 @@ struct pp_pamp_args {
   zd *Hm;              // m x m column-major, owned by workgroup 0
   zd *cvec;            // m, exp(-i dt H) e1 broadcast
   double *diag;        // nsteps x 4: error estimate, basis used, |psi~|, re-orth passes
+  long long *tlog;     // optional: wall-clock stamps of workgroup 0 during step 0
 };
 
 struct pp_pamp {
@@ -62,7 +63,11 @@ struct pp_pamp {
   zd *V, *y, *Hm, *cvec;
   double *part, *diag, *scal;
   int diag_cap, scal_cap, pstr;
+  long long *tlog;
+  int timing;
 };
+
+#define PP_TLOG_MAX 512
 
 // ------------------------------------------------------------------ helpers
 __device__ inline double wave_sum(double v)
@@ -111,6 +116,35 @@ __device__ inline void block_sums(int cnt, F val, double *wave_lds, double *red_
     double acc = 0.0;
     for (int w = 0; w < PP_THREADS / 64; w++) acc += wave_lds[w * cnt + tid];
     red_out[tid] = acc;
+  }
+  __syncthreads();
+}
+
+// out[2k], out[2k+1] = <V_k, W> over one fiber for k < nk (V_k in global
+// memory, W in LDS); with extra_norm also out[2nk] = |W|^2.  Wave w owns
+// k = w, w+4, ...; each lane streams a strided slice of the fiber so the
+// loads of one k are independent and pipeline.
+__device__ inline void fiber_dots(const zd *V, const zd *W, int nk, int N, bool extra_norm, double *out)
+{
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int ntot = nk + (extra_norm ? 1 : 0);
+  for (int k = wave; k < ntot; k += PP_THREADS / 64) {
+    double re = 0.0, im = 0.0;
+    if (k < nk) {
+      const zd *vk = V + (size_t)k * N;
+#pragma unroll 4
+      for (int i = lane; i < N; i += 64) {
+        zd v = vk[i], w = W[i];
+        re = fma(v.x, w.x, re); re = fma(v.y, w.y, re);
+        im = fma(v.x, w.y, im); im = fma(-v.y, w.x, im);
+      }
+    } else {
+#pragma unroll 4
+      for (int i = lane; i < N; i += 64) { zd w = W[i]; re = fma(w.x, w.x, re); re = fma(w.y, w.y, re); }
+    }
+    re = wave_sum(re);
+    im = wave_sum(im);
+    if (lane == 0) { out[2 * k] = re; if (k < nk) out[2 * k + 1] = im; }
   }
   __syncthreads();
 }
@@ -271,6 +305,10 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
   const pp_overlap &ov = A.ov;
   zd *Va = A.V + (size_t)a * m * N;
   double *mypart = A.part + (size_t)a * A.pstr;
+  int tcount = 0;
+  auto stamp = [&](int step) {
+    if (A.tlog && step == 0 && a == 0 && tid == 0 && tcount < PP_TLOG_MAX) A.tlog[tcount++] = (long long)wall_clock64();
+  };
   const int NRM = 2 * m;        // slot of |w|^2 in the partial sums
   const int ONRM = 2 * m + 1;   // slot of the pre-projection norm (re-orthogonalisation)
 
@@ -295,7 +333,9 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
     if (tid < 64) pp_band_solve_wave<KD, MAXB>(ov.solveN, X, false, N);
     __syncthreads();
     for (int i = tid; i < N; i += PP_THREADS) A.y[(size_t)a * N + i] = X[i];
+    stamp(step);
     grid.sync();
+    stamp(step);
 
     // Every workgroup executes the same number of grid barriers whatever the
     // data: breakdown and re-orthogonalisation only predicate work, so a
@@ -349,25 +389,20 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
             __syncthreads();
           }
         }
+        stamp(step);
         if (tid < 64) pp_band_solve_wave<KD, MAXB>(ov.solveC, X, true, N);
         __syncthreads();
+        stamp(step);
         // w' -> W ; partial <v_k, w'> for k <= j ; |w'|^2 for re-orthogonalisation
         for (int i = tid; i < N; i += PP_THREADS) W[i] = X[i];
         __syncthreads();
-        block_sums(ncomp + (A.reorth ? 1 : 0), [&](int c) {
-          double s = 0.0;
-          if (c < ncomp) {
-            const zd *vk = Va + (size_t)(c >> 1) * N;
-            if ((c & 1) == 0) { for (int i = tid; i < N; i += PP_THREADS) { zd v = vk[i], w = W[i]; s += v.x * w.x + v.y * w.y; } }
-            else { for (int i = tid; i < N; i += PP_THREADS) { zd v = vk[i], w = W[i]; s += v.x * w.y - v.y * w.x; } }
-          } else {
-            for (int i = tid; i < N; i += PP_THREADS) s += zabs2(W[i]);
-          }
-          return s; }, wsum, hred);
+        fiber_dots(Va, W, j + 1, N, A.reorth != 0, hred);
         for (int c = tid; c < ncomp; c += PP_THREADS) mypart[c] = hred[c];
         if (A.reorth && tid == 0) mypart[ONRM] = hred[ncomp];
       }
+      stamp(step);
       grid.sync();
+      stamp(step);
 
       // ---------------- phase B: h = sum_a partials ; w' -= V h ; |w'|^2 ; y = U^-1 w'
       if (!broken) {
@@ -378,6 +413,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
       if (!broken) {
         for (int i = tid; i < N; i += PP_THREADS) {
           zd w = W[i];
+#pragma unroll 4
           for (int k = 0; k <= j; k++) {
             zd h = zmk(hred[2 * k], hred[2 * k + 1]);
             zd v = Va[(size_t)k * N + i];
@@ -414,12 +450,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
             else stop = true;
           }
           if (doit) {
-            block_sums(ncomp, [&](int c) {
-              double s = 0.0;
-              const zd *vk = Va + (size_t)(c >> 1) * N;
-              if ((c & 1) == 0) { for (int i = tid; i < N; i += PP_THREADS) { zd v = vk[i], w = W[i]; s += v.x * w.x + v.y * w.y; } }
-              else { for (int i = tid; i < N; i += PP_THREADS) { zd v = vk[i], w = W[i]; s += v.x * w.y - v.y * w.x; } }
-              return s; }, wsum, hred);
+            fiber_dots(Va, W, j + 1, N, false, hred);
             for (int c = tid; c < ncomp; c += PP_THREADS) mypart[c] = hred[c];
           }
           grid.sync();
@@ -449,11 +480,14 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
         if (tid == 0) mypart[NRM] = hred[0];
         for (int i = tid; i < padN; i += PP_THREADS) X[i] = (i < N) ? W[i] : zmk(0.0, 0.0);
         __syncthreads();
+        stamp(step);
         if (tid < 64) pp_band_solve_wave<KD, MAXB>(ov.solveN, X, false, N);
         __syncthreads();
         for (int i = tid; i < N; i += PP_THREADS) A.y[(size_t)a * N + i] = X[i];
       }
+      stamp(step);
       grid.sync();
+      stamp(step);
     }
 
     // ---------------- exponentiate on workgroup 0, broadcast exp(-i dt H) e1
@@ -468,7 +502,9 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
       __syncthreads();
       for (int e = tid; e < m * m; e += PP_THREADS) A.Hm[e] = zmk(0.0, 0.0);
     }
+    stamp(step);
     grid.sync();
+    stamp(step);
     // ---------------- psi~ = |psi~| V c ; psi = U^-1 psi~ (SolveSqrtOverlap)
     for (int i = tid; i < padN; i += PP_THREADS) {
       zd acc = zmk(0.0, 0.0);
@@ -481,7 +517,9 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
     __syncthreads();
     for (int i = tid; i < N; i += PP_THREADS) A.psi[(size_t)a * N + i] = X[i];
     __syncthreads();
+    stamp(step);
   }
+  if (A.tlog && a == 0 && tid == 0) A.tlog[PP_TLOG_MAX] = tcount;
 }
 
 // ------------------------------------------------------------------ host API
@@ -575,6 +613,7 @@ extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re
   A.ov = *ov;
   A.psi = (zd *)psi; A.V = p->V; A.y = p->y; A.part = p->part; A.pstr = p->pstr;
   A.Hm = p->Hm; A.cvec = p->cvec; A.diag = p->diag;
+  A.tlog = p->timing ? p->tlog : nullptr;
   size_t lds = pamp_lds_bytes(p, A.padN);
   void *args[] = {&A};
   int maxb = ov->solveN.B;
@@ -605,9 +644,37 @@ extern "C" int pp_pamp_diagnostics(pp_pamp *p, int nsteps, double *diag, hipStre
   return PP_OK;
 }
 
+// Profiling hook: when enabled, the next launches record wall-clock stamps
+// (100 MHz counter) of workgroup 0 around every grid barrier of step 0.
+extern "C" int pp_pamp_set_timing(pp_pamp *p, int enable)
+{
+  if (!p) return PP_EARG;
+  if (enable && !p->tlog) {
+    PP_HIP(hipMalloc(&p->tlog, sizeof(long long) * (PP_TLOG_MAX + 1)));
+    PP_HIP(hipMemset(p->tlog, 0, sizeof(long long) * (PP_TLOG_MAX + 1)));
+  }
+  p->timing = enable ? 1 : 0;
+  return PP_OK;
+}
+
+// Copy the stamps of the last timed launch (host, max entries); *count = stamps written.
+extern "C" int pp_pamp_timings(pp_pamp *p, long long *out, int max, int *count)
+{
+  if (!p || !out || !count || !p->tlog) return PP_EARG;
+  long long buf[PP_TLOG_MAX + 1];
+  PP_HIP(hipDeviceSynchronize());
+  PP_HIP(hipMemcpy(buf, p->tlog, sizeof(buf), hipMemcpyDeviceToHost));
+  int n = (int)buf[PP_TLOG_MAX];
+  if (n > max) n = max;
+  for (int i = 0; i < n; i++) out[i] = buf[i];
+  *count = n;
+  return PP_OK;
+}
+
 extern "C" int pp_pamp_destroy(pp_pamp *p)
 {
   if (!p) return PP_OK;
+  if (p->tlog) (void)hipFree(p->tlog);
   for (int t = 0; t < p->nterms; t++) if (p->csr_dev[t]) (void)hipFree(p->csr_dev[t]);
   if (p->V) (void)hipFree(p->V);
   if (p->y) (void)hipFree(p->y);

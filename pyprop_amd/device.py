@@ -140,6 +140,17 @@ class PampEngine:
         nat.check(nat.lib().pp_pamp_diagnostics(self.h, n, nat.dptr(d), stream_ptr()), "pp_pamp_diagnostics")
         return d
 
+    def set_timing(self, enable=True):
+        nat.check(nat.lib().pp_pamp_set_timing(self.h, int(bool(enable))), "pp_pamp_set_timing")
+
+    def timings(self):
+        """Wall-clock stamps (seconds, 100 MHz counter) of workgroup 0 during
+        step 0 of the last timed launch."""
+        buf = (ctypes.c_longlong * 512)()
+        cnt = ctypes.c_int(0)
+        nat.check(nat.lib().pp_pamp_timings(self.h, buf, 512, ctypes.byref(cnt)), "pp_pamp_timings")
+        return np.array(buf[:cnt.value], dtype=np.float64) / 1e8
+
     def __del__(self):
         if getattr(self, "h", None) and nat._lib is not None:
             nat._lib.pp_pamp_destroy(self.h)

@@ -48,17 +48,25 @@ def test_tensor_multiply_matches_oracle():
         assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("lmax,nbp,m,nsteps,reorth", [(6, 40, 12, 5, False), (10, 100, 20, 20, False), (6, 40, 12, 5, True)])
-def test_pamp_persistent_matches_oracle(lmax, nbp, m, nsteps, reorth):
+@pytest.mark.parametrize("lmax,nbp,m,nsteps,dt,reorth", [
+    (6, 40, 12, 5, 0.05, False), (10, 100, 20, 20, 0.01, False), (10, 100, 20, 20, 0.05, False),
+    (6, 40, 12, 5, 0.05, True), (63, 120, 24, 6, 0.005, False)])
+def test_pamp_persistent_matches_oracle(lmax, nbp, m, nsteps, dt, reorth):
+    """psi after nsteps pAMP steps, GPU vs oracle.  The bar is the oracle's own
+    sensitivity: the change a 1e-16 relative perturbation of psi0 causes (an
+    unconverged Krylov space amplifies rounding), times 10, floor 1e-12."""
     from pyprop_amd import device as D
     h = _hydrogen(lmax=lmax, nbp=nbp, rmax=float(nbp))
     psi0 = h.ground_state()
     # a little excited component so the dipole term matters from step 0
     psi0[1] = 0.1 * psi0[0]
-    dt = 0.05
     ts = O.step_times(nsteps, dt, t0=10.0)
     sc = np.array([[1.0, h.scaling(t)] for t in ts])
-    ref = h.ham().propagate_pamp(psi0, nsteps, dt, sc, m, reorth)
+    ham = h.ham()
+    ref = ham.propagate_pamp(psi0, nsteps, dt, sc, m, reorth)
+    rng = np.random.default_rng(11)
+    ref2 = ham.propagate_pamp(psi0 * (1 + 1e-16 * rng.standard_normal(psi0.shape)), nsteps, dt, sc, m, reorth)
+    tol = max(1e-12, 10 * np.abs(ref2 - ref).max())
     ov = D.Overlap(h.S, h.N, h.kd)
     eng = D.PampEngine(h.L, h.N, m, ov, reorth=reorth)
     eng.add_term("Diag", D.to_device(h.H0))
@@ -67,5 +75,5 @@ def test_pamp_persistent_matches_oracle(lmax, nbp, m, nsteps, reorth):
     eng.propagate(psi, nsteps, dt, sc)
     got = psi.cpu().numpy()
     err = np.abs(got - ref).max()
-    print("pamp", lmax, nbp, m, nsteps, reorth, err, eng.diagnostics()[-1])
-    assert err <= 1e-11
+    print("pamp", lmax, nbp, m, nsteps, dt, reorth, "err", err, "tol", tol, eng.diagnostics()[-1])
+    assert err <= tol
