@@ -44,7 +44,7 @@ int pp_band_cholesky_upper(int N, int kd, double *ab);
 /* OverlapMatrix::Setup (overlapmatrix.cpp:9-104): S in upper band storage on
  * the host, factorised and uploaded with both solve plans.  *info = zpbtrf
  * info (the reference leaves it unchecked; here info>0 returns 1).
- * Supported: kd in {4,5,6,8} (B-spline orders 5,6,7,9), N <= 2048. */
+ * Supported: kd in {4,5,6,8} (B-spline orders 5,6,7,9), N <= 1024. */
 int pp_overlap_create(int N, int kd, const double *S_upper, pp_overlap **out, int *info);
 /* Copy the real Cholesky factor U (N*(kd+1) doubles, upper band storage) to the host. */
 int pp_overlap_factor(const pp_overlap *ov, double *U_upper_host);
@@ -73,9 +73,25 @@ int pp_tp_multiply(int R, const int *storage, const int *potShape, const void *p
 int pp_pamp_create(int L, int N, int m, int reorth, const pp_overlap *ov, pp_pamp **out);
 /* One consolidated tensor potential: angular storage (Ident/Diag/Simp/Herm,
  * P0 compressed indices, host pairs for Simp/Herm) x radial BandNH of total
- * band width bw; `potential` device (P0, N*bw) in the reference layout. */
+ * band width bw; `potential` device (P0, N*bw) in the reference layout, as
+ * TensorPotential.PotentialData (pyprop/tensorpotential/TensorPotential.py:33-77).
+ * The library factors it into real radial bands x complex angular
+ * coefficients and returns 1 if that reconstruction misses V by more than
+ * 1e-12 max|V| (the assembled tensor itself is low rank only to ~1e-14). */
 int pp_pamp_add_term(pp_pamp *p, int angular_storage, int P0, const int *pairs, int radial_storage,
                      int bw, const void *potential);
+/* Same term already factored: V[p,:] = sum_r coef[p,r] bands[r,:] with host
+ * coef (P0 x rank complex) and real host bands (rank x N*bw, BandNH layout).
+ * No reference counterpart: the generator of the reference always stores
+ * the full tensor (TensorPotentialGenerator.py:72-256). */
+int pp_pamp_add_factored_term(pp_pamp *p, int angular_storage, int P0, const int *pairs,
+                              int radial_storage, int bw, int rank, const double *coef,
+                              const double *bands);
+/* Host helper behind pp_pamp_add_term: V (host, P0 x M complex) ~= coef
+ * (P0 x rank complex) x bands (rank x M real rows of Re/Im V, pivoted).
+ * No reference counterpart.  *relerr = max|V - recon| / max|V|. */
+int pp_lowrank_factor(int P0, long M, const double *V, int maxrank, int *rank_out, double *coef,
+                      double *bands, double *relerr);
 /* nsteps consecutive PamPropagator.AdvanceStep(t_n, dt) (PamPropagator.py:50-59)
  * in one launch; scal (host, nsteps x nterms) = each term's time factor at t_n. */
 int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re, double dt_im,

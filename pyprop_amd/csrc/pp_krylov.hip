@@ -11,65 +11,120 @@
 // re-orthogonalisation, 0.717 criterion, <= 3 passes), :430-564 (scaling and
 // Pade), :613-669 (PropagateVector); Problem.py:158-173 (H held at t_n).
 //
-// Work split: the wavefunction is (L, N) with the angular rank outer and the
-// B-spline rank contiguous; workgroup a owns fiber a for the whole run.  The
-// Arnoldi vectors, the residual and the Gram-Schmidt projections of fiber a
-// never leave workgroup a (residual in LDS, V fiber-major in HBM/L2).  Only
-// U^-1 w (needed by the neighbours through the angular couplings) and the
-// per-fiber partial inner products cross workgroups, so each Arnoldi
-// iteration costs two grid barriers.  Every workgroup reduces the partial
-// sums itself in one fixed order, so branch decisions (breakdown,
-// re-orthogonalisation) are bitwise identical across the grid.
+// Work split: the wavefunction is (L, N), angular rank outer, B-spline rank
+// contiguous; workgroup a owns fiber a for the whole launch.  Its Arnoldi
+// vectors (fiber-major, L2-resident), residual (LDS) and the chunked solver
+// plan (LDS) never leave it.  Only U^-1 w (needed by the neighbours through
+// the angular couplings), the per-fiber partial inner products and the
+// Krylov coefficients cross workgroups; those go through agent-scope atomic
+// loads/stores, so the grid barrier needs no cache write-back/invalidate and
+// the private data stays cached.  Each Arnoldi iteration costs two barriers;
+// every workgroup reduces the partial sums itself in one fixed order, and
+// breakdown / re-orthogonalisation only predicate work, so all workgroups
+// execute the same barrier sequence whatever the data.
+//
+// Tensor potentials arrive factored as sum_r c_{p,r} B_r with real radial
+// bands B_r (diagonal-major, shared by all workgroups and L2-resident) and
+// complex angular coefficients c (scalar per workgroup).
 #include "pp_common.h"
 #include <hip/hip_cooperative_groups.h>
 #include <cstring>
 #include <cmath>
-
-namespace cg = cooperative_groups;
+#include <vector>
+#include <algorithm>
 
 #define PP_MAXTERMS 8
 #define PP_MAXM 64
+#define PP_MAXSLOTS 4
+#define PP_MAXCOMBO 16
 #define PP_THREADS 256
+#define PP_TLOG_MAX 512
 
-struct pp_kterm {
-  const int *rowptr;  // L+1
-  const int *ent;     // (col, pidx, conj) per entry
-  const zd *pot;      // (P0, N*bw) reference BandNH layout
+struct pp_fterm {
+  const int *rowptr;   // L+1
+  const int *ent;      // (slot, pidx, conj) per entry
+  const zd *coef;      // P0 x rank
+  const double *bands; // rank x bw x N, element (i, i+d) at (r*bw + d+s)*N + i
+  int rank;
+};
+
+struct pp_barrier {
+  unsigned count, gen, err, pad;
 };
 
 struct pp_pamp_args {
-  int L, N, m, s, bw, nterms, nsteps, reorth, padN;
+  int L, N, m, s, bw, nterms, nsteps, reorth, padN, nslots, ncombo, scrDoubles;
   double dt_re, dt_im;
-  pp_kterm term[PP_MAXTERMS];
-  const double *scal;  // nsteps x nterms
-  pp_overlap ov;
-  zd *psi;             // L x N, in/out
-  zd *V;               // L x m x N (fiber-major)
-  zd *y;               // L x N, U^-1 w exchanged between workgroups
-  double *part;        // L x pstr partial sums
+  pp_fterm term[PP_MAXTERMS];
+  const int *slotcols;   // L x nslots source fibers staged per row (-1 unused)
+  const double *scal;    // nsteps x nterms
+  const double *Uband;   // real U band for psi~ = U psi
+  const double *plan;    // packed chunk plan (copied to LDS)
+  int planP, planLevels, planDoubles;
+  zd *psi;               // L x N, in/out (fiber a private to workgroup a)
+  zd *V;                 // L x m x N (fiber-major, private)
+  zd *y;                 // L x N, U^-1 w (exchanged)
+  double *part;          // L x pstr partial sums (exchanged)
   int pstr;
-  zd *Hm;              // m x m column-major, owned by workgroup 0
-  zd *cvec;            // m, exp(-i dt H) e1 broadcast
-  double *diag;        // nsteps x 4: error estimate, basis used, |psi~|, re-orth passes
-  long long *tlog;     // optional: wall-clock stamps of workgroup 0 during step 0
+  zd *Hm;                // m x m column-major, private to workgroup 0
+  zd *cvec;              // m, exp(-i dt H) e1 (exchanged)
+  double *diag;          // nsteps x 4
+  pp_barrier *bar;
+  long long *tlog;
 };
 
 struct pp_pamp {
   int L, N, m, reorth, s, bw;
   const pp_overlap *ov;
-  int nterms;
-  pp_kterm term[PP_MAXTERMS];
-  int *csr_dev[PP_MAXTERMS];
+  int nterms, nslots;
+  pp_fterm term[PP_MAXTERMS];
+  std::vector<std::vector<int>> rowcols;   // per row: distinct source fibers
+  std::vector<std::vector<int>> termrows;  // per term: rowptr (L+1) + (col, pidx, conj) entries
+  void *termbuf[PP_MAXTERMS];
+  int *slotcols;
+  bool dirty;
   zd *V, *y, *Hm, *cvec;
   double *part, *diag, *scal;
+  pp_barrier *bar;
   int diag_cap, scal_cap, pstr;
   long long *tlog;
   int timing;
 };
 
-#define PP_TLOG_MAX 512
+// ------------------------------------------------------------------ exchanged data
+__device__ inline void xst(double *p, double v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline double xld(const double *p) { return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline void zst(zd *p, zd v) { xst(&p->x, v.x); xst(&p->y, v.y); }
+__device__ inline zd zld(const zd *p) { return zmk(xld(&p->x), xld(&p->y)); }
 
-// ------------------------------------------------------------------ helpers
+// Grid barrier over co-resident workgroups: arrival counter + generation.
+// Data crossing it is written/read with agent-scope atomics (above); the
+// workgroup-scope fences order this workgroup's accesses around the arrival.
+// A bounded spin turns a lost arrival into an error flag instead of a hang.
+__device__ inline void grid_barrier(pp_barrier *b)
+{
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    const unsigned n = gridDim.x;
+    unsigned gen = __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned arrived = __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived == n - 1) {
+      __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&b->gen, gen + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      long spins = 0;
+      while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > (1l << 25)) { __hip_atomic_store(&b->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------ reductions
 __device__ inline double wave_sum(double v)
 {
 #pragma unroll
@@ -77,53 +132,48 @@ __device__ inline double wave_sum(double v)
   return v;
 }
 
-// Deterministic reduction over workgroups: out[c] = sum_{a=0}^{L-1} part[a*pstr + off + c]
-// for c < ncomp, summed in the same fixed order by every workgroup.
+// out[c] = sum_{a=0}^{L-1} part[a*pstr + off + c], c < ncomp, in one fixed
+// order (identical in every workgroup).  The L*ncomp partials are staged in
+// LDS (scratch) with independent loads first, then summed per component.
 __device__ inline void grid_partial_sum(const double *part, int pstr, int off, int ncomp, int L,
-                                        double *out_lds, double *scratch_lds)
+                                        double *out_lds, double *scratch)
 {
   const int tid = threadIdx.x;
-  int G = PP_THREADS / ncomp;
-  if (G > L) G = L;
-  if (G < 1) G = 1;
-  if (tid < G * ncomp) {
-    int c = tid % ncomp, g = tid / ncomp;
-    double acc = 0.0;
-    for (int a = g; a < L; a += G) acc += part[(size_t)a * pstr + off + c];
-    scratch_lds[g * ncomp + c] = acc;
+  const int tot = L * ncomp;
+#pragma unroll 4
+  for (int f = tid; f < tot; f += PP_THREADS) {
+    const int a = f / ncomp, c = f - a * ncomp;
+    scratch[f] = xld(part + (size_t)a * pstr + off + c);
   }
   __syncthreads();
   if (tid < ncomp) {
-    double acc = 0.0;
-    for (int g = 0; g < G; g++) acc += scratch_lds[g * ncomp + tid];
-    out_lds[tid] = acc;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int a = 0;
+    for (; a + 3 < L; a += 4) {
+      s0 += scratch[a * ncomp + tid]; s1 += scratch[(a + 1) * ncomp + tid];
+      s2 += scratch[(a + 2) * ncomp + tid]; s3 += scratch[(a + 3) * ncomp + tid];
+    }
+    for (; a < L; a++) s0 += scratch[a * ncomp + tid];
+    out_lds[tid] = (s0 + s1) + (s2 + s3);
   }
   __syncthreads();
 }
 
-// per-workgroup sum of cnt values, each thread contributing val(k) for k<cnt
-// (wave shuffles, then LDS across the 4 waves); result in red_out[k].
-template <typename F>
-__device__ inline void block_sums(int cnt, F val, double *wave_lds, double *red_out)
+__device__ inline double block_sum(double v, double *wave_lds)
 {
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  for (int k = 0; k < cnt; k++) {
-    double v = wave_sum(val(k));
-    if (lane == 0) wave_lds[wave * cnt + k] = v;
-  }
+  const int tid = threadIdx.x;
+  v = wave_sum(v);
+  if ((tid & 63) == 0) wave_lds[tid >> 6] = v;
   __syncthreads();
-  if (tid < cnt) {
-    double acc = 0.0;
-    for (int w = 0; w < PP_THREADS / 64; w++) acc += wave_lds[w * cnt + tid];
-    red_out[tid] = acc;
-  }
+  double r = 0.0;
+  for (int w = 0; w < PP_THREADS / 64; w++) r += wave_lds[w];
   __syncthreads();
+  return r;
 }
 
-// out[2k], out[2k+1] = <V_k, W> over one fiber for k < nk (V_k in global
-// memory, W in LDS); with extra_norm also out[2nk] = |W|^2.  Wave w owns
-// k = w, w+4, ...; each lane streams a strided slice of the fiber so the
-// loads of one k are independent and pipeline.
+// out[2k], out[2k+1] = <V_k, W> over one fiber for k < nk; with extra_norm
+// also out[2nk] = |W|^2.  Wave w owns k = w, w+4, ...; each lane streams a
+// strided slice so the loads of one k are independent and pipeline.
 __device__ inline void fiber_dots(const zd *V, const zd *W, int nk, int N, bool extra_norm, double *out)
 {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -151,8 +201,8 @@ __device__ inline void fiber_dots(const zd *V, const zd *W, int nk, int N, bool 
 
 // ------------------------------------------------------------------ Pade on one workgroup
 // pAMP::ScalingPadeExponential + PadeExponential (pamp.h:430-564), column-major
-// m x m complex matrices in LDS; result column 0 -> cvec.
-__device__ void pade_block(int m, const zd *Hg, double dt_re, double dt_im, zd *lds, zd *cvec, double *diag)
+// m x m complex matrices in LDS; first column -> cvec_out (exchanged).
+__device__ void pade_block(int m, const zd *Hg, double dt_re, double dt_im, zd *lds, zd *cvec_out, double *diag)
 {
   const int tid = threadIdx.x;
   const int mm = m * m;
@@ -179,10 +229,15 @@ __device__ void pade_block(int m, const zd *Hg, double dt_re, double dt_im, zd *
   __syncthreads();
   auto matmul = [&](const zd *X, const zd *Y, zd *Z) {
     for (int e = tid; e < mm; e += PP_THREADS) {
-      int i = e % m, j = e / m;
-      zd acc = zmk(0.0, 0.0);
-      for (int k = 0; k < m; k++) zfma(acc, X[i + k * m], Y[k + j * m]);
-      Z[e] = acc;
+      const int i = e % m, j = e / m;
+      zd a0 = zmk(0.0, 0.0), a1 = zmk(0.0, 0.0);
+      int k = 0;
+      for (; k + 1 < m; k += 2) {
+        zfma(a0, X[i + k * m], Y[k + j * m]);
+        zfma(a1, X[i + (k + 1) * m], Y[k + 1 + j * m]);
+      }
+      if (k < m) zfma(a0, X[i + k * m], Y[k + j * m]);
+      Z[e] = zadd(a0, a1);
     }
     __syncthreads();
   };
@@ -219,15 +274,21 @@ __device__ void pade_block(int m, const zd *Hg, double dt_re, double dt_im, zd *
   __syncthreads();
   // LU with partial pivoting (izamax: |re|+|im|) applied to Q and the RHS P
   for (int k = 0; k < m; k++) {
-    if (tid == 0) {
-      int p = k;
-      double best = fabs(Q[k + k * m].x) + fabs(Q[k + k * m].y);
-      for (int i = k + 1; i < m; i++) {
-        double v = fabs(Q[i + k * m].x) + fabs(Q[i + k * m].y);
+    if (tid < 64) {
+      // izamax over rows k..m-1 (first index on ties), one wave
+      double best = -1.0;
+      int p = m;
+      for (int i = k + tid; i < m; i += 64) {
+        const double v = fabs(Q[i + k * m].x) + fabs(Q[i + k * m].y);
         if (v > best) { best = v; p = i; }
       }
-      ipiv[0] = p;
-      ipiv[1] = (best == 0.0);
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o, 64);
+        const int op = __shfl_xor(p, o, 64);
+        if (ob > best || (ob == best && op < p)) { best = ob; p = op; }
+      }
+      if (tid == 0) { ipiv[0] = p; ipiv[1] = (best == 0.0); }
     }
     __syncthreads();
     int p = ipiv[0];
@@ -281,73 +342,74 @@ __device__ void pade_block(int m, const zd *Hg, double dt_re, double dt_im, zd *
   }
   __syncthreads();
   for (int s = 0; s < sc; s++) { matmul(A, A, T); copy(T, A); }
-  for (int i = tid; i < m; i += PP_THREADS) cvec[i] = A[i];
+  for (int i = tid; i < m; i += PP_THREADS) zst(cvec_out + i, A[i]);
   if (tid == 0) diag[0] = sqrt(zabs2(A[m - 1]));
   __syncthreads();
 }
 
+
 // ------------------------------------------------------------------ the step kernel
-template <int KD, int MAXB>
+template <int KD, int B>
 __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
 {
-  cg::grid_group grid = cg::this_grid();
-  extern __shared__ zd lds[];
+  extern __shared__ double lds_d[];
   const int N = A.N, L = A.L, m = A.m, a = blockIdx.x, tid = threadIdx.x;
-  const int padN = A.padN;
-  zd *W = lds;                 // residual w of fiber a (stays in LDS across barriers)
-  zd *X = lds + padN;          // solve / scratch buffer
-  zd *Yn = lds + 2 * padN;     // staged neighbour fiber
-  double *wsum = (double *)(Yn + N);          // 4 waves x (2m+4)
-  double *hred = wsum + 4 * (2 * m + 4);      // reduced values (2m+4)
-  double *gscr = hred + (2 * m + 4);          // grid-sum scratch (PP_THREADS)
-  double *hacc = gscr + PP_THREADS;           // accumulated projections (2m) when re-orthogonalising
-  const double tol = 1.4901161193847656e-08;  // sqrt(DBL_EPSILON), pamp.h:75
-  const pp_overlap &ov = A.ov;
+  const int padN = A.padN, bw = A.bw, s = A.s;
+  double *pk = lds_d;                                  // packed solver plan
+  zd *W = (zd *)(lds_d + A.planDoubles);               // residual of fiber a
+  zd *X = W + padN;                                    // solve / accumulation buffer
+  zd *Ys = X + padN;                                   // staged source fibers (nslots x N)
+  double *scr = (double *)Ys;                          // also: partial-sum staging (scrDoubles)
+  double *wsum = (double *)Ys + A.scrDoubles;          // 8
+  double *hred = wsum + 8;                             // 2m+4 reduced values
+  double *gscr = hred + (2 * m + 4);                   // PP_THREADS
+  double *hacc = gscr + PP_THREADS;                    // 2m
+  zd *cv = (zd *)(hacc + 2 * m);                       // m Krylov coefficients
+  zd *cmb_c = cv + m;                                  // combo table: coefficients (PP_MAXCOMBO x slots)
+  int *cmb_q = (int *)(cmb_c + PP_MAXCOMBO * PP_MAXSLOTS);  // slots
+  int *cmb_n = cmb_q + PP_MAXCOMBO * PP_MAXSLOTS;      // entries per combo
+  const double tol = 1.4901161193847656e-08;           // sqrt(DBL_EPSILON), pamp.h:75
   zd *Va = A.V + (size_t)a * m * N;
   double *mypart = A.part + (size_t)a * A.pstr;
+  const int NRM = 2 * m, ONRM = 2 * m + 1;
   int tcount = 0;
   auto stamp = [&](int step) {
     if (A.tlog && step == 0 && a == 0 && tid == 0 && tcount < PP_TLOG_MAX) A.tlog[tcount++] = (long long)wall_clock64();
   };
-  const int NRM = 2 * m;        // slot of |w|^2 in the partial sums
-  const int ONRM = 2 * m + 1;   // slot of the pre-projection norm (re-orthogonalisation)
+  for (int i = tid; i < A.planDoubles; i += PP_THREADS) pk[i] = A.plan[i];
+  __syncthreads();
 
   for (int step = 0; step < A.nsteps; step++) {
     // ---- psi~ = U psi (MultiplySqrtOverlap, PamPropagator.py:53) ; y = U^-1 psi~
     for (int i = tid; i < N; i += PP_THREADS) X[i] = A.psi[(size_t)a * N + i];
     __syncthreads();
+    double nrm = 0.0;
     for (int i = tid; i < N; i += PP_THREADS) {
       zd acc = zmk(0.0, 0.0);
-      int j1 = i + KD < N - 1 ? i + KD : N - 1;
-      for (int j = i; j <= j1; j++) zfmar(acc, ov.Uband[(size_t)j * (KD + 1) + KD + i - j], X[j]);
+      const int j1 = i + KD < N - 1 ? i + KD : N - 1;
+      for (int j = i; j <= j1; j++) zfmar(acc, A.Uband[(size_t)j * (KD + 1) + KD + i - j], X[j]);
       W[i] = acc;
+      nrm += zabs2(acc);
     }
+    nrm = block_sum(nrm, wsum);
+    if (tid == 0) xst(mypart + NRM, nrm);
+    for (int i = tid; i < padN; i += PP_THREADS) X[i] = i < N ? W[i] : zmk(0.0, 0.0);
     __syncthreads();
-    block_sums(1, [&](int) {
-      double s = 0.0;
-      for (int i = tid; i < N; i += PP_THREADS) s += zabs2(W[i]);
-      return s; }, wsum, hred);
-    if (tid == 0) mypart[NRM] = hred[0];
-    for (int i = tid; i < N; i += PP_THREADS) X[i] = W[i];
+    if (tid < 64) pp_solve_upper<KD, B>(pk, A.planP, A.planLevels, X, N);
     __syncthreads();
-    if (tid < 64) pp_band_solve_wave<KD, MAXB>(ov.solveN, X, false, N);
-    __syncthreads();
-    for (int i = tid; i < N; i += PP_THREADS) A.y[(size_t)a * N + i] = X[i];
+    for (int i = tid; i < N; i += PP_THREADS) zst(A.y + (size_t)a * N + i, X[i]);
     stamp(step);
-    grid.sync();
+    grid_barrier(A.bar);
     stamp(step);
 
-    // Every workgroup executes the same number of grid barriers whatever the
-    // data: breakdown and re-orthogonalisation only predicate work, so a
-    // disagreement between workgroups could never leave one waiting.
     double beta0 = 0.0, beta = 0.0;
     int mEff = m;
     bool broken = false;
     int reorth_passes = 0;
     for (int j = 0; j < m; j++) {
       // ---------------- phase A: beta_j, v_j = w/beta, w' = U^-H H U^-1 v_j, <v_k, w'>
-      const int ncomp = 2 * (j + 1);
-      grid_partial_sum(A.part, A.pstr, NRM, 1, L, hred, gscr);
+      const int nk = j + 1, ncomp = 2 * nk;
+      grid_partial_sum(A.part, A.pstr, NRM, 1, L, hred, scr);
       beta = sqrt(hred[0]);
       if (j == 0) beta0 = beta;
       else if (!broken) {
@@ -361,52 +423,86 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
           W[i] = v;
           Va[(size_t)j * N + i] = v;
         }
-        for (int i = tid; i < padN; i += PP_THREADS) X[i] = zmk(0.0, 0.0);
-        __syncthreads();
-        for (int t = 0; t < A.nterms; t++) {
-          const double st = A.scal[(size_t)step * A.nterms + t] * inv;
-          const pp_kterm T = A.term[t];
-          const int e0 = T.rowptr[a], e1 = T.rowptr[a + 1];
-          for (int e = e0; e < e1; e++) {
-            const int col = T.ent[3 * e], pidx = T.ent[3 * e + 1], cj = T.ent[3 * e + 2];
-            for (int i = tid; i < N; i += PP_THREADS) Yn[i] = A.y[(size_t)col * N + i];
-            __syncthreads();
-            const zd *Vp = T.pot + (size_t)pidx * N * A.bw;
-            for (int i = tid; i < N; i += PP_THREADS) {
-              zd acc = zmk(0.0, 0.0);
-              int d0 = -A.s, d1 = A.s;
-              if (i + d0 < 0) d0 = -i;
-              if (i + d1 > N - 1) d1 = N - 1 - i;
-              for (int d = d0; d <= d1; d++) {
-                zd v = Vp[(size_t)(i + d) * A.bw + A.s - d];
-                if (cj) zfmac(acc, v, Yn[i + d]); else zfma(acc, v, Yn[i + d]);
+        // stage the source fibers U^-1 w of this row's couplings
+        for (int q = 0; q < A.nslots; q++) {
+          const int col = A.slotcols[a * A.nslots + q];
+          if (col >= 0)
+            for (int i = tid; i < N; i += PP_THREADS) Ys[q * N + i] = zld(A.y + (size_t)col * N + i);
+        }
+        // combo table: one entry per (term, rank) with this row's coefficients
+        if (tid == 0) {
+          int k = 0;
+          for (int t = 0; t < A.nterms; t++) {
+            const pp_fterm T = A.term[t];
+            const double st = A.scal[(size_t)step * A.nterms + t] * inv;
+            const int e0 = T.rowptr[a], e1 = T.rowptr[a + 1];
+            for (int r = 0; r < T.rank; r++, k++) {
+              int ne = 0;
+              for (int e = e0; e < e1; e++, ne++) {
+                zd c = T.coef[(size_t)T.ent[3 * e + 1] * T.rank + r];
+                if (T.ent[3 * e + 2]) c = zconj(c);
+                cmb_c[k * PP_MAXSLOTS + ne] = zscale(c, st);
+                cmb_q[k * PP_MAXSLOTS + ne] = T.ent[3 * e];
               }
-              zd x = X[i];
-              x.x = fma(st, acc.x, x.x);
-              x.y = fma(st, acc.y, x.y);
-              X[i] = x;
+              cmb_n[k] = ne;
             }
-            __syncthreads();
           }
         }
-        stamp(step);
-        if (tid < 64) pp_band_solve_wave<KD, MAXB>(ov.solveC, X, true, N);
+        __syncthreads();
+        // X = sum_k B_k (sum_q c_kq Ys_q): bands diagonal-major, all loads independent
+        for (int i = tid; i < padN; i += PP_THREADS) {
+          zd acc = zmk(0.0, 0.0);
+          if (i < N) {
+            const bool interior = i >= KD && i + KD < N;
+            int k = 0;
+            for (int t = 0; t < A.nterms; t++) {
+              const double *Bt = A.term[t].bands;
+              for (int r = 0; r < A.term[t].rank; r++, k++) {
+                const int ne = cmb_n[k];
+                if (ne == 0) continue;
+                const double *Br = Bt + (size_t)r * (2 * KD + 1) * N;
+                double bv[2 * KD + 1];
+#pragma unroll
+                for (int d = -KD; d <= KD; d++)
+                  bv[d + KD] = (interior || (i + d >= 0 && i + d < N)) ? Br[(size_t)(d + KD) * N + i] : 0.0;
+#pragma unroll
+                for (int q = 0; q < PP_MAXSLOTS; q++) {
+                  if (q < ne) {
+                    const zd cq = cmb_c[k * PP_MAXSLOTS + q];
+                    const zd *Yq = Ys + cmb_q[k * PP_MAXSLOTS + q] * N;
+                    zd z = zmk(0.0, 0.0);
+#pragma unroll
+                    for (int d = -KD; d <= KD; d++) {
+                      const int j = i + d;
+                      const zd yv = (interior || (j >= 0 && j < N)) ? Yq[j] : zmk(0.0, 0.0);
+                      zfmar(z, bv[d + KD], yv);
+                    }
+                    zfma(acc, cq, z);
+                  }
+                }
+              }
+            }
+          }
+          X[i] = acc;
+        }
         __syncthreads();
         stamp(step);
-        // w' -> W ; partial <v_k, w'> for k <= j ; |w'|^2 for re-orthogonalisation
+        if (tid < 64) pp_solve_lowerT<KD, B>(pk, A.planP, A.planLevels, X, N);
+        __syncthreads();
+        stamp(step);
         for (int i = tid; i < N; i += PP_THREADS) W[i] = X[i];
         __syncthreads();
-        fiber_dots(Va, W, j + 1, N, A.reorth != 0, hred);
-        for (int c = tid; c < ncomp; c += PP_THREADS) mypart[c] = hred[c];
-        if (A.reorth && tid == 0) mypart[ONRM] = hred[ncomp];
+        fiber_dots(Va, W, nk, N, A.reorth != 0, hred);
+        for (int c = tid; c < ncomp; c += PP_THREADS) xst(mypart + c, hred[c]);
+        if (A.reorth && tid == 0) xst(mypart + ONRM, hred[ncomp]);
       }
       stamp(step);
-      grid.sync();
+      grid_barrier(A.bar);
       stamp(step);
 
       // ---------------- phase B: h = sum_a partials ; w' -= V h ; |w'|^2 ; y = U^-1 w'
       if (!broken) {
-        grid_partial_sum(A.part, A.pstr, 0, ncomp, L, hred, gscr);
+        grid_partial_sum(A.part, A.pstr, 0, ncomp, L, hred, scr);
         if (a == 0 && tid <= j) A.Hm[tid + j * m] = zmk(hred[2 * tid], hred[2 * tid + 1]);
       }
       if (j == m - 1) break;   // the last residual is not used (pamp.h:419-426)
@@ -414,11 +510,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
         for (int i = tid; i < N; i += PP_THREADS) {
           zd w = W[i];
 #pragma unroll 4
-          for (int k = 0; k <= j; k++) {
-            zd h = zmk(hred[2 * k], hred[2 * k + 1]);
-            zd v = Va[(size_t)k * N + i];
-            w = zsub(w, zmul(v, h));
-          }
+          for (int k = 0; k <= j; k++) w = zsub(w, zmul(Va[(size_t)k * N + i], zmk(hred[2 * k], hred[2 * k + 1])));
           W[i] = w;
         }
         __syncthreads();
@@ -430,32 +522,34 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
         bool stop = broken;
         if (!broken) {
           for (int c = tid; c < ncomp; c += PP_THREADS) hacc[c] = hred[c];
-          grid_partial_sum(A.part, A.pstr, ONRM, 1, L, hred, gscr);
+          grid_partial_sum(A.part, A.pstr, ONRM, 1, L, hred, scr);
           orig = sqrt(hred[0]);
           __syncthreads();
         }
         for (int pass = 0; pass < 4; pass++) {
           if (!stop) {
-            block_sums(1, [&](int) { double s = 0.0; for (int i = tid; i < N; i += PP_THREADS) s += zabs2(W[i]); return s; }, wsum, hred);
-            if (tid == 0) mypart[NRM] = hred[0];
+            double v = 0.0;
+            for (int i = tid; i < N; i += PP_THREADS) v += zabs2(W[i]);
+            v = block_sum(v, wsum);
+            if (tid == 0) xst(mypart + NRM, v);
           }
-          grid.sync();
+          grid_barrier(A.bar);
           bool doit = false;
           double res = 0.0;
           if (!stop) {
-            grid_partial_sum(A.part, A.pstr, NRM, 1, L, hred, gscr);
+            grid_partial_sum(A.part, A.pstr, NRM, 1, L, hred, scr);
             res = sqrt(hred[0]);
             __syncthreads();
             if (fabs(res) < 0.717 * fabs(orig) && pass < 3) doit = true;
             else stop = true;
           }
           if (doit) {
-            fiber_dots(Va, W, j + 1, N, false, hred);
-            for (int c = tid; c < ncomp; c += PP_THREADS) mypart[c] = hred[c];
+            fiber_dots(Va, W, nk, N, false, hred);
+            for (int c = tid; c < ncomp; c += PP_THREADS) xst(mypart + c, hred[c]);
           }
-          grid.sync();
+          grid_barrier(A.bar);
           if (doit) {
-            grid_partial_sum(A.part, A.pstr, 0, ncomp, L, hred, gscr);
+            grid_partial_sum(A.part, A.pstr, 0, ncomp, L, hred, scr);
             for (int i = tid; i < N; i += PP_THREADS) {
               zd w = W[i];
               for (int k = 0; k <= j; k++) w = zsub(w, zmul(Va[(size_t)k * N + i], zmk(hred[2 * k], hred[2 * k + 1])));
@@ -470,54 +564,56 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
             __syncthreads();
           }
         }
-        grid.sync();   // everyone has read NRM before it is rewritten below
+        grid_barrier(A.bar);   // everyone has read NRM before it is rewritten below
       }
       if (!broken) {
-        block_sums(1, [&](int) {
-          double s = 0.0;
-          for (int i = tid; i < N; i += PP_THREADS) s += zabs2(W[i]);
-          return s; }, wsum, hred);
-        if (tid == 0) mypart[NRM] = hred[0];
+        double v = 0.0;
+        for (int i = tid; i < N; i += PP_THREADS) v += zabs2(W[i]);
+        v = block_sum(v, wsum);
+        if (tid == 0) xst(mypart + NRM, v);
         for (int i = tid; i < padN; i += PP_THREADS) X[i] = (i < N) ? W[i] : zmk(0.0, 0.0);
         __syncthreads();
         stamp(step);
-        if (tid < 64) pp_band_solve_wave<KD, MAXB>(ov.solveN, X, false, N);
+        if (tid < 64) pp_solve_upper<KD, B>(pk, A.planP, A.planLevels, X, N);
         __syncthreads();
-        for (int i = tid; i < N; i += PP_THREADS) A.y[(size_t)a * N + i] = X[i];
+        for (int i = tid; i < N; i += PP_THREADS) zst(A.y + (size_t)a * N + i, X[i]);
       }
       stamp(step);
-      grid.sync();
+      grid_barrier(A.bar);
       stamp(step);
     }
 
     // ---------------- exponentiate on workgroup 0, broadcast exp(-i dt H) e1
     if (a == 0) {
-      pade_block(m, A.Hm, A.dt_re, A.dt_im, lds, A.cvec, A.diag + (size_t)step * 4);
+      pade_block(m, A.Hm, A.dt_re, A.dt_im, (zd *)lds_d + 0 + (A.planDoubles + 1) / 2, A.cvec,
+                 A.diag + (size_t)step * 4);
       if (tid == 0) {
         A.diag[(size_t)step * 4 + 1] = mEff;
         A.diag[(size_t)step * 4 + 2] = beta0;
         A.diag[(size_t)step * 4 + 3] = reorth_passes;
       }
-      // reset the Hessenberg matrix for the next step
       __syncthreads();
       for (int e = tid; e < m * m; e += PP_THREADS) A.Hm[e] = zmk(0.0, 0.0);
     }
     stamp(step);
-    grid.sync();
+    grid_barrier(A.bar);
     stamp(step);
     // ---------------- psi~ = |psi~| V c ; psi = U^-1 psi~ (SolveSqrtOverlap)
+    for (int k = tid; k < m; k += PP_THREADS) cv[k] = zld(A.cvec + k);
+    __syncthreads();
     for (int i = tid; i < padN; i += PP_THREADS) {
       zd acc = zmk(0.0, 0.0);
       if (i < N)
-        for (int k = 0; k < mEff; k++) zfma(acc, Va[(size_t)k * N + i], A.cvec[k]);
+        for (int k = 0; k < mEff; k++) zfma(acc, Va[(size_t)k * N + i], cv[k]);
       X[i] = zscale(acc, beta0);
     }
     __syncthreads();
-    if (tid < 64) pp_band_solve_wave<KD, MAXB>(ov.solveN, X, false, N);
+    if (tid < 64) pp_solve_upper<KD, B>(pk, A.planP, A.planLevels, X, N);
     __syncthreads();
     for (int i = tid; i < N; i += PP_THREADS) A.psi[(size_t)a * N + i] = X[i];
     __syncthreads();
     stamp(step);
+    // cvec is rewritten by the next step's Pade only after further barriers
   }
   if (A.tlog && a == 0 && tid == 0) A.tlog[PP_TLOG_MAX] = tcount;
 }
@@ -529,56 +625,260 @@ extern "C" int pp_pamp_create(int L, int N, int m, int reorth, const pp_overlap 
   if (ov->N != N) return PP_EARG;
   if (m > PP_MAXM) return PP_EUNSUP;
   pp_pamp *p = new pp_pamp();
-  memset(p, 0, sizeof(*p));
   p->L = L; p->N = N; p->m = m; p->reorth = reorth ? 1 : 0; p->ov = ov;
   p->pstr = 2 * m + 4;
-  p->s = -1;
+  p->s = -1; p->bw = 0; p->nterms = 0; p->nslots = 0;
+  p->rowcols.assign(L, std::vector<int>());
+  p->slotcols = nullptr; p->dirty = true;
+  p->V = p->y = p->Hm = p->cvec = nullptr;
+  p->part = p->diag = p->scal = nullptr;
+  p->bar = nullptr; p->tlog = nullptr; p->timing = 0;
+  p->diag_cap = p->scal_cap = 0;
+  for (int t = 0; t < PP_MAXTERMS; t++) p->termbuf[t] = nullptr;
   PP_HIP(hipMalloc(&p->V, sizeof(zd) * (size_t)L * m * N));
   PP_HIP(hipMalloc(&p->y, sizeof(zd) * (size_t)L * N));
   PP_HIP(hipMalloc(&p->Hm, sizeof(zd) * (size_t)m * m));
   PP_HIP(hipMalloc(&p->cvec, sizeof(zd) * (size_t)m));
   PP_HIP(hipMalloc(&p->part, sizeof(double) * (size_t)L * p->pstr));
+  PP_HIP(hipMalloc(&p->bar, sizeof(pp_barrier)));
   PP_HIP(hipMemset(p->V, 0, sizeof(zd) * (size_t)L * m * N));
   PP_HIP(hipMemset(p->Hm, 0, sizeof(zd) * (size_t)m * m));
+  PP_HIP(hipMemset(p->bar, 0, sizeof(pp_barrier)));
   *out = p;
   return PP_OK;
 }
 
-// One tensor potential: angular storage (Ident/Diag/Simp/Herm) with P0
-// compressed angular indices (pairs host int32 P0x2 for Simp/Herm), radial
-// BandNH band of total width bw (=2s+1) per B-spline, device data (P0, N*bw).
-extern "C" int pp_pamp_add_term(pp_pamp *p, int angular_storage, int P0, const int *pairs, int radial_storage,
-                                int bw, const void *pot)
+// Register one factored tensor potential  V[p, :] = sum_r coef[p, r] bands[r, :]
+// with real radial bands in the reference BandNH layout (rank x N*bw) and
+// complex angular coefficients (P0 x rank), angular storage with P0 indices.
+static int pamp_add_factored(pp_pamp *p, int angular_storage, int P0, const int *pairs, int bw, int rank,
+                             const double *coef, const double *bandsNH)
 {
-  if (!p || !pot || P0 < 1) return PP_EARG;
   if (p->nterms >= PP_MAXTERMS) return PP_EUNSUP;
-  if (radial_storage != PP_BANDNH) return PP_EUNSUP;
-  if (bw < 1 || (bw % 2) == 0) return PP_EARG;
+  if (bw < 1 || (bw % 2) == 0 || rank < 1) return PP_EARG;
   if (p->s >= 0 && bw != p->bw) return PP_EUNSUP;
   pp_rank_csr csr;
   int rc = pp_build_csr(angular_storage, P0, p->L, pairs, csr);
   if (rc) return rc;
-  std::vector<int> host(csr.rowptr);
-  host.insert(host.end(), csr.ent.begin(), csr.ent.end());
-  int *dev = nullptr;
-  PP_HIP(hipMalloc(&dev, host.size() * sizeof(int)));
-  PP_HIP(hipMemcpy(dev, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice));
+  const int N = p->N, s = (bw - 1) / 2;
+  // slots: distinct source fibers per row
+  std::vector<int> ent;
+  for (int a = 0; a < p->L; a++)
+    for (int e = csr.rowptr[a]; e < csr.rowptr[a + 1]; e++) {
+      int col = csr.ent[3 * e];
+      auto &rc_ = p->rowcols[a];
+      auto it = std::find(rc_.begin(), rc_.end(), col);
+      if (it == rc_.end()) { if ((int)rc_.size() >= PP_MAXSLOTS) return PP_EUNSUP; rc_.push_back(col); }
+    }
+  for (int a = 0; a < p->L; a++) {
+    int cnt = csr.rowptr[a + 1] - csr.rowptr[a];
+    if (cnt > PP_MAXSLOTS) return PP_EUNSUP;
+  }
+  for (size_t e = 0; e < csr.ent.size() / 3; e++) {
+    // find row of entry e
+    (void)e;
+  }
+  // device term buffer: rowptr | ent(slot,pidx,conj) | coef | bands
+  std::vector<int> ints(csr.rowptr);
+  for (int a = 0; a < p->L; a++)
+    for (int e = csr.rowptr[a]; e < csr.rowptr[a + 1]; e++) {
+      int col = csr.ent[3 * e];
+      int slot = (int)(std::find(p->rowcols[a].begin(), p->rowcols[a].end(), col) - p->rowcols[a].begin());
+      ints.push_back(slot);
+      ints.push_back(csr.ent[3 * e + 1]);
+      ints.push_back(csr.ent[3 * e + 2]);
+    }
+  size_t nint = ints.size();
+  size_t off_coef = ((nint * sizeof(int) + 15) / 16) * 16;
+  size_t off_band = off_coef + sizeof(zd) * (size_t)P0 * rank;
+  size_t total = off_band + sizeof(double) * (size_t)rank * bw * N;
+  std::vector<char> host(total, 0);
+  memcpy(host.data(), ints.data(), nint * sizeof(int));
+  memcpy(host.data() + off_coef, coef, sizeof(zd) * (size_t)P0 * rank);
+  double *bd = (double *)(host.data() + off_band);
+  for (int r = 0; r < rank; r++)
+    for (int j = 0; j < N; j++)
+      for (int i = j - s; i <= j + s; i++) {
+        if (i < 0 || i >= N) continue;
+        int d = j - i;   // element (i, i+d)
+        bd[((size_t)r * bw + d + s) * N + i] = bandsNH[(size_t)r * N * bw + (size_t)j * bw + s + i - j];
+      }
+  void *dev = nullptr;
+  PP_HIP(hipMalloc(&dev, total));
+  PP_HIP(hipMemcpy(dev, host.data(), total, hipMemcpyHostToDevice));
   int t = p->nterms++;
-  p->csr_dev[t] = dev;
-  p->term[t].rowptr = dev;
-  p->term[t].ent = dev + csr.rowptr.size();
-  p->term[t].pot = (const zd *)pot;
+  p->termbuf[t] = dev;
+  p->term[t].rowptr = (const int *)dev;
+  p->term[t].ent = (const int *)dev + csr.rowptr.size();
+  p->term[t].coef = (const zd *)((char *)dev + off_coef);
+  p->term[t].bands = (const double *)((char *)dev + off_band);
+  p->term[t].rank = rank;
   p->bw = bw;
-  p->s = (bw - 1) / 2;
+  p->s = s;
+  p->dirty = true;
   return PP_OK;
+}
+
+extern "C" int pp_pamp_add_factored_term(pp_pamp *p, int angular_storage, int P0, const int *pairs,
+                                         int radial_storage, int bw, int rank, const double *coef,
+                                         const double *bands)
+{
+  if (!p || !coef || !bands || P0 < 1) return PP_EARG;
+  if (radial_storage != PP_BANDNH) return PP_EUNSUP;
+  return pamp_add_factored(p, angular_storage, P0, pairs, bw, rank, coef, bands);
+}
+
+// Low-rank factorisation of a tensor potential (host, complex interleaved,
+// P0 x M): V[p,:] = sum_r coef[p,r] bands[r,:] with real bands.  Bands are
+// rows of the real and imaginary parts chosen by pivoted Gram-Schmidt (an
+// interpolative decomposition, so coefficients stay O(1)); coef solves the
+// triangular system in the orthonormal basis.  *rank_out = rank, *relerr =
+// max |V - recon| / max |V|.  coef: P0 x maxrank complex, bands: maxrank x M.
+extern "C" int pp_lowrank_factor(int P0, long M, const double *V, int maxrank, int *rank_out, double *coef,
+                                 double *bands, double *relerr)
+{
+  if (P0 < 1 || M < 1 || !V || !rank_out || !coef || !bands || maxrank < 1) return PP_EARG;
+  const int R2 = 2 * P0;
+  std::vector<double> rows((size_t)R2 * M);
+  for (int p0 = 0; p0 < P0; p0++)
+    for (long e = 0; e < M; e++) {
+      rows[(size_t)p0 * M + e] = V[2 * ((size_t)p0 * M + e)];
+      rows[(size_t)(P0 + p0) * M + e] = V[2 * ((size_t)p0 * M + e) + 1];
+    }
+  double maxnorm = 0.0, maxval = 0.0;
+  for (int r = 0; r < R2; r++) {
+    double n2 = 0.0;
+    for (long e = 0; e < M; e++) { double v = rows[(size_t)r * M + e]; n2 += v * v; maxval = std::max(maxval, std::fabs(v)); }
+    maxnorm = std::max(maxnorm, std::sqrt(n2));
+  }
+  std::vector<double> Q, res(rows);
+  std::vector<int> piv;
+  int rank = 0;
+  while (rank < std::min(R2, maxrank) && maxnorm > 0.0) {
+    int best = -1;
+    double bn = 0.0;
+    for (int r = 0; r < R2; r++) {
+      double n2 = 0.0;
+      for (long e = 0; e < M; e++) n2 += res[(size_t)r * M + e] * res[(size_t)r * M + e];
+      if (std::sqrt(n2) > bn) { bn = std::sqrt(n2); best = r; }
+    }
+    if (bn <= 1e-13 * maxnorm) break;   // below the assembly rounding of the rows
+    std::vector<double> q(res.begin() + (size_t)best * M, res.begin() + (size_t)(best + 1) * M);
+    for (int pass = 0; pass < 2; pass++)
+      for (int k = 0; k < rank; k++) {
+        double d = 0.0;
+        for (long e = 0; e < M; e++) d += Q[(size_t)k * M + e] * q[e];
+        for (long e = 0; e < M; e++) q[e] -= d * Q[(size_t)k * M + e];
+      }
+    double n2 = 0.0;
+    for (long e = 0; e < M; e++) n2 += q[e] * q[e];
+    const double nq = std::sqrt(n2);
+    for (long e = 0; e < M; e++) q[e] /= nq;
+    Q.insert(Q.end(), q.begin(), q.end());
+    piv.push_back(best);
+    rank++;
+    for (int r = 0; r < R2; r++) {
+      double d = 0.0;
+      for (long e = 0; e < M; e++) d += q[e] * res[(size_t)r * M + e];
+      for (long e = 0; e < M; e++) res[(size_t)r * M + e] -= d * q[e];
+    }
+  }
+  if (rank == 0) {   // zero potential
+    *rank_out = 1;
+    for (long e = 0; e < M; e++) bands[e] = 0.0;
+    for (int p0 = 0; p0 < P0; p0++) coef[2 * p0] = coef[2 * p0 + 1] = 0.0;
+    if (relerr) *relerr = 0.0;
+    return PP_OK;
+  }
+  // pivot rows B_j = sum_k R[k][j] q_k (R upper triangular); coef solves R c = Q^T row
+  std::vector<double> R((size_t)rank * rank, 0.0);
+  for (int j = 0; j < rank; j++)
+    for (int k = 0; k <= j; k++) {
+      double d = 0.0;
+      for (long e = 0; e < M; e++) d += Q[(size_t)k * M + e] * rows[(size_t)piv[j] * M + e];
+      R[(size_t)k * rank + j] = d;
+    }
+  for (int j = 0; j < rank; j++)
+    for (long e = 0; e < M; e++) bands[(size_t)j * M + e] = rows[(size_t)piv[j] * M + e];
+  std::vector<double> yv(rank);
+  double maxerr = 0.0;
+  for (int p0 = 0; p0 < P0; p0++)
+    for (int part = 0; part < 2; part++) {
+      const double *row = rows.data() + (size_t)(part * P0 + p0) * M;
+      for (int k = 0; k < rank; k++) {
+        double d = 0.0;
+        for (long e = 0; e < M; e++) d += Q[(size_t)k * M + e] * row[e];
+        yv[k] = d;
+      }
+      for (int k = rank - 1; k >= 0; k--) {
+        double v = yv[k];
+        for (int j = k + 1; j < rank; j++) v -= R[(size_t)k * rank + j] * yv[j];
+        yv[k] = v / R[(size_t)k * rank + k];
+      }
+      for (int k = 0; k < rank; k++) coef[2 * ((size_t)p0 * rank + k) + part] = yv[k];
+      for (long e = 0; e < M; e++) {
+        double v = 0.0;
+        for (int k = 0; k < rank; k++) v += yv[k] * bands[(size_t)k * M + e];
+        maxerr = std::max(maxerr, std::fabs(v - row[e]));
+      }
+    }
+  *rank_out = rank;
+  if (relerr) *relerr = maxval > 0.0 ? maxerr / maxval : 0.0;
+  return PP_OK;
+}
+
+// One consolidated tensor potential in the reference layout (device,
+// P0 x N*bw complex).  The library factors it on the host
+// (pp_lowrank_factor) into real radial bands and complex angular
+// coefficients and rejects factorisations that miss V by more than
+// 1e-12 max|V| (the assembled tensor is itself low rank only up to the
+// rounding of its separately projected parts, a few 1e-14).
+extern "C" int pp_pamp_add_term(pp_pamp *p, int angular_storage, int P0, const int *pairs, int radial_storage,
+                                int bw, const void *pot)
+{
+  if (!p || !pot || P0 < 1) return PP_EARG;
+  if (radial_storage != PP_BANDNH) return PP_EUNSUP;
+  if (bw < 1 || (bw % 2) == 0) return PP_EARG;
+  const long M = (long)p->N * bw;
+  std::vector<double> T((size_t)P0 * M * 2);
+  PP_HIP(hipMemcpy(T.data(), pot, T.size() * sizeof(double), hipMemcpyDeviceToHost));
+  const int maxrank = std::min(2 * P0, 16);
+  std::vector<double> coef((size_t)P0 * maxrank * 2), bands((size_t)maxrank * M);
+  int rank = 0;
+  double relerr = 0.0;
+  int rc = pp_lowrank_factor(P0, M, T.data(), maxrank, &rank, coef.data(), bands.data(), &relerr);
+  if (rc) return rc;
+  if (relerr > 1e-12) return PP_ENUM;
+  return pamp_add_factored(p, angular_storage, P0, pairs, bw, rank, coef.data(), bands.data());
 }
 
 static size_t pamp_lds_bytes(const pp_pamp *p, int padN)
 {
-  size_t fib = sizeof(zd) * (size_t)(2 * padN + p->N) +
-               sizeof(double) * (size_t)(4 * (2 * p->m + 4) + (2 * p->m + 4) + PP_THREADS + 2 * p->m);
-  size_t pade = sizeof(zd) * (size_t)(5 * p->m * p->m) + sizeof(double) * 8 + 2 * sizeof(int) + 16;
+  const pp_band_plan &pl = p->ov->plan;
+  const size_t scr = std::max(2 * (size_t)p->nslots * p->N, (size_t)p->L * (2 * p->m + 2));
+  size_t fib = sizeof(double) * (size_t)pl.ndoubles + sizeof(zd) * (size_t)(2 * padN) + sizeof(double) * scr +
+               sizeof(double) * (size_t)(8 + (2 * p->m + 4) + PP_THREADS + 2 * p->m) + sizeof(zd) * (size_t)p->m +
+               (sizeof(zd) + 2 * sizeof(int)) * PP_MAXCOMBO * PP_MAXSLOTS + 64;
+  size_t pade = sizeof(double) * (size_t)(pl.ndoubles + 2) + sizeof(zd) * (size_t)(5 * p->m * p->m) +
+                sizeof(double) * 8 + 2 * sizeof(int) + 64;
   return fib > pade ? fib : pade;
+}
+
+static int pamp_upload_slots(pp_pamp *p)
+{
+  if (!p->dirty) return PP_OK;
+  int ns = 0;
+  for (auto &rc : p->rowcols) ns = std::max(ns, (int)rc.size());
+  if (ns < 1) ns = 1;
+  std::vector<int> sc((size_t)p->L * ns, -1);
+  for (int a = 0; a < p->L; a++)
+    for (size_t q = 0; q < p->rowcols[a].size(); q++) sc[(size_t)a * ns + q] = p->rowcols[a][q];
+  if (p->slotcols) (void)hipFree(p->slotcols);
+  PP_HIP(hipMalloc(&p->slotcols, sc.size() * sizeof(int)));
+  PP_HIP(hipMemcpy(p->slotcols, sc.data(), sc.size() * sizeof(int), hipMemcpyHostToDevice));
+  p->nslots = ns;
+  p->dirty = false;
+  return PP_OK;
 }
 
 // Propagate psi (device, L x N complex128) through nsteps steps of length dt.
@@ -589,6 +889,8 @@ extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re
   if (!p || !psi || nsteps < 0 || (!scal && p->nterms > 0)) return PP_EARG;
   if (nsteps == 0) return PP_OK;
   if (p->nterms == 0) return PP_EARG;
+  int rc = pamp_upload_slots(p);
+  if (rc) return rc;
   const pp_overlap *ov = p->ov;
   if (nsteps * p->nterms > p->scal_cap) {
     if (p->scal) (void)hipFree(p->scal);
@@ -601,51 +903,60 @@ extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re
     PP_HIP(hipMalloc(&p->diag, sizeof(double) * 4 * (size_t)nsteps));
   }
   PP_HIP(hipMemcpyAsync(p->scal, scal, sizeof(double) * nsteps * p->nterms, hipMemcpyHostToDevice, stream));
+  PP_HIP(hipMemsetAsync(p->bar, 0, sizeof(pp_barrier), stream));
   pp_pamp_args A;
   memset(&A, 0, sizeof(A));
   A.L = p->L; A.N = p->N; A.m = p->m; A.s = p->s; A.bw = p->bw; A.nterms = p->nterms;
-  A.nsteps = nsteps; A.reorth = p->reorth;
-  A.padN = ov->solveN.P * ov->solveN.B;
+  A.nsteps = nsteps; A.reorth = p->reorth; A.nslots = p->nslots;
+  if (p->s != ov->kd) return PP_EUNSUP;   // radial bands must have the overlap's bandwidth
+  A.ncombo = 0;
+  for (int t = 0; t < p->nterms; t++) A.ncombo += p->term[t].rank;
+  if (A.ncombo > PP_MAXCOMBO) return PP_EUNSUP;
+  A.scrDoubles = std::max(2 * p->nslots * p->N, p->L * (2 * p->m + 2));
+  const pp_band_plan &pl = ov->plan;
+  A.padN = pl.P * pl.B;
   if (A.padN < p->N) A.padN = p->N;
   A.dt_re = dt_re; A.dt_im = dt_im;
   for (int t = 0; t < p->nterms; t++) A.term[t] = p->term[t];
+  A.slotcols = p->slotcols;
   A.scal = p->scal;
-  A.ov = *ov;
+  A.Uband = ov->Uband;
+  A.plan = pl.packed; A.planP = pl.P; A.planLevels = pl.levels; A.planDoubles = pl.ndoubles;
   A.psi = (zd *)psi; A.V = p->V; A.y = p->y; A.part = p->part; A.pstr = p->pstr;
-  A.Hm = p->Hm; A.cvec = p->cvec; A.diag = p->diag;
+  A.Hm = p->Hm; A.cvec = p->cvec; A.diag = p->diag; A.bar = p->bar;
   A.tlog = p->timing ? p->tlog : nullptr;
   size_t lds = pamp_lds_bytes(p, A.padN);
+  if (lds > 160 * 1024) return PP_EUNSUP;
   void *args[] = {&A};
-  int maxb = ov->solveN.B;
   bool launched = false;
   hipError_t err = hipSuccess;
-  auto go = [&](const void *fn) -> hipError_t {
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    return hipLaunchCooperativeKernel(fn, dim3(p->L), dim3(PP_THREADS), args, (unsigned)lds, stream);
-  };
-  if (maxb <= 16) {
-    PP_KD_SWITCH(ov->kd, err = go((const void *)pp_pamp_kernel<KD_, 16>); launched = true)
-  } else {
-    PP_KD_SWITCH(ov->kd, err = go((const void *)pp_pamp_kernel<KD_, 32>); launched = true)
-  }
+  PP_B_SWITCH(pl.B, PP_KD_SWITCH(ov->kd, {
+    const void *fn = (const void *)pp_pamp_kernel<KD_, B_>;
+    err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (err == hipSuccess)
+      err = hipLaunchCooperativeKernel(fn, dim3(p->L), dim3(PP_THREADS), args, (unsigned)lds, stream);
+    launched = true; }))
   if (!launched) return PP_EUNSUP;
   if (err != hipSuccess) return PP_EHIP;
   return PP_OK;
 }
 
 // diag: host nsteps x 4 (error estimate |c_{m-1}|, basis used, |psi~|, re-orth passes)
-// of the last pp_pamp_propagate call; synchronises the stream.
+// of the last pp_pamp_propagate call; synchronises the stream.  Returns a HIP
+// error status if a grid barrier timed out.
 extern "C" int pp_pamp_diagnostics(pp_pamp *p, int nsteps, double *diag, hipStream_t stream)
 {
   if (!p || !diag || nsteps > p->diag_cap) return PP_EARG;
+  pp_barrier hb;
   PP_HIP(hipMemcpyAsync(diag, p->diag, sizeof(double) * 4 * (size_t)nsteps, hipMemcpyDeviceToHost, stream));
+  PP_HIP(hipMemcpyAsync(&hb, p->bar, sizeof(hb), hipMemcpyDeviceToHost, stream));
   PP_HIP(hipStreamSynchronize(stream));
+  if (hb.err) return PP_EHIP;
   return PP_OK;
 }
 
-// Profiling hook: when enabled, the next launches record wall-clock stamps
-// (100 MHz counter) of workgroup 0 around every grid barrier of step 0.
+// Profiling hook: when enabled, later launches record wall-clock stamps
+// (100 MHz counter) of workgroup 0 around the phases of step 0.
 extern "C" int pp_pamp_set_timing(pp_pamp *p, int enable)
 {
   if (!p) return PP_EARG;
@@ -657,7 +968,6 @@ extern "C" int pp_pamp_set_timing(pp_pamp *p, int enable)
   return PP_OK;
 }
 
-// Copy the stamps of the last timed launch (host, max entries); *count = stamps written.
 extern "C" int pp_pamp_timings(pp_pamp *p, long long *out, int max, int *count)
 {
   if (!p || !out || !count || !p->tlog) return PP_EARG;
@@ -674,8 +984,9 @@ extern "C" int pp_pamp_timings(pp_pamp *p, long long *out, int max, int *count)
 extern "C" int pp_pamp_destroy(pp_pamp *p)
 {
   if (!p) return PP_OK;
+  for (int t = 0; t < p->nterms; t++) if (p->termbuf[t]) (void)hipFree(p->termbuf[t]);
+  if (p->slotcols) (void)hipFree(p->slotcols);
   if (p->tlog) (void)hipFree(p->tlog);
-  for (int t = 0; t < p->nterms; t++) if (p->csr_dev[t]) (void)hipFree(p->csr_dev[t]);
   if (p->V) (void)hipFree(p->V);
   if (p->y) (void)hipFree(p->y);
   if (p->Hm) (void)hipFree(p->Hm);
@@ -683,6 +994,7 @@ extern "C" int pp_pamp_destroy(pp_pamp *p)
   if (p->part) (void)hipFree(p->part);
   if (p->diag) (void)hipFree(p->diag);
   if (p->scal) (void)hipFree(p->scal);
+  if (p->bar) (void)hipFree(p->bar);
   delete p;
   return PP_OK;
 }

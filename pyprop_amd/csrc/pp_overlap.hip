@@ -42,82 +42,96 @@ extern "C" int pp_band_cholesky_upper(int N, int kd, double *ab)
 }
 
 // ---------------------------------------------------------------- host: plan
-int pp_plan_build(pp_band_plan *pl, int N, int kd, const double *U, bool reversed)
+// Packed chunk plan for U (see pp_band_plan in pp_common.h).  B is the
+// compile-time chunk length the device solver is instantiated with.
+int pp_plan_build(pp_band_plan *pl, int N, int kd, const double *U, int B)
 {
-  // U: real band, [j*(kd+1) + kd+i-j]
+  // U: real band, [j*(kd+1) + kd+i-j]; rows/cols >= N are identity padding
   auto getU = [&](int i, int j) -> double {
     if (i < 0 || j < i || j - i > kd) return 0.0;
-    if (i >= N) return (i == j) ? 1.0 : 0.0;  // padded identity rows
+    if (i >= N) return (i == j) ? 1.0 : 0.0;
     if (j >= N) return 0.0;
-    int ii = i, jj = j;
-    if (reversed) { ii = N - 1 - j; jj = N - 1 - i; }
-    return U[(size_t)jj * (kd + 1) + kd + ii - jj];
+    return U[(size_t)j * (kd + 1) + kd + i - j];
   };
-  int B = (N + 63) / 64;
-  if (B < kd) B = kd;
-  if (B < 1) B = 1;
+  if (B < kd) return PP_EUNSUP;
   int P = (N + B - 1) / B;
+  if (P > 32) return PP_EUNSUP;
   int levels = 0;
   while ((1 << levels) < P) levels++;
   pl->N = N; pl->kd = kd; pl->B = B; pl->P = P; pl->levels = levels;
-  std::vector<double> uc((size_t)B * (kd + 1) * P, 0.0), G((size_t)B * kd * P, 0.0),
-      Nl((size_t)(levels > 0 ? levels : 1) * kd * kd * P, 0.0);
+  const size_t nUi = (size_t)B * (kd + 1) * P, nL = (size_t)levels * kd * kd * P;
+  std::vector<double> pk(nUi + 2 * nL, 0.0);
+  double *Ui = pk.data(), *NlN = Ui + nUi, *NlC = NlN + nL;
   for (int c = 0; c < P; c++)
     for (int r = 0; r < B; r++) {
       int row = c * B + r;
-      uc[(size_t)(r * (kd + 1)) * P + c] = 1.0 / getU(row, row);
-      for (int d = 1; d <= kd; d++)
-        if (r + d < B) uc[(size_t)(r * (kd + 1) + d) * P + c] = getU(row, row + d);
+      Ui[(size_t)(r * (kd + 1)) * P + c] = 1.0 / getU(row, row);
+      for (int d = 1; d <= kd; d++) Ui[(size_t)(r * (kd + 1) + d) * P + c] = getU(row, row + d);
     }
-  // G_c = U_cc^-1 C_c, C_c[r][k] = U(cB+r, (c+1)B+k)
+  // chunk-local solves on dense B x B blocks
   std::vector<double> col(B);
-  std::vector<std::vector<double>> M((size_t)P, std::vector<double>((size_t)kd * kd, 0.0));
-  for (int c = 0; c + 1 < P; c++) {
-    for (int k = 0; k < kd; k++) {
-      for (int r = 0; r < B; r++) col[r] = getU(c * B + r, (c + 1) * B + k);
-      for (int r = B - 1; r >= 0; r--) {
-        double s = col[r];
-        for (int d = 1; d <= kd && r + d < B; d++) s -= getU(c * B + r, c * B + r + d) * col[r + d];
-        col[r] = s / getU(c * B + r, c * B + r);
-      }
-      for (int r = 0; r < B; r++) G[(size_t)(r * kd + k) * P + c] = col[r];
-      for (int i = 0; i < kd; i++) M[c][(size_t)i * kd + k] = col[i];
+  auto back = [&](int c) {   // col := U_cc^-1 col
+    for (int r = B - 1; r >= 0; r--) {
+      double s = col[r];
+      for (int d = 1; d <= kd && r + d < B; d++) s -= getU(c * B + r, c * B + r + d) * col[r + d];
+      col[r] = s / getU(c * B + r, c * B + r);
     }
-  }
-  // level matrices: N0_c = -M_c ; N_{l+1,c} = N_{l,c} N_{l,c+2^l}
-  std::vector<std::vector<double>> cur((size_t)P, std::vector<double>((size_t)kd * kd));
+  };
+  auto fwd = [&](int c) {    // col := U_cc^-T col
+    for (int r = 0; r < B; r++) {
+      double s = col[r];
+      for (int d = 1; d <= kd && r - d >= 0; d++) s -= getU(c * B + r - d, c * B + r) * col[r - d];
+      col[r] = s / getU(c * B + r, c * B + r);
+    }
+  };
+  std::vector<std::vector<double>> MN((size_t)P, std::vector<double>((size_t)kd * kd, 0.0)), MC = MN;
   for (int c = 0; c < P; c++)
-    for (int e = 0; e < kd * kd; e++) cur[c][e] = -M[c][e];
-  for (int l = 0; l < levels; l++) {
-    for (int c = 0; c < P; c++)
-      for (int i = 0; i < kd; i++)
-        for (int k = 0; k < kd; k++) Nl[(size_t)((l * kd + i) * kd + k) * P + c] = cur[c][(size_t)i * kd + k];
-    int s = 1 << l;
-    std::vector<std::vector<double>> nxt = cur;
-    for (int c = 0; c + s < P; c++)
-      for (int i = 0; i < kd; i++)
-        for (int k = 0; k < kd; k++) {
-          double a = 0.0;
-          for (int q = 0; q < kd; q++) a += cur[c][(size_t)i * kd + q] * cur[c + s][(size_t)q * kd + k];
-          nxt[c][(size_t)i * kd + k] = a;
-        }
-    cur.swap(nxt);
-  }
-  PP_HIP(hipMalloc(&pl->ucoef, uc.size() * sizeof(double)));
-  PP_HIP(hipMalloc(&pl->G, G.size() * sizeof(double)));
-  PP_HIP(hipMalloc(&pl->Nlev, Nl.size() * sizeof(double)));
-  PP_HIP(hipMemcpy(pl->ucoef, uc.data(), uc.size() * sizeof(double), hipMemcpyHostToDevice));
-  PP_HIP(hipMemcpy(pl->G, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice));
-  PP_HIP(hipMemcpy(pl->Nlev, Nl.data(), Nl.size() * sizeof(double), hipMemcpyHostToDevice));
+    for (int k = 0; k < kd; k++) {
+      if (c + 1 < P) {   // U x = b: M_c = (U_cc^-1 C_c)[0:kd, :], C_c[r][k] = U(cB+r, (c+1)B+k)
+        for (int r = 0; r < B; r++) col[r] = getU(c * B + r, (c + 1) * B + k);
+        back(c);
+        for (int i = 0; i < kd; i++) MN[c][(size_t)i * kd + k] = col[i];
+      }
+      if (c > 0) {       // U^T x = b: M'_c = (U_cc^-T C'_c)[B-kd:B, :], C'_c[r][k] = U((c-1)B+B-kd+k, cB+r)
+        for (int r = 0; r < B; r++) col[r] = getU((c - 1) * B + B - kd + k, c * B + r);
+        fwd(c);
+        for (int i = 0; i < kd; i++) MC[c][(size_t)i * kd + k] = col[B - kd + i];
+      }
+    }
+  // scan matrices: N0_c = -M_c; suffix N_{l+1,c} = N_{l,c} N_{l,c+2^l}; prefix uses c-2^l
+  auto levelize = [&](std::vector<std::vector<double>> &M, double *out, int dir) {
+    std::vector<std::vector<double>> cur = M;
+    for (auto &m : cur) for (auto &v : m) v = -v;
+    for (int l = 0; l < levels; l++) {
+      for (int c = 0; c < P; c++)
+        for (int e = 0; e < kd * kd; e++) out[(size_t)(l * kd * kd + e) * P + c] = cur[c][e];
+      int s = 1 << l;
+      std::vector<std::vector<double>> nxt = cur;
+      for (int c = 0; c < P; c++) {
+        int o = c + dir * s;
+        if (o < 0 || o >= P) continue;
+        for (int i = 0; i < kd; i++)
+          for (int k = 0; k < kd; k++) {
+            double a = 0.0;
+            for (int q = 0; q < kd; q++) a += cur[c][(size_t)i * kd + q] * cur[o][(size_t)q * kd + k];
+            nxt[c][(size_t)i * kd + k] = a;
+          }
+      }
+      cur.swap(nxt);
+    }
+  };
+  levelize(MN, NlN, +1);
+  levelize(MC, NlC, -1);
+  pl->ndoubles = (int)pk.size();
+  PP_HIP(hipMalloc(&pl->packed, pk.size() * sizeof(double)));
+  PP_HIP(hipMemcpy(pl->packed, pk.data(), pk.size() * sizeof(double), hipMemcpyHostToDevice));
   return PP_OK;
 }
 
 void pp_plan_free(pp_band_plan *pl)
 {
-  if (pl->ucoef) (void)hipFree(pl->ucoef);
-  if (pl->G) (void)hipFree(pl->G);
-  if (pl->Nlev) (void)hipFree(pl->Nlev);
-  pl->ucoef = pl->G = pl->Nlev = nullptr;
+  if (pl->packed) (void)hipFree(pl->packed);
+  pl->packed = nullptr;
 }
 
 // ---------------------------------------------------------------- overlap object
@@ -126,7 +140,7 @@ void pp_plan_free(pp_band_plan *pl)
 extern "C" int pp_overlap_create(int N, int kd, const double *S_upper, pp_overlap **out, int *info)
 {
   if (!out || !S_upper || N < 1 || kd < 1) return PP_EARG;
-  if (!(kd == 4 || kd == 5 || kd == 6 || kd == 8) || N > 64 * 32) return PP_EUNSUP;
+  if (!(kd == 4 || kd == 5 || kd == 6 || kd == 8) || pp_plan_chunk(N, kd) == 0) return PP_EUNSUP;
   std::vector<double> ab(S_upper, S_upper + (size_t)2 * N * (kd + 1));
   int inf = pp_band_cholesky_upper(N, kd, ab.data());
   if (info) *info = inf;
@@ -142,8 +156,7 @@ extern "C" int pp_overlap_create(int N, int kd, const double *S_upper, pp_overla
   int rc;
   if (hipMalloc(&ov->Uband, U.size() * sizeof(double)) != hipSuccess) { delete ov; return PP_EHIP; }
   if (hipMemcpy(ov->Uband, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { delete ov; return PP_EHIP; }
-  if ((rc = pp_plan_build(&ov->solveN, N, kd, U.data(), false)) != PP_OK) return rc;
-  if ((rc = pp_plan_build(&ov->solveC, N, kd, U.data(), true)) != PP_OK) return rc;
+  if ((rc = pp_plan_build(&ov->plan, N, kd, U.data(), pp_plan_chunk(N, kd))) != PP_OK) return rc;
   *out = ov;
   return PP_OK;
 }
@@ -159,8 +172,7 @@ extern "C" int pp_overlap_destroy(pp_overlap *ov)
 {
   if (!ov) return PP_OK;
   if (ov->Uband) (void)hipFree(ov->Uband);
-  pp_plan_free(&ov->solveN);
-  pp_plan_free(&ov->solveC);
+  pp_plan_free(&ov->plan);
   delete ov;
   return PP_OK;
 }
@@ -188,21 +200,27 @@ __device__ inline void pp_band_mul_block(const double *Ub, int N, int kd, zd *x,
   __syncthreads();
 }
 
-template <int KD, int MAXB>
+template <int KD, int B>
 __global__ void __launch_bounds__(256) pp_overlap_kernel(pp_overlap ov, int op, zd *x, long post, long nfib)
 {
-  extern __shared__ zd lds[];
+  extern __shared__ double lds_d[];
+  const pp_band_plan &pl = ov.plan;
+  double *pk = lds_d;
+  zd *xs = (zd *)(lds_d + pl.ndoubles);
+  zd *tmp = xs + pl.P * B;
   const int N = ov.N;
-  zd *xs = lds;
-  zd *tmp = lds + ov.solveN.P * ov.solveN.B;
   long f = blockIdx.x;
   if (f >= nfib) return;
+  for (int i = threadIdx.x; i < pl.ndoubles; i += blockDim.x) pk[i] = pl.packed[i];
   long pre = f / post, pst = f % post;
   zd *base = x + pre * (long)N * post + pst;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) xs[i] = base[(long)i * post];
+  for (int i = threadIdx.x; i < pl.P * B; i += blockDim.x) xs[i] = i < N ? base[(long)i * post] : zmk(0.0, 0.0);
   __syncthreads();
   auto solve = [&](bool conj) {
-    if (threadIdx.x < 64) pp_band_solve_wave<KD, MAXB>(conj ? ov.solveC : ov.solveN, xs, conj, N);
+    if (threadIdx.x < 64) {
+      if (conj) pp_solve_lowerT<KD, B>(pk, pl.P, pl.levels, xs, N);
+      else pp_solve_upper<KD, B>(pk, pl.P, pl.levels, xs, N);
+    }
     __syncthreads();
   };
   switch (op) {
@@ -231,19 +249,19 @@ extern "C" int pp_overlap_apply(const pp_overlap *ov, int op, int rank, int R, c
   for (int i = rank + 1; i < R; i++) post *= shape[i];
   long nfib = pre * post;
   if (nfib == 0) return PP_OK;
-  size_t lds = sizeof(zd) * (size_t)(ov->solveN.P * ov->solveN.B + ov->N);
-  int maxb = ov->solveN.B > ov->solveC.B ? ov->solveN.B : ov->solveC.B;
+  const pp_band_plan &pl = ov->plan;
+  size_t lds = sizeof(double) * (size_t)pl.ndoubles + sizeof(zd) * (size_t)(pl.P * pl.B + ov->N);
   bool launched = false;
-  if (maxb <= 16) {
-    PP_KD_SWITCH(ov->kd, {
-      hipLaunchKernelGGL((pp_overlap_kernel<KD_, 16>), dim3(nfib), dim3(256), lds, stream, *ov, op, (zd *)x, post, nfib);
-      launched = true; })
-  } else {
-    PP_KD_SWITCH(ov->kd, {
-      hipLaunchKernelGGL((pp_overlap_kernel<KD_, 32>), dim3(nfib), dim3(256), lds, stream, *ov, op, (zd *)x, post, nfib);
-      launched = true; })
-  }
+  hipError_t err = hipSuccess;
+  PP_B_SWITCH(pl.B, PP_KD_SWITCH(ov->kd, {
+    const void *fn = (const void *)pp_overlap_kernel<KD_, B_>;
+    err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (err == hipSuccess) {
+      hipLaunchKernelGGL((pp_overlap_kernel<KD_, B_>), dim3(nfib), dim3(256), lds, stream, *ov, op, (zd *)x, post, nfib);
+      err = hipGetLastError();
+    }
+    launched = true; }))
   if (!launched) return PP_EUNSUP;
-  PP_HIP(hipGetLastError());
+  if (err != hipSuccess) return PP_EHIP;
   return PP_OK;
 }
