@@ -56,23 +56,26 @@ __host__ __device__ inline double zabs2(zd a) { return a.x * a.x + a.y * a.y; }
 enum pp_storage { PP_IDENT = 0, PP_DIAG = 1, PP_SIMP = 2, PP_HERM = 3, PP_BANDNH = 4, PP_BAND = 5 };
 
 // Chunked parallel solver for one real upper-banded triangular factor U
-// (kd super-diagonals), both for U x = b (backward) and U^T x = b (forward,
-// = U^H x = b since U is real).  Rows are padded to P*B, B a compile-time
-// chunk length, one wavefront lane per chunk (P <= 32).  Each lane solves
-// its chunk locally; the chunk couplings form a kd-dimensional affine
-// recurrence (suffix for U, prefix for U^T) solved by a log-depth scan whose
-// data-independent matrix products are precomputed on the host; a second
-// local pass with the coupling folded into the right-hand side gives x.
-// Padded rows are identity rows with zero right-hand side.
-//
-// Packed plan (doubles), read by lane c at [..]*P + c (conflict-free LDS):
-//   Ui    [(r*(kd+1)+d)*P + c] : d=0: 1/U(i,i); d>0: U(i, i+d) (also across chunks)
-//   NlevN [((l*kd+i)*kd+k)*P + c] suffix-scan matrices of U x = b
-//   NlevC [((l*kd+i)*kd+k)*P + c] prefix-scan matrices of U^T x = b
+// (kd super-diagonals), for U x = b and U^T x = b (= U^H x = b, U real).
+// Rows are padded to P*B (chunk length B a power of two >= kd, P <= 64
+// chunks; padded rows are identity rows).
+//   1. y_c = U_cc^-1 b_c for every chunk at once: a block-diagonal matvec
+//      with the precomputed triangular inverse blocks (all threads; the
+//      blocks are read-only and stay in L2).
+//   2. The chunk couplings give t_c = y_c[0:kd] - M_c t_{c+1} (U) or
+//      t_c = y_c[B-kd:B] - M'_c t_{c-1} (U^T), solved by a log-depth scan
+//      on one wavefront (lane = chunk) whose matrix products are
+//      precomputed (LDS).
+//   3. x_c = y_c - G_c t_{c+1} (resp. G'_c t_{c-1}) for all rows at once.
 struct pp_band_plan {
-  int N, kd, B, P, levels;
-  int ndoubles;      // size of the packed plan
-  double *packed;    // device copy
+  int N, kd, B, lb, P, levels;
+  int nlev;          // doubles of NlevN + NlevC (copied to LDS by the kernels)
+  double *packed;    // device allocation holding the arrays below
+  double *nlevs;     // [NlevN | NlevC], [((l*kd+i)*kd+k)*P + c]
+  double *binv;      // [(c*B + j)*B + i] = (U_cc^-1)(i, j)
+  double *binvT;     // [(c*B + j)*B + i] = (U_cc^-1)(j, i)
+  double *G;         // [k*P*B + c*B + i] = (U_cc^-1 C_c)(i, k)
+  double *GT;        // [k*P*B + c*B + i] = (U_cc^-T C'_c)(i, k)
 };
 
 // Overlap factor on the device: S = U^H U for one B-spline rank.
@@ -82,191 +85,96 @@ struct pp_overlap {
   pp_band_plan plan;
 };
 
-// U x = b for one fiber; x (LDS, >= P*B complex, zero beyond N) holds b on
-// entry and x on exit.  Executed by the 64 lanes of the calling wavefront
-// (lanes >= P idle).  Each lane keeps a sliding window of kd unknowns in
-// registers; the rows touching a chunk boundary are peeled off so the
-// steady-state loop is branch-free.
-struct pp_win_ops {
-  template <int KD>
-  __device__ static inline void shift_in(zd (&win)[KD], zd v)
-  {
-#pragma unroll
-    for (int d = KD - 1; d >= 1; d--) win[d] = win[d - 1];
-    win[0] = v;
-  }
-};
-
-// one backward row: acc = b - sum_{d valid} U(r, r+d) win[d-1] - coupling; returns acc / U(r,r)
-template <int KD, int B, bool EDGE>
-__device__ inline zd pp_back_row(const double *Ui, int P, int c, int r, zd b, const zd (&win)[KD], const zd *t)
+// Solve U x = b (UPPER) or U^T x = b (!UPPER) for one fiber with the whole
+// workgroup (NT threads).  in: b (LDS, P*B, zero beyond N); out: x (LDS,
+// P*B, distinct from in; zero beyond N on return); nl: LDS copy of
+// pl.nlevs; tb: LDS scratch of P*KD complex.  Ends with a __syncthreads.
+template <int KD, bool UPPER, int NT>
+__device__ inline void pp_band_solve(const pp_band_plan &pl, const double *nl, const zd *in, zd *out, zd *tb)
 {
-  zd acc = b;
-  if (EDGE && t) {
+  const int tid = threadIdx.x, P = pl.P, B = pl.B, lb = pl.lb, npad = P << lb;
+  const double *Bm = UPPER ? pl.binv : pl.binvT;
+  // 1. block-diagonal inverse; column blocks of 8 that are structurally zero
+  //    for this row are skipped (upper: j < i, lower: j > i)
+  for (int row = tid; row < npad; row += NT) {
+    const int c = row >> lb, i = row & (B - 1);
+    const double *col = Bm + ((size_t)c << (2 * lb)) + i;
+    const zd *bb = in + (c << lb);
+    zd a0 = zmk(0.0, 0.0), a1 = zmk(0.0, 0.0);
+    for (int j0 = 0; j0 < B; j0 += 8) {
+      if (UPPER ? (j0 + 7 < i) : (j0 > i)) continue;
+      const double *cj = col + ((size_t)j0 << lb);
+#pragma unroll
+      for (int jj = 0; jj < 8; jj += 2) {
+        zfmar(a0, cj[(size_t)jj << lb], bb[j0 + jj]);
+        zfmar(a1, cj[(size_t)(jj + 1) << lb], bb[j0 + jj + 1]);
+      }
+    }
+    out[row] = zadd(a0, a1);
+  }
+  __syncthreads();
+  // 2. scan of the chunk couplings on wave 0
+  if (tid < 64) {
+    const int c = tid;
+    const bool act = c < P;
+    const double *Nl = UPPER ? nl : nl + pl.levels * KD * KD * P;
+    zd a[KD];
+#pragma unroll
+    for (int k = 0; k < KD; k++) a[k] = act ? out[(c << lb) + (UPPER ? k : B - KD + k)] : zmk(0.0, 0.0);
+    for (int l = 0; l < pl.levels; l++) {
+      const int s = 1 << l;
+      zd an[KD];
+#pragma unroll
+      for (int k = 0; k < KD; k++) {
+        if (UPPER) { an[k].x = __shfl_down(a[k].x, s, 64); an[k].y = __shfl_down(a[k].y, s, 64); }
+        else { an[k].x = __shfl_up(a[k].x, s, 64); an[k].y = __shfl_up(a[k].y, s, 64); }
+      }
+      const bool has = UPPER ? (c + s < P) : (c - s >= 0);
+      if (act && has) {
+        const double *Nc = Nl + (size_t)l * KD * KD * P + c;
+#pragma unroll
+        for (int i = 0; i < KD; i++) {
+          zd acc = a[i];
+#pragma unroll
+          for (int k = 0; k < KD; k++) zfmar(acc, Nc[(i * KD + k) * P], an[k]);
+          a[i] = acc;
+        }
+      }
+    }
 #pragma unroll
     for (int k = 0; k < KD; k++) {
-      const int d = B + k - r;
-      if (d >= 1 && d <= KD) { const double u = Ui[(r * (KD + 1) + d) * P + c]; acc.x = fma(-u, t[k].x, acc.x); acc.y = fma(-u, t[k].y, acc.y); }
+      zd t;
+      if (UPPER) { t.x = __shfl_down(a[k].x, 1, 64); t.y = __shfl_down(a[k].y, 1, 64); if (c + 1 >= P) t = zmk(0.0, 0.0); }
+      else { t.x = __shfl_up(a[k].x, 1, 64); t.y = __shfl_up(a[k].y, 1, 64); if (c == 0) t = zmk(0.0, 0.0); }
+      if (act) tb[c * KD + k] = t;
     }
   }
+  __syncthreads();
+  // 3. fix-up with the neighbouring chunk's boundary unknowns
+  const double *Gm = UPPER ? pl.G : pl.GT;
+  for (int row = tid; row < npad; row += NT) {
+    const zd *t = tb + (row >> lb) * KD;
+    zd v = out[row];
 #pragma unroll
-  for (int d = KD; d >= 1; d--)
-    if (!EDGE || r + d < B) { const double u = Ui[(r * (KD + 1) + d) * P + c]; acc.x = fma(-u, win[d - 1].x, acc.x); acc.y = fma(-u, win[d - 1].y, acc.y); }
-  return zscale(acc, Ui[(r * (KD + 1)) * P + c]);
+    for (int k = 0; k < KD; k++) zfmar(v, -Gm[(size_t)k * npad + row], t[k]);
+    out[row] = v;
+  }
+  __syncthreads();
 }
 
-template <int KD, int B>
-__device__ inline void pp_back_pass(const double *Ui, int P, int c, zd *x, zd (&win)[KD], const zd *t, bool write)
-{
-#pragma unroll
-  for (int k = 0; k < KD; k++) win[k] = zmk(0.0, 0.0);
-  // peeled rows B-1 .. B-KD (window partly empty, coupling to the next chunk)
-#pragma unroll
-  for (int r = B - 1; r >= B - KD; r--) {
-    zd v = pp_back_row<KD, B, true>(Ui, P, c, r, x[c * B + r], win, t);
-    pp_win_ops::shift_in<KD>(win, v);
-    if (write) x[c * B + r] = v;
-  }
-#pragma unroll 4
-  for (int r = B - KD - 1; r >= 0; r--) {
-    zd v = pp_back_row<KD, B, false>(Ui, P, c, r, x[c * B + r], win, t);
-    pp_win_ops::shift_in<KD>(win, v);
-    if (write) x[c * B + r] = v;
-  }
-}
-
-template <int KD, int B>
-__device__ inline void pp_solve_upper(const double *pk, int P, int levels, zd *x, int N)
-{
-  const int c = threadIdx.x & 63;
-  if (c >= P) return;
-  const double *Ui = pk;
-  const double *Nl = pk + B * (KD + 1) * P;
-  zd win[KD];
-  pp_back_pass<KD, B>(Ui, P, c, x, win, nullptr, false);   // window ends on rows 0..kd-1
-  zd a[KD];
-#pragma unroll
-  for (int k = 0; k < KD; k++) a[k] = win[k];                // row 0 went in last: win[k] = y of row k
-  for (int l = 0; l < levels; l++) {
-    const int s = 1 << l;
-    zd an[KD];
-#pragma unroll
-    for (int k = 0; k < KD; k++) { an[k].x = __shfl_down(a[k].x, s, 64); an[k].y = __shfl_down(a[k].y, s, 64); }
-    if (c + s < P) {
-#pragma unroll
-      for (int i = 0; i < KD; i++) {
-        zd acc = a[i];
-#pragma unroll
-        for (int k = 0; k < KD; k++) { const double nv = Nl[((l * KD + i) * KD + k) * P + c]; acc.x = fma(nv, an[k].x, acc.x); acc.y = fma(nv, an[k].y, acc.y); }
-        a[i] = acc;
-      }
-    }
-  }
-  zd t[KD];
-#pragma unroll
-  for (int k = 0; k < KD; k++) {
-    t[k].x = __shfl_down(a[k].x, 1, 64); t[k].y = __shfl_down(a[k].y, 1, 64);
-    if (c + 1 >= P) t[k] = zmk(0.0, 0.0);
-  }
-  pp_back_pass<KD, B>(Ui, P, c, x, win, t, true);
-  (void)N;
-}
-
-// one forward row of U^T x = b: acc = b - sum_{d valid} U(r-d, r) win[d-1] - coupling
-template <int KD, int B, bool EDGE>
-__device__ inline zd pp_fwd_row(const double *Ui, int P, int c, int r, zd b, const zd (&win)[KD], const zd *t)
-{
-  zd acc = b;
-  if (EDGE && t && c > 0) {
-#pragma unroll
-    for (int k = 0; k < KD; k++) {
-      if (k >= r) {
-        const int rr = B - KD + k, d = r + KD - k;
-        const double u = Ui[(rr * (KD + 1) + d) * P + c - 1];
-        acc.x = fma(-u, t[k].x, acc.x); acc.y = fma(-u, t[k].y, acc.y);
-      }
-    }
-  }
-#pragma unroll
-  for (int d = KD; d >= 1; d--)
-    if (!EDGE || r - d >= 0) { const double u = Ui[((r - d) * (KD + 1) + d) * P + c]; acc.x = fma(-u, win[d - 1].x, acc.x); acc.y = fma(-u, win[d - 1].y, acc.y); }
-  return zscale(acc, Ui[(r * (KD + 1)) * P + c]);
-}
-
-template <int KD, int B>
-__device__ inline void pp_fwd_pass(const double *Ui, int P, int c, zd *x, zd (&win)[KD], const zd *t, bool write)
-{
-#pragma unroll
-  for (int k = 0; k < KD; k++) win[k] = zmk(0.0, 0.0);
-#pragma unroll
-  for (int r = 0; r < KD; r++) {
-    zd v = pp_fwd_row<KD, B, true>(Ui, P, c, r, x[c * B + r], win, t);
-    pp_win_ops::shift_in<KD>(win, v);
-    if (write) x[c * B + r] = v;
-  }
-#pragma unroll 4
-  for (int r = KD; r < B; r++) {
-    zd v = pp_fwd_row<KD, B, false>(Ui, P, c, r, x[c * B + r], win, t);
-    pp_win_ops::shift_in<KD>(win, v);
-    if (write) x[c * B + r] = v;
-  }
-}
-
-// U^T x = b (forward substitution) with the same packed plan.
-template <int KD, int B>
-__device__ inline void pp_solve_lowerT(const double *pk, int P, int levels, zd *x, int N)
-{
-  const int c = threadIdx.x & 63;
-  if (c >= P) return;
-  const double *Ui = pk;
-  const double *Nl = pk + B * (KD + 1) * P + levels * KD * KD * P;
-  zd win[KD];
-  pp_fwd_pass<KD, B>(Ui, P, c, x, win, nullptr, false);    // window ends on rows B-1 .. B-kd
-  zd a[KD];
-#pragma unroll
-  for (int k = 0; k < KD; k++) a[k] = win[KD - 1 - k];      // z of row B-kd+k
-  for (int l = 0; l < levels; l++) {
-    const int s = 1 << l;
-    zd an[KD];
-#pragma unroll
-    for (int k = 0; k < KD; k++) { an[k].x = __shfl_up(a[k].x, s, 64); an[k].y = __shfl_up(a[k].y, s, 64); }
-    if (c - s >= 0) {
-#pragma unroll
-      for (int i = 0; i < KD; i++) {
-        zd acc = a[i];
-#pragma unroll
-        for (int k = 0; k < KD; k++) { const double nv = Nl[((l * KD + i) * KD + k) * P + c]; acc.x = fma(nv, an[k].x, acc.x); acc.y = fma(nv, an[k].y, acc.y); }
-        a[i] = acc;
-      }
-    }
-  }
-  zd t[KD];
-#pragma unroll
-  for (int k = 0; k < KD; k++) {
-    t[k].x = __shfl_up(a[k].x, 1, 64); t[k].y = __shfl_up(a[k].y, 1, 64);
-    if (c == 0) t[k] = zmk(0.0, 0.0);
-  }
-  pp_fwd_pass<KD, B>(Ui, P, c, x, win, t, true);
-  (void)N;
-}
-
-// Chunk length for N rows: the smallest of 8/16/32/64 giving P <= 16 chunks
-// (B >= kd); 16 chunks keep the packed plan small enough to live in LDS next
-// to the Krylov buffers of a fiber of up to 1024 B-splines.
+// Chunk length for N rows: the smallest power of two B >= max(kd, 16) with
+// at most 64 chunks whose scan matrices (both directions) fit in 72 KB of LDS.
 inline int pp_plan_chunk(int N, int kd)
 {
-  for (int B : {8, 16, 32, 64})
-    if (B >= kd && (N + B - 1) / B <= 16) return B;
+  for (int B = 16; B <= 256; B *= 2) {
+    if (B < kd) continue;
+    const int P = (N + B - 1) / B;
+    int levels = 0;
+    while ((1 << levels) < P) levels++;
+    if (P <= 64 && 2.0 * levels * kd * kd * P * sizeof(double) <= 72.0 * 1024) return B;
+  }
   return 0;
 }
-
-#define PP_B_SWITCH(Bv, ...) \
-  switch (Bv) {              \
-    case 8: { constexpr int B_ = 8; __VA_ARGS__; } break; \
-    case 16: { constexpr int B_ = 16; __VA_ARGS__; } break; \
-    case 32: { constexpr int B_ = 32; __VA_ARGS__; } break; \
-    case 64: { constexpr int B_ = 64; __VA_ARGS__; } break; \
-    default: break; }
 
 // Dispatch helper over the supported super-diagonal counts (B-spline order-1).
 #define PP_KD_SWITCH(kd, ...) \

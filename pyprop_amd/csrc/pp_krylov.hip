@@ -53,14 +53,13 @@ struct pp_barrier {
 };
 
 struct pp_pamp_args {
-  int L, N, m, s, bw, nterms, nsteps, reorth, padN, nslots, ncombo, scrDoubles;
+  int L, N, m, s, bw, nterms, nsteps, reorth, padN, nslots, ncombo, scrDoubles, nlevEven;
   double dt_re, dt_im;
   pp_fterm term[PP_MAXTERMS];
   const int *slotcols;   // L x nslots source fibers staged per row (-1 unused)
   const double *scal;    // nsteps x nterms
   const double *Uband;   // real U band for psi~ = U psi
-  const double *plan;    // packed chunk plan (copied to LDS)
-  int planP, planLevels, planDoubles;
+  pp_band_plan plan;     // chunk solver plan of U (scan matrices copied to LDS)
   zd *psi;               // L x N, in/out (fiber a private to workgroup a)
   zd *V;                 // L x m x N (fiber-major, private)
   zd *y;                 // L x N, U^-1 w (exchanged)
@@ -349,16 +348,19 @@ __device__ void pade_block(int m, const zd *Hg, double dt_re, double dt_im, zd *
 
 
 // ------------------------------------------------------------------ the step kernel
-template <int KD, int B>
+template <int KD>
 __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
 {
   extern __shared__ double lds_d[];
   const int N = A.N, L = A.L, m = A.m, a = blockIdx.x, tid = threadIdx.x;
-  const int padN = A.padN, bw = A.bw, s = A.s;
-  double *pk = lds_d;                                  // packed solver plan
-  zd *W = (zd *)(lds_d + A.planDoubles);               // residual of fiber a
-  zd *X = W + padN;                                    // solve / accumulation buffer
-  zd *Ys = X + padN;                                   // staged source fibers (nslots x N)
+  const int padN = A.padN;
+  const pp_band_plan &pl = A.plan;
+  double *nl = lds_d;                                  // scan matrices of the solver plan
+  zd *W = (zd *)(lds_d + A.nlevEven);                  // residual of fiber a (padN, zero beyond N)
+  zd *X = W + padN;                                    // solve input (padN)
+  zd *tb = X + padN;                                   // solver scratch (P*KD)
+  zd *Ys = tb + pl.P * KD;                             // staged source fibers (nslots x N)
+  zd *Z = Ys;                                          // also: solve output (padN)
   double *scr = (double *)Ys;                          // also: partial-sum staging (scrDoubles)
   double *wsum = (double *)Ys + A.scrDoubles;          // 8
   double *hred = wsum + 8;                             // 2m+4 reduced values
@@ -376,28 +378,28 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
   auto stamp = [&](int step) {
     if (A.tlog && step == 0 && a == 0 && tid == 0 && tcount < PP_TLOG_MAX) A.tlog[tcount++] = (long long)wall_clock64();
   };
-  for (int i = tid; i < A.planDoubles; i += PP_THREADS) pk[i] = A.plan[i];
+  for (int i = tid; i < pl.nlev; i += PP_THREADS) nl[i] = pl.nlevs[i];
+  for (int i = N + tid; i < padN; i += PP_THREADS) W[i] = zmk(0.0, 0.0);
   __syncthreads();
 
   for (int step = 0; step < A.nsteps; step++) {
     // ---- psi~ = U psi (MultiplySqrtOverlap, PamPropagator.py:53) ; y = U^-1 psi~
-    for (int i = tid; i < N; i += PP_THREADS) X[i] = A.psi[(size_t)a * N + i];
+    for (int i = tid; i < N; i += PP_THREADS) Ys[i] = A.psi[(size_t)a * N + i];
     __syncthreads();
     double nrm = 0.0;
     for (int i = tid; i < N; i += PP_THREADS) {
       zd acc = zmk(0.0, 0.0);
       const int j1 = i + KD < N - 1 ? i + KD : N - 1;
-      for (int j = i; j <= j1; j++) zfmar(acc, A.Uband[(size_t)j * (KD + 1) + KD + i - j], X[j]);
+      for (int j = i; j <= j1; j++) zfmar(acc, A.Uband[(size_t)j * (KD + 1) + KD + i - j], Ys[j]);
       W[i] = acc;
       nrm += zabs2(acc);
     }
     nrm = block_sum(nrm, wsum);
     if (tid == 0) xst(mypart + NRM, nrm);
-    for (int i = tid; i < padN; i += PP_THREADS) X[i] = i < N ? W[i] : zmk(0.0, 0.0);
+    for (int i = tid; i < padN; i += PP_THREADS) X[i] = W[i];
     __syncthreads();
-    if (tid < 64) pp_solve_upper<KD, B>(pk, A.planP, A.planLevels, X, N);
-    __syncthreads();
-    for (int i = tid; i < N; i += PP_THREADS) zst(A.y + (size_t)a * N + i, X[i]);
+    pp_band_solve<KD, true, PP_THREADS>(pl, nl, X, Z, tb);
+    for (int i = tid; i < N; i += PP_THREADS) zst(A.y + (size_t)a * N + i, Z[i]);
     stamp(step);
     grid_barrier(A.bar);
     stamp(step);
@@ -487,11 +489,8 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
         }
         __syncthreads();
         stamp(step);
-        if (tid < 64) pp_solve_lowerT<KD, B>(pk, A.planP, A.planLevels, X, N);
-        __syncthreads();
+        pp_band_solve<KD, false, PP_THREADS>(pl, nl, X, W, tb);
         stamp(step);
-        for (int i = tid; i < N; i += PP_THREADS) W[i] = X[i];
-        __syncthreads();
         fiber_dots(Va, W, nk, N, A.reorth != 0, hred);
         for (int c = tid; c < ncomp; c += PP_THREADS) xst(mypart + c, hred[c]);
         if (A.reorth && tid == 0) xst(mypart + ONRM, hred[ncomp]);
@@ -571,12 +570,11 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
         for (int i = tid; i < N; i += PP_THREADS) v += zabs2(W[i]);
         v = block_sum(v, wsum);
         if (tid == 0) xst(mypart + NRM, v);
-        for (int i = tid; i < padN; i += PP_THREADS) X[i] = (i < N) ? W[i] : zmk(0.0, 0.0);
+        for (int i = tid; i < padN; i += PP_THREADS) X[i] = W[i];
         __syncthreads();
         stamp(step);
-        if (tid < 64) pp_solve_upper<KD, B>(pk, A.planP, A.planLevels, X, N);
-        __syncthreads();
-        for (int i = tid; i < N; i += PP_THREADS) zst(A.y + (size_t)a * N + i, X[i]);
+        pp_band_solve<KD, true, PP_THREADS>(pl, nl, X, Z, tb);
+        for (int i = tid; i < N; i += PP_THREADS) zst(A.y + (size_t)a * N + i, Z[i]);
       }
       stamp(step);
       grid_barrier(A.bar);
@@ -585,7 +583,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
 
     // ---------------- exponentiate on workgroup 0, broadcast exp(-i dt H) e1
     if (a == 0) {
-      pade_block(m, A.Hm, A.dt_re, A.dt_im, (zd *)lds_d + 0 + (A.planDoubles + 1) / 2, A.cvec,
+      pade_block(m, A.Hm, A.dt_re, A.dt_im, (zd *)(lds_d + A.nlevEven), A.cvec,
                  A.diag + (size_t)step * 4);
       if (tid == 0) {
         A.diag[(size_t)step * 4 + 1] = mEff;
@@ -608,9 +606,8 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
       X[i] = zscale(acc, beta0);
     }
     __syncthreads();
-    if (tid < 64) pp_solve_upper<KD, B>(pk, A.planP, A.planLevels, X, N);
-    __syncthreads();
-    for (int i = tid; i < N; i += PP_THREADS) A.psi[(size_t)a * N + i] = X[i];
+    pp_band_solve<KD, true, PP_THREADS>(pl, nl, X, Z, tb);
+    for (int i = tid; i < N; i += PP_THREADS) A.psi[(size_t)a * N + i] = Z[i];
     __syncthreads();
     stamp(step);
     // cvec is rewritten by the next step's Pade only after further barriers
@@ -852,14 +849,20 @@ extern "C" int pp_pamp_add_term(pp_pamp *p, int angular_storage, int P0, const i
   return pamp_add_factored(p, angular_storage, P0, pairs, bw, rank, coef.data(), bands.data());
 }
 
+static int pamp_scr_doubles(const pp_pamp *p, int padN)
+{
+  return std::max(std::max(2 * p->nslots * p->N, p->L * (2 * p->m + 2)), 2 * padN);
+}
+
 static size_t pamp_lds_bytes(const pp_pamp *p, int padN)
 {
   const pp_band_plan &pl = p->ov->plan;
-  const size_t scr = std::max(2 * (size_t)p->nslots * p->N, (size_t)p->L * (2 * p->m + 2));
-  size_t fib = sizeof(double) * (size_t)pl.ndoubles + sizeof(zd) * (size_t)(2 * padN) + sizeof(double) * scr +
+  const size_t nle = (size_t)((pl.nlev + 1) & ~1);
+  size_t fib = sizeof(double) * nle + sizeof(zd) * (size_t)(2 * padN + pl.P * p->ov->kd) +
+               sizeof(double) * (size_t)pamp_scr_doubles(p, padN) +
                sizeof(double) * (size_t)(8 + (2 * p->m + 4) + PP_THREADS + 2 * p->m) + sizeof(zd) * (size_t)p->m +
                (sizeof(zd) + 2 * sizeof(int)) * PP_MAXCOMBO * PP_MAXSLOTS + 64;
-  size_t pade = sizeof(double) * (size_t)(pl.ndoubles + 2) + sizeof(zd) * (size_t)(5 * p->m * p->m) +
+  size_t pade = sizeof(double) * nle + sizeof(zd) * (size_t)(5 * p->m * p->m) +
                 sizeof(double) * 8 + 2 * sizeof(int) + 64;
   return fib > pade ? fib : pade;
 }
@@ -912,16 +915,16 @@ extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re
   A.ncombo = 0;
   for (int t = 0; t < p->nterms; t++) A.ncombo += p->term[t].rank;
   if (A.ncombo > PP_MAXCOMBO) return PP_EUNSUP;
-  A.scrDoubles = std::max(2 * p->nslots * p->N, p->L * (2 * p->m + 2));
   const pp_band_plan &pl = ov->plan;
   A.padN = pl.P * pl.B;
-  if (A.padN < p->N) A.padN = p->N;
+  A.scrDoubles = pamp_scr_doubles(p, A.padN);
+  A.nlevEven = (pl.nlev + 1) & ~1;
   A.dt_re = dt_re; A.dt_im = dt_im;
   for (int t = 0; t < p->nterms; t++) A.term[t] = p->term[t];
   A.slotcols = p->slotcols;
   A.scal = p->scal;
   A.Uband = ov->Uband;
-  A.plan = pl.packed; A.planP = pl.P; A.planLevels = pl.levels; A.planDoubles = pl.ndoubles;
+  A.plan = pl;
   A.psi = (zd *)psi; A.V = p->V; A.y = p->y; A.part = p->part; A.pstr = p->pstr;
   A.Hm = p->Hm; A.cvec = p->cvec; A.diag = p->diag; A.bar = p->bar;
   A.tlog = p->timing ? p->tlog : nullptr;
@@ -930,12 +933,12 @@ extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re
   void *args[] = {&A};
   bool launched = false;
   hipError_t err = hipSuccess;
-  PP_B_SWITCH(pl.B, PP_KD_SWITCH(ov->kd, {
-    const void *fn = (const void *)pp_pamp_kernel<KD_, B_>;
+  PP_KD_SWITCH(ov->kd, {
+    const void *fn = (const void *)pp_pamp_kernel<KD_>;
     err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (err == hipSuccess)
       err = hipLaunchCooperativeKernel(fn, dim3(p->L), dim3(PP_THREADS), args, (unsigned)lds, stream);
-    launched = true; }))
+    launched = true; })
   if (!launched) return PP_EUNSUP;
   if (err != hipSuccess) return PP_EHIP;
   return PP_OK;

@@ -3,8 +3,8 @@
 // Reference: core/representation/overlapmatrix.cpp:9-37 (upper band storage,
 // zpbtrf), :137-149 (ztbmv / ztbsv per fiber), :207-229 (tensor fibers via
 // MapToRank3).  The Cholesky factor of a B-spline overlap matrix is real, so
-// the device keeps it as doubles; solves run one wavefront per fiber with the
-// chunked scan of pp_band_solve_wave (see pp_common.h).
+// the device keeps it as doubles; solves run one workgroup per fiber with the
+// chunked block-inverse solver pp_band_solve (pp_common.h).
 #include "pp_common.h"
 #include <vector>
 #include <cmath>
@@ -42,8 +42,9 @@ extern "C" int pp_band_cholesky_upper(int N, int kd, double *ab)
 }
 
 // ---------------------------------------------------------------- host: plan
-// Packed chunk plan for U (see pp_band_plan in pp_common.h).  B is the
-// compile-time chunk length the device solver is instantiated with.
+// Chunk plan of U for pp_band_solve (pp_common.h): per chunk the dense
+// triangular inverse U_cc^-1, the coupling columns G_c = U_cc^-1 C_c and
+// G'_c = U_cc^-T C'_c, and the level matrices of both boundary scans.
 int pp_plan_build(pp_band_plan *pl, int N, int kd, const double *U, int B)
 {
   // U: real band, [j*(kd+1) + kd+i-j]; rows/cols >= N are identity padding
@@ -53,22 +54,18 @@ int pp_plan_build(pp_band_plan *pl, int N, int kd, const double *U, int B)
     if (j >= N) return 0.0;
     return U[(size_t)j * (kd + 1) + kd + i - j];
   };
-  if (B < kd) return PP_EUNSUP;
-  int P = (N + B - 1) / B;
-  if (P > 32) return PP_EUNSUP;
+  int lb = 0;
+  while ((1 << lb) < B) lb++;
+  if (B < kd || B < 8 || (1 << lb) != B) return PP_EUNSUP;
+  const int P = (N + B - 1) / B;
+  if (P > 64) return PP_EUNSUP;
   int levels = 0;
   while ((1 << levels) < P) levels++;
-  pl->N = N; pl->kd = kd; pl->B = B; pl->P = P; pl->levels = levels;
-  const size_t nUi = (size_t)B * (kd + 1) * P, nL = (size_t)levels * kd * kd * P;
-  std::vector<double> pk(nUi + 2 * nL, 0.0);
-  double *Ui = pk.data(), *NlN = Ui + nUi, *NlC = NlN + nL;
-  for (int c = 0; c < P; c++)
-    for (int r = 0; r < B; r++) {
-      int row = c * B + r;
-      Ui[(size_t)(r * (kd + 1)) * P + c] = 1.0 / getU(row, row);
-      for (int d = 1; d <= kd; d++) Ui[(size_t)(r * (kd + 1) + d) * P + c] = getU(row, row + d);
-    }
-  // chunk-local solves on dense B x B blocks
+  const int npad = P * B;
+  pl->N = N; pl->kd = kd; pl->B = B; pl->lb = lb; pl->P = P; pl->levels = levels;
+  const size_t nL = (size_t)levels * kd * kd * P, nB = (size_t)P * B * B, nG = (size_t)kd * npad;
+  std::vector<double> pk(2 * nL + 2 * nB + 2 * nG, 0.0);
+  double *NlN = pk.data(), *NlC = NlN + nL, *Bi = NlC + nL, *BiT = Bi + nB, *G = BiT + nB, *GT = G + nG;
   std::vector<double> col(B);
   auto back = [&](int c) {   // col := U_cc^-1 col
     for (int r = B - 1; r >= 0; r--) {
@@ -85,19 +82,31 @@ int pp_plan_build(pp_band_plan *pl, int N, int kd, const double *U, int B)
     }
   };
   std::vector<std::vector<double>> MN((size_t)P, std::vector<double>((size_t)kd * kd, 0.0)), MC = MN;
-  for (int c = 0; c < P; c++)
+  for (int c = 0; c < P; c++) {
+    for (int j = 0; j < B; j++) {   // column j of U_cc^-1
+      std::fill(col.begin(), col.end(), 0.0);
+      col[j] = 1.0;
+      back(c);
+      for (int i = 0; i < B; i++) {
+        Bi[((size_t)c * B + j) * B + i] = col[i];    // (i, j)
+        BiT[((size_t)c * B + i) * B + j] = col[i];   // (j, i) of the transpose
+      }
+    }
     for (int k = 0; k < kd; k++) {
-      if (c + 1 < P) {   // U x = b: M_c = (U_cc^-1 C_c)[0:kd, :], C_c[r][k] = U(cB+r, (c+1)B+k)
+      if (c + 1 < P) {   // U x = b: C_c[r][k] = U(cB+r, (c+1)B+k)
         for (int r = 0; r < B; r++) col[r] = getU(c * B + r, (c + 1) * B + k);
         back(c);
+        for (int i = 0; i < B; i++) G[(size_t)k * npad + c * B + i] = col[i];
         for (int i = 0; i < kd; i++) MN[c][(size_t)i * kd + k] = col[i];
       }
-      if (c > 0) {       // U^T x = b: M'_c = (U_cc^-T C'_c)[B-kd:B, :], C'_c[r][k] = U((c-1)B+B-kd+k, cB+r)
+      if (c > 0) {       // U^T x = b: C'_c[r][k] = U((c-1)B+B-kd+k, cB+r)
         for (int r = 0; r < B; r++) col[r] = getU((c - 1) * B + B - kd + k, c * B + r);
         fwd(c);
+        for (int i = 0; i < B; i++) GT[(size_t)k * npad + c * B + i] = col[i];
         for (int i = 0; i < kd; i++) MC[c][(size_t)i * kd + k] = col[B - kd + i];
       }
     }
+  }
   // scan matrices: N0_c = -M_c; suffix N_{l+1,c} = N_{l,c} N_{l,c+2^l}; prefix uses c-2^l
   auto levelize = [&](std::vector<std::vector<double>> &M, double *out, int dir) {
     std::vector<std::vector<double>> cur = M;
@@ -122,9 +131,14 @@ int pp_plan_build(pp_band_plan *pl, int N, int kd, const double *U, int B)
   };
   levelize(MN, NlN, +1);
   levelize(MC, NlC, -1);
-  pl->ndoubles = (int)pk.size();
+  pl->nlev = (int)(2 * nL);
   PP_HIP(hipMalloc(&pl->packed, pk.size() * sizeof(double)));
   PP_HIP(hipMemcpy(pl->packed, pk.data(), pk.size() * sizeof(double), hipMemcpyHostToDevice));
+  pl->nlevs = pl->packed;
+  pl->binv = pl->packed + 2 * nL;
+  pl->binvT = pl->binv + nB;
+  pl->G = pl->binvT + nB;
+  pl->GT = pl->G + nG;
   return PP_OK;
 }
 
@@ -200,38 +214,38 @@ __device__ inline void pp_band_mul_block(const double *Ub, int N, int kd, zd *x,
   __syncthreads();
 }
 
-template <int KD, int B>
+template <int KD>
 __global__ void __launch_bounds__(256) pp_overlap_kernel(pp_overlap ov, int op, zd *x, long post, long nfib)
 {
   extern __shared__ double lds_d[];
   const pp_band_plan &pl = ov.plan;
-  double *pk = lds_d;
-  zd *xs = (zd *)(lds_d + pl.ndoubles);
-  zd *tmp = xs + pl.P * B;
-  const int N = ov.N;
+  const int N = ov.N, npad = pl.P * pl.B;
+  double *nl = lds_d;
+  zd *xs = (zd *)(lds_d + ((pl.nlev + 1) & ~1));   // npad, zero beyond N
+  zd *xp = xs + npad;                               // npad
+  zd *tb = xp + npad;                               // P*KD
   long f = blockIdx.x;
   if (f >= nfib) return;
-  for (int i = threadIdx.x; i < pl.ndoubles; i += blockDim.x) pk[i] = pl.packed[i];
+  for (int i = threadIdx.x; i < pl.nlev; i += blockDim.x) nl[i] = pl.nlevs[i];
   long pre = f / post, pst = f % post;
   zd *base = x + pre * (long)N * post + pst;
-  for (int i = threadIdx.x; i < pl.P * B; i += blockDim.x) xs[i] = i < N ? base[(long)i * post] : zmk(0.0, 0.0);
+  for (int i = threadIdx.x; i < npad; i += blockDim.x) xs[i] = i < N ? base[(long)i * post] : zmk(0.0, 0.0);
   __syncthreads();
   auto solve = [&](bool conj) {
-    if (threadIdx.x < 64) {
-      if (conj) pp_solve_lowerT<KD, B>(pk, pl.P, pl.levels, xs, N);
-      else pp_solve_upper<KD, B>(pk, pl.P, pl.levels, xs, N);
-    }
+    if (conj) pp_band_solve<KD, false, 256>(pl, nl, xs, xp, tb);
+    else pp_band_solve<KD, true, 256>(pl, nl, xs, xp, tb);
+    for (int i = threadIdx.x; i < npad; i += blockDim.x) xs[i] = xp[i];
     __syncthreads();
   };
   switch (op) {
     case PP_OV_SOLVE_SQRT: solve(false); break;
     case PP_OV_SOLVE_SQRT_CONJ: solve(true); break;
-    case PP_OV_MUL_SQRT: pp_band_mul_block(ov.Uband, N, ov.kd, xs, tmp, false); break;
-    case PP_OV_MUL_SQRT_CONJ: pp_band_mul_block(ov.Uband, N, ov.kd, xs, tmp, true); break;
+    case PP_OV_MUL_SQRT: pp_band_mul_block(ov.Uband, N, ov.kd, xs, xp, false); break;
+    case PP_OV_MUL_SQRT_CONJ: pp_band_mul_block(ov.Uband, N, ov.kd, xs, xp, true); break;
     case PP_OV_SOLVE: solve(true); solve(false); break;     // S^-1 = U^-1 U^-H (overlapmatrix.cpp:131-135)
     case PP_OV_MUL:                                         // S = U^H U
-      pp_band_mul_block(ov.Uband, N, ov.kd, xs, tmp, false);
-      pp_band_mul_block(ov.Uband, N, ov.kd, xs, tmp, true);
+      pp_band_mul_block(ov.Uband, N, ov.kd, xs, xp, false);
+      pp_band_mul_block(ov.Uband, N, ov.kd, xs, xp, true);
       break;
   }
   for (int i = threadIdx.x; i < N; i += blockDim.x) base[(long)i * post] = xs[i];
@@ -250,17 +264,17 @@ extern "C" int pp_overlap_apply(const pp_overlap *ov, int op, int rank, int R, c
   long nfib = pre * post;
   if (nfib == 0) return PP_OK;
   const pp_band_plan &pl = ov->plan;
-  size_t lds = sizeof(double) * (size_t)pl.ndoubles + sizeof(zd) * (size_t)(pl.P * pl.B + ov->N);
+  size_t lds = sizeof(double) * (size_t)((pl.nlev + 1) & ~1) + sizeof(zd) * (size_t)(2 * pl.P * pl.B + pl.P * ov->kd);
   bool launched = false;
   hipError_t err = hipSuccess;
-  PP_B_SWITCH(pl.B, PP_KD_SWITCH(ov->kd, {
-    const void *fn = (const void *)pp_overlap_kernel<KD_, B_>;
+  PP_KD_SWITCH(ov->kd, {
+    const void *fn = (const void *)pp_overlap_kernel<KD_>;
     err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (err == hipSuccess) {
-      hipLaunchKernelGGL((pp_overlap_kernel<KD_, B_>), dim3(nfib), dim3(256), lds, stream, *ov, op, (zd *)x, post, nfib);
+      hipLaunchKernelGGL((pp_overlap_kernel<KD_>), dim3(nfib), dim3(256), lds, stream, *ov, op, (zd *)x, post, nfib);
       err = hipGetLastError();
     }
-    launched = true; }))
+    launched = true; })
   if (!launched) return PP_EUNSUP;
   if (err != hipSuccess) return PP_EHIP;
   return PP_OK;
