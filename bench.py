@@ -41,6 +41,9 @@ def main():
     ap.add_argument("--workload", default="hydrogen_large")
     ap.add_argument("--cpu-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one plain device copy of this many MiB before the workload, so that a "
+                         "rocprofv3 --pmc run can scale FETCH_SIZE/WRITE_SIZE by a copy of known size")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs one GPU holding the whole wavefunction; "
@@ -51,6 +54,15 @@ def main():
     import torch
     import pyprop_amd
     from pyprop_amd.examples import hydrogen as H
+
+    if args.copy_calibration_mib > 0:
+        nb = args.copy_calibration_mib << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     conf = H.load(args.workload)
     prob = pyprop_amd.Problem(conf)
