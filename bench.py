@@ -24,6 +24,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM-side bytes per step of pp_pamp_kernel on each workload, read by the
+# FETCH_SIZE and WRITE_SIZE counter passes recorded in
+# profiles/pp_pamp_kernel.hydrogen_large.pmc.md (FETCH_SIZE scaled x2 and
+# WRITE_SIZE x1 by a 512 MiB device copy of known size).
+PMC_TRAFFIC_PER_STEP = {"hydrogen_large": {"fetch": 67.8e6, "write": 60.1e6}}
+
 
 def step_bytes(n, m, N, s_mv=16, s_gs=16):
     """Algorithmic HBM bytes of one Krylov step:
@@ -39,7 +45,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="hydrogen_large")
-    ap.add_argument("--cpu-steps", type=int, default=50)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB before the workload, so that a "
@@ -94,6 +100,8 @@ def main():
     norm = prob.psi.GetNorm()
 
     Bstep = step_bytes(n, m, N)
+    pmc = PMC_TRAFFIC_PER_STEP.get(args.workload)
+    traffic = None if pmc is None else (pmc["fetch"] + pmc["write"]) * args.steps
     achieved = Bstep * args.steps / kernel_s / 1e9
 
     cpu = None
@@ -129,9 +137,10 @@ def main():
         "config": {"workload": args.workload, "n": n, "L": L, "N": N, "order": 7, "krylov_dim": m,
                    "dt": dt.real, "form": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "pp_pamp_kernel", "algorithmic_bytes_per_step": Bstep,
-                     "kernel_ms_per_step": 1e3 * kernel_s / args.steps},
+                     "kernel_ms_per_step": 1e3 * kernel_s / args.steps,
+                     "launch": "one launch of %d steps; traffic from counter passes" % args.steps},
         "cpu_baseline": cpu,
         "checks": {"norm_after": norm, "krylov_error_estimate_last": float(diag[-1, 0]),
                    "basis_used_last": int(diag[-1, 1])},
