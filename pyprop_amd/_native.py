@@ -12,6 +12,8 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpyprop_amd.so")
+if os.environ.get("PYPROP_AMD_LIB"):   # an alternative in-tree build (launch-shape tuning)
+    LIB_PATH = os.path.abspath(os.environ["PYPROP_AMD_LIB"])
 
 PP_OK, PP_EARG, PP_EUNSUP, PP_EHIP = 0, -1, -2, -3
 

@@ -85,17 +85,18 @@ struct pp_overlap {
   pp_band_plan plan;
 };
 
-// Solve U x = b (UPPER) or U^T x = b (!UPPER) for one fiber with the whole
-// workgroup (NT threads).  in: b (LDS, P*B, zero beyond N); out: x (LDS,
-// P*B, distinct from in; zero beyond N on return); nl: LDS copy of
-// pl.nlevs; tb: LDS scratch of P*KD complex.  Ends with a __syncthreads.
-template <int KD, bool UPPER, int NT>
-__device__ inline void pp_band_solve(const pp_band_plan &pl, const double *nl, const zd *in, zd *out, zd *tb)
+struct pp_nostamp {
+  __device__ void operator()() const {}
+};
+
+// Stage 1: out_c = U_cc^-1 in_c (UPPER) or U_cc^-T in_c for every chunk c;
+// 8-column blocks that are structurally zero for a row are skipped (upper:
+// j < i, lower: j > i).  The blocks are read-only and stream from L2.
+template <bool UPPER, int NT>
+__device__ inline void pp_solve_blocks(const pp_band_plan &pl, const zd *in, zd *out)
 {
-  const int tid = threadIdx.x, P = pl.P, B = pl.B, lb = pl.lb, npad = P << lb;
+  const int tid = threadIdx.x, B = pl.B, lb = pl.lb, npad = pl.P << lb;
   const double *Bm = UPPER ? pl.binv : pl.binvT;
-  // 1. block-diagonal inverse; column blocks of 8 that are structurally zero
-  //    for this row are skipped (upper: j < i, lower: j > i)
   for (int row = tid; row < npad; row += NT) {
     const int c = row >> lb, i = row & (B - 1);
     const double *col = Bm + ((size_t)c << (2 * lb)) + i;
@@ -112,53 +113,82 @@ __device__ inline void pp_band_solve(const pp_band_plan &pl, const double *nl, c
     }
     out[row] = zadd(a0, a1);
   }
-  __syncthreads();
-  // 2. scan of the chunk couplings on wave 0
-  if (tid < 64) {
-    const int c = tid;
-    const bool act = c < P;
-    const double *Nl = UPPER ? nl : nl + pl.levels * KD * KD * P;
-    zd a[KD];
+}
+
+// Stage 2 (wavefront 0 only): boundary unknowns of every chunk from the
+// chunk couplings, t_c = y_c[0:kd] - M_c t_{c+1} (UPPER) or
+// t_c = y_c[B-kd:B] - M'_c t_{c-1}, by a log-depth scan (lane = chunk);
+// tb[c*KD + k] receives the neighbour's t (t_{c+1} resp. t_{c-1}).
+template <int KD, bool UPPER>
+__device__ inline void pp_solve_scan(const pp_band_plan &pl, const double *nl, const zd *y, zd *tb)
+{
+  const int P = pl.P, B = pl.B, lb = pl.lb;
+  const int c = threadIdx.x;
+  const bool act = c < P;
+  const double *Nl = UPPER ? nl : nl + pl.levels * KD * KD * P;
+  zd a[KD];
 #pragma unroll
-    for (int k = 0; k < KD; k++) a[k] = act ? out[(c << lb) + (UPPER ? k : B - KD + k)] : zmk(0.0, 0.0);
-    for (int l = 0; l < pl.levels; l++) {
-      const int s = 1 << l;
-      zd an[KD];
-#pragma unroll
-      for (int k = 0; k < KD; k++) {
-        if (UPPER) { an[k].x = __shfl_down(a[k].x, s, 64); an[k].y = __shfl_down(a[k].y, s, 64); }
-        else { an[k].x = __shfl_up(a[k].x, s, 64); an[k].y = __shfl_up(a[k].y, s, 64); }
-      }
-      const bool has = UPPER ? (c + s < P) : (c - s >= 0);
-      if (act && has) {
-        const double *Nc = Nl + (size_t)l * KD * KD * P + c;
-#pragma unroll
-        for (int i = 0; i < KD; i++) {
-          zd acc = a[i];
-#pragma unroll
-          for (int k = 0; k < KD; k++) zfmar(acc, Nc[(i * KD + k) * P], an[k]);
-          a[i] = acc;
-        }
-      }
-    }
+  for (int k = 0; k < KD; k++) a[k] = act ? y[(c << lb) + (UPPER ? k : B - KD + k)] : zmk(0.0, 0.0);
+  for (int l = 0; l < pl.levels; l++) {
+    const int s = 1 << l;
+    zd an[KD];
 #pragma unroll
     for (int k = 0; k < KD; k++) {
-      zd t;
-      if (UPPER) { t.x = __shfl_down(a[k].x, 1, 64); t.y = __shfl_down(a[k].y, 1, 64); if (c + 1 >= P) t = zmk(0.0, 0.0); }
-      else { t.x = __shfl_up(a[k].x, 1, 64); t.y = __shfl_up(a[k].y, 1, 64); if (c == 0) t = zmk(0.0, 0.0); }
-      if (act) tb[c * KD + k] = t;
+      if (UPPER) { an[k].x = __shfl_down(a[k].x, s, 64); an[k].y = __shfl_down(a[k].y, s, 64); }
+      else { an[k].x = __shfl_up(a[k].x, s, 64); an[k].y = __shfl_up(a[k].y, s, 64); }
+    }
+    const bool has = UPPER ? (c + s < P) : (c - s >= 0);
+    if (act && has) {
+      const double *Nc = Nl + (size_t)l * KD * KD * P + c;
+#pragma unroll
+      for (int i = 0; i < KD; i++) {
+        zd acc = a[i];
+#pragma unroll
+        for (int k = 0; k < KD; k++) zfmar(acc, Nc[(i * KD + k) * P], an[k]);
+        a[i] = acc;
+      }
     }
   }
-  __syncthreads();
-  // 3. fix-up with the neighbouring chunk's boundary unknowns
+#pragma unroll
+  for (int k = 0; k < KD; k++) {
+    zd t;
+    if (UPPER) { t.x = __shfl_down(a[k].x, 1, 64); t.y = __shfl_down(a[k].y, 1, 64); if (c + 1 >= P) t = zmk(0.0, 0.0); }
+    else { t.x = __shfl_up(a[k].x, 1, 64); t.y = __shfl_up(a[k].y, 1, 64); if (c == 0) t = zmk(0.0, 0.0); }
+    if (act) tb[c * KD + k] = t;
+  }
+}
+
+// Stage 3: x_c = y_c - G_c t_{c+1} (UPPER) or y_c - G'_c t_{c-1}, in place.
+template <int KD, bool UPPER, int NT>
+__device__ inline void pp_solve_fixup(const pp_band_plan &pl, const zd *tb, zd *out)
+{
+  const int npad = pl.P << pl.lb, lb = pl.lb;
   const double *Gm = UPPER ? pl.G : pl.GT;
-  for (int row = tid; row < npad; row += NT) {
+  for (int row = threadIdx.x; row < npad; row += NT) {
     const zd *t = tb + (row >> lb) * KD;
     zd v = out[row];
 #pragma unroll
     for (int k = 0; k < KD; k++) zfmar(v, -Gm[(size_t)k * npad + row], t[k]);
     out[row] = v;
   }
+}
+
+// Solve U x = b (UPPER) or U^T x = b (!UPPER) for one fiber with the first
+// NT threads of the workgroup (all threads must call it).  in: b (LDS, P*B, zero beyond N); out: x (LDS,
+// P*B, distinct from in; zero beyond N on return); nl: LDS copy of
+// pl.nlevs; tb: LDS scratch of P*KD complex.  Ends with a __syncthreads.
+// `stamp` is called after stages 1 and 2 (profiling hook).
+template <int KD, bool UPPER, int NT, class Stamp = pp_nostamp>
+__device__ inline void pp_band_solve(const pp_band_plan &pl, const double *nl, const zd *in, zd *out, zd *tb,
+                                     Stamp stamp = Stamp())
+{
+  if (threadIdx.x < NT) pp_solve_blocks<UPPER, NT>(pl, in, out);
+  __syncthreads();
+  stamp();
+  if (threadIdx.x < 64) pp_solve_scan<KD, UPPER>(pl, nl, out, tb);
+  __syncthreads();
+  stamp();
+  if (threadIdx.x < NT) pp_solve_fixup<KD, UPPER, NT>(pl, tb, out);
   __syncthreads();
 }
 

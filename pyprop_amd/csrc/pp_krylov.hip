@@ -37,8 +37,15 @@
 #define PP_MAXM 64
 #define PP_MAXSLOTS 4
 #define PP_MAXCOMBO 16
-#define PP_THREADS 256
+#ifndef PP_THREADS
+#define PP_THREADS 256   // threads per workgroup (one workgroup per fiber)
+#endif
+#ifndef PP_SOLVE_THREADS
+#define PP_SOLVE_THREADS (PP_THREADS < 256 ? PP_THREADS : 256)   // threads in the band solves
+#endif
 #define PP_TLOG_MAX 512
+// LDS doubles of the per-row combo table (coefficients, slots, counts, terms)
+#define PP_CMB_DOUBLES (PP_MAXCOMBO * PP_MAXSLOTS * 5 + 2 * PP_MAXCOMBO)
 
 struct pp_fterm {
   const int *rowptr;   // L+1
@@ -201,13 +208,21 @@ __device__ inline void fiber_dots(const zd *V, const zd *W, int nk, int N, bool 
 // ------------------------------------------------------------------ Pade on one workgroup
 // pAMP::ScalingPadeExponential + PadeExponential (pamp.h:430-564), column-major
 // m x m complex matrices in LDS; first column -> cvec_out (exchanged).
-__device__ void pade_block(int m, const zd *Hg, double dt_re, double dt_im, zd *lds, zd *cvec_out, double *diag)
+#ifdef PP_PADE_INLINE
+template <class Stamp>
+__device__ void pade_block(int m, const zd *Hg, double dt_re, double dt_im, zd *lds, zd *cvec_out, double *diag,
+                           Stamp &&stamp)
+#else
+#define stamp()
+__device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg, double dt_re, double dt_im, zd *lds,
+                                                     zd *cvec_out, double *diag)
+#endif
 {
   const int tid = threadIdx.x;
   const int mm = m * m;
   zd *A = lds, *A2 = lds + mm, *P = lds + 2 * mm, *Q = lds + 3 * mm, *T = lds + 4 * mm;
   double *red = (double *)(lds + 5 * mm);
-  int *ipiv = (int *)(red + 8);
+  int *ipiv = (int *)(red + PP_THREADS / 64);
   const zd fac = zmk(dt_im, -dt_re);  // -i * dt
   for (int e = tid; e < mm; e += PP_THREADS) A[e] = zmul(fac, Hg[e]);
   __syncthreads();
@@ -218,7 +233,8 @@ __device__ void pade_block(int m, const zd *Hg, double dt_re, double dt_im, zd *
   for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
   if ((tid & 63) == 0) red[tid >> 6] = mx;
   __syncthreads();
-  mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  mx = red[0];
+  for (int w = 1; w < PP_THREADS / 64; w++) mx = fmax(mx, red[w]);
   int sc = 0;
   if (mx > 0.0) sc = (int)ceil(log2(mx));
   if (sc < 0) sc = 0;
@@ -269,6 +285,7 @@ __device__ void pade_block(int m, const zd *Hg, double dt_re, double dt_im, zd *
   }
   if (odd) { matmul(Q, A, T); copy(T, Q); }
   else { matmul(P, A, T); copy(T, P); }
+  stamp();
   for (int e = tid; e < mm; e += PP_THREADS) Q[e] = zsub(Q[e], P[e]);
   __syncthreads();
   // LU with partial pivoting (izamax: |re|+|im|) applied to Q and the RHS P
@@ -319,6 +336,7 @@ __device__ void pade_block(int m, const zd *Hg, double dt_re, double dt_im, zd *
     }
     __syncthreads();
   }
+  stamp();
   // back substitution U X = P
   for (int k = m - 1; k >= 0; k--) {
     zd piv = Q[k + k * m];
@@ -332,6 +350,7 @@ __device__ void pade_block(int m, const zd *Hg, double dt_re, double dt_im, zd *
     }
     __syncthreads();
   }
+  stamp();
   // E = +-(I + 2X)
   for (int e = tid; e < mm; e += PP_THREADS) {
     zd v = zscale(P[e], 2.0);
@@ -345,6 +364,9 @@ __device__ void pade_block(int m, const zd *Hg, double dt_re, double dt_im, zd *
   if (tid == 0) diag[0] = sqrt(zabs2(A[m - 1]));
   __syncthreads();
 }
+#ifndef PP_PADE_INLINE
+#undef stamp
+#endif
 
 
 // ------------------------------------------------------------------ the step kernel
@@ -355,21 +377,29 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
   const int N = A.N, L = A.L, m = A.m, a = blockIdx.x, tid = threadIdx.x;
   const int padN = A.padN;
   const pp_band_plan &pl = A.plan;
+  // persistent for the whole launch: scan matrices and the combo table
   double *nl = lds_d;                                  // scan matrices of the solver plan
-  zd *W = (zd *)(lds_d + A.nlevEven);                  // residual of fiber a (padN, zero beyond N)
+  zd *cmb_c = (zd *)(lds_d + A.nlevEven);              // combo table: coefficients (PP_MAXCOMBO x slots)
+  int *cmb_q = (int *)(cmb_c + PP_MAXCOMBO * PP_MAXSLOTS);  // slots
+  int *cmb_n = cmb_q + PP_MAXCOMBO * PP_MAXSLOTS;      // entries per combo
+  int *cmb_t = cmb_n + PP_MAXCOMBO;                    // term of each combo
+  // rewritten before every matvec (so no table entry lives in registers
+  // across the step loop): scaled coefficients, slots, counts
+  zd *cur_c = (zd *)(cmb_t + PP_MAXCOMBO);
+  int *cur_q = (int *)(cur_c + PP_MAXCOMBO * PP_MAXSLOTS);
+  int *cur_n = cur_q + PP_MAXCOMBO * PP_MAXSLOTS;
+  // per step (workgroup 0 reuses this region for the Pade exponential)
+  zd *W = (zd *)(lds_d + A.nlevEven + PP_CMB_DOUBLES); // residual of fiber a (N)
   zd *X = W + padN;                                    // solve input (padN)
   zd *tb = X + padN;                                   // solver scratch (P*KD)
   zd *Ys = tb + pl.P * KD;                             // staged source fibers (nslots x N)
   zd *Z = Ys;                                          // also: solve output (padN)
   double *scr = (double *)Ys;                          // also: partial-sum staging (scrDoubles)
-  double *wsum = (double *)Ys + A.scrDoubles;          // 8
-  double *hred = wsum + 8;                             // 2m+4 reduced values
+  double *wsum = (double *)Ys + A.scrDoubles;          // 16
+  double *hred = wsum + 16;                            // 2m+4 reduced values
   double *gscr = hred + (2 * m + 4);                   // PP_THREADS
   double *hacc = gscr + PP_THREADS;                    // 2m
   zd *cv = (zd *)(hacc + 2 * m);                       // m Krylov coefficients
-  zd *cmb_c = cv + m;                                  // combo table: coefficients (PP_MAXCOMBO x slots)
-  int *cmb_q = (int *)(cmb_c + PP_MAXCOMBO * PP_MAXSLOTS);  // slots
-  int *cmb_n = cmb_q + PP_MAXCOMBO * PP_MAXSLOTS;      // entries per combo
   const double tol = 1.4901161193847656e-08;           // sqrt(DBL_EPSILON), pamp.h:75
   zd *Va = A.V + (size_t)a * m * N;
   double *mypart = A.part + (size_t)a * A.pstr;
@@ -379,7 +409,26 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
     if (A.tlog && step == 0 && a == 0 && tid == 0 && tcount < PP_TLOG_MAX) A.tlog[tcount++] = (long long)wall_clock64();
   };
   for (int i = tid; i < pl.nlev; i += PP_THREADS) nl[i] = pl.nlevs[i];
-  for (int i = N + tid; i < padN; i += PP_THREADS) W[i] = zmk(0.0, 0.0);
+  // combo table of this row: one entry per (term, rank) with the angular
+  // coefficients of its couplings (time and 1/beta scalings applied later)
+  if (tid == 0) {
+    int k = 0;
+    for (int t = 0; t < A.nterms; t++) {
+      const pp_fterm T = A.term[t];
+      const int e0 = T.rowptr[a], e1 = T.rowptr[a + 1];
+      for (int r = 0; r < T.rank; r++, k++) {
+        int ne = 0;
+        for (int e = e0; e < e1; e++, ne++) {
+          zd c = T.coef[(size_t)T.ent[3 * e + 1] * T.rank + r];
+          if (T.ent[3 * e + 2]) c = zconj(c);
+          cmb_c[k * PP_MAXSLOTS + ne] = c;
+          cmb_q[k * PP_MAXSLOTS + ne] = T.ent[3 * e];
+        }
+        cmb_n[k] = ne;
+        cmb_t[k] = t;
+      }
+    }
+  }
   __syncthreads();
 
   for (int step = 0; step < A.nsteps; step++) {
@@ -396,9 +445,9 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
     }
     nrm = block_sum(nrm, wsum);
     if (tid == 0) xst(mypart + NRM, nrm);
-    for (int i = tid; i < padN; i += PP_THREADS) X[i] = W[i];
+    for (int i = tid; i < padN; i += PP_THREADS) X[i] = i < N ? W[i] : zmk(0.0, 0.0);
     __syncthreads();
-    pp_band_solve<KD, true, PP_THREADS>(pl, nl, X, Z, tb);
+    pp_band_solve<KD, true, PP_SOLVE_THREADS>(pl, nl, X, Z, tb);
     for (int i = tid; i < N; i += PP_THREADS) zst(A.y + (size_t)a * N + i, Z[i]);
     stamp(step);
     grid_barrier(A.bar);
@@ -431,24 +480,11 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
           if (col >= 0)
             for (int i = tid; i < N; i += PP_THREADS) Ys[q * N + i] = zld(A.y + (size_t)col * N + i);
         }
-        // combo table: one entry per (term, rank) with this row's coefficients
-        if (tid == 0) {
-          int k = 0;
-          for (int t = 0; t < A.nterms; t++) {
-            const pp_fterm T = A.term[t];
-            const double st = A.scal[(size_t)step * A.nterms + t] * inv;
-            const int e0 = T.rowptr[a], e1 = T.rowptr[a + 1];
-            for (int r = 0; r < T.rank; r++, k++) {
-              int ne = 0;
-              for (int e = e0; e < e1; e++, ne++) {
-                zd c = T.coef[(size_t)T.ent[3 * e + 1] * T.rank + r];
-                if (T.ent[3 * e + 2]) c = zconj(c);
-                cmb_c[k * PP_MAXSLOTS + ne] = zscale(c, st);
-                cmb_q[k * PP_MAXSLOTS + ne] = T.ent[3 * e];
-              }
-              cmb_n[k] = ne;
-            }
-          }
+        // coefficients times the terms' time factors at t_n and 1/beta
+        for (int e = tid; e < A.ncombo * PP_MAXSLOTS; e += PP_THREADS) {
+          cur_c[e] = zscale(cmb_c[e], A.scal[(size_t)step * A.nterms + cmb_t[e / PP_MAXSLOTS]] * inv);
+          cur_q[e] = cmb_q[e];
+          if (e % PP_MAXSLOTS == 0) cur_n[e / PP_MAXSLOTS] = cmb_n[e / PP_MAXSLOTS];
         }
         __syncthreads();
         // X = sum_k B_k (sum_q c_kq Ys_q): bands diagonal-major, all loads independent
@@ -460,7 +496,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
             for (int t = 0; t < A.nterms; t++) {
               const double *Bt = A.term[t].bands;
               for (int r = 0; r < A.term[t].rank; r++, k++) {
-                const int ne = cmb_n[k];
+                const int ne = cur_n[k];
                 if (ne == 0) continue;
                 const double *Br = Bt + (size_t)r * (2 * KD + 1) * N;
                 double bv[2 * KD + 1];
@@ -470,8 +506,8 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
 #pragma unroll
                 for (int q = 0; q < PP_MAXSLOTS; q++) {
                   if (q < ne) {
-                    const zd cq = cmb_c[k * PP_MAXSLOTS + q];
-                    const zd *Yq = Ys + cmb_q[k * PP_MAXSLOTS + q] * N;
+                    const zd cq = cur_c[k * PP_MAXSLOTS + q];
+                    const zd *Yq = Ys + cur_q[k * PP_MAXSLOTS + q] * N;
                     zd z = zmk(0.0, 0.0);
 #pragma unroll
                     for (int d = -KD; d <= KD; d++) {
@@ -489,7 +525,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
         }
         __syncthreads();
         stamp(step);
-        pp_band_solve<KD, false, PP_THREADS>(pl, nl, X, W, tb);
+        pp_band_solve<KD, false, PP_SOLVE_THREADS>(pl, nl, X, W, tb, [&]() { stamp(step); });
         stamp(step);
         fiber_dots(Va, W, nk, N, A.reorth != 0, hred);
         for (int c = tid; c < ncomp; c += PP_THREADS) xst(mypart + c, hred[c]);
@@ -570,10 +606,10 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
         for (int i = tid; i < N; i += PP_THREADS) v += zabs2(W[i]);
         v = block_sum(v, wsum);
         if (tid == 0) xst(mypart + NRM, v);
-        for (int i = tid; i < padN; i += PP_THREADS) X[i] = W[i];
+        for (int i = tid; i < padN; i += PP_THREADS) X[i] = i < N ? W[i] : zmk(0.0, 0.0);
         __syncthreads();
         stamp(step);
-        pp_band_solve<KD, true, PP_THREADS>(pl, nl, X, Z, tb);
+        pp_band_solve<KD, true, PP_SOLVE_THREADS>(pl, nl, X, Z, tb);
         for (int i = tid; i < N; i += PP_THREADS) zst(A.y + (size_t)a * N + i, Z[i]);
       }
       stamp(step);
@@ -583,8 +619,13 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
 
     // ---------------- exponentiate on workgroup 0, broadcast exp(-i dt H) e1
     if (a == 0) {
-      pade_block(m, A.Hm, A.dt_re, A.dt_im, (zd *)(lds_d + A.nlevEven), A.cvec,
+#ifdef PP_PADE_INLINE
+      pade_block(m, A.Hm, A.dt_re, A.dt_im, (zd *)(lds_d + A.nlevEven + PP_CMB_DOUBLES), A.cvec,
+                 A.diag + (size_t)step * 4, [&]() { stamp(step); });
+#else
+      pade_block(m, A.Hm, A.dt_re, A.dt_im, (zd *)(lds_d + A.nlevEven + PP_CMB_DOUBLES), A.cvec,
                  A.diag + (size_t)step * 4);
+#endif
       if (tid == 0) {
         A.diag[(size_t)step * 4 + 1] = mEff;
         A.diag[(size_t)step * 4 + 2] = beta0;
@@ -606,7 +647,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
       X[i] = zscale(acc, beta0);
     }
     __syncthreads();
-    pp_band_solve<KD, true, PP_THREADS>(pl, nl, X, Z, tb);
+    pp_band_solve<KD, true, PP_SOLVE_THREADS>(pl, nl, X, Z, tb);
     for (int i = tid; i < N; i += PP_THREADS) A.psi[(size_t)a * N + i] = Z[i];
     __syncthreads();
     stamp(step);
@@ -857,13 +898,12 @@ static int pamp_scr_doubles(const pp_pamp *p, int padN)
 static size_t pamp_lds_bytes(const pp_pamp *p, int padN)
 {
   const pp_band_plan &pl = p->ov->plan;
-  const size_t nle = (size_t)((pl.nlev + 1) & ~1);
+  const size_t nle = (size_t)((pl.nlev + 1) & ~1) + PP_CMB_DOUBLES;
   size_t fib = sizeof(double) * nle + sizeof(zd) * (size_t)(2 * padN + pl.P * p->ov->kd) +
                sizeof(double) * (size_t)pamp_scr_doubles(p, padN) +
-               sizeof(double) * (size_t)(8 + (2 * p->m + 4) + PP_THREADS + 2 * p->m) + sizeof(zd) * (size_t)p->m +
-               (sizeof(zd) + 2 * sizeof(int)) * PP_MAXCOMBO * PP_MAXSLOTS + 64;
+               sizeof(double) * (size_t)(16 + (2 * p->m + 4) + PP_THREADS + 2 * p->m) + sizeof(zd) * (size_t)p->m + 64;
   size_t pade = sizeof(double) * nle + sizeof(zd) * (size_t)(5 * p->m * p->m) +
-                sizeof(double) * 8 + 2 * sizeof(int) + 64;
+                sizeof(double) * (PP_THREADS / 64) + 2 * sizeof(int) + 64;
   return fib > pade ? fib : pade;
 }
 
