@@ -103,26 +103,36 @@ __device__ inline double xld(const double *p) { return __hip_atomic_load(const_c
 __device__ inline void zst(zd *p, zd v) { xst(&p->x, v.x); xst(&p->y, v.y); }
 __device__ inline zd zld(const zd *p) { return zmk(xld(&p->x), xld(&p->y)); }
 
-// Grid barrier over co-resident workgroups: arrival counter + generation.
-// Data crossing it is written/read with agent-scope atomics (above); the
-// workgroup-scope fences order this workgroup's accesses around the arrival.
-// A bounded spin turns a lost arrival into an error flag instead of a hang.
-__device__ inline void grid_barrier(pp_barrier *b)
+// Grid barrier over co-resident workgroups: every workgroup publishes the
+// number of barriers it has reached in its own arrival word (the words follow
+// the pp_barrier header) and wavefront 0 polls all words until each has
+// reached the same count, so a barrier costs one store plus the polling
+// round trips.  Data crossing it is written/read with agent-scope atomics
+// (above); the workgroup-scope fences order this workgroup's accesses around
+// the arrival.  A bounded spin turns a lost arrival into an error flag
+// instead of a hang.  `ep` counts the barriers of this launch (identical in
+// every workgroup, since all execute the same barrier sequence).
+__device__ inline void grid_barrier(pp_barrier *b, unsigned &ep)
 {
+  unsigned *word = (unsigned *)(b + 1);
+  const int n = gridDim.x, tid = threadIdx.x;
+  ep++;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    const unsigned n = gridDim.x;
-    unsigned gen = __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned arrived = __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == n - 1) {
-      __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&b->gen, gen + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      long spins = 0;
-      while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        __builtin_amdgcn_s_sleep(1);
-        if (++spins > (1l << 25)) { __hip_atomic_store(&b->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
+  if (tid < 64) {
+    if (tid == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __hip_atomic_store(word + blockIdx.x, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    long spins = 0;
+    while (true) {
+      bool ok = true;
+      for (int f = tid; f < n; f += 64)
+        ok = ok && (int)(__hip_atomic_load(word + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - ep) >= 0;
+      if (__all(ok)) break;
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > (1l << 24)) {
+        if (tid == 0) __hip_atomic_store(&b->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -404,6 +414,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
   zd *Va = A.V + (size_t)a * m * N;
   double *mypart = A.part + (size_t)a * A.pstr;
   const int NRM = 2 * m, ONRM = 2 * m + 1;
+  unsigned ep = 0;   // grid barriers passed in this launch
   int tcount = 0;
   auto stamp = [&](int step) {
     if (A.tlog && step == 0 && a == 0 && tid == 0 && tcount < PP_TLOG_MAX) A.tlog[tcount++] = (long long)wall_clock64();
@@ -450,7 +461,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
     pp_band_solve<KD, true, PP_SOLVE_THREADS>(pl, nl, X, Z, tb);
     for (int i = tid; i < N; i += PP_THREADS) zst(A.y + (size_t)a * N + i, Z[i]);
     stamp(step);
-    grid_barrier(A.bar);
+    grid_barrier(A.bar, ep);
     stamp(step);
 
     double beta0 = 0.0, beta = 0.0;
@@ -532,7 +543,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
         if (A.reorth && tid == 0) xst(mypart + ONRM, hred[ncomp]);
       }
       stamp(step);
-      grid_barrier(A.bar);
+      grid_barrier(A.bar, ep);
       stamp(step);
 
       // ---------------- phase B: h = sum_a partials ; w' -= V h ; |w'|^2 ; y = U^-1 w'
@@ -568,7 +579,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
             v = block_sum(v, wsum);
             if (tid == 0) xst(mypart + NRM, v);
           }
-          grid_barrier(A.bar);
+          grid_barrier(A.bar, ep);
           bool doit = false;
           double res = 0.0;
           if (!stop) {
@@ -582,7 +593,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
             fiber_dots(Va, W, nk, N, false, hred);
             for (int c = tid; c < ncomp; c += PP_THREADS) xst(mypart + c, hred[c]);
           }
-          grid_barrier(A.bar);
+          grid_barrier(A.bar, ep);
           if (doit) {
             grid_partial_sum(A.part, A.pstr, 0, ncomp, L, hred, scr);
             for (int i = tid; i < N; i += PP_THREADS) {
@@ -599,7 +610,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
             __syncthreads();
           }
         }
-        grid_barrier(A.bar);   // everyone has read NRM before it is rewritten below
+        grid_barrier(A.bar, ep);   // everyone has read NRM before it is rewritten below
       }
       if (!broken) {
         double v = 0.0;
@@ -613,7 +624,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
         for (int i = tid; i < N; i += PP_THREADS) zst(A.y + (size_t)a * N + i, Z[i]);
       }
       stamp(step);
-      grid_barrier(A.bar);
+      grid_barrier(A.bar, ep);
       stamp(step);
     }
 
@@ -635,7 +646,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
       for (int e = tid; e < m * m; e += PP_THREADS) A.Hm[e] = zmk(0.0, 0.0);
     }
     stamp(step);
-    grid_barrier(A.bar);
+    grid_barrier(A.bar, ep);
     stamp(step);
     // ---------------- psi~ = |psi~| V c ; psi = U^-1 psi~ (SolveSqrtOverlap)
     for (int k = tid; k < m; k += PP_THREADS) cv[k] = zld(A.cvec + k);
@@ -678,10 +689,10 @@ extern "C" int pp_pamp_create(int L, int N, int m, int reorth, const pp_overlap 
   PP_HIP(hipMalloc(&p->Hm, sizeof(zd) * (size_t)m * m));
   PP_HIP(hipMalloc(&p->cvec, sizeof(zd) * (size_t)m));
   PP_HIP(hipMalloc(&p->part, sizeof(double) * (size_t)L * p->pstr));
-  PP_HIP(hipMalloc(&p->bar, sizeof(pp_barrier)));
+  PP_HIP(hipMalloc(&p->bar, sizeof(pp_barrier) + sizeof(unsigned) * (size_t)L));
   PP_HIP(hipMemset(p->V, 0, sizeof(zd) * (size_t)L * m * N));
   PP_HIP(hipMemset(p->Hm, 0, sizeof(zd) * (size_t)m * m));
-  PP_HIP(hipMemset(p->bar, 0, sizeof(pp_barrier)));
+  PP_HIP(hipMemset(p->bar, 0, sizeof(pp_barrier) + sizeof(unsigned) * (size_t)L));
   *out = p;
   return PP_OK;
 }
@@ -946,7 +957,7 @@ extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re
     PP_HIP(hipMalloc(&p->diag, sizeof(double) * 4 * (size_t)nsteps));
   }
   PP_HIP(hipMemcpyAsync(p->scal, scal, sizeof(double) * nsteps * p->nterms, hipMemcpyHostToDevice, stream));
-  PP_HIP(hipMemsetAsync(p->bar, 0, sizeof(pp_barrier), stream));
+  PP_HIP(hipMemsetAsync(p->bar, 0, sizeof(pp_barrier) + sizeof(unsigned) * (size_t)p->L, stream));
   pp_pamp_args A;
   memset(&A, 0, sizeof(A));
   A.L = p->L; A.N = p->N; A.m = p->m; A.s = p->s; A.bw = p->bw; A.nterms = p->nterms;
