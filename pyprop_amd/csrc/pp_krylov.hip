@@ -216,27 +216,29 @@ __device__ inline void fiber_dots(const zd *V, const zd *W, int nk, int N, bool 
 }
 
 // ------------------------------------------------------------------ Pade on one workgroup
-// pAMP::ScalingPadeExponential + PadeExponential (pamp.h:430-564), column-major
-// m x m complex matrices in LDS; first column -> cvec_out (exchanged).
-#ifdef PP_PADE_INLINE
-template <class Stamp>
-__device__ void pade_block(int m, const zd *Hg, double dt_re, double dt_im, zd *lds, zd *cvec_out, double *diag,
-                           Stamp &&stamp)
-#else
-#define stamp()
-__device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg, double dt_re, double dt_im, zd *lds,
+// pAMP::ScalingPadeExponential + PadeExponential (pamp.h:430-564): A = -i dt H
+// scaled by 2^-s with s = max(0, ceil(log2 max|A_ij|)), Pade(6) numerator
+// and denominator by Horner in A^2, X = (Q - P)^-1 P, E = +-(I + 2X), then s
+// squarings; the first column of E goes to cvec_out (exchanged).  Column-major
+// m x m complex matrices in the dynamic LDS at double offset `off`.  The
+// linear solve is Gauss-Jordan with the same partial pivoting as zgesv
+// (largest |re|+|im|, first index on ties) on ping-pong buffers, one barrier
+// per column.
+__device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg, double dt_re, double dt_im, int off,
                                                      zd *cvec_out, double *diag)
-#endif
 {
+  extern __shared__ double pp_lds[];
+  zd *lds = (zd *)(pp_lds + off);
   const int tid = threadIdx.x;
   const int mm = m * m;
   zd *A = lds, *A2 = lds + mm, *P = lds + 2 * mm, *Q = lds + 3 * mm, *T = lds + 4 * mm;
-  double *red = (double *)(lds + 5 * mm);
-  int *ipiv = (int *)(red + PP_THREADS / 64);
+  zd *G0 = lds + 5 * mm, *G1 = G0 + 2 * mm;          // Gauss-Jordan buffers (m x 2m)
+  double *red = (double *)(G1 + 2 * mm);
+  // element e = tid + r*PP_THREADS of an m x m matrix: row i = e % m, column j = e / m
+  const int i0 = tid % m, j0 = tid / m, di = PP_THREADS % m, dj = PP_THREADS / m;
   const zd fac = zmk(dt_im, -dt_re);  // -i * dt
   for (int e = tid; e < mm; e += PP_THREADS) A[e] = zmul(fac, Hg[e]);
   __syncthreads();
-  // max |A_ij|
   double mx = 0.0;
   for (int e = tid; e < mm; e += PP_THREADS) mx = fmax(mx, sqrt(zabs2(A[e])));
 #pragma unroll
@@ -248,26 +250,24 @@ __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg, double
   int sc = 0;
   if (mx > 0.0) sc = (int)ceil(log2(mx));
   if (sc < 0) sc = 0;
-  double inv = 1.0 / pow(2.0, (double)sc);
-  __syncthreads();
+  const double inv = 1.0 / pow(2.0, (double)sc);
   for (int e = tid; e < mm; e += PP_THREADS) A[e] = zscale(A[e], inv);
   __syncthreads();
-  auto matmul = [&](const zd *X, const zd *Y, zd *Z) {
+  auto matmul = [&](const zd *X, const zd *Y, zd *Z) {   // Z = X Y
+    int i = i0, j = j0;
     for (int e = tid; e < mm; e += PP_THREADS) {
-      const int i = e % m, j = e / m;
+      const zd *xr = X + i, *yc = Y + j * m;
       zd a0 = zmk(0.0, 0.0), a1 = zmk(0.0, 0.0);
       int k = 0;
       for (; k + 1 < m; k += 2) {
-        zfma(a0, X[i + k * m], Y[k + j * m]);
-        zfma(a1, X[i + (k + 1) * m], Y[k + 1 + j * m]);
+        zfma(a0, xr[k * m], yc[k]);
+        zfma(a1, xr[(k + 1) * m], yc[k + 1]);
       }
-      if (k < m) zfma(a0, X[i + k * m], Y[k + j * m]);
+      if (k < m) zfma(a0, xr[k * m], yc[k]);
       Z[e] = zadd(a0, a1);
+      i += di; j += dj;
+      if (i >= m) { i -= m; j++; }
     }
-    __syncthreads();
-  };
-  auto copy = [&](const zd *X, zd *Z) {
-    for (int e = tid; e < mm; e += PP_THREADS) Z[e] = X[e];
     __syncthreads();
   };
   matmul(A, A, A2);
@@ -275,109 +275,110 @@ __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg, double
   double c[order + 1];
   c[0] = 1.0;
   for (int i = 0; i < order; i++) c[i + 1] = c[i] * (double)(order - i) / (double)((i + 1) * (2 * order - i));
-  for (int e = tid; e < mm; e += PP_THREADS) {
-    int i = e % m, j = e / m;
-    Q[e] = zmk(i == j ? c[order] : 0.0, 0.0);
-    P[e] = zmk(i == j ? c[order - 1] : 0.0, 0.0);
+  {
+    int i = i0, j = j0;
+    for (int e = tid; e < mm; e += PP_THREADS) {
+      Q[e] = zmk(i == j ? c[order] : 0.0, 0.0);
+      P[e] = zmk(i == j ? c[order - 1] : 0.0, 0.0);
+      i += di; j += dj;
+      if (i >= m) { i -= m; j++; }
+    }
   }
   __syncthreads();
   bool odd = true;
-  for (int i = order - 2; i >= 0; i--) {
+  for (int q = order - 2; q >= 0; q--) {
     zd *X = odd ? Q : P;
     matmul(X, A2, T);
+    int i = i0, j = j0;
     for (int e = tid; e < mm; e += PP_THREADS) {
       zd v = T[e];
-      if (e % m == e / m) v.x += c[i];
+      if (i == j) v.x += c[q];
       X[e] = v;
+      i += di; j += dj;
+      if (i >= m) { i -= m; j++; }
     }
     __syncthreads();
     odd = !odd;
   }
-  if (odd) { matmul(Q, A, T); copy(T, Q); }
-  else { matmul(P, A, T); copy(T, P); }
-  stamp();
-  for (int e = tid; e < mm; e += PP_THREADS) Q[e] = zsub(Q[e], P[e]);
-  __syncthreads();
-  // LU with partial pivoting (izamax: |re|+|im|) applied to Q and the RHS P
-  for (int k = 0; k < m; k++) {
-    if (tid < 64) {
-      // izamax over rows k..m-1 (first index on ties), one wave
-      double best = -1.0;
-      int p = m;
-      for (int i = k + tid; i < m; i += 64) {
-        const double v = fabs(Q[i + k * m].x) + fabs(Q[i + k * m].y);
-        if (v > best) { best = v; p = i; }
-      }
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        const double ob = __shfl_xor(best, o, 64);
-        const int op = __shfl_xor(p, o, 64);
-        if (ob > best || (ob == best && op < p)) { best = ob; p = op; }
-      }
-      if (tid == 0) { ipiv[0] = p; ipiv[1] = (best == 0.0); }
-    }
-    __syncthreads();
-    int p = ipiv[0];
-    if (ipiv[1]) { if (tid == 0) diag[0] = -1.0; return; }
-    if (p != k)
-      for (int j = tid; j < 2 * m; j += PP_THREADS) {
-        zd *M = j < m ? Q : P;
-        int jj = j < m ? j : j - m;
-        zd t = M[k + jj * m]; M[k + jj * m] = M[p + jj * m]; M[p + jj * m] = t;
-      }
-    __syncthreads();
-    zd piv = Q[k + k * m];
-    double dn = zabs2(piv);
-    zd pinv = zmk(piv.x / dn, -piv.y / dn);
-    for (int i = k + 1 + tid; i < m; i += PP_THREADS) Q[i + k * m] = zmul(Q[i + k * m], pinv);
-    __syncthreads();
-    int rem = m - k - 1;
-    for (int e = tid; e < rem * rem; e += PP_THREADS) {
-      int i = k + 1 + e % rem, j = k + 1 + e / rem;
-      zd v = Q[i + j * m];
-      zd l = Q[i + k * m], u = Q[k + j * m];
-      v = zsub(v, zmul(l, u));
-      Q[i + j * m] = v;
-    }
-    // forward elimination of the RHS column block for step k
-    for (int e = tid; e < rem * m; e += PP_THREADS) {
-      int i = k + 1 + e % rem, j = e / rem;
-      P[i + j * m] = zsub(P[i + j * m], zmul(Q[i + k * m], P[k + j * m]));
-    }
-    __syncthreads();
-  }
-  stamp();
-  // back substitution U X = P
-  for (int k = m - 1; k >= 0; k--) {
-    zd piv = Q[k + k * m];
-    double dn = zabs2(piv);
-    zd pinv = zmk(piv.x / dn, -piv.y / dn);
-    for (int j = tid; j < m; j += PP_THREADS) P[k + j * m] = zmul(P[k + j * m], pinv);
-    __syncthreads();
-    for (int e = tid; e < k * m; e += PP_THREADS) {
-      int i = e % k, j = e / k;
-      P[i + j * m] = zsub(P[i + j * m], zmul(Q[i + k * m], P[k + j * m]));
-    }
-    __syncthreads();
-  }
-  stamp();
-  // E = +-(I + 2X)
+  zd *Pm = odd ? Q : P;   // the odd polynomial gets the extra factor A
+  matmul(Pm, A, T);
+  // Gauss-Jordan on [Q - P | P] (m x 2m, column-major)
   for (int e = tid; e < mm; e += PP_THREADS) {
-    zd v = zscale(P[e], 2.0);
-    if (e % m == e / m) v.x += 1.0;
-    if (odd) v = zmk(-v.x, -v.y);
-    A[e] = v;
+    const zd qv = odd ? T[e] : Q[e];       // Q after its final update
+    const zd pv = odd ? P[e] : T[e];       // P after its final update
+    G0[e] = zsub(qv, pv);
+    G0[mm + e] = pv;
   }
   __syncthreads();
-  for (int s = 0; s < sc; s++) { matmul(A, A, T); copy(T, A); }
+  zd *gc = G0, *gn = G1;
+  const int w2 = 2 * m * m;
+  for (int k = 0; k < m; k++) {
+    // every thread finds the pivot itself: rows k..m-1 of column k
+    int p = k;
+    double best = fabs(gc[k + k * m].x) + fabs(gc[k + k * m].y);
+    for (int i = k + 1; i < m; i++) {
+      const double v = fabs(gc[i + k * m].x) + fabs(gc[i + k * m].y);
+      if (v > best) { best = v; p = i; }
+    }
+    if (best == 0.0) {   // singular denominator (pamp.h:544-550)
+      if (tid == 0) diag[0] = -1.0;
+      return;
+    }
+    const zd piv = gc[p + k * m];
+    const double dn = zabs2(piv);
+    const zd pinv = zmk(piv.x / dn, -piv.y / dn);
+    int i = i0, j = j0;
+    for (int e = tid; e < w2; e += PP_THREADS) {
+      if (j >= k) {   // eliminated columns are not read again
+        const zd rk = zmul(gc[p + j * m], pinv);   // new pivot row
+        zd v;
+        if (i == k) v = rk;
+        else {
+          const int si = (i == p) ? k : i;       // row swap k <-> p
+          v = zsub(gc[si + j * m], zmul(gc[si + k * m], rk));
+        }
+        gn[e] = v;
+      }
+      i += di; j += dj;
+      if (i >= m) { i -= m; j++; }
+    }
+    __syncthreads();
+    zd *t = gc; gc = gn; gn = t;
+  }
+  // X = right half of gc;  E = +-(I + 2X)
+  const bool neg = odd;
+  if (sc == 0) {   // only the first column of E is needed
+    for (int i = tid; i < m; i += PP_THREADS) {
+      zd v = zscale(gc[mm + i], 2.0);
+      if (i == 0) v.x += 1.0;
+      if (neg) v = zmk(-v.x, -v.y);
+      zst(cvec_out + i, v);
+      if (i == m - 1) diag[0] = sqrt(zabs2(v));
+    }
+    __syncthreads();
+    return;
+  }
+  {
+    int i = i0, j = j0;
+    for (int e = tid; e < mm; e += PP_THREADS) {
+      zd v = zscale(gc[mm + e], 2.0);
+      if (i == j) v.x += 1.0;
+      if (neg) v = zmk(-v.x, -v.y);
+      A[e] = v;
+      i += di; j += dj;
+      if (i >= m) { i -= m; j++; }
+    }
+  }
+  __syncthreads();
+  for (int s = 0; s < sc; s++) {
+    matmul(A, A, T);
+    for (int e = tid; e < mm; e += PP_THREADS) A[e] = T[e];
+    __syncthreads();
+  }
   for (int i = tid; i < m; i += PP_THREADS) zst(cvec_out + i, A[i]);
   if (tid == 0) diag[0] = sqrt(zabs2(A[m - 1]));
   __syncthreads();
 }
-#ifndef PP_PADE_INLINE
-#undef stamp
-#endif
-
 
 // ------------------------------------------------------------------ the step kernel
 template <int KD>
@@ -630,13 +631,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
 
     // ---------------- exponentiate on workgroup 0, broadcast exp(-i dt H) e1
     if (a == 0) {
-#ifdef PP_PADE_INLINE
-      pade_block(m, A.Hm, A.dt_re, A.dt_im, (zd *)(lds_d + A.nlevEven + PP_CMB_DOUBLES), A.cvec,
-                 A.diag + (size_t)step * 4, [&]() { stamp(step); });
-#else
-      pade_block(m, A.Hm, A.dt_re, A.dt_im, (zd *)(lds_d + A.nlevEven + PP_CMB_DOUBLES), A.cvec,
-                 A.diag + (size_t)step * 4);
-#endif
+      pade_block(m, A.Hm, A.dt_re, A.dt_im, A.nlevEven + PP_CMB_DOUBLES, A.cvec, A.diag + (size_t)step * 4);
       if (tid == 0) {
         A.diag[(size_t)step * 4 + 1] = mEff;
         A.diag[(size_t)step * 4 + 2] = beta0;
@@ -913,8 +908,8 @@ static size_t pamp_lds_bytes(const pp_pamp *p, int padN)
   size_t fib = sizeof(double) * nle + sizeof(zd) * (size_t)(2 * padN + pl.P * p->ov->kd) +
                sizeof(double) * (size_t)pamp_scr_doubles(p, padN) +
                sizeof(double) * (size_t)(16 + (2 * p->m + 4) + PP_THREADS + 2 * p->m) + sizeof(zd) * (size_t)p->m + 64;
-  size_t pade = sizeof(double) * nle + sizeof(zd) * (size_t)(5 * p->m * p->m) +
-                sizeof(double) * (PP_THREADS / 64) + 2 * sizeof(int) + 64;
+  size_t pade = sizeof(double) * nle + sizeof(zd) * (size_t)(9 * p->m * p->m) +
+                sizeof(double) * (PP_THREADS / 64) + 64;
   return fib > pade ? fib : pade;
 }
 
