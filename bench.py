@@ -42,11 +42,12 @@ def step_bytes(n, m, N, s_mv=16, s_gs=16):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="hydrogen_large")
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB before the workload, so that a "
                          "rocprofv3 --pmc run can scale FETCH_SIZE/WRITE_SIZE by a copy of known size")
@@ -70,34 +71,60 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    conf = H.load(args.workload)
-    prob = pyprop_amd.Problem(conf)
-    prob.SetupStep()
-    psi0 = H.ground_state(prob)
-    prob.SetWavefunction(psi0)
+    def run_form(mixed):
+        """W untimed + K timed steps from the 1s state; returns the final
+        state, its norm, the diagnostics and the timings."""
+        over = {("Propagation", "matvec_precision"): "mixed"} if mixed else {}
+        conf = H.load(args.workload, over)
+        prob = pyprop_amd.Problem(conf)
+        prob.SetupStep()
+        psi0 = H.ground_state(prob)
+        prob.SetWavefunction(psi0)
+        prob.AdvanceSteps(args.warmup)
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        prob.AdvanceSteps(args.steps)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kernel_s = ev0.elapsed_time(ev1) / 1e3
+        diag = prob.Propagator.GetDiagnostics()
+        return dict(conf=conf, prob=prob, psi0=psi0, psi=prob.psi.GetData().cpu().numpy(),
+                    norm=prob.psi.GetNorm(), diag=diag, wall=wall, kernel_s=kernel_s)
+
+    fp = run_form(False)
+    conf, prob, psi0 = fp["conf"], fp["prob"], fp["psi0"]
     L, N = prob.psi.GetRepresentation().GetFullShape()
     n = L * N
     m = prob.Propagator.BasisSize
     dt = prob.TimeStep
-
-    # warmup (also compiles nothing: the library is prebuilt)
-    prob.AdvanceSteps(args.warmup)
-    torch.cuda.synchronize()
-
-    stream = torch.cuda.current_stream()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    prob.AdvanceSteps(args.steps)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_s = ev0.elapsed_time(ev1) / 1e3
+    wall, kernel_s, diag, norm = fp["wall"], fp["kernel_s"], fp["diag"], fp["norm"]
     steps_per_s = args.steps / wall
-    diag = prob.Propagator.GetDiagnostics()
-    norm = prob.psi.GetNorm()
+
+    mixed = None
+    if not args.no_mixed:
+        mx = run_form(True)
+        # bound-state populations of both final states against the same
+        # eigenvectors of (H0_l, S) (SURVEY d1: error vs the FP64 result)
+        per64 = H.populations(prob, fp["psi"])[1]
+        per32 = H.populations(prob, mx["psi"])[1]
+        perr = max(float(np.max(np.abs(a - b))) if len(a) else 0.0 for a, b in zip(per64, per32))
+        S = prob.psi.GetRepresentation().GetRepresentation(1).GetBSplineObject().overlap_dense()
+        d = fp["psi"] - mx["psi"]
+        dnorm = float(np.sqrt(sum(np.real(np.conj(d[l]) @ S @ d[l]) for l in range(L))))
+        mixed = {"value": args.steps / mx["wall"], "unit": "steps/s", "ms_per_step": 1e3 * mx["wall"] / args.steps,
+                 "speedup_vs_fp64": (args.steps / mx["wall"]) / steps_per_s,
+                 "population_error_max": perr,
+                 "bound_population_fp64": float(sum(p.sum() for p in per64)),
+                 "bound_population_mixed": float(sum(p.sum() for p in per32)),
+                 "norm_fp64": fp["norm"], "norm_mixed": mx["norm"],
+                 "psi_diff_snorm": dnorm, "steps_compared": args.warmup + args.steps,
+                 "dtype": "c64 operator apply, c128 Gram-Schmidt / overlap solves / Pade"}
 
     Bstep = step_bytes(n, m, N)
     pmc = PMC_TRAFFIC_PER_STEP.get(args.workload)
@@ -144,6 +171,7 @@ def main():
         "cpu_baseline": cpu,
         "checks": {"norm_after": norm, "krylov_error_estimate_last": float(diag[-1, 0]),
                    "basis_used_last": int(diag[-1, 1])},
+        "mixed": mixed,
     }
     print(json.dumps(line))
 

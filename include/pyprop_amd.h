@@ -100,6 +100,12 @@ int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re, double dt
  * (pamp.h:666; -1 if the Pade denominator was singular, pamp.h:544-550),
  * basis size used (< m after happy breakdown), |U psi|, re-orthogonalisation passes. */
 int pp_pamp_diagnostics(pp_pamp *p, int nsteps, double *diag, pp_stream stream);
+/* Mixed-precision form (no reference counterpart; the reference is FP64
+ * throughout): later launches apply the terms whose bit is set in term_mask
+ * (bit t = t-th added term) in complex FP32 (FP32 radial bands, FP32 band
+ * sums); Gram-Schmidt, the overlap solves, the exchanged U^-1 w and the Pade
+ * exponential stay FP64.  0 restores the all-FP64 drop-in. */
+int pp_pamp_set_mixed(pp_pamp *p, unsigned term_mask);
 /* Profiling hook (no reference counterpart; the reference keeps per-phase
  * Timers in pAMP, pamp.h:41,125,295-349): record wall-clock stamps (100 MHz)
  * of workgroup 0 around every grid barrier of step 0 of later launches. */

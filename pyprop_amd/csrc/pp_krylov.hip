@@ -52,6 +52,8 @@ struct pp_fterm {
   const int *ent;      // (slot, pidx, conj) per entry
   const zd *coef;      // P0 x rank
   const double *bands; // rank x bw x N, element (i, i+d) at (r*bw + d+s)*N + i
+  const float *bands32; // the same in FP32 (mixed-precision form)
+  int fp32;             // apply this term in complex FP32
   int rank;
 };
 
@@ -90,6 +92,7 @@ struct pp_pamp {
   int *slotcols;
   bool dirty;
   zd *V, *y, *Hm, *cvec;
+  unsigned mixed;                          // bit t: term t applied in complex FP32
   double *part, *diag, *scal;
   pp_barrier *bar;
   int diag_cap, scal_cap, pstr;
@@ -257,14 +260,18 @@ __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg, double
     int i = i0, j = j0;
     for (int e = tid; e < mm; e += PP_THREADS) {
       const zd *xr = X + i, *yc = Y + j * m;
-      zd a0 = zmk(0.0, 0.0), a1 = zmk(0.0, 0.0);
+      zd a0 = zmk(0.0, 0.0), a1 = zmk(0.0, 0.0), a2 = zmk(0.0, 0.0), a3 = zmk(0.0, 0.0);
       int k = 0;
-      for (; k + 1 < m; k += 2) {
-        zfma(a0, xr[k * m], yc[k]);
-        zfma(a1, xr[(k + 1) * m], yc[k + 1]);
+      for (; k + 3 < m; k += 4) {   // four independent chains keep 8 LDS reads in flight
+        const zd x0 = xr[k * m], x1 = xr[(k + 1) * m], x2 = xr[(k + 2) * m], x3 = xr[(k + 3) * m];
+        const zd y0 = yc[k], y1 = yc[k + 1], y2 = yc[k + 2], y3 = yc[k + 3];
+        zfma(a0, x0, y0);
+        zfma(a1, x1, y1);
+        zfma(a2, x2, y2);
+        zfma(a3, x3, y3);
       }
-      if (k < m) zfma(a0, xr[k * m], yc[k]);
-      Z[e] = zadd(a0, a1);
+      for (; k < m; k++) zfma(a0, xr[k * m], yc[k]);
+      Z[e] = zadd(zadd(a0, a1), zadd(a2, a3));
       i += di; j += dj;
       if (i >= m) { i -= m; j++; }
     }
@@ -313,12 +320,20 @@ __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg, double
   zd *gc = G0, *gn = G1;
   const int w2 = 2 * m * m;
   for (int k = 0; k < m; k++) {
-    // every thread finds the pivot itself: rows k..m-1 of column k
-    int p = k;
-    double best = fabs(gc[k + k * m].x) + fabs(gc[k + k * m].y);
-    for (int i = k + 1; i < m; i++) {
+    // every wavefront finds the pivot itself: rows k..m-1 of column k, one
+    // per lane, reduced by shuffles (first index on ties, like izamax)
+    int p = m;
+    double best = -1.0;
+    const int lane = tid & 63;
+    for (int i = k + lane; i < m; i += 64) {
       const double v = fabs(gc[i + k * m].x) + fabs(gc[i + k * m].y);
       if (v > best) { best = v; p = i; }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, 64);
+      const int op = __shfl_xor(p, o, 64);
+      if (ob > best || (ob == best && op < p)) { best = ob; p = op; }
     }
     if (best == 0.0) {   // singular denominator (pamp.h:544-550)
       if (tid == 0) diag[0] = -1.0;
@@ -381,7 +396,10 @@ __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg, double
 }
 
 // ------------------------------------------------------------------ the step kernel
-template <int KD>
+// MIXED: terms flagged fp32 are applied in complex FP32 (FP32 radial bands,
+// FP32 band sums); Gram-Schmidt, the overlap solves, the exchanged U^-1 w and
+// the Pade exponential stay FP64.
+template <int KD, bool MIXED>
 __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
 {
   extern __shared__ double lds_d[];
@@ -500,7 +518,64 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
         }
         __syncthreads();
         // X = sum_k B_k (sum_q c_kq Ys_q): bands diagonal-major, all loads independent
-        for (int i = tid; i < padN; i += PP_THREADS) {
+        if (MIXED) for (int i = tid; i < padN; i += PP_THREADS) {
+          zd acc = zmk(0.0, 0.0);
+          if (i < N) {
+            const bool interior = i >= KD && i + KD < N;
+            int k = 0;
+            for (int t = 0; t < A.nterms; t++) {
+              const bool f32 = A.term[t].fp32 != 0;
+              for (int r = 0; r < A.term[t].rank; r++, k++) {
+                const int ne = cur_n[k];
+                if (ne == 0) continue;
+                if (f32) {
+                  const float *Br = A.term[t].bands32 + (size_t)r * (2 * KD + 1) * N;
+                  float bv[2 * KD + 1];
+#pragma unroll
+                  for (int d = -KD; d <= KD; d++)
+                    bv[d + KD] = (interior || (i + d >= 0 && i + d < N)) ? Br[(size_t)(d + KD) * N + i] : 0.f;
+#pragma unroll
+                  for (int q = 0; q < PP_MAXSLOTS; q++) {
+                    if (q < ne) {
+                      const zd *Yq = Ys + cur_q[k * PP_MAXSLOTS + q] * N;
+                      float zr = 0.f, zi = 0.f;
+#pragma unroll
+                      for (int d = -KD; d <= KD; d++) {
+                        const int j = i + d;
+                        const zd yv = (interior || (j >= 0 && j < N)) ? Yq[j] : zmk(0.0, 0.0);
+                        zr = fmaf(bv[d + KD], (float)yv.x, zr);
+                        zi = fmaf(bv[d + KD], (float)yv.y, zi);
+                      }
+                      zfma(acc, cur_c[k * PP_MAXSLOTS + q], zmk((double)zr, (double)zi));
+                    }
+                  }
+                } else {
+                  const double *Br = A.term[t].bands + (size_t)r * (2 * KD + 1) * N;
+                  double bv[2 * KD + 1];
+#pragma unroll
+                  for (int d = -KD; d <= KD; d++)
+                    bv[d + KD] = (interior || (i + d >= 0 && i + d < N)) ? Br[(size_t)(d + KD) * N + i] : 0.0;
+#pragma unroll
+                  for (int q = 0; q < PP_MAXSLOTS; q++) {
+                    if (q < ne) {
+                      const zd *Yq = Ys + cur_q[k * PP_MAXSLOTS + q] * N;
+                      zd z = zmk(0.0, 0.0);
+#pragma unroll
+                      for (int d = -KD; d <= KD; d++) {
+                        const int j = i + d;
+                        const zd yv = (interior || (j >= 0 && j < N)) ? Yq[j] : zmk(0.0, 0.0);
+                        zfmar(z, bv[d + KD], yv);
+                      }
+                      zfma(acc, cur_c[k * PP_MAXSLOTS + q], z);
+                    }
+                  }
+                }
+              }
+            }
+          }
+          X[i] = acc;
+        }
+        else for (int i = tid; i < padN; i += PP_THREADS) {
           zd acc = zmk(0.0, 0.0);
           if (i < N) {
             const bool interior = i >= KD && i + KD < N;
@@ -675,6 +750,7 @@ extern "C" int pp_pamp_create(int L, int N, int m, int reorth, const pp_overlap 
   p->rowcols.assign(L, std::vector<int>());
   p->slotcols = nullptr; p->dirty = true;
   p->V = p->y = p->Hm = p->cvec = nullptr;
+  p->mixed = 0;
   p->part = p->diag = p->scal = nullptr;
   p->bar = nullptr; p->tlog = nullptr; p->timing = 0;
   p->diag_cap = p->scal_cap = 0;
@@ -735,7 +811,8 @@ static int pamp_add_factored(pp_pamp *p, int angular_storage, int P0, const int 
   size_t nint = ints.size();
   size_t off_coef = ((nint * sizeof(int) + 15) / 16) * 16;
   size_t off_band = off_coef + sizeof(zd) * (size_t)P0 * rank;
-  size_t total = off_band + sizeof(double) * (size_t)rank * bw * N;
+  size_t off_band32 = off_band + sizeof(double) * (size_t)rank * bw * N;
+  size_t total = off_band32 + sizeof(float) * (size_t)rank * bw * N;
   std::vector<char> host(total, 0);
   memcpy(host.data(), ints.data(), nint * sizeof(int));
   memcpy(host.data() + off_coef, coef, sizeof(zd) * (size_t)P0 * rank);
@@ -747,6 +824,8 @@ static int pamp_add_factored(pp_pamp *p, int angular_storage, int P0, const int 
         int d = j - i;   // element (i, i+d)
         bd[((size_t)r * bw + d + s) * N + i] = bandsNH[(size_t)r * N * bw + (size_t)j * bw + s + i - j];
       }
+  float *bf = (float *)(host.data() + off_band32);
+  for (size_t e = 0; e < (size_t)rank * bw * N; e++) bf[e] = (float)bd[e];
   void *dev = nullptr;
   PP_HIP(hipMalloc(&dev, total));
   PP_HIP(hipMemcpy(dev, host.data(), total, hipMemcpyHostToDevice));
@@ -756,6 +835,7 @@ static int pamp_add_factored(pp_pamp *p, int angular_storage, int P0, const int 
   p->term[t].ent = (const int *)dev + csr.rowptr.size();
   p->term[t].coef = (const zd *)((char *)dev + off_coef);
   p->term[t].bands = (const double *)((char *)dev + off_band);
+  p->term[t].bands32 = (const float *)((char *)dev + off_band32);
   p->term[t].rank = rank;
   p->bw = bw;
   p->s = s;
@@ -966,7 +1046,10 @@ extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re
   A.scrDoubles = pamp_scr_doubles(p, A.padN);
   A.nlevEven = (pl.nlev + 1) & ~1;
   A.dt_re = dt_re; A.dt_im = dt_im;
-  for (int t = 0; t < p->nterms; t++) A.term[t] = p->term[t];
+  for (int t = 0; t < p->nterms; t++) {
+    A.term[t] = p->term[t];
+    A.term[t].fp32 = (p->mixed >> t) & 1u;
+  }
   A.slotcols = p->slotcols;
   A.scal = p->scal;
   A.Uband = ov->Uband;
@@ -980,7 +1063,7 @@ extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re
   bool launched = false;
   hipError_t err = hipSuccess;
   PP_KD_SWITCH(ov->kd, {
-    const void *fn = (const void *)pp_pamp_kernel<KD_>;
+    const void *fn = p->mixed ? (const void *)pp_pamp_kernel<KD_, true> : (const void *)pp_pamp_kernel<KD_, false>;
     err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (err == hipSuccess)
       err = hipLaunchCooperativeKernel(fn, dim3(p->L), dim3(PP_THREADS), args, (unsigned)lds, stream);
@@ -1001,6 +1084,17 @@ extern "C" int pp_pamp_diagnostics(pp_pamp *p, int nsteps, double *diag, hipStre
   PP_HIP(hipMemcpyAsync(&hb, p->bar, sizeof(hb), hipMemcpyDeviceToHost, stream));
   PP_HIP(hipStreamSynchronize(stream));
   if (hb.err) return PP_EHIP;
+  return PP_OK;
+}
+
+// Mixed-precision form: later launches apply the terms whose bit is set in
+// term_mask in complex FP32 (FP32 radial bands and band sums); 0 restores the
+// all-FP64 drop-in.
+extern "C" int pp_pamp_set_mixed(pp_pamp *p, unsigned term_mask)
+{
+  if (!p) return PP_EARG;
+  if (term_mask >> PP_MAXTERMS) return PP_EARG;
+  p->mixed = term_mask;
   return PP_OK;
 }
 

@@ -140,6 +140,16 @@ class PampEngine:
         nat.check(nat.lib().pp_pamp_diagnostics(self.h, n, nat.dptr(d), stream_ptr()), "pp_pamp_diagnostics")
         return d
 
+    def set_mixed(self, terms=()):
+        """Mixed-precision form: the listed term indices (order of add_term)
+        are applied in complex FP32, everything else stays FP64."""
+        mask = 0
+        for t in terms:
+            if not 0 <= int(t) < len(self.terms):
+                raise ValueError("no term %r" % (t,))
+            mask |= 1 << int(t)
+        nat.check(nat.lib().pp_pamp_set_mixed(self.h, mask), "pp_pamp_set_mixed")
+
     def set_timing(self, enable=True):
         nat.check(nat.lib().pp_pamp_set_timing(self.h, int(bool(enable))), "pp_pamp_set_timing")
 

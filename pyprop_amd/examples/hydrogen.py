@@ -40,11 +40,12 @@ def register():
     pyprop_amd.ProjectNamespace["GroundStateInverseIteration"] = GroundStateInverseIteration
 
 
-def load(name, **overrides):
+def load(name, overrides=None):
+    """Config of a named workload; overrides: {(section, key): value}."""
     import pyprop_amd
     register()
     conf = pyprop_amd.Load(os.path.join(CONFIGS, WORKLOADS[name][0]))
-    for (section, key), value in overrides.items():
+    for (section, key), value in (overrides or {}).items():
         conf.SetValue(section, key, value)
     return conf
 

@@ -60,6 +60,7 @@ class PamPropagator(PropagatorBase):
         self.BasisSize = 20
         self.ExponentiationMethod = 0
         self.DoubleOrthogonalization = False
+        self.MatvecPrecision = "fp64"
         self.Engine = None
 
     def ApplyConfig(self, config):
@@ -80,6 +81,15 @@ class PamPropagator(PropagatorBase):
             self.ExponentiationMethod = int(section.krylov_exponentiation_method)
         if hasattr(section, "krylov_double_orthogonalization"):
             self.DoubleOrthogonalization = bool(section.krylov_double_orthogonalization)
+        # extension key (no reference counterpart): "fp64" (default, the
+        # drop-in) or "mixed" (time-dependent potentials applied in complex
+        # FP32; the static Hamiltonian, Gram-Schmidt, overlap solves and Pade
+        # stay FP64)
+        if hasattr(section, "matvec_precision"):
+            prec = str(section.matvec_precision).lower()
+            if prec not in ("fp64", "mixed"):
+                raise ValueError("matvec_precision must be 'fp64' or 'mixed', got %r" % prec)
+            self.MatvecPrecision = prec
 
     def SetupStep(self, dt):
         self.BasePropagator.SetupStep(dt)
@@ -106,6 +116,12 @@ class PamPropagator(PropagatorBase):
                 raise NotImplementedError("storage %s is not on the pAMP HIP engine" % "_".join(ids))
             pairs = pot.GeometryList[0].GetBasisPairs() if ids[0] in ("Simp", "Herm") else None
             eng.add_term(ids[0], pot.DeviceData(), pairs=pairs)
+        if self.MatvecPrecision == "mixed":
+            # the static part carries the stiff spectrum (centrifugal and
+            # kinetic terms reach ~1e4 at large l near the origin), so FP32
+            # rounding there would dominate the error; the time-dependent
+            # couplings are small and take the FP32 path
+            eng.set_mixed([i for i, p in enumerate(pots) if p.IsTimeDependent])
         self._pots = pots
         return eng
 
