@@ -100,6 +100,23 @@ int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re, double dt
  * (pamp.h:666; -1 if the Pade denominator was singular, pamp.h:544-550),
  * basis size used (< m after happy breakdown), |U psi|, re-orthogonalisation passes. */
 int pp_pamp_diagnostics(pp_pamp *p, int nsteps, double *diag, pp_stream stream);
+/* nsteps consecutive ExpokitPropagator.AdvanceStep(t_n, dt)
+ * (pyprop/propagator/KrylovPropagator.py:41-45, core.krylov_ExpokitPropagator_R,
+ * core/krylov/expokit/expokitpropagator.h:29-31, expokitpropagator.cpp:36-103)
+ * on the same engine: psi~ = U psi, w = exp(|dt| A) psi~ by zgexpv
+ * (zgexpv.cpp:32-586, adaptive substeps with error control), psi = U^-1 w,
+ * with A = -i H^ (real dt) or -H^ (dt_re == 0, imaginary time).  tol =
+ * krylov_tolerance (<= eps means sqrt(eps)), anorm = krylov_norm.  Complex
+ * dt and reorth != 0 return -2.  The reference passes dt^2 as the Krylov time
+ * for real dt (expokitpropagator.cpp:61); here it is |dt|. */
+int pp_expokit_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re, double dt_im,
+                         const double *scal, double tol, double anorm, pp_stream stream);
+/* Per step of the last pp_expokit_propagate (host, nsteps x 6): matvecs,
+ * exponentials, substeps, rejected substeps, basis size of the last substep,
+ * zgexpv iflag (0 ok, 1 more than 500 substeps, -9 Pade failure, -8 more than
+ * 100 rejections of one substep, a bound zgexpv does not have); the
+ * reference throws on iflag != 0 (expokitpropagator.cpp:96-100). */
+int pp_expokit_diagnostics(pp_pamp *p, int nsteps, double *stats, pp_stream stream);
 /* Mixed-precision form (no reference counterpart; the reference is FP64
  * throughout): later launches apply the terms whose bit is set in term_mask
  * (bit t = t-th added term) in complex FP32 (FP32 radial bands, FP32 band

@@ -179,7 +179,7 @@ __device__ inline void pp_solve_fixup(const pp_band_plan &pl, const zd *tb, zd *
 // pl.nlevs; tb: LDS scratch of P*KD complex.  Ends with a __syncthreads.
 // `stamp` is called after stages 1 and 2 (profiling hook).
 template <int KD, bool UPPER, int NT, class Stamp = pp_nostamp>
-__device__ inline void pp_band_solve(const pp_band_plan &pl, const double *nl, const zd *in, zd *out, zd *tb,
+__device__ __attribute__((always_inline)) inline void pp_band_solve(const pp_band_plan &pl, const double *nl, const zd *in, zd *out, zd *tb,
                                      Stamp stamp = Stamp())
 {
   if (threadIdx.x < NT) pp_solve_blocks<UPPER, NT>(pl, in, out);

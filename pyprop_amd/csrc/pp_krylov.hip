@@ -70,7 +70,7 @@ struct pp_pamp_args {
   const double *Uband;   // real U band for psi~ = U psi
   pp_band_plan plan;     // chunk solver plan of U (scan matrices copied to LDS)
   zd *psi;               // L x N, in/out (fiber a private to workgroup a)
-  zd *V;                 // L x m x N (fiber-major, private)
+  zd *V;                 // L x (m+1) x N (fiber-major, private)
   zd *y;                 // L x N, U^-1 w (exchanged)
   double *part;          // L x pstr partial sums (exchanged)
   int pstr;
@@ -396,51 +396,54 @@ __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg, double
 }
 
 // ------------------------------------------------------------------ the step kernel
-// MIXED: terms flagged fp32 are applied in complex FP32 (FP32 radial bands,
-// FP32 band sums); Gram-Schmidt, the overlap solves, the exchanged U^-1 w and
-// the Pade exponential stay FP64.
-template <int KD, bool MIXED>
-__global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
+// ------------------------------------------------------------------ LDS map and the operator apply
+// LDS carve-up shared by the step kernels.  Persistent for a launch: the
+// solver's scan matrices and the combo table; the rest is per step (and
+// reused by the Pade exponential).
+struct pp_lds_map {
+  double *nl;
+  zd *cmb_c; int *cmb_q, *cmb_n, *cmb_t;   // per row: coefficients, slots, counts, terms
+  zd *cur_c; int *cur_q, *cur_n;           // rewritten before every matvec
+  zd *W, *X, *tb, *Ys, *Z;                 // residual, solve in/out, solver scratch, staged fibers
+  double *scr, *wsum, *hred, *gscr, *hacc;
+  zd *cv;
+};
+
+template <int KD>
+__device__ __attribute__((always_inline)) inline pp_lds_map pp_map_lds(const pp_pamp_args &A, double *lds_d,
+                                                                      int extra = 0)
 {
-  extern __shared__ double lds_d[];
-  const int N = A.N, L = A.L, m = A.m, a = blockIdx.x, tid = threadIdx.x;
-  const int padN = A.padN;
-  const pp_band_plan &pl = A.plan;
-  // persistent for the whole launch: scan matrices and the combo table
-  double *nl = lds_d;                                  // scan matrices of the solver plan
-  zd *cmb_c = (zd *)(lds_d + A.nlevEven);              // combo table: coefficients (PP_MAXCOMBO x slots)
-  int *cmb_q = (int *)(cmb_c + PP_MAXCOMBO * PP_MAXSLOTS);  // slots
-  int *cmb_n = cmb_q + PP_MAXCOMBO * PP_MAXSLOTS;      // entries per combo
-  int *cmb_t = cmb_n + PP_MAXCOMBO;                    // term of each combo
-  // rewritten before every matvec (so no table entry lives in registers
-  // across the step loop): scaled coefficients, slots, counts
-  zd *cur_c = (zd *)(cmb_t + PP_MAXCOMBO);
-  int *cur_q = (int *)(cur_c + PP_MAXCOMBO * PP_MAXSLOTS);
-  int *cur_n = cur_q + PP_MAXCOMBO * PP_MAXSLOTS;
-  // per step (workgroup 0 reuses this region for the Pade exponential)
-  zd *W = (zd *)(lds_d + A.nlevEven + PP_CMB_DOUBLES); // residual of fiber a (N)
-  zd *X = W + padN;                                    // solve input (padN)
-  zd *tb = X + padN;                                   // solver scratch (P*KD)
-  zd *Ys = tb + pl.P * KD;                             // staged source fibers (nslots x N)
-  zd *Z = Ys;                                          // also: solve output (padN)
-  double *scr = (double *)Ys;                          // also: partial-sum staging (scrDoubles)
-  double *wsum = (double *)Ys + A.scrDoubles;          // 16
-  double *hred = wsum + 16;                            // 2m+4 reduced values
-  double *gscr = hred + (2 * m + 4);                   // PP_THREADS
-  double *hacc = gscr + PP_THREADS;                    // 2m
-  zd *cv = (zd *)(hacc + 2 * m);                       // m Krylov coefficients
-  const double tol = 1.4901161193847656e-08;           // sqrt(DBL_EPSILON), pamp.h:75
-  zd *Va = A.V + (size_t)a * m * N;
-  double *mypart = A.part + (size_t)a * A.pstr;
-  const int NRM = 2 * m, ONRM = 2 * m + 1;
-  unsigned ep = 0;   // grid barriers passed in this launch
-  int tcount = 0;
-  auto stamp = [&](int step) {
-    if (A.tlog && step == 0 && a == 0 && tid == 0 && tcount < PP_TLOG_MAX) A.tlog[tcount++] = (long long)wall_clock64();
-  };
-  for (int i = tid; i < pl.nlev; i += PP_THREADS) nl[i] = pl.nlevs[i];
-  // combo table of this row: one entry per (term, rank) with the angular
-  // coefficients of its couplings (time and 1/beta scalings applied later)
+  pp_lds_map M;
+  const int padN = A.padN, m = A.m;
+  M.nl = lds_d;
+  M.cmb_c = (zd *)(lds_d + A.nlevEven);
+  M.cmb_q = (int *)(M.cmb_c + PP_MAXCOMBO * PP_MAXSLOTS);
+  M.cmb_n = M.cmb_q + PP_MAXCOMBO * PP_MAXSLOTS;
+  M.cmb_t = M.cmb_n + PP_MAXCOMBO;
+  M.cur_c = (zd *)(M.cmb_t + PP_MAXCOMBO);
+  M.cur_q = (int *)(M.cur_c + PP_MAXCOMBO * PP_MAXSLOTS);
+  M.cur_n = M.cur_q + PP_MAXCOMBO * PP_MAXSLOTS;
+  M.W = (zd *)(lds_d + A.nlevEven + PP_CMB_DOUBLES + extra);   // extra: kernel-specific persistent doubles
+  M.X = M.W + padN;
+  M.tb = M.X + padN;
+  M.Ys = M.tb + A.plan.P * KD;
+  M.Z = M.Ys;
+  M.scr = (double *)M.Ys;
+  M.wsum = (double *)M.Ys + A.scrDoubles;
+  M.hred = M.wsum + 16;
+  M.gscr = M.hred + (2 * m + 4);
+  M.hacc = M.gscr + PP_THREADS;
+  M.cv = (zd *)(M.hacc + 2 * m);
+  return M;
+}
+
+// Scan matrices to LDS and the combo table of row a: one entry per
+// (term, rank) with the angular coefficients of its couplings (time factors
+// applied per matvec).
+__device__ __attribute__((always_inline)) inline void pp_load_persistent(const pp_pamp_args &A, const pp_lds_map &M, int a)
+{
+  const int tid = threadIdx.x;
+  for (int i = tid; i < A.plan.nlev; i += PP_THREADS) M.nl[i] = A.plan.nlevs[i];
   if (tid == 0) {
     int k = 0;
     for (int t = 0; t < A.nterms; t++) {
@@ -451,68 +454,40 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
         for (int e = e0; e < e1; e++, ne++) {
           zd c = T.coef[(size_t)T.ent[3 * e + 1] * T.rank + r];
           if (T.ent[3 * e + 2]) c = zconj(c);
-          cmb_c[k * PP_MAXSLOTS + ne] = c;
-          cmb_q[k * PP_MAXSLOTS + ne] = T.ent[3 * e];
+          M.cmb_c[k * PP_MAXSLOTS + ne] = c;
+          M.cmb_q[k * PP_MAXSLOTS + ne] = T.ent[3 * e];
         }
-        cmb_n[k] = ne;
-        cmb_t[k] = t;
+        M.cmb_n[k] = ne;
+        M.cmb_t[k] = t;
       }
     }
   }
   __syncthreads();
+}
 
-  for (int step = 0; step < A.nsteps; step++) {
-    // ---- psi~ = U psi (MultiplySqrtOverlap, PamPropagator.py:53) ; y = U^-1 psi~
-    for (int i = tid; i < N; i += PP_THREADS) Ys[i] = A.psi[(size_t)a * N + i];
-    __syncthreads();
-    double nrm = 0.0;
-    for (int i = tid; i < N; i += PP_THREADS) {
-      zd acc = zmk(0.0, 0.0);
-      const int j1 = i + KD < N - 1 ? i + KD : N - 1;
-      for (int j = i; j <= j1; j++) zfmar(acc, A.Uband[(size_t)j * (KD + 1) + KD + i - j], Ys[j]);
-      W[i] = acc;
-      nrm += zabs2(acc);
-    }
-    nrm = block_sum(nrm, wsum);
-    if (tid == 0) xst(mypart + NRM, nrm);
-    for (int i = tid; i < padN; i += PP_THREADS) X[i] = i < N ? W[i] : zmk(0.0, 0.0);
-    __syncthreads();
-    pp_band_solve<KD, true, PP_SOLVE_THREADS>(pl, nl, X, Z, tb);
-    for (int i = tid; i < N; i += PP_THREADS) zst(A.y + (size_t)a * N + i, Z[i]);
-    stamp(step);
-    grid_barrier(A.bar, ep);
-    stamp(step);
-
-    double beta0 = 0.0, beta = 0.0;
-    int mEff = m;
-    bool broken = false;
-    int reorth_passes = 0;
-    for (int j = 0; j < m; j++) {
-      // ---------------- phase A: beta_j, v_j = w/beta, w' = U^-H H U^-1 v_j, <v_k, w'>
-      const int nk = j + 1, ncomp = 2 * nk;
-      grid_partial_sum(A.part, A.pstr, NRM, 1, L, hred, scr);
-      beta = sqrt(hred[0]);
-      if (j == 0) beta0 = beta;
-      else if (!broken) {
-        if (fabs(beta) < tol) { broken = true; mEff = j; }   // happy breakdown (pamp.h:305-312)
-        else if (a == 0 && tid == 0) A.Hm[j + (j - 1) * m] = zmk(beta, 0.0);
-      }
-      if (!broken) {
-        const double inv = 1.0 / beta;
-        for (int i = tid; i < N; i += PP_THREADS) {
-          zd v = zscale(W[i], inv);
-          W[i] = v;
-          Va[(size_t)j * N + i] = v;
-        }
+// W := fac * U^-H (sum_t s_t(t_step) V_t) y for fiber a = blockIdx.x, with y
+// the exchanged U^-1 v of every fiber (A.y) staged for this row's couplings.
+// The balanced-overlap matvec of BasisPropagator.py:131-159.  `stamp` is a
+// profiling hook (after the apply and inside the solve).
+template <int KD, bool MIXED, class Stamp>
+__device__ __attribute__((always_inline)) inline void pp_matvec(const pp_pamp_args &A, const pp_lds_map &M, int step, zd fac, Stamp stamp)
+{
+  const int N = A.N, a = blockIdx.x, tid = threadIdx.x, padN = A.padN;
+  const pp_band_plan &pl = A.plan;
+  zd *Ys = M.Ys, *X = M.X, *W = M.W, *tb = M.tb, *cur_c = M.cur_c;
+  int *cur_q = M.cur_q, *cur_n = M.cur_n;
+  const zd *cmb_c = M.cmb_c;
+  const int *cmb_q = M.cmb_q, *cmb_n = M.cmb_n, *cmb_t = M.cmb_t;
+  const double *nl = M.nl;
         // stage the source fibers U^-1 w of this row's couplings
         for (int q = 0; q < A.nslots; q++) {
           const int col = A.slotcols[a * A.nslots + q];
           if (col >= 0)
             for (int i = tid; i < N; i += PP_THREADS) Ys[q * N + i] = zld(A.y + (size_t)col * N + i);
         }
-        // coefficients times the terms' time factors at t_n and 1/beta
+        // coefficients times the terms' time factors at t_n and the factor fac
         for (int e = tid; e < A.ncombo * PP_MAXSLOTS; e += PP_THREADS) {
-          cur_c[e] = zscale(cmb_c[e], A.scal[(size_t)step * A.nterms + cmb_t[e / PP_MAXSLOTS]] * inv);
+          cur_c[e] = zmul(cmb_c[e], zscale(fac, A.scal[(size_t)step * A.nterms + cmb_t[e / PP_MAXSLOTS]]));
           cur_q[e] = cmb_q[e];
           if (e % PP_MAXSLOTS == 0) cur_n[e / PP_MAXSLOTS] = cmb_n[e / PP_MAXSLOTS];
         }
@@ -611,8 +586,79 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
           X[i] = acc;
         }
         __syncthreads();
-        stamp(step);
-        pp_band_solve<KD, false, PP_SOLVE_THREADS>(pl, nl, X, W, tb, [&]() { stamp(step); });
+        stamp();
+        pp_band_solve<KD, false, PP_SOLVE_THREADS>(pl, nl, X, W, tb, stamp);
+}
+
+// MIXED: terms flagged fp32 are applied in complex FP32 (FP32 radial bands,
+// FP32 band sums); Gram-Schmidt, the overlap solves, the exchanged U^-1 w and
+// the Pade exponential stay FP64.
+template <int KD, bool MIXED>
+__global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
+{
+  extern __shared__ double lds_d[];
+  const int N = A.N, L = A.L, m = A.m, a = blockIdx.x, tid = threadIdx.x;
+  const int padN = A.padN;
+  const pp_band_plan &pl = A.plan;
+  const pp_lds_map M = pp_map_lds<KD>(A, lds_d);
+  double *nl = M.nl;
+  zd *W = M.W, *X = M.X, *tb = M.tb, *Ys = M.Ys, *Z = M.Z, *cv = M.cv;
+  double *scr = M.scr, *wsum = M.wsum, *hred = M.hred, *hacc = M.hacc;
+  const double tol = 1.4901161193847656e-08;           // sqrt(DBL_EPSILON), pamp.h:75
+  zd *Va = A.V + (size_t)a * (m + 1) * N;
+  double *mypart = A.part + (size_t)a * A.pstr;
+  const int NRM = 2 * m, ONRM = 2 * m + 1;
+  unsigned ep = 0;   // grid barriers passed in this launch
+  int tcount = 0;
+  auto stamp = [&](int step) {
+    if (A.tlog && step == 0 && a == 0 && tid == 0 && tcount < PP_TLOG_MAX) A.tlog[tcount++] = (long long)wall_clock64();
+  };
+  pp_load_persistent(A, M, a);
+
+  for (int step = 0; step < A.nsteps; step++) {
+    // ---- psi~ = U psi (MultiplySqrtOverlap, PamPropagator.py:53) ; y = U^-1 psi~
+    for (int i = tid; i < N; i += PP_THREADS) Ys[i] = A.psi[(size_t)a * N + i];
+    __syncthreads();
+    double nrm = 0.0;
+    for (int i = tid; i < N; i += PP_THREADS) {
+      zd acc = zmk(0.0, 0.0);
+      const int j1 = i + KD < N - 1 ? i + KD : N - 1;
+      for (int j = i; j <= j1; j++) zfmar(acc, A.Uband[(size_t)j * (KD + 1) + KD + i - j], Ys[j]);
+      W[i] = acc;
+      nrm += zabs2(acc);
+    }
+    nrm = block_sum(nrm, wsum);
+    if (tid == 0) xst(mypart + NRM, nrm);
+    for (int i = tid; i < padN; i += PP_THREADS) X[i] = i < N ? W[i] : zmk(0.0, 0.0);
+    __syncthreads();
+    pp_band_solve<KD, true, PP_SOLVE_THREADS>(pl, nl, X, Z, tb);
+    for (int i = tid; i < N; i += PP_THREADS) zst(A.y + (size_t)a * N + i, Z[i]);
+    stamp(step);
+    grid_barrier(A.bar, ep);
+    stamp(step);
+
+    double beta0 = 0.0, beta = 0.0;
+    int mEff = m;
+    bool broken = false;
+    int reorth_passes = 0;
+    for (int j = 0; j < m; j++) {
+      // ---------------- phase A: beta_j, v_j = w/beta, w' = U^-H H U^-1 v_j, <v_k, w'>
+      const int nk = j + 1, ncomp = 2 * nk;
+      grid_partial_sum(A.part, A.pstr, NRM, 1, L, hred, scr);
+      beta = sqrt(hred[0]);
+      if (j == 0) beta0 = beta;
+      else if (!broken) {
+        if (fabs(beta) < tol) { broken = true; mEff = j; }   // happy breakdown (pamp.h:305-312)
+        else if (a == 0 && tid == 0) A.Hm[j + (j - 1) * m] = zmk(beta, 0.0);
+      }
+      if (!broken) {
+        const double inv = 1.0 / beta;
+        for (int i = tid; i < N; i += PP_THREADS) {
+          zd v = zscale(W[i], inv);
+          W[i] = v;
+          Va[(size_t)j * N + i] = v;
+        }
+        pp_matvec<KD, MIXED>(A, M, step, zmk(inv, 0.0), [&]() { stamp(step); });
         stamp(step);
         fiber_dots(Va, W, nk, N, A.reorth != 0, hred);
         for (int c = tid; c < ncomp; c += PP_THREADS) xst(mypart + c, hred[c]);
@@ -737,6 +783,381 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
   if (A.tlog && a == 0 && tid == 0) A.tlog[PP_TLOG_MAX] = tcount;
 }
 
+// ------------------------------------------------------------------ Expokit
+// zgpadm (Expokit, Sidje 1998; core/krylov/expokit/f2c/zgpadm.cpp:28-285):
+// exp(t H) of the leading mx x mx block of Hm (column-major, leading
+// dimension mh, LDS), irreducible Pade(6) with ns = max(0, int(log2 ||tH||_inf)
+// + 2) squarings.  Writes the first column to ecol (LDS) and returns ns, or
+// -1 for a null H or a singular denominator.  Scratch: 5 mx^2 + 5 mx complex
+// and PP_THREADS/64 doubles at double offset `off` of the dynamic LDS.  The
+// solve of zgesv is an in-place Gauss-Jordan with the same partial pivoting.
+__device__ __attribute__((noinline)) int pp_zgpadm_block(int mx, int mh, const zd *Hm, double t, int off, zd *ecol)
+{
+  extern __shared__ double pp_lds[];
+  const int tid = threadIdx.x, mm = mx * mx;
+  zd *Hs = (zd *)(pp_lds + off), *P = Hs + mm, *Q = P + mm, *H2 = Q + mm, *F = H2 + mm;
+  zd *G = H2;                                   // [Q - P | P], mx x 2mx, after H2 and F are dead
+  zd *rk = F + mm, *rowk = rk + 2 * mx, *cm = rowk + 2 * mx;
+  double *red = (double *)(cm + mx);
+  const int i0 = tid % mx, j0 = tid / mx, di = PP_THREADS % mx, dj = PP_THREADS / mx;
+  for (int e = tid; e < mm; e += PP_THREADS) Hs[e] = Hm[(e % mx) + (e / mx) * mh];
+  __syncthreads();
+  // infinity norm
+  double rs = 0.0;
+  if (tid < mx)
+    for (int j = 0; j < mx; j++) rs += sqrt(zabs2(Hs[tid + j * mx]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) rs = fmax(rs, __shfl_xor(rs, o, 64));
+  if ((tid & 63) == 0) red[tid >> 6] = rs;
+  __syncthreads();
+  double hnorm = red[0];
+  for (int w = 1; w < PP_THREADS / 64; w++) hnorm = fmax(hnorm, red[w]);
+  hnorm = fabs(t * hnorm);
+  if (hnorm == 0.0) return -1;
+  int ns = (int)(log(hnorm) / log(2.0)) + 2;
+  if (ns < 0) ns = 0;
+  const double scale = t / pow(2.0, (double)ns), scale2 = scale * scale;
+  const int ideg = 6;
+  double coef[ideg + 1];
+  coef[0] = 1.0;
+  for (int k = 1; k <= ideg; k++) coef[k] = coef[k - 1] * (double)(ideg + 1 - k) / (double)(k * (2 * ideg + 1 - k));
+  auto matmul = [&](const zd *X, const zd *Y, zd *Z, double sc) {   // Z = sc X Y
+    int i = i0, j = j0;
+    for (int e = tid; e < mm; e += PP_THREADS) {
+      const zd *xr = X + i, *yc = Y + j * mx;
+      zd a0 = zmk(0.0, 0.0), a1 = zmk(0.0, 0.0), a2 = zmk(0.0, 0.0), a3 = zmk(0.0, 0.0);
+      int k = 0;
+      for (; k + 3 < mx; k += 4) {
+        const zd x0 = xr[k * mx], x1 = xr[(k + 1) * mx], x2 = xr[(k + 2) * mx], x3 = xr[(k + 3) * mx];
+        const zd y0 = yc[k], y1 = yc[k + 1], y2 = yc[k + 2], y3 = yc[k + 3];
+        zfma(a0, x0, y0);
+        zfma(a1, x1, y1);
+        zfma(a2, x2, y2);
+        zfma(a3, x3, y3);
+      }
+      for (; k < mx; k++) zfma(a0, xr[k * mx], yc[k]);
+      Z[e] = zscale(zadd(zadd(a0, a1), zadd(a2, a3)), sc);
+      i += di; j += dj;
+      if (i >= mx) { i -= mx; j++; }
+    }
+    __syncthreads();
+  };
+  matmul(Hs, Hs, H2, scale2);
+  {
+    int i = i0, j = j0;
+    for (int e = tid; e < mm; e += PP_THREADS) {
+      P[e] = zmk(i == j ? coef[ideg - 1] : 0.0, 0.0);
+      Q[e] = zmk(i == j ? coef[ideg] : 0.0, 0.0);
+      i += di; j += dj;
+      if (i >= mx) { i -= mx; j++; }
+    }
+  }
+  __syncthreads();
+  int iodd = 1;
+  for (int k = ideg - 1; k > 0; k--) {
+    zd *U = iodd ? Q : P;
+    matmul(U, H2, F, 1.0);
+    int i = i0, j = j0;
+    for (int e = tid; e < mm; e += PP_THREADS) {
+      zd v = F[e];
+      if (i == j) v.x += coef[k - 1];
+      U[e] = v;
+      i += di; j += dj;
+      if (i >= mx) { i -= mx; j++; }
+    }
+    __syncthreads();
+    iodd = 1 - iodd;
+  }
+  zd *Uo = iodd ? Q : P;
+  matmul(Uo, Hs, F, scale);
+  for (int e = tid; e < mm; e += PP_THREADS) Uo[e] = F[e];
+  __syncthreads();
+  for (int e = tid; e < mm; e += PP_THREADS) {
+    G[e] = zsub(Q[e], P[e]);
+    G[mm + e] = P[e];
+  }
+  __syncthreads();
+  // in-place Gauss-Jordan on G: the pivot row, the old row k and the
+  // multipliers go to small buffers first, so every element update reads
+  // only its own entry and the buffers
+  const int w2 = 2 * mm;
+  for (int k = 0; k < mx; k++) {
+    int p = mx;
+    double best = -1.0;
+    for (int i = k + (tid & 63); i < mx; i += 64) {
+      const double v = fabs(G[i + k * mx].x) + fabs(G[i + k * mx].y);
+      if (v > best) { best = v; p = i; }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, 64);
+      const int op = __shfl_xor(p, o, 64);
+      if (ob > best || (ob == best && op < p)) { best = ob; p = op; }
+    }
+    if (best == 0.0) return -1;
+    const zd piv = G[p + k * mx];
+    const double dn = zabs2(piv);
+    const zd pinv = zmk(piv.x / dn, -piv.y / dn);
+    if (tid < 2 * mx) {
+      rowk[tid] = G[k + tid * mx];
+      rk[tid] = zmul(G[p + tid * mx], pinv);
+    }
+    if (tid < mx) {
+      const int si = (tid == p) ? k : tid;
+      cm[tid] = (tid == k) ? zmk(0.0, 0.0) : G[si + k * mx];
+    }
+    __syncthreads();
+    int i = i0, j = j0;
+    for (int e = tid; e < w2; e += PP_THREADS) {
+      if (j >= k) {
+        if (i == k) G[e] = rk[j];
+        else {
+          const zd src = (i == p) ? rowk[j] : G[e];
+          G[e] = zsub(src, zmul(cm[i], rk[j]));
+        }
+      }
+      i += di; j += dj;
+      if (i >= mx) { i -= mx; j++; }
+    }
+    __syncthreads();
+  }
+  // E = 2X + I (X = right half of G), sign / squarings as zgpadm
+  if (ns == 0) {
+    if (tid < mx) {
+      zd v = zscale(G[mm + tid], 2.0);
+      if (tid == 0) v.x += 1.0;
+      if (iodd) v = zmk(-v.x, -v.y);
+      ecol[tid] = v;
+    }
+    __syncthreads();
+    return 0;
+  }
+  {
+    int i = i0, j = j0;
+    for (int e = tid; e < mm; e += PP_THREADS) {
+      zd v = zscale(G[mm + e], 2.0);
+      if (i == j) v.x += 1.0;
+      P[e] = v;
+      i += di; j += dj;
+      if (i >= mx) { i -= mx; j++; }
+    }
+  }
+  __syncthreads();
+  for (int k = 0; k < ns; k++) {
+    matmul(P, P, Q, 1.0);
+    for (int e = tid; e < mm; e += PP_THREADS) P[e] = Q[e];
+    __syncthreads();
+  }
+  if (tid < mx) ecol[tid] = P[tid];
+  __syncthreads();
+  return ns;
+}
+
+// x_roundstep of zgexpv (zgexpv.cpp:297-310): keep two significant digits.
+__device__ inline double pp_roundstep(double t)
+{
+  const double e = log10(t) - sqrt(0.1);
+  const int ie = (int)(e >= 0 ? floor(e + 0.5) : -floor(0.5 - e)) - 1;
+  const double p1 = pow(10.0, (double)ie);
+  return trunc(t / p1 + 0.55) * p1;
+}
+
+struct pp_expo_args {
+  double tol, anorm;   // krylov_tolerance (<= eps: sqrt(eps)), krylov_norm
+  double t_out;        // |dt|
+  int imag;            // imaginary time: A = -H^ (else A = -i H^)
+  double *stats;       // nsteps x 6: nmult, nexph, nstep, nreject, mbrkdwn, iflag
+};
+
+// nsteps consecutive ExpokitPropagator.AdvanceStep's (expokitpropagator.cpp:
+// 36-103 inside KrylovPropagator.py:41-45): psi~ = U psi; w = exp(t A) psi~ by
+// zgexpv (zgexpv.cpp:32-586: adaptive substeps, m+1 matvecs each, 2-corrected
+// error estimate, step rejection, at most 500 substeps); psi = U^-1 w.  The
+// Arnoldi basis is orthogonalised by classical Gram-Schmidt with one
+// re-orthogonalisation pass (two grid reductions) in place of zgexpv's
+// modified Gram-Schmidt (one reduction per basis vector); the two agree in
+// exact arithmetic.  Every workgroup holds the Hessenberg matrix and runs the
+// exponential itself on identical, identically ordered data, so the step
+// control takes the same branches in every workgroup without a broadcast.
+template <int KD, bool MIXED>
+__global__ void __launch_bounds__(PP_THREADS) pp_expokit_kernel(pp_pamp_args A, pp_expo_args E)
+{
+  extern __shared__ double lds_d[];
+  const int N = A.N, L = A.L, m = A.m, a = blockIdx.x, tid = threadIdx.x, padN = A.padN, mh = m + 2;
+  const pp_band_plan &pl = A.plan;
+  const int hm_doubles = 2 * (mh * mh + mh);
+  const pp_lds_map M = pp_map_lds<KD>(A, lds_d, hm_doubles);
+  zd *Hm = (zd *)(lds_d + A.nlevEven + PP_CMB_DOUBLES);   // (m+2)^2, column-major
+  zd *ecol = Hm + mh * mh;                                 // first column of exp(t H)
+  double *nl = M.nl;
+  zd *W = M.W, *X = M.X, *tb = M.tb, *Z = M.Z;
+  double *scr = M.scr, *wsum = M.wsum, *hred = M.hred, *hacc = M.hacc;
+  zd *Va = A.V + (size_t)a * (m + 1) * N;
+  double *mypart = A.part + (size_t)a * A.pstr;
+  const int D1 = 0, D2 = 2 * m, NRM = 4 * m + 2, ANRM = 4 * m + 3, P0NRM = 4 * m + 4;
+  const int pade_off = A.nlevEven + PP_CMB_DOUBLES + hm_doubles;
+  unsigned ep = 0;
+  pp_load_persistent(A, M, a);
+  const zd fac = E.imag ? zmk(-1.0, 0.0) : zmk(0.0, -1.0);
+  const double eps = 2.220446049250313e-16;
+  const double tol = E.tol <= eps ? sqrt(eps) : E.tol;
+  const double rndoff = eps * E.anorm, break_tol = 1e-7, t_out = E.t_out;
+  auto reduce1 = [&](int slot) {
+    grid_partial_sum(A.part, A.pstr, slot, 1, L, hred, scr);
+    return hred[0];
+  };
+  auto exchange_y = [&]() {   // X holds v (padded); y = U^-1 v to the other fibers
+    __syncthreads();
+    pp_band_solve<KD, true, PP_SOLVE_THREADS>(pl, nl, X, Z, tb);
+    for (int i = tid; i < N; i += PP_THREADS) zst(A.y + (size_t)a * N + i, Z[i]);
+  };
+  for (int step = 0; step < A.nsteps; step++) {
+    // ---- psi~ = U psi into W, |psi~|^2
+    for (int i = tid; i < N; i += PP_THREADS) Z[i] = A.psi[(size_t)a * N + i];
+    __syncthreads();
+    double nrm = 0.0;
+    for (int i = tid; i < N; i += PP_THREADS) {
+      zd acc = zmk(0.0, 0.0);
+      const int j1 = i + KD < N - 1 ? i + KD : N - 1;
+      for (int j = i; j <= j1; j++) zfmar(acc, A.Uband[(size_t)j * (KD + 1) + KD + i - j], Z[j]);
+      W[i] = acc;
+      nrm += zabs2(acc);
+    }
+    nrm = block_sum(nrm, wsum);
+    if (tid == 0) xst(mypart + P0NRM, nrm);
+    grid_barrier(A.bar, ep);
+    double beta = sqrt(reduce1(P0NRM));
+    int nmult = 0, nexph = 0, nstep = 0, nreject = 0, mbrkdwn = m, k1 = 2, iflag = 0;
+    double t_now = 0.0, err_loc = 0.0, t_step = 0.0, avnorm = 0.0, xmloc = 1.0 / m;
+    const double p2 = tol * pow((m + 1) / 2.72, (double)(m + 1)) * sqrt((m + 1) * 6.28);
+    double t_new = pp_roundstep((1.0 / E.anorm) * pow(p2 / (4.0 * beta * E.anorm), 1.0 / m));
+    while (t_now < t_out) {
+      nstep++;
+      t_step = fmin(t_out - t_now, t_new);
+      const double inv = 1.0 / beta;
+      for (int i = tid; i < padN; i += PP_THREADS) {
+        const zd v = i < N ? zscale(W[i], inv) : zmk(0.0, 0.0);
+        if (i < N) Va[i] = v;
+        X[i] = v;
+      }
+      for (int e = tid; e < mh * mh; e += PP_THREADS) Hm[e] = zmk(0.0, 0.0);
+      exchange_y();
+      grid_barrier(A.bar, ep);
+      k1 = 2;
+      mbrkdwn = m;
+      for (int j = 1; j <= m; j++) {
+        nmult++;
+        pp_matvec<KD, MIXED>(A, M, step, fac, pp_nostamp());
+        // first pass: h1 = V^H z
+        fiber_dots(Va, W, j, N, false, hred);
+        for (int c = tid; c < 2 * j; c += PP_THREADS) xst(mypart + D1 + c, hred[c]);
+        grid_barrier(A.bar, ep);
+        grid_partial_sum(A.part, A.pstr, D1, 2 * j, L, hred, scr);
+        for (int c = tid; c < 2 * j; c += PP_THREADS) hacc[c] = hred[c];
+        for (int i = tid; i < N; i += PP_THREADS) {
+          zd w = W[i];
+          for (int k = 0; k < j; k++) w = zsub(w, zmul(Va[(size_t)k * N + i], zmk(hred[2 * k], hred[2 * k + 1])));
+          W[i] = w;
+        }
+        __syncthreads();
+        // second pass: h2 = V^H z', |z'|^2
+        fiber_dots(Va, W, j, N, true, hred);
+        for (int c = tid; c <= 2 * j; c += PP_THREADS) xst(mypart + D2 + c, hred[c]);
+        grid_barrier(A.bar, ep);
+        grid_partial_sum(A.part, A.pstr, D2, 2 * j + 1, L, hred, scr);
+        double h2n = 0.0;
+        for (int k = 0; k < j; k++) h2n += hred[2 * k] * hred[2 * k] + hred[2 * k + 1] * hred[2 * k + 1];
+        const double hj1j = sqrt(fmax(hred[2 * j] - h2n, 0.0));
+        for (int i = tid; i < N; i += PP_THREADS) {
+          zd w = W[i];
+          for (int k = 0; k < j; k++) w = zsub(w, zmul(Va[(size_t)k * N + i], zmk(hred[2 * k], hred[2 * k + 1])));
+          W[i] = w;
+        }
+        if (tid < j) Hm[tid + (j - 1) * mh] = zmk(hacc[2 * tid] + hred[2 * tid], hacc[2 * tid + 1] + hred[2 * tid + 1]);
+        __syncthreads();
+        if (hj1j <= break_tol) {   // happy breakdown (zgexpv.cpp:366-372)
+          k1 = 0;
+          mbrkdwn = j;
+          t_step = t_out - t_now;
+          break;
+        }
+        if (tid == 0) Hm[j + (j - 1) * mh] = zmk(hj1j, 0.0);
+        const double hinv = 1.0 / hj1j;
+        for (int i = tid; i < padN; i += PP_THREADS) {
+          const zd v = i < N ? zscale(W[i], hinv) : zmk(0.0, 0.0);
+          if (i < N) Va[(size_t)j * N + i] = v;
+          X[i] = v;
+        }
+        exchange_y();
+        grid_barrier(A.bar, ep);
+      }
+      if (k1 != 0) {   // the extra matvec of the 2-corrected error estimate
+        nmult++;
+        pp_matvec<KD, MIXED>(A, M, step, fac, pp_nostamp());
+        double v = 0.0;
+        for (int i = tid; i < N; i += PP_THREADS) v += zabs2(W[i]);
+        v = block_sum(v, wsum);
+        if (tid == 0) xst(mypart + ANRM, v);
+        grid_barrier(A.bar, ep);
+        avnorm = sqrt(reduce1(ANRM));
+      }
+      if (tid == 0) Hm[(m + 1) + m * mh] = zmk(1.0, 0.0);
+      __syncthreads();
+      // exponential with step rejection (zgexpv.cpp:397-470)
+      int ireject = 0;
+      for (;;) {
+        nexph++;
+        const int mx = mbrkdwn + k1;
+        const int ns = pp_zgpadm_block(mx, mh, Hm, t_step, pade_off, ecol);
+        if (ns < 0) { iflag = -9; break; }
+        if (k1 == 0) err_loc = tol;
+        else {
+          const double q1 = sqrt(zabs2(ecol[m])) * beta, q2 = sqrt(zabs2(ecol[m + 1])) * beta * avnorm;
+          if (q1 > 10.0 * q2) { err_loc = q2; xmloc = 1.0 / m; }
+          else if (q1 > q2) { err_loc = q1 * q2 / (q1 - q2); xmloc = 1.0 / m; }
+          else { err_loc = q1; xmloc = 1.0 / (m - 1); }
+        }
+        if (k1 != 0 && err_loc > 1.2 * t_step * tol) {
+          t_step = pp_roundstep(0.9 * t_step * pow(t_step * tol / err_loc, xmloc));
+          nreject++;
+          if (++ireject > 100) { iflag = -8; break; }   // bound (not in zgexpv): no accepted step
+          continue;
+        }
+        break;
+      }
+      if (iflag) break;
+      // w = beta V exp(t H) e1, |w|
+      const int mx = mbrkdwn + (k1 - 1 > 0 ? k1 - 1 : 0);
+      double nw = 0.0;
+      for (int i = tid; i < N; i += PP_THREADS) {
+        zd acc = zmk(0.0, 0.0);
+        for (int k = 0; k < mx; k++) zfma(acc, Va[(size_t)k * N + i], ecol[k]);
+        acc = zscale(acc, beta);
+        W[i] = acc;
+        nw += zabs2(acc);
+      }
+      nw = block_sum(nw, wsum);
+      if (tid == 0) xst(mypart + NRM, nw);
+      grid_barrier(A.bar, ep);
+      beta = sqrt(reduce1(NRM));
+      t_new = pp_roundstep(0.9 * t_step * pow(t_step * tol / err_loc, xmloc));
+      err_loc = fmax(err_loc, rndoff);
+      t_now += t_step;
+      if (nstep >= 500 && t_now < t_out) { iflag = 1; break; }
+    }
+    // ---- psi = U^-1 w (SolveSqrtOverlap)
+    for (int i = tid; i < padN; i += PP_THREADS) X[i] = i < N ? W[i] : zmk(0.0, 0.0);
+    __syncthreads();
+    pp_band_solve<KD, true, PP_SOLVE_THREADS>(pl, nl, X, Z, tb);
+    for (int i = tid; i < N; i += PP_THREADS) A.psi[(size_t)a * N + i] = Z[i];
+    if (a == 0 && tid == 0) {
+      double *st = E.stats + (size_t)step * 6;
+      st[0] = nmult; st[1] = nexph; st[2] = nstep; st[3] = nreject; st[4] = mbrkdwn; st[5] = iflag;
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ host API
 extern "C" int pp_pamp_create(int L, int N, int m, int reorth, const pp_overlap *ov, pp_pamp **out)
 {
@@ -745,7 +1166,7 @@ extern "C" int pp_pamp_create(int L, int N, int m, int reorth, const pp_overlap 
   if (m > PP_MAXM) return PP_EUNSUP;
   pp_pamp *p = new pp_pamp();
   p->L = L; p->N = N; p->m = m; p->reorth = reorth ? 1 : 0; p->ov = ov;
-  p->pstr = 2 * m + 4;
+  p->pstr = 4 * m + 6;
   p->s = -1; p->bw = 0; p->nterms = 0; p->nslots = 0;
   p->rowcols.assign(L, std::vector<int>());
   p->slotcols = nullptr; p->dirty = true;
@@ -755,13 +1176,13 @@ extern "C" int pp_pamp_create(int L, int N, int m, int reorth, const pp_overlap 
   p->bar = nullptr; p->tlog = nullptr; p->timing = 0;
   p->diag_cap = p->scal_cap = 0;
   for (int t = 0; t < PP_MAXTERMS; t++) p->termbuf[t] = nullptr;
-  PP_HIP(hipMalloc(&p->V, sizeof(zd) * (size_t)L * m * N));
+  PP_HIP(hipMalloc(&p->V, sizeof(zd) * (size_t)L * (m + 1) * N));
   PP_HIP(hipMalloc(&p->y, sizeof(zd) * (size_t)L * N));
   PP_HIP(hipMalloc(&p->Hm, sizeof(zd) * (size_t)m * m));
   PP_HIP(hipMalloc(&p->cvec, sizeof(zd) * (size_t)m));
   PP_HIP(hipMalloc(&p->part, sizeof(double) * (size_t)L * p->pstr));
   PP_HIP(hipMalloc(&p->bar, sizeof(pp_barrier) + sizeof(unsigned) * (size_t)L));
-  PP_HIP(hipMemset(p->V, 0, sizeof(zd) * (size_t)L * m * N));
+  PP_HIP(hipMemset(p->V, 0, sizeof(zd) * (size_t)L * (m + 1) * N));
   PP_HIP(hipMemset(p->Hm, 0, sizeof(zd) * (size_t)m * m));
   PP_HIP(hipMemset(p->bar, 0, sizeof(pp_barrier) + sizeof(unsigned) * (size_t)L));
   *out = p;
@@ -1010,17 +1431,15 @@ static int pamp_upload_slots(pp_pamp *p)
   return PP_OK;
 }
 
-// Propagate psi (device, L x N complex128) through nsteps steps of length dt.
-// scal: host array nsteps x nterms with each term's time factor at t_n.
-extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re, double dt_im,
-                                 const double *scal, hipStream_t stream)
+// Common launch set-up of the step kernels: uploads the time factors,
+// clears the barrier words and fills the kernel arguments.
+static int pamp_prepare(pp_pamp *p, void *psi, int nsteps, double dt_re, double dt_im, const double *scal,
+                        hipStream_t stream, pp_pamp_args &A)
 {
-  if (!p || !psi || nsteps < 0 || (!scal && p->nterms > 0)) return PP_EARG;
-  if (nsteps == 0) return PP_OK;
-  if (p->nterms == 0) return PP_EARG;
   int rc = pamp_upload_slots(p);
   if (rc) return rc;
   const pp_overlap *ov = p->ov;
+  if (p->s != ov->kd) return PP_EUNSUP;   // radial bands must have the overlap's bandwidth
   if (nsteps * p->nterms > p->scal_cap) {
     if (p->scal) (void)hipFree(p->scal);
     p->scal_cap = nsteps * p->nterms;
@@ -1029,15 +1448,13 @@ extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re
   if (nsteps > p->diag_cap) {
     if (p->diag) (void)hipFree(p->diag);
     p->diag_cap = nsteps;
-    PP_HIP(hipMalloc(&p->diag, sizeof(double) * 4 * (size_t)nsteps));
+    PP_HIP(hipMalloc(&p->diag, sizeof(double) * 6 * (size_t)nsteps));
   }
   PP_HIP(hipMemcpyAsync(p->scal, scal, sizeof(double) * nsteps * p->nterms, hipMemcpyHostToDevice, stream));
   PP_HIP(hipMemsetAsync(p->bar, 0, sizeof(pp_barrier) + sizeof(unsigned) * (size_t)p->L, stream));
-  pp_pamp_args A;
   memset(&A, 0, sizeof(A));
   A.L = p->L; A.N = p->N; A.m = p->m; A.s = p->s; A.bw = p->bw; A.nterms = p->nterms;
   A.nsteps = nsteps; A.reorth = p->reorth; A.nslots = p->nslots;
-  if (p->s != ov->kd) return PP_EUNSUP;   // radial bands must have the overlap's bandwidth
   A.ncombo = 0;
   for (int t = 0; t < p->nterms; t++) A.ncombo += p->term[t].rank;
   if (A.ncombo > PP_MAXCOMBO) return PP_EUNSUP;
@@ -1057,19 +1474,85 @@ extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re
   A.psi = (zd *)psi; A.V = p->V; A.y = p->y; A.part = p->part; A.pstr = p->pstr;
   A.Hm = p->Hm; A.cvec = p->cvec; A.diag = p->diag; A.bar = p->bar;
   A.tlog = p->timing ? p->tlog : nullptr;
-  size_t lds = pamp_lds_bytes(p, A.padN);
+  return PP_OK;
+}
+
+static int pamp_launch(const void *fn, int L, void **args, size_t lds, hipStream_t stream)
+{
   if (lds > 160 * 1024) return PP_EUNSUP;
+  hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (err == hipSuccess) err = hipLaunchCooperativeKernel(fn, dim3(L), dim3(PP_THREADS), args, (unsigned)lds, stream);
+  return err == hipSuccess ? PP_OK : PP_EHIP;
+}
+
+// Propagate psi (device, L x N complex128) through nsteps steps of length dt.
+// scal: host array nsteps x nterms with each term's time factor at t_n.
+extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re, double dt_im,
+                                 const double *scal, hipStream_t stream)
+{
+  if (!p || !psi || nsteps < 0 || (!scal && p->nterms > 0)) return PP_EARG;
+  if (nsteps == 0) return PP_OK;
+  if (p->nterms == 0) return PP_EARG;
+  pp_pamp_args A;
+  int rc = pamp_prepare(p, psi, nsteps, dt_re, dt_im, scal, stream, A);
+  if (rc) return rc;
+  const size_t lds = pamp_lds_bytes(p, A.padN);
   void *args[] = {&A};
-  bool launched = false;
-  hipError_t err = hipSuccess;
-  PP_KD_SWITCH(ov->kd, {
-    const void *fn = p->mixed ? (const void *)pp_pamp_kernel<KD_, true> : (const void *)pp_pamp_kernel<KD_, false>;
-    err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (err == hipSuccess)
-      err = hipLaunchCooperativeKernel(fn, dim3(p->L), dim3(PP_THREADS), args, (unsigned)lds, stream);
-    launched = true; })
-  if (!launched) return PP_EUNSUP;
-  if (err != hipSuccess) return PP_EHIP;
+  rc = PP_EUNSUP;
+  PP_KD_SWITCH(p->ov->kd, {
+    rc = pamp_launch(p->mixed ? (const void *)pp_pamp_kernel<KD_, true> : (const void *)pp_pamp_kernel<KD_, false>,
+                     p->L, args, lds, stream); })
+  return rc;
+}
+
+// Expokit form of the same engine: nsteps ExpokitPropagator.AdvanceStep's
+// with zgexpv (tol <= eps means sqrt(eps), anorm = krylov_norm).  Real dt:
+// A = -i H^ over |dt|; imaginary dt (dt_re == 0): A = -H^ over |dt_im|.
+extern "C" int pp_expokit_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re, double dt_im,
+                                    const double *scal, double tol, double anorm, hipStream_t stream)
+{
+  if (!p || !psi || nsteps < 0 || (!scal && p->nterms > 0) || !(anorm > 0.0)) return PP_EARG;
+  if (nsteps == 0) return PP_OK;
+  if (p->nterms == 0 || p->m < 2) return PP_EARG;
+  if (dt_re != 0.0 && dt_im != 0.0) return PP_EUNSUP;   // complex time steps
+  if (p->reorth) return PP_EUNSUP;                      // pAMP-only option
+  pp_pamp_args A;
+  int rc = pamp_prepare(p, psi, nsteps, dt_re, dt_im, scal, stream, A);
+  if (rc) return rc;
+  pp_expo_args E;
+  E.tol = tol;
+  E.anorm = anorm;
+  E.imag = dt_re == 0.0 ? 1 : 0;
+  E.t_out = E.imag ? fabs(dt_im) : fabs(dt_re);
+  E.stats = p->diag;
+  const int mh = p->m + 2;
+  const size_t hm = sizeof(double) * 2 * (size_t)(mh * mh + mh);
+  const size_t nle = sizeof(double) * ((size_t)((p->ov->plan.nlev + 1) & ~1) + PP_CMB_DOUBLES);
+  size_t fib = pamp_lds_bytes(p, A.padN) + hm;
+  size_t pade = nle + hm + sizeof(zd) * (size_t)(5 * mh * mh + 5 * mh) + sizeof(double) * (PP_THREADS / 64) + 64;
+  const size_t lds = fib > pade ? fib : pade;
+  void *args[] = {&A, &E};
+  rc = PP_EUNSUP;
+  PP_KD_SWITCH(p->ov->kd, {
+    rc = pamp_launch(p->mixed ? (const void *)pp_expokit_kernel<KD_, true> : (const void *)pp_expokit_kernel<KD_, false>,
+                     p->L, args, lds, stream); })
+  return rc;
+}
+
+// stats: host nsteps x 6 of the last pp_expokit_propagate call (matvecs,
+// exponentials, substeps, rejected substeps, basis size of the last substep,
+// zgexpv iflag: 0 ok, 1 more than 500 substeps, -9 Pade failure, -8 more
+// than 100 rejections of one substep);
+// synchronises the stream.  Returns a HIP error status if a grid barrier
+// timed out.
+extern "C" int pp_expokit_diagnostics(pp_pamp *p, int nsteps, double *stats, hipStream_t stream)
+{
+  if (!p || !stats || nsteps > p->diag_cap) return PP_EARG;
+  pp_barrier hb;
+  PP_HIP(hipMemcpyAsync(stats, p->diag, sizeof(double) * 6 * (size_t)nsteps, hipMemcpyDeviceToHost, stream));
+  PP_HIP(hipMemcpyAsync(&hb, p->bar, sizeof(hb), hipMemcpyDeviceToHost, stream));
+  PP_HIP(hipStreamSynchronize(stream));
+  if (hb.err) return PP_EHIP;
   return PP_OK;
 }
 

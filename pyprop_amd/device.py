@@ -134,6 +134,24 @@ class PampEngine:
         self._last = nsteps
         return psi
 
+    def propagate_expokit(self, psi, nsteps, dt, scalings, tol=0.0, anorm=1e6):
+        """nsteps ExpokitPropagator.AdvanceStep's in one launch (zgexpv)."""
+        dt = complex(dt)
+        sc = np.ascontiguousarray(np.asarray(scalings, dtype=np.float64).reshape(nsteps, len(self.terms)))
+        nat.check(nat.lib().pp_expokit_propagate(self.h, ptr(psi), int(nsteps), dt.real, dt.imag, nat.dptr(sc),
+                                                 float(tol), float(anorm), stream_ptr()), "pp_expokit_propagate")
+        self._last = nsteps
+        return psi
+
+    def expokit_stats(self, nsteps=None):
+        """Per step: matvecs, exponentials, substeps, rejections, basis size, iflag."""
+        n = self._last if nsteps is None else nsteps
+        d = np.zeros((n, 6))
+        nat.check(nat.lib().pp_expokit_diagnostics(self.h, n, nat.dptr(d), stream_ptr()), "pp_expokit_diagnostics")
+        if np.any(d[:, 5] != 0):
+            raise RuntimeError("Expokit returned nonzero value %d" % int(d[d[:, 5] != 0][0, 5]))
+        return d
+
     def diagnostics(self, nsteps=None):
         n = self._last if nsteps is None else nsteps
         d = np.zeros((n, 4))

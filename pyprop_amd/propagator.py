@@ -1,4 +1,4 @@
-"""Propagators: PropagatorBase and the Krylov PamPropagator.
+"""Propagators: PropagatorBase and the Krylov PamPropagator / ExpokitPropagator.
 
 PropagatorBase follows pyprop/propagator/Propagator.py:5-80.  PamPropagator
 keeps the reference's config keys and its balanced-overlap stepping
@@ -143,3 +143,39 @@ class PamPropagator(PropagatorBase):
         """Per step of the last launch: |last Krylov coefficient| (pamp.h:666),
         basis size used, |U psi|, re-orthogonalisation passes."""
         return self.Engine.diagnostics()
+
+
+class ExpokitPropagator(PamPropagator):
+    """Expokit zgexpv propagator (pyprop/propagator/KrylovPropagator.py:1-60,
+    core/krylov/expokit/expokitpropagator.cpp:9-103): config keys
+    krylov_basis_size, krylov_tolerance, krylov_norm.  Same balanced-overlap
+    wrapping and persistent engine as PamPropagator; the exponential is the
+    adaptive zgexpv step with error control.  The Krylov time is |dt| (the
+    reference's expokitpropagator.cpp:61 passes dt^2 for real dt)."""
+
+    def __init__(self, psi):
+        PamPropagator.__init__(self, psi)
+        self.Tolerance = 0.0
+        self.MatrixNorm = 1e6
+
+    def _apply_krylov(self, section):
+        # expokitpropagator.cpp:13-20
+        self.BasisSize = int(section.krylov_basis_size)
+        if hasattr(section, "krylov_tolerance"):
+            self.Tolerance = float(section.krylov_tolerance)
+        if hasattr(section, "krylov_norm"):
+            self.MatrixNorm = float(section.krylov_norm)
+        if hasattr(section, "matvec_precision"):
+            prec = str(section.matvec_precision).lower()
+            if prec not in ("fp64", "mixed"):
+                raise ValueError("matvec_precision must be 'fp64' or 'mixed', got %r" % prec)
+            self.MatvecPrecision = prec
+
+    def AdvanceSteps(self, times, dt):
+        sc = self.TimeScalings(times)
+        self.Engine.propagate_expokit(self.psi.GetData(), len(times), dt, sc, self.Tolerance, self.MatrixNorm)
+
+    def GetDiagnostics(self):
+        """Per step of the last launch: matvecs, exponentials, substeps,
+        rejected substeps, basis size of the last substep, zgexpv iflag."""
+        return self.Engine.expokit_stats()
