@@ -46,6 +46,10 @@ int pp_band_cholesky_upper(int N, int kd, double *ab);
  * info (the reference leaves it unchecked; here info>0 returns 1).
  * Supported: kd in {4,5,6,8} (B-spline orders 5,6,7,9), N <= 1024. */
 int pp_overlap_create(int N, int kd, const double *S_upper, pp_overlap **out, int *info);
+/* Same with an explicit solver chunk length B (power of two >= max(kd, 8),
+ * at most 64 chunks; 0 = default).  No reference counterpart: the reference's
+ * solve is a sequential ztbsv; the batch autotuner picks B per shape. */
+int pp_overlap_create_chunked(int N, int kd, const double *S_upper, int B, pp_overlap **out, int *info);
 /* Copy the real Cholesky factor U (N*(kd+1) doubles, upper band storage) to the host. */
 int pp_overlap_factor(const pp_overlap *ov, double *U_upper_host);
 int pp_overlap_destroy(pp_overlap *ov);
@@ -117,6 +121,14 @@ int pp_expokit_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re, double
  * 100 rejections of one substep, a bound zgexpv does not have); the
  * reference throws on iflag != 0 (expokitpropagator.cpp:96-100). */
 int pp_expokit_diagnostics(pp_pamp *p, int nsteps, double *stats, pp_stream stream);
+/* Batched form (no reference counterpart; the reference runs one problem per
+ * process): nsteps pAMP steps of each of ninst independent engines (own
+ * operator, overlap, psi[i]; dt: ninst x 2 re/im; scal[i]: host nsteps x
+ * nterms_i), packed in order into cooperative launches whose workgroups (one
+ * per fiber) are all co-resident; larger batches take several launches on the
+ * stream.  Engines must share the B-spline order and the precision form. */
+int pp_pamp_batch_propagate(int ninst, pp_pamp *const *inst, void *const *psi, int nsteps, const double *dt,
+                            const double *const *scal, pp_stream stream);
 /* Mixed-precision form (no reference counterpart; the reference is FP64
  * throughout): later launches apply the terms whose bit is set in term_mask
  * (bit t = t-th added term) in complex FP32 (FP32 radial bands, FP32 band

@@ -24,9 +24,9 @@ STORAGE_IDS = {"Ident": 0, "Diag": 1, "Simp": 2, "Herm": 3, "BandNH": 4, "Band":
 OV_SOLVE_SQRT, OV_SOLVE_SQRT_CONJ, OV_MUL_SQRT, OV_MUL_SQRT_CONJ, OV_SOLVE, OV_MUL = range(6)
 
 EXPORTS = [
-    "pp_band_cholesky_upper", "pp_overlap_create", "pp_overlap_factor", "pp_overlap_destroy",
+    "pp_band_cholesky_upper", "pp_overlap_create", "pp_overlap_create_chunked", "pp_overlap_factor", "pp_overlap_destroy",
     "pp_overlap_apply", "pp_tp_multiply", "pp_pamp_create", "pp_pamp_add_term", "pp_pamp_add_factored_term",
-    "pp_lowrank_factor", "pp_pamp_propagate", "pp_pamp_diagnostics", "pp_pamp_set_mixed", "pp_expokit_propagate", "pp_expokit_diagnostics", "pp_pamp_set_timing", "pp_pamp_timings", "pp_pamp_destroy",
+    "pp_lowrank_factor", "pp_pamp_propagate", "pp_pamp_diagnostics", "pp_pamp_set_mixed", "pp_expokit_propagate", "pp_expokit_diagnostics", "pp_pamp_batch_propagate", "pp_pamp_set_timing", "pp_pamp_timings", "pp_pamp_destroy",
 ]
 
 _lib = None
@@ -58,6 +58,7 @@ def lib():
     ip = ctypes.POINTER(ctypes.c_int)
     L.pp_band_cholesky_upper.argtypes = [I, I, dp]
     L.pp_overlap_create.argtypes = [I, I, dp, ctypes.POINTER(P), ip]
+    L.pp_overlap_create_chunked.argtypes = [I, I, dp, I, ctypes.POINTER(P), ip]
     L.pp_overlap_factor.argtypes = [P, dp]
     L.pp_overlap_destroy.argtypes = [P]
     L.pp_overlap_apply.argtypes = [P, I, I, I, ip, P, P]
@@ -71,6 +72,7 @@ def lib():
     L.pp_pamp_set_mixed.argtypes = [P, ctypes.c_uint]
     L.pp_expokit_propagate.argtypes = [P, P, I, D, D, dp, D, D, P]
     L.pp_expokit_diagnostics.argtypes = [P, I, dp, P]
+    L.pp_pamp_batch_propagate.argtypes = [I, ctypes.POINTER(P), ctypes.POINTER(P), I, dp, ctypes.POINTER(dp), P]
     L.pp_pamp_set_timing.argtypes = [P, I]
     L.pp_pamp_timings.argtypes = [P, ctypes.POINTER(ctypes.c_longlong), I, ip]
     L.pp_pamp_destroy.argtypes = [P]

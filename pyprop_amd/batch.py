@@ -1,0 +1,109 @@
+"""Batched propagation of many independent problems in packed persistent
+launches, plus the chunk-length autotuner of the band solves.
+
+No reference counterpart: pyprop runs one problem per process (MPI ranks
+split one wavefunction).  Each problem keeps its own config, operator,
+overlap and wavefunction exactly as a single Problem; ``Batch.AdvanceSteps``
+hands all of them to ``pp_pamp_batch_propagate``, which packs their fibers
+into cooperative launches and synchronises each problem's workgroups only
+among themselves.
+"""
+import numpy as np
+
+from . import _native as nat
+from . import device as dev
+
+
+def instance_draws(count=512, seed=20261203):
+    """Seeded draws of the batched hydrogen scan: N ~ U{300..600} B-splines,
+    lmax ~ U{20..60}, omega ~ U[0.05, 0.6], log10 E0 ~ U[log10 0.01, log10 0.2],
+    cycles ~ U{4..12} (numpy PCG64)."""
+    rng = np.random.default_rng(seed)
+    N = rng.integers(300, 601, size=count)
+    lmax = rng.integers(20, 61, size=count)
+    omega = rng.uniform(0.05, 0.6, size=count)
+    E0 = 10.0 ** rng.uniform(np.log10(0.01), np.log10(0.2), size=count)
+    cycles = rng.integers(4, 13, size=count)
+    return [dict(N=int(N[i]), lmax=int(lmax[i]), omega=float(omega[i]), E0=float(E0[i]), cycles=int(cycles[i]))
+            for i in range(count)]
+
+
+def autotune_chunks(shapes, kd, candidates=(16, 32, 64), fibers=64, reps=10):
+    """Pick the band-solver chunk length per radial size N by timing the
+    overlap solves U^-1 and U^-H on a (fibers, N) tensor for each candidate
+    (HIP events); records the winners in device.CHUNK_TABLE and returns
+    {N: (best chunk, {chunk: microseconds per solve pair})}."""
+    torch = dev.torch()
+    from .bspline import BSplineBasis, LinearBreakpointSequence
+    result = {}
+    for N in sorted(set(int(n) for n in shapes)):
+        # any SPD band of the right shape exercises the same code path; the
+        # B-spline overlap of a linear grid is the one the problems use
+        k = kd + 1
+        basis = BSplineBasis(LinearBreakpointSequence(0.0, 0.5 * N, N - k + 4), k, "zero")
+        S = basis.overlap_upper()
+        x = torch.zeros((fibers, N), dtype=torch.complex128, device=dev.device())
+        x[:, :] = 1.0
+        times = {}
+        for B in candidates:
+            # the step kernels keep the scan matrices of both directions in LDS
+            # (2 levels kd^2 P doubles, at most 72 KB, as the library default)
+            P = -(-N // B)
+            levels = int(np.ceil(np.log2(P))) if P > 1 else 0
+            if B < kd or P > 64 or 2 * levels * kd * kd * P * 8 > 72 * 1024:
+                continue
+            try:
+                ov = dev.Overlap(S, N, kd, chunk=B)
+            except nat.NativeError:
+                continue
+            ov.apply(nat.OV_SOLVE_SQRT, 1, x)
+            ov.apply(nat.OV_SOLVE_SQRT_CONJ, 1, x)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                ov.apply(nat.OV_SOLVE_SQRT, 1, x)
+                ov.apply(nat.OV_SOLVE_SQRT_CONJ, 1, x)
+            e1.record()
+            torch.cuda.synchronize()
+            times[B] = 1e3 * e0.elapsed_time(e1) / reps
+        if not times:
+            continue
+        best = min(times, key=times.get)
+        dev.CHUNK_TABLE[(N, kd)] = best
+        result[N] = (best, times)
+    return result
+
+
+class Batch:
+    """A set of set-up Problems whose propagators are PamPropagators on the
+    HIP engine; AdvanceSteps(n) advances every problem by n steps."""
+
+    def __init__(self, problems):
+        from .propagator import ExpokitPropagator, PamPropagator
+        for p in problems:
+            if not isinstance(p.Propagator, PamPropagator) or isinstance(p.Propagator, ExpokitPropagator):
+                raise NotImplementedError("batched propagation runs PamPropagator problems")
+            if p.Propagator.Engine is None:
+                raise RuntimeError("call SetupStep() on every problem first")
+            if p.Propagator.RenormalizeActive:
+                raise NotImplementedError("renormalisation between steps is not batched")
+        self.Problems = list(problems)
+
+    def AdvanceSteps(self, n):
+        if n <= 0:
+            return
+        engines, psis, dts, scs = [], [], [], []
+        for p in self.Problems:
+            times = []
+            for _ in range(n):
+                times.append(p.PropagatedTime)
+                p._advance_time()
+            engines.append(p.Propagator.Engine)
+            psis.append(p.psi.GetData())
+            dts.append(p.TimeStep)
+            scs.append(p.Propagator.TimeScalings(times))
+        dev.pamp_batch_propagate(engines, psis, n, dts, scs)
+
+    def GetDiagnostics(self):
+        return [p.Propagator.GetDiagnostics() for p in self.Problems]

@@ -160,6 +160,42 @@ class BSplineBasis:
         return {0: self.B, 1: self.D1, 2: self.D2}[d]
 
     # ------------------------------------------------------------ integrals
+    def _projector(self, d):
+        """Sparse (G, N*(2k-1)) map from weighted samples w*f on the global
+        quadrature grid to BandNH storage of <B_i| f d^d |B_j>."""
+        cache = self.__dict__.setdefault("_proj_cache", {})
+        if d in cache:
+            return cache[d]
+        import scipy.sparse
+        k = self.MaxSplineOrder
+        N = self.NumberOfBSplines
+        s = k - 1
+        bw = 2 * s + 1
+        Bt = self.B
+        Dt = self.table(d)
+        span = self.Span
+        G = Bt.shape[0]
+        sign = (-1.0) ** d
+        rows, cols, vals = [], [], []
+        q = np.arange(G)
+        for a_ in range(k):
+            for b_ in range(k):
+                i = span + a_
+                j = span + b_
+                valid = (i >= 0) & (i < N) & (j >= 0) & (j < N)
+                if a_ <= b_:
+                    c = Bt[:, a_] * Dt[:, b_]          # element (i, j), i <= j
+                else:
+                    c = sign * (Bt[:, b_] * Dt[:, a_])  # i > j: (-1)^d * B_j D_i
+                idx = j * bw + s + i - j
+                rows.append(q[valid])
+                cols.append(idx[valid])
+                vals.append(c[valid])
+        M = scipy.sparse.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                                    shape=(G, N * bw))
+        cache[d] = M
+        return M
+
     def project_bandnh(self, f, d=0):
         """<B_i| f d^d |B_j> in BandNH storage, shape (..., N*(2k-1)).
 
@@ -167,44 +203,12 @@ class BSplineBasis:
         (i,j) is stored at j*(2k-1) + (k-1)+i-j (GeometryInfo.py:433-454).
         For i > j the reference integrates the transposed pair and multiplies
         by (-1)^d (bspline.cpp:380-392)."""
-        k = self.MaxSplineOrder
-        N = self.NumberOfBSplines
-        s = k - 1
-        bw = 2 * s + 1
         f = np.asarray(f)
         lead = f.shape[:-1]
-        fw = f * self.QuadratureWeightsGlobal
-        Bt = self.B
-        Dt = self.table(d)
-        # T[..., q, a, b] = w f B_a D_b at each point; sum within spans
-        out = np.zeros(lead + (N * bw,), dtype=np.result_type(f.dtype, np.float64))
-        span = self.Span
-        sign = (-1.0) ** d
-        # accumulate per (a, b) offset pair
-        for a_ in range(k):
-            for b_ in range(k):
-                i = span + a_
-                j = span + b_
-                valid = (i >= 0) & (i < N) & (j >= 0) & (j < N)
-                if a_ <= b_:
-                    contrib = fw * (Bt[:, a_] * Dt[:, b_])
-                    ii, jj = i, j                      # element (i, j), i <= j
-                    c = contrib
-                else:
-                    # element (i, j) with i > j: (-1)^d * sum w f B_j D_i
-                    contrib = fw * (Bt[:, b_] * Dt[:, a_])
-                    ii, jj = i, j
-                    c = sign * contrib
-                idx = jj * bw + s + ii - jj
-                idx = np.where(valid, idx, 0)
-                c = np.where(valid, c, 0.0)
-                flat = out.reshape(-1, N * bw)
-                cc = c.reshape(-1, c.shape[-1])
-                for r in range(flat.shape[0]):
-                    flat[r] += np.bincount(idx, weights=cc[r].real, minlength=N * bw)
-                    if np.iscomplexobj(out):
-                        flat[r] += 1j * np.bincount(idx, weights=cc[r].imag, minlength=N * bw)
-        return out
+        fw = (f * self.QuadratureWeightsGlobal).reshape(-1, f.shape[-1])
+        M = self._projector(d)
+        out = np.asarray((M.T @ fw.T).T)
+        return out.reshape(lead + (M.shape[1],))
 
     def overlap_upper(self):
         """S in upper hermitian band storage (N, k): [j*k + (k-1)+i-j] = S(i,j), i<=j

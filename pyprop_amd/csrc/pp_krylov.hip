@@ -29,6 +29,7 @@
 #include "pp_common.h"
 #include <hip/hip_cooperative_groups.h>
 #include <cstring>
+#include <cstdio>
 #include <cmath>
 #include <vector>
 #include <algorithm>
@@ -106,7 +107,8 @@ __device__ inline double xld(const double *p) { return __hip_atomic_load(const_c
 __device__ inline void zst(zd *p, zd v) { xst(&p->x, v.x); xst(&p->y, v.y); }
 __device__ inline zd zld(const zd *p) { return zmk(xld(&p->x), xld(&p->y)); }
 
-// Grid barrier over co-resident workgroups: every workgroup publishes the
+// Barrier over the n co-resident workgroups of one instance (member `me`):
+// every workgroup publishes the
 // number of barriers it has reached in its own arrival word (the words follow
 // the pp_barrier header) and wavefront 0 polls all words until each has
 // reached the same count, so a barrier costs one store plus the polling
@@ -115,16 +117,16 @@ __device__ inline zd zld(const zd *p) { return zmk(xld(&p->x), xld(&p->y)); }
 // the arrival.  A bounded spin turns a lost arrival into an error flag
 // instead of a hang.  `ep` counts the barriers of this launch (identical in
 // every workgroup, since all execute the same barrier sequence).
-__device__ inline void grid_barrier(pp_barrier *b, unsigned &ep)
+__device__ inline void grid_barrier(pp_barrier *b, unsigned &ep, int n, int me)
 {
   unsigned *word = (unsigned *)(b + 1);
-  const int n = gridDim.x, tid = threadIdx.x;
+  const int tid = threadIdx.x;
   ep++;
   __syncthreads();
   if (tid < 64) {
     if (tid == 0) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __hip_atomic_store(word + blockIdx.x, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(word + me, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     long spins = 0;
     while (true) {
@@ -465,14 +467,15 @@ __device__ __attribute__((always_inline)) inline void pp_load_persistent(const p
   __syncthreads();
 }
 
-// W := fac * U^-H (sum_t s_t(t_step) V_t) y for fiber a = blockIdx.x, with y
+// W := fac * U^-H (sum_t s_t(t_step) V_t) y for fiber a, with y
 // the exchanged U^-1 v of every fiber (A.y) staged for this row's couplings.
 // The balanced-overlap matvec of BasisPropagator.py:131-159.  `stamp` is a
 // profiling hook (after the apply and inside the solve).
 template <int KD, bool MIXED, class Stamp>
-__device__ __attribute__((always_inline)) inline void pp_matvec(const pp_pamp_args &A, const pp_lds_map &M, int step, zd fac, Stamp stamp)
+__device__ __attribute__((always_inline)) inline void pp_matvec(const pp_pamp_args &A, const pp_lds_map &M, int a, int step, zd fac,
+                                                                Stamp stamp)
 {
-  const int N = A.N, a = blockIdx.x, tid = threadIdx.x, padN = A.padN;
+  const int N = A.N, tid = threadIdx.x, padN = A.padN;
   const pp_band_plan &pl = A.plan;
   zd *Ys = M.Ys, *X = M.X, *W = M.W, *tb = M.tb, *cur_c = M.cur_c;
   int *cur_q = M.cur_q, *cur_n = M.cur_n;
@@ -594,10 +597,10 @@ __device__ __attribute__((always_inline)) inline void pp_matvec(const pp_pamp_ar
 // FP32 band sums); Gram-Schmidt, the overlap solves, the exchanged U^-1 w and
 // the Pade exponential stay FP64.
 template <int KD, bool MIXED>
-__global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
+__device__ __attribute__((always_inline)) inline void pp_pamp_body(const pp_pamp_args &A, int a)
 {
   extern __shared__ double lds_d[];
-  const int N = A.N, L = A.L, m = A.m, a = blockIdx.x, tid = threadIdx.x;
+  const int N = A.N, L = A.L, m = A.m, tid = threadIdx.x;
   const int padN = A.padN;
   const pp_band_plan &pl = A.plan;
   const pp_lds_map M = pp_map_lds<KD>(A, lds_d);
@@ -634,7 +637,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
     pp_band_solve<KD, true, PP_SOLVE_THREADS>(pl, nl, X, Z, tb);
     for (int i = tid; i < N; i += PP_THREADS) zst(A.y + (size_t)a * N + i, Z[i]);
     stamp(step);
-    grid_barrier(A.bar, ep);
+    grid_barrier(A.bar, ep, L, a);
     stamp(step);
 
     double beta0 = 0.0, beta = 0.0;
@@ -658,14 +661,14 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
           W[i] = v;
           Va[(size_t)j * N + i] = v;
         }
-        pp_matvec<KD, MIXED>(A, M, step, zmk(inv, 0.0), [&]() { stamp(step); });
+        pp_matvec<KD, MIXED>(A, M, a, step, zmk(inv, 0.0), [&]() { stamp(step); });
         stamp(step);
         fiber_dots(Va, W, nk, N, A.reorth != 0, hred);
         for (int c = tid; c < ncomp; c += PP_THREADS) xst(mypart + c, hred[c]);
         if (A.reorth && tid == 0) xst(mypart + ONRM, hred[ncomp]);
       }
       stamp(step);
-      grid_barrier(A.bar, ep);
+      grid_barrier(A.bar, ep, L, a);
       stamp(step);
 
       // ---------------- phase B: h = sum_a partials ; w' -= V h ; |w'|^2 ; y = U^-1 w'
@@ -701,7 +704,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
             v = block_sum(v, wsum);
             if (tid == 0) xst(mypart + NRM, v);
           }
-          grid_barrier(A.bar, ep);
+          grid_barrier(A.bar, ep, L, a);
           bool doit = false;
           double res = 0.0;
           if (!stop) {
@@ -715,7 +718,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
             fiber_dots(Va, W, nk, N, false, hred);
             for (int c = tid; c < ncomp; c += PP_THREADS) xst(mypart + c, hred[c]);
           }
-          grid_barrier(A.bar, ep);
+          grid_barrier(A.bar, ep, L, a);
           if (doit) {
             grid_partial_sum(A.part, A.pstr, 0, ncomp, L, hred, scr);
             for (int i = tid; i < N; i += PP_THREADS) {
@@ -732,7 +735,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
             __syncthreads();
           }
         }
-        grid_barrier(A.bar, ep);   // everyone has read NRM before it is rewritten below
+        grid_barrier(A.bar, ep, L, a);   // everyone has read NRM before it is rewritten below
       }
       if (!broken) {
         double v = 0.0;
@@ -746,7 +749,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
         for (int i = tid; i < N; i += PP_THREADS) zst(A.y + (size_t)a * N + i, Z[i]);
       }
       stamp(step);
-      grid_barrier(A.bar, ep);
+      grid_barrier(A.bar, ep, L, a);
       stamp(step);
     }
 
@@ -762,7 +765,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
       for (int e = tid; e < m * m; e += PP_THREADS) A.Hm[e] = zmk(0.0, 0.0);
     }
     stamp(step);
-    grid_barrier(A.bar, ep);
+    grid_barrier(A.bar, ep, L, a);
     stamp(step);
     // ---------------- psi~ = |psi~| V c ; psi = U^-1 psi~ (SolveSqrtOverlap)
     for (int k = tid; k < m; k += PP_THREADS) cv[k] = zld(A.cvec + k);
@@ -781,6 +784,25 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
     // cvec is rewritten by the next step's Pade only after further barriers
   }
   if (A.tlog && a == 0 && tid == 0) A.tlog[PP_TLOG_MAX] = tcount;
+}
+
+// One instance: workgroup a owns fiber a.
+template <int KD, bool MIXED>
+__global__ void __launch_bounds__(PP_THREADS) pp_pamp_kernel(pp_pamp_args A)
+{
+  pp_pamp_body<KD, MIXED>(A, blockIdx.x);
+}
+
+// Many independent instances in one cooperative grid: workgroup w runs fiber
+// map[w].y of instance map[w].x; every instance has its own arguments
+// (sizes, operator, solver plan, buffers, barrier words) and synchronises
+// only its own workgroups.
+template <int KD, bool MIXED>
+__global__ void __launch_bounds__(PP_THREADS) pp_pamp_batch_kernel(const pp_pamp_args *args, const int2 *map)
+{
+  const int2 w = map[blockIdx.x];
+  const pp_pamp_args A = args[w.x];
+  pp_pamp_body<KD, MIXED>(A, w.y);
 }
 
 // ------------------------------------------------------------------ Expokit
@@ -1025,7 +1047,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_expokit_kernel(pp_pamp_args A, 
     }
     nrm = block_sum(nrm, wsum);
     if (tid == 0) xst(mypart + P0NRM, nrm);
-    grid_barrier(A.bar, ep);
+    grid_barrier(A.bar, ep, L, a);
     double beta = sqrt(reduce1(P0NRM));
     int nmult = 0, nexph = 0, nstep = 0, nreject = 0, mbrkdwn = m, k1 = 2, iflag = 0;
     double t_now = 0.0, err_loc = 0.0, t_step = 0.0, avnorm = 0.0, xmloc = 1.0 / m;
@@ -1042,16 +1064,16 @@ __global__ void __launch_bounds__(PP_THREADS) pp_expokit_kernel(pp_pamp_args A, 
       }
       for (int e = tid; e < mh * mh; e += PP_THREADS) Hm[e] = zmk(0.0, 0.0);
       exchange_y();
-      grid_barrier(A.bar, ep);
+      grid_barrier(A.bar, ep, L, a);
       k1 = 2;
       mbrkdwn = m;
       for (int j = 1; j <= m; j++) {
         nmult++;
-        pp_matvec<KD, MIXED>(A, M, step, fac, pp_nostamp());
+        pp_matvec<KD, MIXED>(A, M, a, step, fac, pp_nostamp());
         // first pass: h1 = V^H z
         fiber_dots(Va, W, j, N, false, hred);
         for (int c = tid; c < 2 * j; c += PP_THREADS) xst(mypart + D1 + c, hred[c]);
-        grid_barrier(A.bar, ep);
+        grid_barrier(A.bar, ep, L, a);
         grid_partial_sum(A.part, A.pstr, D1, 2 * j, L, hred, scr);
         for (int c = tid; c < 2 * j; c += PP_THREADS) hacc[c] = hred[c];
         for (int i = tid; i < N; i += PP_THREADS) {
@@ -1063,7 +1085,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_expokit_kernel(pp_pamp_args A, 
         // second pass: h2 = V^H z', |z'|^2
         fiber_dots(Va, W, j, N, true, hred);
         for (int c = tid; c <= 2 * j; c += PP_THREADS) xst(mypart + D2 + c, hred[c]);
-        grid_barrier(A.bar, ep);
+        grid_barrier(A.bar, ep, L, a);
         grid_partial_sum(A.part, A.pstr, D2, 2 * j + 1, L, hred, scr);
         double h2n = 0.0;
         for (int k = 0; k < j; k++) h2n += hred[2 * k] * hred[2 * k] + hred[2 * k + 1] * hred[2 * k + 1];
@@ -1089,16 +1111,16 @@ __global__ void __launch_bounds__(PP_THREADS) pp_expokit_kernel(pp_pamp_args A, 
           X[i] = v;
         }
         exchange_y();
-        grid_barrier(A.bar, ep);
+        grid_barrier(A.bar, ep, L, a);
       }
       if (k1 != 0) {   // the extra matvec of the 2-corrected error estimate
         nmult++;
-        pp_matvec<KD, MIXED>(A, M, step, fac, pp_nostamp());
+        pp_matvec<KD, MIXED>(A, M, a, step, fac, pp_nostamp());
         double v = 0.0;
         for (int i = tid; i < N; i += PP_THREADS) v += zabs2(W[i]);
         v = block_sum(v, wsum);
         if (tid == 0) xst(mypart + ANRM, v);
-        grid_barrier(A.bar, ep);
+        grid_barrier(A.bar, ep, L, a);
         avnorm = sqrt(reduce1(ANRM));
       }
       if (tid == 0) Hm[(m + 1) + m * mh] = zmk(1.0, 0.0);
@@ -1138,7 +1160,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_expokit_kernel(pp_pamp_args A, 
       }
       nw = block_sum(nw, wsum);
       if (tid == 0) xst(mypart + NRM, nw);
-      grid_barrier(A.bar, ep);
+      grid_barrier(A.bar, ep, L, a);
       beta = sqrt(reduce1(NRM));
       t_new = pp_roundstep(0.9 * t_step * pow(t_step * tol / err_loc, xmloc));
       err_loc = fmax(err_loc, rndoff);
@@ -1322,10 +1344,17 @@ extern "C" int pp_lowrank_factor(int P0, long M, const double *V, int maxrank, i
     Q.insert(Q.end(), q.begin(), q.end());
     piv.push_back(best);
     rank++;
+    // project the new direction out of every residual row twice (one pass
+    // leaves ~eps*sqrt(M) of the row behind, which would pass for rank);
+    // the pivot row lies in the span exactly
     for (int r = 0; r < R2; r++) {
-      double d = 0.0;
-      for (long e = 0; e < M; e++) d += q[e] * res[(size_t)r * M + e];
-      for (long e = 0; e < M; e++) res[(size_t)r * M + e] -= d * q[e];
+      double *rr = res.data() + (size_t)r * M;
+      if (r == best) { std::fill(rr, rr + M, 0.0); continue; }
+      for (int pass = 0; pass < 2; pass++) {
+        double d = 0.0;
+        for (long e = 0; e < M; e++) d += q[e] * rr[e];
+        for (long e = 0; e < M; e++) rr[e] -= d * q[e];
+      }
     }
   }
   if (rank == 0) {   // zero potential
@@ -1503,6 +1532,74 @@ extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re
     rc = pamp_launch(p->mixed ? (const void *)pp_pamp_kernel<KD_, true> : (const void *)pp_pamp_kernel<KD_, false>,
                      p->L, args, lds, stream); })
   return rc;
+}
+
+// Many independent instances (each a pp_pamp with its own operator, overlap
+// and psi) advanced nsteps steps each.  Instances are packed in order into
+// cooperative launches whose workgroups (one per fiber) are all co-resident;
+// a batch larger than the GPU holds at once becomes several launches on the
+// stream.  dt: ninst x 2 (re, im); scal[i]: host nsteps x nterms_i.  All
+// instances need the same B-spline order and precision form.
+extern "C" int pp_pamp_batch_propagate(int ninst, pp_pamp *const *inst, void *const *psi, int nsteps,
+                                       const double *dt, const double *const *scal, hipStream_t stream)
+{
+  if (ninst < 0 || (ninst > 0 && (!inst || !psi || !dt || !scal)) || nsteps < 0) return PP_EARG;
+  if (ninst == 0 || nsteps == 0) return PP_OK;
+  const int kd = inst[0]->ov->kd;
+  const bool mixed = inst[0]->mixed != 0;
+  for (int i = 0; i < ninst; i++) {
+    if (!inst[i] || !psi[i] || inst[i]->nterms == 0 || !scal[i]) return PP_EARG;
+    if (inst[i]->ov->kd != kd || (inst[i]->mixed != 0) != mixed) return PP_EUNSUP;
+  }
+  std::vector<pp_pamp_args> args(ninst);
+  std::vector<size_t> lds(ninst);
+  for (int i = 0; i < ninst; i++) {
+    int rc = pamp_prepare(inst[i], psi[i], nsteps, dt[2 * i], dt[2 * i + 1], scal[i], stream, args[i]);
+    if (rc) return rc;
+    lds[i] = pamp_lds_bytes(inst[i], args[i].padN);
+    if (lds[i] > 160 * 1024) return PP_EUNSUP;
+  }
+  const void *fn = nullptr;
+  PP_KD_SWITCH(kd, { fn = mixed ? (const void *)pp_pamp_batch_kernel<KD_, true> : (const void *)pp_pamp_batch_kernel<KD_, false>; })
+  if (!fn) return PP_EUNSUP;
+  int dev = 0, ncu = 0, reg_per_cu = 0;
+  PP_HIP(hipGetDevice(&dev));
+  PP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  // workgroups per CU: the register limit from the runtime, the LDS limit
+  // (160 KB per CU) from the group's largest allocation
+  PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&reg_per_cu, fn, PP_THREADS, 0));
+  int i0 = 0;
+  while (i0 < ninst) {
+    // grow the group while all its workgroups fit on the GPU at once
+    size_t lmax = 0;
+    int wgs = 0, i1 = i0;
+    while (i1 < ninst) {
+      const size_t l2 = std::max(lmax, lds[i1]);
+      const int per_cu = std::min(reg_per_cu, (int)((160 * 1024) / l2));
+      if (wgs + inst[i1]->L > per_cu * ncu) break;
+      lmax = l2;
+      wgs += inst[i1]->L;
+      i1++;
+    }
+    if (i1 == i0) return PP_EUNSUP;   // one instance alone does not fit
+    std::vector<int2> map;
+    map.reserve(wgs);
+    for (int i = i0; i < i1; i++)
+      for (int a = 0; a < inst[i]->L; a++) map.push_back(make_int2(i - i0, a));
+    pp_pamp_args *dargs = nullptr;
+    int2 *dmap = nullptr;
+    PP_HIP(hipMallocAsync((void **)&dargs, sizeof(pp_pamp_args) * (i1 - i0), stream));
+    PP_HIP(hipMallocAsync((void **)&dmap, sizeof(int2) * map.size(), stream));
+    PP_HIP(hipMemcpyAsync(dargs, args.data() + i0, sizeof(pp_pamp_args) * (i1 - i0), hipMemcpyHostToDevice, stream));
+    PP_HIP(hipMemcpyAsync(dmap, map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice, stream));
+    PP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lmax));
+    void *kargs[] = {&dargs, &dmap};
+    PP_HIP(hipLaunchCooperativeKernel(fn, dim3(wgs), dim3(PP_THREADS), kargs, (unsigned)lmax, stream));
+    PP_HIP(hipFreeAsync(dargs, stream));
+    PP_HIP(hipFreeAsync(dmap, stream));
+    i0 = i1;
+  }
+  return PP_OK;
 }
 
 // Expokit form of the same engine: nsteps ExpokitPropagator.AdvanceStep's

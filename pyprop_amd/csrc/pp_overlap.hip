@@ -151,10 +151,23 @@ void pp_plan_free(pp_band_plan *pl)
 // ---------------------------------------------------------------- overlap object
 // Takes S in upper hermitian band storage (host, complex interleaved, N*(kd+1)),
 // factorises it (overlapmatrix.cpp:35-37) and uploads U plus both solve plans.
+extern "C" int pp_overlap_create_chunked(int N, int kd, const double *S_upper, int B, pp_overlap **out, int *info);
+extern "C" int pp_overlap_destroy(pp_overlap *ov);
+
 extern "C" int pp_overlap_create(int N, int kd, const double *S_upper, pp_overlap **out, int *info)
 {
-  if (!out || !S_upper || N < 1 || kd < 1) return PP_EARG;
-  if (!(kd == 4 || kd == 5 || kd == 6 || kd == 8) || pp_plan_chunk(N, kd) == 0) return PP_EUNSUP;
+  return pp_overlap_create_chunked(N, kd, S_upper, 0, out, info);
+}
+
+// Same with an explicit solver chunk length B (a power of two >= max(kd, 8),
+// at most 64 chunks); B = 0 picks the default of pp_plan_chunk.  The
+// autotuner times the candidates.
+extern "C" int pp_overlap_create_chunked(int N, int kd, const double *S_upper, int B, pp_overlap **out, int *info)
+{
+  if (!out || !S_upper || N < 1 || kd < 1 || B < 0) return PP_EARG;
+  if (!(kd == 4 || kd == 5 || kd == 6 || kd == 8)) return PP_EUNSUP;
+  if (B == 0) B = pp_plan_chunk(N, kd);
+  if (B == 0 || B < kd || B < 8 || (B & (B - 1)) || (N + B - 1) / B > 64) return PP_EUNSUP;
   std::vector<double> ab(S_upper, S_upper + (size_t)2 * N * (kd + 1));
   int inf = pp_band_cholesky_upper(N, kd, ab.data());
   if (info) *info = inf;
@@ -169,8 +182,8 @@ extern "C" int pp_overlap_create(int N, int kd, const double *S_upper, pp_overla
   ov->N = N; ov->kd = kd;
   int rc;
   if (hipMalloc(&ov->Uband, U.size() * sizeof(double)) != hipSuccess) { delete ov; return PP_EHIP; }
-  if (hipMemcpy(ov->Uband, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { delete ov; return PP_EHIP; }
-  if ((rc = pp_plan_build(&ov->plan, N, kd, U.data(), pp_plan_chunk(N, kd))) != PP_OK) return rc;
+  if (hipMemcpy(ov->Uband, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { pp_overlap_destroy(ov); return PP_EHIP; }
+  if ((rc = pp_plan_build(&ov->plan, N, kd, U.data(), B)) != PP_OK) { pp_overlap_destroy(ov); return rc; }
   *out = ov;
   return PP_OK;
 }

@@ -37,23 +37,32 @@ def ptr(tensor):
     return ctypes.c_void_p(tensor.data_ptr())
 
 
+# Solver chunk length per (N, kd) chosen by the autotuner (batch.autotune_chunks);
+# shapes without an entry use the library default.
+CHUNK_TABLE = {}
+
+
 class Overlap:
     """Banded overlap S = U^H U of one B-spline rank, factorised on creation
-    (OverlapMatrix::Setup, overlapmatrix.cpp:9-104)."""
+    (OverlapMatrix::Setup, overlapmatrix.cpp:9-104).  chunk: solver chunk
+    length (None: the tuned value for (N, kd) if any, else the default)."""
 
-    def __init__(self, S_upper, N, kd):
+    def __init__(self, S_upper, N, kd, chunk=None):
         L = nat.lib()
         S = np.ascontiguousarray(S_upper, dtype=np.complex128)
         if S.size != N * (kd + 1):
             raise ValueError("overlap band has %d entries, expected %d" % (S.size, N * (kd + 1)))
+        if chunk is None:
+            chunk = CHUNK_TABLE.get((N, kd), 0)
         h = ctypes.c_void_p()
         info = ctypes.c_int(0)
-        rc = L.pp_overlap_create(N, kd, nat.dptr(S.view(np.float64)), ctypes.byref(h), ctypes.byref(info))
+        rc = L.pp_overlap_create_chunked(N, kd, nat.dptr(S.view(np.float64)), int(chunk), ctypes.byref(h),
+                                         ctypes.byref(info))
         if rc > 0:
             raise RuntimeError("overlap matrix is not positive definite (zpbtrf info=%d)" % info.value)
         nat.check(rc, "pp_overlap_create")
         self.h = h
-        self.N, self.kd = N, kd
+        self.N, self.kd, self.chunk = N, kd, chunk
 
     def factor(self):
         U = np.zeros(self.N * (self.kd + 1))
@@ -183,3 +192,21 @@ class PampEngine:
         if getattr(self, "h", None) and nat._lib is not None:
             nat._lib.pp_pamp_destroy(self.h)
             self.h = None
+
+
+def pamp_batch_propagate(engines, psis, nsteps, dts, scalings):
+    """nsteps steps of many independent engines in packed cooperative
+    launches (pp_pamp_batch_propagate).  scalings[i]: (nsteps, nterms_i)."""
+    n = len(engines)
+    if not (len(psis) == len(dts) == len(scalings) == n):
+        raise ValueError("engines, psis, dts and scalings must have the same length")
+    H = (ctypes.c_void_p * n)(*[e.h.value for e in engines])
+    Pp = (ctypes.c_void_p * n)(*[ptr(p).value for p in psis])
+    dt = np.ascontiguousarray([[complex(d).real, complex(d).imag] for d in dts], dtype=np.float64)
+    keep = [np.ascontiguousarray(np.asarray(sc, dtype=np.float64).reshape(nsteps, len(e.terms)))
+            for e, sc in zip(engines, scalings)]
+    S = (ctypes.POINTER(ctypes.c_double) * n)(*[nat.dptr(k) for k in keep])
+    nat.check(nat.lib().pp_pamp_batch_propagate(n, H, Pp, int(nsteps), nat.dptr(dt), S, stream_ptr()),
+              "pp_pamp_batch_propagate")
+    for e in engines:
+        e._last = nsteps

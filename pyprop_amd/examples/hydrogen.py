@@ -50,6 +50,32 @@ def load(name, overrides=None):
     return conf
 
 
+def batch_instance_config(draw, krylov=16, dt=0.01, precision="mixed", dr=0.5):
+    """Config of one instance of the batched scan: hydrogen_base.ini with the
+    drawn radial size N (linear grid, r in [0, dr*N]), lmax and pulse, Krylov
+    dimension 16 and dt = 0.01; precision "mixed" or "fp64"."""
+    import pyprop_amd
+    register()
+    k = 7
+    extra = """
+[AngularRepresentation]
+maxl = {lmax}
+[RadialRepresentation]
+xmax = {rmax!r}
+xsize = {xsize}
+[PulseParameters]
+frequency = {omega!r}
+cycles = {cycles}
+field_strength = {E0!r}
+[Propagation]
+krylov_basis_size = {krylov}
+timestep = {dt!r}
+matvec_precision = "{precision}"
+""".format(lmax=draw["lmax"], rmax=dr * draw["N"], xsize=draw["N"] - k + 4, omega=draw["omega"],
+           cycles=draw["cycles"], E0=draw["E0"], krylov=krylov, dt=dt, precision=precision)
+    return pyprop_amd.Load(os.path.join(CONFIGS, "hydrogen_base.ini"), extraText=extra)
+
+
 def _block(pot, l_index, N, bw):
     s = (bw - 1) // 2
     band = pot.PotentialData[l_index]
@@ -68,22 +94,35 @@ def GroundStateInverseIteration(psi, conf):
 
 
 def ground_state(problem, l=0, shift=-0.5 + 1e-3, iterations=5):
+    """Inverse iteration (H0_l - shift S) x_{k+1} = S x_k on the l block of the
+    static Diag x BandNH potentials, S-normalised, phase of the largest
+    component real (analysis.py:134-138); banded LU of width 2(k-1)."""
+    import scipy.linalg
     prop = problem.Propagator.BasePropagator
     repr = problem.psi.GetRepresentation()
     L, N = repr.GetFullShape()
     bs = repr.GetRepresentation(1).GetBSplineObject()
+    kd = bs.MaxSplineOrder - 1
     S = bs.overlap_dense()
-    H = np.zeros((N, N), dtype=np.complex128)
     li = l - repr.GetRepresentation(0).M
+    bw = 2 * kd + 1
+    H = np.zeros(N * bw, dtype=np.complex128)
     for pot in prop.PotentialList:
         if pot.IsTimeDependent or pot.StorageIds() != ["Diag", "BandNH"]:
             continue
-        bw = pot.PotentialData.shape[1] // N
-        H += _block(pot, li, N, bw)
+        if pot.PotentialData.shape[1] != N * bw:
+            raise ValueError("radial band width differs from the overlap's")
+        H += pot.PotentialData[li]
+    # BandNH (j*bw + kd + i - j) -> LAPACK general band ab[kd + i - j, j]
+    ab = H.reshape(N, bw).T.copy()
+    Sb = np.zeros_like(ab)
+    for dgl in range(-kd, kd + 1):
+        i = np.arange(max(0, -dgl), min(N, N - dgl))
+        Sb[kd - dgl, i + dgl] = S[i, i + dgl]
+    M = ab - shift * Sb
     x = np.ones(N, dtype=np.complex128)
-    M = H - shift * S
     for _ in range(iterations):
-        x = np.linalg.solve(M, S @ x)
+        x = scipy.linalg.solve_banded((kd, kd), M, S @ x)
         x /= np.sqrt(np.real(np.conj(x) @ S @ x))
     i = np.argmax(np.abs(x))
     x *= np.abs(x[i]) / x[i]

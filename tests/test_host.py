@@ -96,3 +96,29 @@ def test_tensor_multiply_name_dispatch():
     assert f.__name__ == "TensorPotentialMultiply_Simp_BandNH"
     with pytest.raises(AttributeError):
         core.TensorPotentialMultiply_Bogus_BandNH
+
+
+def test_lowrank_factor_finds_exact_rank_of_batch_operators():
+    """The library's host factorisation (pp_lowrank_factor) of assembled
+    hydrogen operators: rank 2 for the static and for the dipole tensor of
+    batch instances with 500-600 B-splines (regression: one-pass residual
+    updates used to leave ~1e-13 of the pivot rows and inflate the rank to
+    the cap), reconstruction within 1e-13 of max|V|."""
+    import ctypes
+    from pyprop_amd import _native as nat
+    from pyprop_amd import batch as Bt
+    lib = nat.lib()
+    for d in Bt.instance_draws(3):
+        conf = H.batch_instance_config(d)
+        repr = CreateRepresentation(conf)
+        bp = P.BasisPropagator(Wavefunction(repr))
+        conf.Apply(bp)
+        for pot in bp.PotentialList:
+            V = np.ascontiguousarray(pot.PotentialData, dtype=np.complex128)
+            P0, M = V.shape
+            mr = min(2 * P0, 16)
+            coef, bands = np.zeros(P0 * mr * 2), np.zeros(mr * M)
+            rk, err = ctypes.c_int(), ctypes.c_double()
+            rc = lib.pp_lowrank_factor(P0, M, nat.dptr(V.view(np.float64)), mr, ctypes.byref(rk), nat.dptr(coef),
+                                       nat.dptr(bands), ctypes.byref(err))
+            assert rc == 0 and rk.value == 2 and err.value < 1e-13, (d, pot.StorageIds(), rk.value, err.value)
