@@ -53,7 +53,7 @@ def run_batch(args):
     count = int(args.workload[len("batch"):] or 512)
     draws = Bt.instance_draws(count)
     t0 = time.perf_counter()
-    tuned = Bt.autotune_chunks([d["N"] for d in draws], kd=6)
+    tuned = Bt.autotune_chunks([(d["N"], d["lmax"] + 1) for d in draws], kd=6)
     tune_s = time.perf_counter() - t0
     print("autotuned %d radial sizes in %.1f s" % (len(tuned), tune_s), file=sys.stderr, flush=True)
     probs, psi0s = [], []
@@ -142,7 +142,91 @@ def run_batch(args):
         "mixed_vs_fp64": {"psi_diff_snorm_max": max(dn), "psi_diff_snorm_median": float(np.median(dn)),
                           "population_error_max_sample16": max(perr), "sample": sample},
         "autotune": {"sizes": len(tuned), "seconds": tune_s,
-                     "chosen": {str(c): sum(1 for b, _ in tuned.values() if b == c) for c in (16, 32, 64)}},
+                     "chosen": {str(c): sum(1 for b, _, _ in tuned.values() if b == c) for c in (16, 32, 64)},
+                     "two_workgroups_per_cu": sum(1 for _, _, o in tuned.values() if o >= 2)},
+    }
+    print(json.dumps(line))
+
+
+def adjoint_samples(K, dt, seed=20261203, E0=0.05, omega=0.3, sigma=0.005):
+    """Control samples of the adjoint workload: the interaction time factor
+    (velocity-gauge vector potential) at t_k = k dt, a sin^2-enveloped carrier
+    A0 sin^2(pi t/T) (-cos(omega t)), A0 = E0/omega, T = K dt, plus seeded
+    Gaussian noise of sigma on every sample."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t = dt * np.arange(K)
+    T = dt * K
+    return (E0 / omega) * np.sin(np.pi * t / T) ** 2 * (-np.cos(omega * t)) + sigma * rng.standard_normal(K)
+
+
+def run_adjoint(args):
+    """Adjoint gradient (config 4): hydrogen N=400, l=0..31, 4000 control
+    samples, objective the final 2p population; dP/ds_k for every sample by
+    the adjoint sweep (checkpoints every 50 steps), checked against central
+    differences h=1e-6 on 32 seeded samples, evaluated as one batched launch
+    set of 64 forward runs."""
+    import numpy as np
+    import torch
+    import pyprop_amd
+    from pyprop_amd import device as D
+    from pyprop_amd.adjoint import PopulationAdjoint
+    from pyprop_amd.examples import hydrogen as H
+
+    conf = H.load("hydrogen_adjoint")
+    prob = pyprop_amd.Problem(conf)
+    prob.SetupStep()
+    prob.SetWavefunction(H.ground_state(prob))
+    target, _ = H.bound_state(prob, 1, 0)
+    dt = prob.TimeStep.real
+    K = args.steps
+    s = adjoint_samples(K, dt)
+    adj = PopulationAdjoint(prob, target, checkpoint=50)
+    adj.gradient(s[:60])   # warm up both engines
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    P, g = adj.gradient(s)
+    grad_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    adj.population(s)
+    fwd_s = time.perf_counter() - t0
+    # finite differences: 64 forward runs (32 samples, +-h) as one batch
+    h = 1e-6
+    rng = np.random.default_rng(7)
+    idx = np.sort(rng.choice(K, size=32, replace=False))
+    eng0 = prob.Propagator.Engine
+    pots = prob.Propagator._pots
+    engines, psis, scs = [], [], []
+    for k in idx:
+        for sgn in (1.0, -1.0):
+            e = D.PampEngine(eng0.L, eng0.N, eng0.m, eng0.overlap)
+            e.add_term("Diag", pots[0].DeviceData())
+            e.add_term("Simp", pots[1].DeviceData(), pairs=pots[1].GeometryList[0].GetBasisPairs())
+            sv = s.copy()
+            sv[k] += sgn * h
+            engines.append(e)
+            psis.append(adj.psi0.clone())
+            scs.append(np.stack([np.ones(K), sv], axis=1))
+    t0 = time.perf_counter()
+    D.pamp_batch_propagate(engines, psis, K, [dt] * len(engines), scs)
+    torch.cuda.synchronize()
+    fd_s = time.perf_counter() - t0
+    pops = [abs(adj._sdot(adj.target, p)) ** 2 for p in psis]
+    fd = np.array([(pops[2 * i] - pops[2 * i + 1]) / (2 * h) for i in range(len(idx))])
+    err = np.abs(g[idx] - fd)
+    line = {
+        "metric": "Krylov steps/s + matvec HBM GB/s, FP64 vs mixed at <=1e-8 population error",
+        "value": K / grad_s, "unit": "adjoint-sweep steps/s",
+        "n_gpus": 1, "steps": K, "warmup": 60, "ms_per_step": 1e3 * grad_s / K,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic (seeded control samples, 1s start state by inverse iteration)",
+        "config": {"workload": "hydrogen_adjoint", "n": adj.L * adj.N, "L": adj.L, "N": adj.N,
+                   "krylov_dim": adj.m, "dt": dt, "checkpoint": 50, "objective": "final 2p population"},
+        "population_2p": P, "forward_s": fwd_s, "gradient_s": grad_s,
+        "fd_check": {"samples": len(idx), "h": h, "max_abs_err": float(err.max()),
+                     "max_rel_err_vs_max_grad": float(err.max() / np.abs(g).max()),
+                     "batched_forward_runs": len(engines), "seconds": fd_s},
+        "grad_norm": float(np.linalg.norm(g)),
     }
     print(json.dumps(line))
 
@@ -150,7 +234,8 @@ def run_batch(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: the workload's own count; 20000 for hydrogen_large)")
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="hydrogen_large")
     ap.add_argument("--cpu-steps", type=int, default=100)
@@ -160,6 +245,8 @@ def main():
                     help="run one plain device copy of this many MiB before the workload, so that a "
                          "rocprofv3 --pmc run can scale FETCH_SIZE/WRITE_SIZE by a copy of known size")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = {"hydrogen_adjoint": 4000}.get(args.workload, 20 if args.workload.startswith("batch") else 20000)
     if args.gpus != 1:
         print("bench.py: this build runs one GPU holding the whole wavefunction; "
               "--gpus %d is not supported (sharding is design-only)" % args.gpus, file=sys.stderr)
@@ -167,6 +254,8 @@ def main():
 
     if args.workload.startswith("batch"):
         return run_batch(args)
+    if args.workload == "hydrogen_adjoint":
+        return run_adjoint(args)
 
     import numpy as np
     import torch

@@ -129,6 +129,11 @@ int pp_expokit_diagnostics(pp_pamp *p, int nsteps, double *stats, pp_stream stre
  * stream.  Engines must share the B-spline order and the precision form. */
 int pp_pamp_batch_propagate(int ninst, pp_pamp *const *inst, void *const *psi, int nsteps, const double *dt,
                             const double *const *scal, pp_stream stream);
+/* LDS bytes per workgroup a pAMP launch of an (L, N) problem with Krylov
+ * dimension m, nslots coupled fibers per row, kd super-diagonals and solver
+ * chunk length B needs (-1: invalid B).  No reference counterpart: the batch
+ * autotuner uses it to keep two workgroups per CU. */
+long pp_pamp_lds_estimate(int L, int N, int m, int nslots, int kd, int B);
 /* Mixed-precision form (no reference counterpart; the reference is FP64
  * throughout): later launches apply the terms whose bit is set in term_mask
  * (bit t = t-th added term) in complex FP32 (FP32 radial bands, FP32 band

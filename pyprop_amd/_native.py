@@ -26,7 +26,7 @@ OV_SOLVE_SQRT, OV_SOLVE_SQRT_CONJ, OV_MUL_SQRT, OV_MUL_SQRT_CONJ, OV_SOLVE, OV_M
 EXPORTS = [
     "pp_band_cholesky_upper", "pp_overlap_create", "pp_overlap_create_chunked", "pp_overlap_factor", "pp_overlap_destroy",
     "pp_overlap_apply", "pp_tp_multiply", "pp_pamp_create", "pp_pamp_add_term", "pp_pamp_add_factored_term",
-    "pp_lowrank_factor", "pp_pamp_propagate", "pp_pamp_diagnostics", "pp_pamp_set_mixed", "pp_expokit_propagate", "pp_expokit_diagnostics", "pp_pamp_batch_propagate", "pp_pamp_set_timing", "pp_pamp_timings", "pp_pamp_destroy",
+    "pp_lowrank_factor", "pp_pamp_propagate", "pp_pamp_diagnostics", "pp_pamp_set_mixed", "pp_expokit_propagate", "pp_expokit_diagnostics", "pp_pamp_batch_propagate", "pp_pamp_lds_estimate", "pp_pamp_set_timing", "pp_pamp_timings", "pp_pamp_destroy",
 ]
 
 _lib = None
@@ -76,8 +76,10 @@ def lib():
     L.pp_pamp_set_timing.argtypes = [P, I]
     L.pp_pamp_timings.argtypes = [P, ctypes.POINTER(ctypes.c_longlong), I, ip]
     L.pp_pamp_destroy.argtypes = [P]
+    L.pp_pamp_lds_estimate.argtypes = [I, I, I, I, I, I]
     for name in EXPORTS:
         getattr(L, name).restype = ctypes.c_int
+    L.pp_pamp_lds_estimate.restype = ctypes.c_long
     _lib = L
     return L
 

@@ -28,30 +28,35 @@ def instance_draws(count=512, seed=20261203):
             for i in range(count)]
 
 
-def autotune_chunks(shapes, kd, candidates=(16, 32, 64), fibers=64, reps=10):
-    """Pick the band-solver chunk length per radial size N by timing the
-    overlap solves U^-1 and U^-H on a (fibers, N) tensor for each candidate
-    (HIP events); records the winners in device.CHUNK_TABLE and returns
-    {N: (best chunk, {chunk: microseconds per solve pair})}."""
+def autotune_chunks(shapes, kd, m=16, nslots=3, candidates=(16, 32, 64), fibers=64, reps=10):
+    """Pick the band-solver chunk length per radial size N.  shapes: (N, L)
+    of the problems.  A candidate must keep the step kernel's LDS within the
+    GPU's 160 KB per CU; among the candidates that give the most workgroups
+    per CU (at most two: the kernel's register use) the fastest overlap solve
+    pair U^-1, U^-H on a (fibers, N) tensor wins (HIP events).  Records the
+    winners in device.CHUNK_TABLE; returns {N: (chunk, {chunk: us}, wg/CU)}."""
     torch = dev.torch()
     from .bspline import BSplineBasis, LinearBreakpointSequence
+    lib = nat.lib()
+    Lmax = {}
+    for N, L in shapes:
+        Lmax[int(N)] = max(Lmax.get(int(N), 0), int(L))
     result = {}
-    for N in sorted(set(int(n) for n in shapes)):
-        # any SPD band of the right shape exercises the same code path; the
-        # B-spline overlap of a linear grid is the one the problems use
+    for N in sorted(Lmax):
+        occ = {}
+        for B in candidates:
+            lds = lib.pp_pamp_lds_estimate(Lmax[N], N, m, nslots, kd, B)
+            if lds > 0 and lds <= 160 * 1024:
+                occ[B] = min(2, (160 * 1024) // lds)
+        if not occ:
+            continue
+        best_occ = max(occ.values())
         k = kd + 1
         basis = BSplineBasis(LinearBreakpointSequence(0.0, 0.5 * N, N - k + 4), k, "zero")
         S = basis.overlap_upper()
-        x = torch.zeros((fibers, N), dtype=torch.complex128, device=dev.device())
-        x[:, :] = 1.0
+        x = torch.ones((fibers, N), dtype=torch.complex128, device=dev.device())
         times = {}
-        for B in candidates:
-            # the step kernels keep the scan matrices of both directions in LDS
-            # (2 levels kd^2 P doubles, at most 72 KB, as the library default)
-            P = -(-N // B)
-            levels = int(np.ceil(np.log2(P))) if P > 1 else 0
-            if B < kd or P > 64 or 2 * levels * kd * kd * P * 8 > 72 * 1024:
-                continue
+        for B in sorted(b for b in occ if occ[b] == best_occ):
             try:
                 ov = dev.Overlap(S, N, kd, chunk=B)
             except nat.NativeError:
@@ -71,7 +76,7 @@ def autotune_chunks(shapes, kd, candidates=(16, 32, 64), fibers=64, reps=10):
             continue
         best = min(times, key=times.get)
         dev.CHUNK_TABLE[(N, kd)] = best
-        result[N] = (best, times)
+        result[N] = (best, times, best_occ)
     return result
 
 

@@ -36,17 +36,21 @@
 
 #define PP_MAXTERMS 8
 #define PP_MAXM 64
-#define PP_MAXSLOTS 4
+#define PP_MAXSLOTS 8    // distinct source fibers per row (all terms)
+#define PP_MAXENT 4      // couplings per row of one term
 #define PP_MAXCOMBO 16
 #ifndef PP_THREADS
 #define PP_THREADS 256   // threads per workgroup (one workgroup per fiber)
+#endif
+#ifndef PP_VUNROLL
+#define PP_VUNROLL 4     // Krylov-vector loads in flight per thread in the dot and projection loops
 #endif
 #ifndef PP_SOLVE_THREADS
 #define PP_SOLVE_THREADS (PP_THREADS < 256 ? PP_THREADS : 256)   // threads in the band solves
 #endif
 #define PP_TLOG_MAX 512
 // LDS doubles of the per-row combo table (coefficients, slots, counts, terms)
-#define PP_CMB_DOUBLES (PP_MAXCOMBO * PP_MAXSLOTS * 5 + 2 * PP_MAXCOMBO)
+#define PP_CMB_DOUBLES (PP_MAXCOMBO * PP_MAXENT * 5 + 2 * PP_MAXCOMBO)
 
 struct pp_fterm {
   const int *rowptr;   // L+1
@@ -203,14 +207,14 @@ __device__ inline void fiber_dots(const zd *V, const zd *W, int nk, int N, bool 
     double re = 0.0, im = 0.0;
     if (k < nk) {
       const zd *vk = V + (size_t)k * N;
-#pragma unroll 4
+#pragma unroll PP_VUNROLL
       for (int i = lane; i < N; i += 64) {
         zd v = vk[i], w = W[i];
         re = fma(v.x, w.x, re); re = fma(v.y, w.y, re);
         im = fma(v.x, w.y, im); im = fma(-v.y, w.x, im);
       }
     } else {
-#pragma unroll 4
+#pragma unroll PP_VUNROLL
       for (int i = lane; i < N; i += 64) { zd w = W[i]; re = fma(w.x, w.x, re); re = fma(w.y, w.y, re); }
     }
     re = wave_sum(re);
@@ -419,12 +423,12 @@ __device__ __attribute__((always_inline)) inline pp_lds_map pp_map_lds(const pp_
   const int padN = A.padN, m = A.m;
   M.nl = lds_d;
   M.cmb_c = (zd *)(lds_d + A.nlevEven);
-  M.cmb_q = (int *)(M.cmb_c + PP_MAXCOMBO * PP_MAXSLOTS);
-  M.cmb_n = M.cmb_q + PP_MAXCOMBO * PP_MAXSLOTS;
+  M.cmb_q = (int *)(M.cmb_c + PP_MAXCOMBO * PP_MAXENT);
+  M.cmb_n = M.cmb_q + PP_MAXCOMBO * PP_MAXENT;
   M.cmb_t = M.cmb_n + PP_MAXCOMBO;
   M.cur_c = (zd *)(M.cmb_t + PP_MAXCOMBO);
-  M.cur_q = (int *)(M.cur_c + PP_MAXCOMBO * PP_MAXSLOTS);
-  M.cur_n = M.cur_q + PP_MAXCOMBO * PP_MAXSLOTS;
+  M.cur_q = (int *)(M.cur_c + PP_MAXCOMBO * PP_MAXENT);
+  M.cur_n = M.cur_q + PP_MAXCOMBO * PP_MAXENT;
   M.W = (zd *)(lds_d + A.nlevEven + PP_CMB_DOUBLES + extra);   // extra: kernel-specific persistent doubles
   M.X = M.W + padN;
   M.tb = M.X + padN;
@@ -456,8 +460,8 @@ __device__ __attribute__((always_inline)) inline void pp_load_persistent(const p
         for (int e = e0; e < e1; e++, ne++) {
           zd c = T.coef[(size_t)T.ent[3 * e + 1] * T.rank + r];
           if (T.ent[3 * e + 2]) c = zconj(c);
-          M.cmb_c[k * PP_MAXSLOTS + ne] = c;
-          M.cmb_q[k * PP_MAXSLOTS + ne] = T.ent[3 * e];
+          M.cmb_c[k * PP_MAXENT + ne] = c;
+          M.cmb_q[k * PP_MAXENT + ne] = T.ent[3 * e];
         }
         M.cmb_n[k] = ne;
         M.cmb_t[k] = t;
@@ -489,10 +493,10 @@ __device__ __attribute__((always_inline)) inline void pp_matvec(const pp_pamp_ar
             for (int i = tid; i < N; i += PP_THREADS) Ys[q * N + i] = zld(A.y + (size_t)col * N + i);
         }
         // coefficients times the terms' time factors at t_n and the factor fac
-        for (int e = tid; e < A.ncombo * PP_MAXSLOTS; e += PP_THREADS) {
-          cur_c[e] = zmul(cmb_c[e], zscale(fac, A.scal[(size_t)step * A.nterms + cmb_t[e / PP_MAXSLOTS]]));
+        for (int e = tid; e < A.ncombo * PP_MAXENT; e += PP_THREADS) {
+          cur_c[e] = zmul(cmb_c[e], zscale(fac, A.scal[(size_t)step * A.nterms + cmb_t[e / PP_MAXENT]]));
           cur_q[e] = cmb_q[e];
-          if (e % PP_MAXSLOTS == 0) cur_n[e / PP_MAXSLOTS] = cmb_n[e / PP_MAXSLOTS];
+          if (e % PP_MAXENT == 0) cur_n[e / PP_MAXENT] = cmb_n[e / PP_MAXENT];
         }
         __syncthreads();
         // X = sum_k B_k (sum_q c_kq Ys_q): bands diagonal-major, all loads independent
@@ -513,9 +517,9 @@ __device__ __attribute__((always_inline)) inline void pp_matvec(const pp_pamp_ar
                   for (int d = -KD; d <= KD; d++)
                     bv[d + KD] = (interior || (i + d >= 0 && i + d < N)) ? Br[(size_t)(d + KD) * N + i] : 0.f;
 #pragma unroll
-                  for (int q = 0; q < PP_MAXSLOTS; q++) {
+                  for (int q = 0; q < PP_MAXENT; q++) {
                     if (q < ne) {
-                      const zd *Yq = Ys + cur_q[k * PP_MAXSLOTS + q] * N;
+                      const zd *Yq = Ys + cur_q[k * PP_MAXENT + q] * N;
                       float zr = 0.f, zi = 0.f;
 #pragma unroll
                       for (int d = -KD; d <= KD; d++) {
@@ -524,7 +528,7 @@ __device__ __attribute__((always_inline)) inline void pp_matvec(const pp_pamp_ar
                         zr = fmaf(bv[d + KD], (float)yv.x, zr);
                         zi = fmaf(bv[d + KD], (float)yv.y, zi);
                       }
-                      zfma(acc, cur_c[k * PP_MAXSLOTS + q], zmk((double)zr, (double)zi));
+                      zfma(acc, cur_c[k * PP_MAXENT + q], zmk((double)zr, (double)zi));
                     }
                   }
                 } else {
@@ -534,9 +538,9 @@ __device__ __attribute__((always_inline)) inline void pp_matvec(const pp_pamp_ar
                   for (int d = -KD; d <= KD; d++)
                     bv[d + KD] = (interior || (i + d >= 0 && i + d < N)) ? Br[(size_t)(d + KD) * N + i] : 0.0;
 #pragma unroll
-                  for (int q = 0; q < PP_MAXSLOTS; q++) {
+                  for (int q = 0; q < PP_MAXENT; q++) {
                     if (q < ne) {
-                      const zd *Yq = Ys + cur_q[k * PP_MAXSLOTS + q] * N;
+                      const zd *Yq = Ys + cur_q[k * PP_MAXENT + q] * N;
                       zd z = zmk(0.0, 0.0);
 #pragma unroll
                       for (int d = -KD; d <= KD; d++) {
@@ -544,7 +548,7 @@ __device__ __attribute__((always_inline)) inline void pp_matvec(const pp_pamp_ar
                         const zd yv = (interior || (j >= 0 && j < N)) ? Yq[j] : zmk(0.0, 0.0);
                         zfmar(z, bv[d + KD], yv);
                       }
-                      zfma(acc, cur_c[k * PP_MAXSLOTS + q], z);
+                      zfma(acc, cur_c[k * PP_MAXENT + q], z);
                     }
                   }
                 }
@@ -569,10 +573,10 @@ __device__ __attribute__((always_inline)) inline void pp_matvec(const pp_pamp_ar
                 for (int d = -KD; d <= KD; d++)
                   bv[d + KD] = (interior || (i + d >= 0 && i + d < N)) ? Br[(size_t)(d + KD) * N + i] : 0.0;
 #pragma unroll
-                for (int q = 0; q < PP_MAXSLOTS; q++) {
+                for (int q = 0; q < PP_MAXENT; q++) {
                   if (q < ne) {
-                    const zd cq = cur_c[k * PP_MAXSLOTS + q];
-                    const zd *Yq = Ys + cur_q[k * PP_MAXSLOTS + q] * N;
+                    const zd cq = cur_c[k * PP_MAXENT + q];
+                    const zd *Yq = Ys + cur_q[k * PP_MAXENT + q] * N;
                     zd z = zmk(0.0, 0.0);
 #pragma unroll
                     for (int d = -KD; d <= KD; d++) {
@@ -680,7 +684,7 @@ __device__ __attribute__((always_inline)) inline void pp_pamp_body(const pp_pamp
       if (!broken) {
         for (int i = tid; i < N; i += PP_THREADS) {
           zd w = W[i];
-#pragma unroll 4
+#pragma unroll PP_VUNROLL
           for (int k = 0; k <= j; k++) w = zsub(w, zmul(Va[(size_t)k * N + i], zmk(hred[2 * k], hred[2 * k + 1])));
           W[i] = w;
         }
@@ -1235,7 +1239,7 @@ static int pamp_add_factored(pp_pamp *p, int angular_storage, int P0, const int 
     }
   for (int a = 0; a < p->L; a++) {
     int cnt = csr.rowptr[a + 1] - csr.rowptr[a];
-    if (cnt > PP_MAXSLOTS) return PP_EUNSUP;
+    if (cnt > PP_MAXENT) return PP_EUNSUP;
   }
   for (size_t e = 0; e < csr.ent.size() / 3; e++) {
     // find row of entry e
@@ -1426,21 +1430,48 @@ extern "C" int pp_pamp_add_term(pp_pamp *p, int angular_storage, int P0, const i
   return pamp_add_factored(p, angular_storage, P0, pairs, bw, rank, coef.data(), bands.data());
 }
 
+static int pamp_scr_doubles(int L, int N, int m, int nslots, int padN)
+{
+  return std::max(std::max(2 * nslots * N, L * (2 * m + 2)), 2 * padN);
+}
+
 static int pamp_scr_doubles(const pp_pamp *p, int padN)
 {
-  return std::max(std::max(2 * p->nslots * p->N, p->L * (2 * p->m + 2)), 2 * padN);
+  return pamp_scr_doubles(p->L, p->N, p->m, p->nslots, padN);
+}
+
+// LDS bytes of one pAMP step workgroup: per-fiber buffers, or the Pade
+// scratch on workgroup 0, above the persistent scan matrices and combo table.
+static size_t pamp_lds_bytes(int L, int N, int m, int nslots, int kd, int P, int B, int nlev)
+{
+  const int padN = P * B;
+  const size_t nle = (size_t)((nlev + 1) & ~1) + PP_CMB_DOUBLES;
+  size_t fib = sizeof(double) * nle + sizeof(zd) * (size_t)(2 * padN + P * kd) +
+               sizeof(double) * (size_t)pamp_scr_doubles(L, N, m, nslots, padN) +
+               sizeof(double) * (size_t)(16 + (2 * m + 4) + PP_THREADS + 2 * m) + sizeof(zd) * (size_t)m + 64;
+  size_t pade = sizeof(double) * nle + sizeof(zd) * (size_t)(9 * m * m) + sizeof(double) * (PP_THREADS / 64) + 64;
+  return fib > pade ? fib : pade;
 }
 
 static size_t pamp_lds_bytes(const pp_pamp *p, int padN)
 {
   const pp_band_plan &pl = p->ov->plan;
-  const size_t nle = (size_t)((pl.nlev + 1) & ~1) + PP_CMB_DOUBLES;
-  size_t fib = sizeof(double) * nle + sizeof(zd) * (size_t)(2 * padN + pl.P * p->ov->kd) +
-               sizeof(double) * (size_t)pamp_scr_doubles(p, padN) +
-               sizeof(double) * (size_t)(16 + (2 * p->m + 4) + PP_THREADS + 2 * p->m) + sizeof(zd) * (size_t)p->m + 64;
-  size_t pade = sizeof(double) * nle + sizeof(zd) * (size_t)(9 * p->m * p->m) +
-                sizeof(double) * (PP_THREADS / 64) + 64;
-  return fib > pade ? fib : pade;
+  (void)padN;
+  return pamp_lds_bytes(p->L, p->N, p->m, p->nslots, p->ov->kd, pl.P, pl.B, pl.nlev);
+}
+
+// LDS bytes a pAMP launch of an (L angular, N radial) problem with Krylov
+// dimension m, nslots coupled fibers per row and solver chunk length B needs
+// per workgroup (-1 if B is not a valid chunk length).  No reference
+// counterpart: the batch autotuner uses it to keep two workgroups per CU.
+extern "C" long pp_pamp_lds_estimate(int L, int N, int m, int nslots, int kd, int B)
+{
+  if (L < 1 || N < 1 || m < 1 || nslots < 1 || kd < 1 || B < 8 || B < kd || (B & (B - 1))) return -1;
+  const int P = (N + B - 1) / B;
+  if (P > 64) return -1;
+  int levels = 0;
+  while ((1 << levels) < P) levels++;
+  return (long)pamp_lds_bytes(L, N, m, nslots, kd, P, B, 2 * levels * kd * kd * P);
 }
 
 static int pamp_upload_slots(pp_pamp *p)

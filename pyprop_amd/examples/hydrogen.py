@@ -30,6 +30,7 @@ def LaserFunctionVelocity(conf, t):
 WORKLOADS = {
     "hydrogen_small": ("hydrogen_small.ini", 2000),   # order 7, 100 breakpoints, l<=10, m=20, dt=0.01
     "hydrogen_large": ("hydrogen_large.ini", 20000),  # order 7, N=800, l<=63, m=24, dt=0.005
+    "hydrogen_adjoint": ("hydrogen_adjoint.ini", 4000),  # order 7, N=400, l<=31, m=20, dt=0.05
 }
 
 
@@ -129,6 +130,26 @@ def ground_state(problem, l=0, shift=-0.5 + 1e-3, iterations=5):
     out = np.zeros((L, N), dtype=np.complex128)
     out[li] = x
     return out
+
+
+def bound_state(problem, l, index):
+    """The index-th (from 0) S-normalised eigenvector of (H0_l, S) placed in
+    channel l of an (L, N) array (index 0 of l = 1 is the 2p state)."""
+    import scipy.linalg
+    prop = problem.Propagator.BasePropagator
+    repr = problem.psi.GetRepresentation()
+    L, N = repr.GetFullShape()
+    S = repr.GetRepresentation(1).GetBSplineObject().overlap_dense()
+    li = l - repr.GetRepresentation(0).M
+    Hl = np.zeros((N, N), dtype=np.complex128)
+    for pot in prop.PotentialList:
+        if pot.IsTimeDependent or pot.StorageIds() != ["Diag", "BandNH"]:
+            continue
+        Hl += _block(pot, li, N, pot.PotentialData.shape[1] // N)
+    E, V = scipy.linalg.eigh(Hl, S)
+    out = np.zeros((L, N), dtype=np.complex128)
+    out[li] = V[:, index]
+    return out, E[index]
 
 
 def populations(problem, psi, emax=0.0):

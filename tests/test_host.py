@@ -83,7 +83,7 @@ def test_initial_state_matches_oracle():
 def test_capi_exports_match_header():
     from pyprop_amd import _native as nat
     decl = open(os.path.join(ROOT, "include", "pyprop_amd.h")).read()
-    names = set(re.findall(r"\bint\s+(pp_\w+)\s*\(", decl))
+    names = set(re.findall(r"\b(?:int|long)\s+(pp_\w+)\s*\(", decl))
     assert names == set(nat.EXPORTS)
     lib = ctypes.CDLL(nat.LIB_PATH)
     for n in names:
