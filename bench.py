@@ -190,30 +190,51 @@ def run_adjoint(args):
     t0 = time.perf_counter()
     adj.population(s)
     fwd_s = time.perf_counter() - t0
-    # finite differences: 64 forward runs (32 samples, +-h) as one batch
-    h = 1e-6
-    rng = np.random.default_rng(7)
-    idx = np.sort(rng.choice(K, size=32, replace=False))
-    eng0 = prob.Propagator.Engine
-    pots = prob.Propagator._pots
-    engines, psis, scs = [], [], []
-    for k in idx:
-        for sgn in (1.0, -1.0):
-            e = D.PampEngine(eng0.L, eng0.N, eng0.m, eng0.overlap)
-            e.add_term("Diag", pots[0].DeviceData())
-            e.add_term("Simp", pots[1].DeviceData(), pairs=pots[1].GeometryList[0].GetBasisPairs())
-            sv = s.copy()
-            sv[k] += sgn * h
-            engines.append(e)
-            psis.append(adj.psi0.clone())
-            scs.append(np.stack([np.ones(K), sv], axis=1))
-    t0 = time.perf_counter()
-    D.pamp_batch_propagate(engines, psis, K, [dt] * len(engines), scs)
-    torch.cuda.synchronize()
-    fd_s = time.perf_counter() - t0
-    pops = [abs(adj._sdot(adj.target, p)) ** 2 for p in psis]
-    fd = np.array([(pops[2 * i] - pops[2 * i + 1]) / (2 * h) for i in range(len(idx))])
-    err = np.abs(g[idx] - fd)
+    def fd_check(prob, adj, s, g, nsamp=32, h=1e-6):
+        """central differences on nsamp seeded samples: 2*nsamp forward runs
+        of the whole sample sequence as one batch"""
+        K = len(s)
+        dt = prob.TimeStep.real
+        idx = np.sort(np.random.default_rng(7).choice(K, size=min(nsamp, K), replace=False))
+        eng0 = prob.Propagator.Engine
+        pots = prob.Propagator._pots
+        engines, psis, scs = [], [], []
+        for k in idx:
+            for sgn in (1.0, -1.0):
+                e = D.PampEngine(eng0.L, eng0.N, eng0.m, eng0.overlap)
+                e.add_term("Diag", pots[0].DeviceData())
+                e.add_term("Simp", pots[1].DeviceData(), pairs=pots[1].GeometryList[0].GetBasisPairs())
+                e.set_pade_scaling(1)
+                sv = s.copy()
+                sv[k] += sgn * h
+                engines.append(e)
+                psis.append(adj.psi0.clone())
+                scs.append(np.stack([np.ones(K), sv], axis=1))
+        t0 = time.perf_counter()
+        D.pamp_batch_propagate(engines, psis, K, [dt] * len(engines), scs)
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t0
+        pops = [abs(adj._sdot(adj.target, p)) ** 2 for p in psis]
+        fd = np.array([(pops[2 * i] - pops[2 * i + 1]) / (2 * h) for i in range(len(idx))])
+        err = np.abs(g[idx] - fd)
+        return {"samples": len(idx), "h": h, "max_abs_err": float(err.max()),
+                "max_rel_err_vs_max_grad": float(err.max() / np.abs(g).max()),
+                "batched_forward_runs": len(engines), "seconds": secs}
+
+    fd_full = fd_check(prob, adj, s, g)
+    krylov_est = adj.engine.diagnostics()
+    # the same check where the forward map is well conditioned: lmax 5 on a
+    # coarse grid (|H^| dt small), 400 samples
+    small = pyprop_amd.Problem(H.load("hydrogen_small", {
+        ("AngularRepresentation", "maxl"): 5, ("RadialRepresentation", "xsize"): 37,
+        ("RadialRepresentation", "xmax"): 40.0, ("Propagation", "krylov_basis_size"): 12,
+        ("Propagation", "timestep"): 0.05}))
+    small.SetupStep()
+    small.SetWavefunction(H.ground_state(small))
+    adj_s = PopulationAdjoint(small, H.bound_state(small, 1, 0)[0], checkpoint=50)
+    s_small = adjoint_samples(400, 0.05)
+    _, g_small = adj_s.gradient(s_small)
+    fd_small = fd_check(small, adj_s, s_small, g_small)
     line = {
         "metric": "Krylov steps/s + matvec HBM GB/s, FP64 vs mixed at <=1e-8 population error",
         "value": K / grad_s, "unit": "adjoint-sweep steps/s",
@@ -223,9 +244,9 @@ def run_adjoint(args):
         "config": {"workload": "hydrogen_adjoint", "n": adj.L * adj.N, "L": adj.L, "N": adj.N,
                    "krylov_dim": adj.m, "dt": dt, "checkpoint": 50, "objective": "final 2p population"},
         "population_2p": P, "forward_s": fwd_s, "gradient_s": grad_s,
-        "fd_check": {"samples": len(idx), "h": h, "max_abs_err": float(err.max()),
-                     "max_rel_err_vs_max_grad": float(err.max() / np.abs(g).max()),
-                     "batched_forward_runs": len(engines), "seconds": fd_s},
+        "fd_check": fd_full,
+        "krylov_error_estimate_median": float(np.median(krylov_est[:, 0])),
+        "fd_check_well_conditioned": dict(fd_small, config="l<=5, N=40, dt=0.05, 400 samples, m=12"),
         "grad_norm": float(np.linalg.norm(g)),
     }
     print(json.dumps(line))

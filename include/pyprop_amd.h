@@ -134,6 +134,12 @@ int pp_pamp_batch_propagate(int ninst, pp_pamp *const *inst, void *const *psi, i
  * chunk length B needs (-1: invalid B).  No reference counterpart: the batch
  * autotuner uses it to keep two workgroups per CU. */
 long pp_pamp_lds_estimate(int L, int N, int m, int nslots, int kd, int B);
+/* Pade scaling of later pp_pamp_propagate launches (no reference
+ * counterpart): 0 (default) scales A = -i dt H_m by its largest element as
+ * pAMP does (pamp.h:442-445); 1 scales until the 1-norm is at most 1/2, which
+ * keeps the Pade(6) error at rounding level and makes a step a smooth
+ * function of its inputs (used by the adjoint gradients). */
+int pp_pamp_set_pade_scaling(pp_pamp *p, int mode);
 /* Mixed-precision form (no reference counterpart; the reference is FP64
  * throughout): later launches apply the terms whose bit is set in term_mask
  * (bit t = t-th added term) in complex FP32 (FP32 radial bands, FP32 band

@@ -15,7 +15,9 @@ dF_k/ds_k psi = U^-1 L(A, E) U psi is the Frechet derivative of the
 exponential (A = -i dt H^_k, E = -i dt V^), read off the upper block of
 exp([[A, E], [0, A]]) [0; U psi]: one step of the same engine on a doubled
 problem (2L fibers: the operator on both halves plus V from the lower to the
-upper half).  Forward states are recomputed from checkpoints.
+upper half).  Forward states are recomputed from checkpoints.  All steps use
+the 1-norm Pade scaling (accurate to rounding), so the forward map is smooth in
+the samples and its finite differences match the gradient.
 """
 import numpy as np
 
@@ -57,6 +59,11 @@ class PopulationAdjoint:
         aug.add_term("Simp", torch.cat([V, V]).contiguous(), pairs=np.concatenate([pairs, pairs + L]))
         aug.add_term("Simp", V, pairs=np.stack([pairs[:, 0], pairs[:, 1] + L], axis=1))
         self.aug = aug
+        # pAMP's largest-element Pade scaling makes a step jump by the Pade
+        # error whenever the scaling exponent changes; derivatives need the
+        # accurate, smooth variant on every engine of the sweep
+        self.engine.set_pade_scaling(1)
+        aug.set_pade_scaling(1)
 
     # ---------------------------------------------------------------- helpers
     def _sdot(self, a, b):

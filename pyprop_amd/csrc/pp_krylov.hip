@@ -68,6 +68,7 @@ struct pp_barrier {
 
 struct pp_pamp_args {
   int L, N, m, s, bw, nterms, nsteps, reorth, padN, nslots, ncombo, scrDoubles, nlevEven;
+  int normscale;         // Pade scaling by the 1-norm instead of the largest element
   double dt_re, dt_im;
   pp_fterm term[PP_MAXTERMS];
   const int *slotcols;   // L x nslots source fibers staged per row (-1 unused)
@@ -98,6 +99,7 @@ struct pp_pamp {
   bool dirty;
   zd *V, *y, *Hm, *cvec;
   unsigned mixed;                          // bit t: term t applied in complex FP32
+  int normscale;                           // Pade scaling by the 1-norm
   double *part, *diag, *scal;
   pp_barrier *bar;
   int diag_cap, scal_cap, pstr;
@@ -233,8 +235,13 @@ __device__ inline void fiber_dots(const zd *V, const zd *W, int nk, int N, bool 
 // linear solve is Gauss-Jordan with the same partial pivoting as zgesv
 // (largest |re|+|im|, first index on ties) on ping-pong buffers, one barrier
 // per column.
+// normscale: 0 scales by the largest |A_ij| as pAMP does; 1 scales until the
+// 1-norm of A / 2^s is at most 1/2, which keeps the Pade(6) error near
+// rounding and makes the step a smooth function of its inputs (the largest
+// element can be several times smaller than the norm, and a change of s
+// then moves the result by the Pade error).
 __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg, double dt_re, double dt_im, int off,
-                                                     zd *cvec_out, double *diag)
+                                                     zd *cvec_out, double *diag, int normscale)
 {
   extern __shared__ double pp_lds[];
   zd *lds = (zd *)(pp_lds + off);
@@ -249,7 +256,12 @@ __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg, double
   for (int e = tid; e < mm; e += PP_THREADS) A[e] = zmul(fac, Hg[e]);
   __syncthreads();
   double mx = 0.0;
-  for (int e = tid; e < mm; e += PP_THREADS) mx = fmax(mx, sqrt(zabs2(A[e])));
+  if (normscale) {   // largest column sum of |A_ij|
+    if (tid < m)
+      for (int i = 0; i < m; i++) mx += sqrt(zabs2(A[i + tid * m]));
+  } else {
+    for (int e = tid; e < mm; e += PP_THREADS) mx = fmax(mx, sqrt(zabs2(A[e])));
+  }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
   if ((tid & 63) == 0) red[tid >> 6] = mx;
@@ -257,7 +269,7 @@ __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg, double
   mx = red[0];
   for (int w = 1; w < PP_THREADS / 64; w++) mx = fmax(mx, red[w]);
   int sc = 0;
-  if (mx > 0.0) sc = (int)ceil(log2(mx));
+  if (mx > 0.0) sc = (int)ceil(log2(mx)) + (normscale ? 1 : 0);
   if (sc < 0) sc = 0;
   const double inv = 1.0 / pow(2.0, (double)sc);
   for (int e = tid; e < mm; e += PP_THREADS) A[e] = zscale(A[e], inv);
@@ -759,7 +771,8 @@ __device__ __attribute__((always_inline)) inline void pp_pamp_body(const pp_pamp
 
     // ---------------- exponentiate on workgroup 0, broadcast exp(-i dt H) e1
     if (a == 0) {
-      pade_block(m, A.Hm, A.dt_re, A.dt_im, A.nlevEven + PP_CMB_DOUBLES, A.cvec, A.diag + (size_t)step * 4);
+      pade_block(m, A.Hm, A.dt_re, A.dt_im, A.nlevEven + PP_CMB_DOUBLES, A.cvec, A.diag + (size_t)step * 4,
+                 A.normscale);
       if (tid == 0) {
         A.diag[(size_t)step * 4 + 1] = mEff;
         A.diag[(size_t)step * 4 + 2] = beta0;
@@ -1198,6 +1211,7 @@ extern "C" int pp_pamp_create(int L, int N, int m, int reorth, const pp_overlap 
   p->slotcols = nullptr; p->dirty = true;
   p->V = p->y = p->Hm = p->cvec = nullptr;
   p->mixed = 0;
+  p->normscale = 0;
   p->part = p->diag = p->scal = nullptr;
   p->bar = nullptr; p->tlog = nullptr; p->timing = 0;
   p->diag_cap = p->scal_cap = 0;
@@ -1514,7 +1528,7 @@ static int pamp_prepare(pp_pamp *p, void *psi, int nsteps, double dt_re, double 
   PP_HIP(hipMemsetAsync(p->bar, 0, sizeof(pp_barrier) + sizeof(unsigned) * (size_t)p->L, stream));
   memset(&A, 0, sizeof(A));
   A.L = p->L; A.N = p->N; A.m = p->m; A.s = p->s; A.bw = p->bw; A.nterms = p->nterms;
-  A.nsteps = nsteps; A.reorth = p->reorth; A.nslots = p->nslots;
+  A.nsteps = nsteps; A.reorth = p->reorth; A.nslots = p->nslots; A.normscale = p->normscale;
   A.ncombo = 0;
   for (int t = 0; t < p->nterms; t++) A.ncombo += p->term[t].rank;
   if (A.ncombo > PP_MAXCOMBO) return PP_EUNSUP;
@@ -1695,6 +1709,16 @@ extern "C" int pp_pamp_diagnostics(pp_pamp *p, int nsteps, double *diag, hipStre
   PP_HIP(hipMemcpyAsync(&hb, p->bar, sizeof(hb), hipMemcpyDeviceToHost, stream));
   PP_HIP(hipStreamSynchronize(stream));
   if (hb.err) return PP_EHIP;
+  return PP_OK;
+}
+
+// Pade scaling of later launches: 0 (default) the largest |A_ij| as pAMP
+// (pamp.h:442-445); 1 the 1-norm with one extra halving, so the exponential
+// is accurate to rounding and smooth in the inputs (needed by derivatives).
+extern "C" int pp_pamp_set_pade_scaling(pp_pamp *p, int mode)
+{
+  if (!p || mode < 0 || mode > 1) return PP_EARG;
+  p->normscale = mode;
   return PP_OK;
 }
 

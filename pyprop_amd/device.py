@@ -177,6 +177,10 @@ class PampEngine:
             mask |= 1 << int(t)
         nat.check(nat.lib().pp_pamp_set_mixed(self.h, mask), "pp_pamp_set_mixed")
 
+    def set_pade_scaling(self, mode):
+        """0: largest element (pAMP); 1: 1-norm, accurate and smooth."""
+        nat.check(nat.lib().pp_pamp_set_pade_scaling(self.h, int(mode)), "pp_pamp_set_pade_scaling")
+
     def set_timing(self, enable=True):
         nat.check(nat.lib().pp_pamp_set_timing(self.h, int(bool(enable))), "pp_pamp_set_timing")
 

@@ -30,7 +30,7 @@ def LaserFunctionVelocity(conf, t):
 WORKLOADS = {
     "hydrogen_small": ("hydrogen_small.ini", 2000),   # order 7, 100 breakpoints, l<=10, m=20, dt=0.01
     "hydrogen_large": ("hydrogen_large.ini", 20000),  # order 7, N=800, l<=63, m=24, dt=0.005
-    "hydrogen_adjoint": ("hydrogen_adjoint.ini", 4000),  # order 7, N=400, l<=31, m=20, dt=0.05
+    "hydrogen_adjoint": ("hydrogen_adjoint.ini", 4000),  # order 7, N=400, l<=31, m=20, dt=0.01
 }
 
 

@@ -254,7 +254,7 @@ def test_adjoint_gradient_matches_finite_differences():
         psi = ham.propagate_pamp(psi0, K, 0.05, np.stack([np.ones(K), sv], axis=1), 12)
         return abs(np.sum(np.conj(target) * (psi @ S.T))) ** 2
 
-    assert abs(pop(s) - P) < 1e-10 * max(P, 1e-30) + 1e-14
+    assert abs(pop(s) - P) < 1e-8 * max(P, 1e-30) + 1e-14   # Pade scaling differs (1-norm here)
     for k in (7, 29):
         sp, sm = s.copy(), s.copy()
         sp[k] += h
