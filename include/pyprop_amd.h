@@ -21,6 +21,7 @@ extern "C" {
 
 typedef struct pp_overlap pp_overlap;
 typedef struct pp_pamp pp_pamp;
+typedef struct pp_pamp3 pp_pamp3;
 typedef void *pp_stream;  /* hipStream_t */
 
 /* storage ids (core/tensorpotential/tensorpotentialmultiply_generator.py:1336) */
@@ -152,6 +153,50 @@ int pp_pamp_set_mixed(pp_pamp *p, unsigned term_mask);
 int pp_pamp_set_timing(pp_pamp *p, int enable);
 int pp_pamp_timings(pp_pamp *p, long long *out, int max, int *count);
 int pp_pamp_destroy(pp_pamp *p);
+
+/* ---- two-electron states: coupled channels x two B-spline radial ranks ----
+ * Replaces core.krylov_PampWrapper_3 driven by PamPropagator.AdvanceStep
+ * (pyprop/propagator/PamPropagator.py:50-59, pampwrapper.cpp:51-108) with
+ * the balanced rank-3 matvec BasisPropagator.MultiplyHamiltonianBalancedOverlap
+ * (BasisPropagator.py:131-159) over the potentials of
+ * examples/tensor/helium_stabilization/config.ini (TensorPotentialMultiply_
+ * Simp_Simp_Simp and _Simp_Simp_BandNH).  psi is (C, N, N) complex128, the
+ * second radial rank contiguous; both radial ranks use the overlap ov (order
+ * 7 B-splines).  reorth != 0 returns -2. */
+int pp_pamp3_create(int C, int N, int m, int reorth, const pp_overlap *ov, pp_pamp3 **out);
+/* A factored potential term sum_p coef[p] |pairs[p,0]><pairs[p,1]| (x) A (x) B
+ * (coef: P complex as 2P doubles; A, B real N x 13 row-major bands, element
+ * (i, j) at i*13 + 6 + j - i; slot: column of the per-step time scalings,
+ * -1 = static).  One-electron terms and the velocity-gauge dipole of the
+ * reference's helium potentials are sums of such terms
+ * (coupledspherical.cpp:120-205, potential.cpp:7-77, coupledvelocity.cpp). */
+int pp_pamp3_add_kron_term(pp_pamp3 *p, int P, const int *pairs, const double *coef, const double *A,
+                           const double *B, int slot);
+/* A multipole term sum_p coef[p] |pairs[p,0]><pairs[p,1]| (x) R with the real
+ * 4-index radial tensor R[i1][d1][i2][d2] (N x 13 x N x 13, j = i + d - 6):
+ * one l of CustomPotentialEvaluationR12 (coupledspherical.cpp:16-114). */
+int pp_pamp3_add_multipole_term(pp_pamp3 *p, int P, const int *pairs, const double *coef, const double *R,
+                                int slot);
+/* dst = H src, or U^-H H U^-1 src with balanced != 0 (src and dst device,
+ * scal: nslots time scalings on the host).  Synchronises the stream. */
+int pp_pamp3_apply(pp_pamp3 *p, const void *src, void *dst, const double *scal, int balanced, pp_stream stream);
+/* nsteps PamPropagator.AdvanceStep's of psi (device): psi~ = (U x U) psi,
+ * pAMP on the balanced operator with scal[n*nslots + slot] at step n,
+ * psi = (U x U)^-1 psi~.  Synchronises the stream. */
+int pp_pamp3_propagate(pp_pamp3 *p, void *psi, int nsteps, double dt_re, double dt_im, const double *scal,
+                       pp_stream stream);
+/* Per step of the last pp_pamp3_propagate (host, nsteps x 4): |last Krylov
+ * coefficient|, basis size used, |psi~|, 0. */
+int pp_pamp3_diagnostics(pp_pamp3 *p, int nsteps, double *diag, pp_stream stream);
+/* Measurement hooks (no reference counterpart): with timing enabled, every
+ * multipole contraction launch of later calls is bracketed by HIP events on
+ * its stream; pp_pamp3_kernel_time returns their summed duration (ms) and
+ * count since timing was enabled; pp_pamp3_multipole_flops the algorithmic
+ * flops of one such launch. */
+int pp_pamp3_set_timing(pp_pamp3 *p, int enable);
+int pp_pamp3_kernel_time(pp_pamp3 *p, double *ms_total, long *launches);
+int pp_pamp3_multipole_flops(pp_pamp3 *p, double *flops);
+int pp_pamp3_destroy(pp_pamp3 *p);
 
 #ifdef __cplusplus
 }

@@ -494,3 +494,226 @@ def scaling_pade_exp(A):
     out = np.zeros(m * m, dtype=np.complex128)
     info = lib().orc_scaling_pade_exp(m, _dp(a.view(np.float64)), _dp(out.view(np.float64)))
     return out.reshape((m, m), order="F"), info
+
+
+# --------------------------------------------------------------------------
+# helium: coupled spherical harmonics x two B-spline radial ranks
+# (examples/tensor/helium_stabilization/config.ini, coupledspherical.cpp,
+# coupledvelocity.cpp, potential.cpp; pyprop/CoupledSphericalHarmonics.py)
+# --------------------------------------------------------------------------
+def coupled_channels(lmax, Ls, Ms=(0,)):
+    """DefaultCoupledIndexIterator order (CoupledSphericalHarmonics.py:52-63)."""
+    out = []
+    for M in Ms:
+        for L in Ls:
+            for l1 in range(lmax + 1):
+                for l2 in range(lmax + 1):
+                    if abs(l1 - l2) <= L <= l1 + l2:
+                        out.append((l1, l2, L, M))
+    return out
+
+
+def _fact(n):
+    return float(np.prod(np.arange(1, n + 1, dtype=np.float64))) if n > 1 else 1.0
+
+
+def threej(j1, j2, j3, m1, m2, m3):
+    """Wigner 3j by the Racah sum in floating point (GSL's
+    gsl_sf_coupling_3j is what the reference calls, coupledrange.h:17-32)."""
+    if m1 + m2 + m3 != 0 or abs(m1) > j1 or abs(m2) > j2 or abs(m3) > j3:
+        return 0.0
+    if j3 > j1 + j2 or j3 < abs(j1 - j2):
+        return 0.0
+    tri = _fact(j1 + j2 - j3) * _fact(j1 - j2 + j3) * _fact(-j1 + j2 + j3) / _fact(j1 + j2 + j3 + 1)
+    pre = np.sqrt(tri * _fact(j1 + m1) * _fact(j1 - m1) * _fact(j2 + m2) * _fact(j2 - m2)
+                  * _fact(j3 + m3) * _fact(j3 - m3))
+    s = 0.0
+    for k in range(0, j1 + j2 + j3 + 1):
+        a = [k, j3 - j2 + k + m1, j3 - j1 + k - m2, j1 + j2 - j3 - k, j1 - k - m1, j2 - k + m2]
+        if min(a) < 0:
+            continue
+        s += (-1.0) ** k / np.prod([_fact(x) for x in a])
+    return (-1.0) ** (j1 - j2 - m3) * pre * s
+
+
+def cg(l1, l2, m1, m2, L, M):
+    """ClebschGordan::operator() (coupledrange.h:21-31)."""
+    return (-1.0) ** (M + l1 - l2) * np.sqrt(2 * L + 1.0) * threej(l1, l2, L, m1, m2, -M)
+
+
+def ee_l3sum(left, right, l3):
+    """The l3 coefficient of CustomPotentialEvaluationR12::UpdatePotentialData
+    (coupledspherical.cpp:53-90): left = (l1', l2', L, M), right = (l1, l2, L, M)."""
+    l1p, l2p, L, M = left
+    l1, l2 = right[0], right[1]
+    coeff = np.sqrt((2. * l1p + 1.) * (2. * l1 + 1.) / (2. * l3 + 1.))
+    coeff *= np.sqrt((2. * l2p + 1.) * (2. * l2 + 1.) / (2. * l3 + 1.))
+    coeff *= 1. / (2 * l3 + 1.)
+    coeff *= cg(l1p, l1, 0, 0, l3, 0)
+    coeff *= cg(l2p, l2, 0, 0, l3, 0)
+    s = 0.0
+    lstop = max(l1, l1p, l2, l2p)
+    for m1p in range(-lstop, lstop + 1):
+        m2p = M - m1p
+        for m1 in range(-lstop, lstop + 1):
+            m2 = M - m1
+            m3 = m1 - m1p
+            cur = cg(l1p, l2p, m1p, m2p, L, M) * cg(l1, l2, m1, m2, L, M)
+            cur *= cg(l1p, l1, -m1p, m1, l3, m3) * cg(l2p, l2, -m2p, m2, l3, -m3)
+            cur *= (-1.0) ** (m1p + m2p + m3)
+            s += cur
+    return s * coeff
+
+
+def _lh(l, m):
+    C = l * np.sqrt(((l + 1 + m) * (l + 1 - m)) / ((2 * l + 1.) * (2 * l + 3)))
+    D = -(l + 1) * np.sqrt(max(0.0, ((l + m) * (l - m)) / ((2 * l + 1.) * (2 * l - 1)))) if l > 0 else 0.0
+    E = np.sqrt(((l + 1 + m) * (l + 1 - m)) / ((2 * l + 1.) * (2 * l + 3)))
+    F = np.sqrt(max(0.0, ((l + m) * (l - m)) / ((2 * l + 1.) * (2 * l - 1)))) if l > 0 else 0.0
+    return C, D, E, F
+
+
+def velocity_coefficients(left, right):
+    """(coupling1, coupling2) of CustomPotential_LaserVelocity and the
+    couplings of the DerivativeR1 / DerivativeR2 parts
+    (coupledvelocity.cpp:96-146, :258-276, and the R2 part)."""
+    l1, l2, L, M = left
+    l1p, l2p, Lp, Mp = right
+    if M != Mp:
+        return 0.0, 0.0, 0.0, 0.0
+    lstop = max(l1, l1p, l2, l2p)
+    c1 = c2 = d1 = d2 = 0.0
+    for m in range(-lstop, lstop + 1):
+        cur = cg(l1, l2, m, M - m, L, M) * cg(l1p, l2p, m, Mp - m, Lp, Mp)
+        if l2 == l2p and abs(m) <= l1 and abs(m) <= l1p:
+            C, D, E, F = _lh(l1p, m)
+            C *= (l1 == l1p + 1); E *= (l1 == l1p + 1)
+            D *= (l1 == l1p - 1); F *= (l1 == l1p - 1)
+            c1 += (-(C + D) - (E + F)) * cur
+            d1 += (E + F) * cur
+        if l1 == l1p and abs(m) <= l2 and abs(m) <= l2p:
+            C, D, E, F = _lh(l2p, M - m)
+            C *= (l2 == l2p + 1); E *= (l2 == l2p + 1)
+            D *= (l2 == l2p - 1); F *= (l2 == l2p - 1)
+            c2 += (-(C + D) - (E + F)) * cur
+            d2 += (E + F) * cur
+    return c1, c2, d1, d2
+
+
+class Helium:
+    """Reference-layout helium operator for small cases: every potential of
+    the helium config evaluated on (angular pairs, r1 grid, r2 grid) and
+    projected on r2 then r1 (TensorPotentialGenerator.py:72-256)."""
+
+    def __init__(self, lmax=1, Ls=(0, 1), nbp=8, rmax=10.0, order=7, z=2.0, mass=1.0,
+                 frequency=2.0, cycles=8, E0=0.05, bpstype="linear", gamma=1.0, quad_add=0,
+                 laser=True):
+        bp = linear_breakpoints(0.0, rmax, nbp) if bpstype == "linear" else \
+            exponential_breakpoints(0.0, rmax, nbp, gamma)
+        self.bs = BSpline(bp, order, "zero", quad_add)
+        self.N = N = self.bs.N
+        self.k, self.kd = order, order - 1
+        s = order - 1
+        self.ch = ch = coupled_channels(lmax, Ls)
+        self.C = C = len(ch)
+        r = self.bs.x
+        G = len(r)
+        # basis tables at the quadrature points
+        val = np.array([[self.bs.value(x, i, 0) for x in r] for i in range(N)])
+        der = {d: np.array([[self.bs.value(x, i, d) for x in r] for i in range(N)]) for d in (1, 2)}
+        der[0] = val
+        self._tab = der
+        self.packed = np.array([(i, j) for i in range(N) for j in range(max(0, i - order + 1), min(i + order, N))])
+        bwid = 2 * s + 1
+        nh = np.zeros((N * bwid, 2), dtype=np.int64)
+        nh[:, 1] = N - 1
+        for i in range(N):
+            for j in range(i - s, i + s + 1):
+                if 0 <= j < N:
+                    nh[j * bwid + s + i - j] = (i, j)
+        self.bandnh = nh
+        diag = [(a, a) for a in range(C)]
+        linpol = [(a, b) for a in range(C) for b in range(C)
+                  if abs(ch[a][2] - ch[b][2]) == 1 and ch[a][3] == ch[b][3]
+                  and ((abs(ch[a][0] - ch[b][0]) == 1 and ch[a][1] == ch[b][1])
+                       or (abs(ch[a][1] - ch[b][1]) == 1 and ch[a][0] == ch[b][0]))]
+        r12 = [(a, b) for a in range(C) for b in range(C)
+               if ch[a][2] == ch[b][2] and ch[a][3] == ch[b][3]]
+        self.pairs = {"diag": np.array(diag), "linpol": np.array(linpol).reshape(-1, 2), "r12": np.array(r12)}
+        R1 = r[:, None] * np.ones((1, G))
+        R2 = np.ones((G, 1)) * r[None, :]
+        one = np.ones((G, G))
+        terms = []
+        # RadialKineticEnergy1/2 (KineticEnergyPotential, differentiation 2 on one rank)
+        kin = np.full((C, G, G), -1.0 / (2 * mass))
+        terms.append(("static", "diag", self._project(kin, diag, 2, 0, "packed")))
+        terms.append(("static", "diag", self._project(kin, diag, 0, 2, "packed")))
+        # AngularKineticEnergy (coupledspherical.cpp:160-205)
+        ang = np.array([ch[a][0] * (ch[a][0] + 1.) / (2 * mass) / R1 ** 2
+                        + ch[a][1] * (ch[a][1] + 1.) / (2 * mass) / R2 ** 2 for a, b in diag])
+        terms.append(("static", "diag", self._project(ang, diag, 0, 0, "packed")))
+        # CoupledSphericalCoulombPotential (potential.cpp:71-76)
+        cou = np.array([-z / R1 - z / R2 for a, b in diag])
+        terms.append(("static", "diag", self._project(cou, diag, 0, 0, "packed")))
+        # CustomPotentialEvaluationR12 (coupledspherical.cpp:16-114), r2 banded-nonhermitian
+        rmin = np.minimum(R1, R2)
+        rmax_ = np.maximum(R1, R2)
+        ee = np.zeros((len(r12), G, G))
+        for n, (a, b) in enumerate(r12):
+            l1p, l2p = ch[a][0], ch[a][1]
+            l1, l2 = ch[b][0], ch[b][1]
+            lo = max(abs(l1 - l1p), abs(l2 - l2p))
+            hi = min(l1 + l1p, l2 + l2p)
+            for l3 in range(lo, hi + 1):
+                ee[n] += ee_l3sum(ch[a], ch[b], l3) * (rmin / rmax_) ** l3 / rmax_
+        terms.append(("static", "r12", self._project(ee, r12, 0, 0, "bandnh")))
+        self.laser = laser and len(linpol) > 0
+        if self.laser:
+            vc = np.array([velocity_coefficients(ch[a], ch[b]) for a, b in linpol])
+            v0 = np.array([-1j * (vc[n, 0] / R1 + vc[n, 1] / R2) for n in range(len(linpol))])
+            terms.append(("laser", "linpol", self._project(v0, linpol, 0, 0, "packed")))
+            v1 = np.array([-1j * vc[n, 2] * one for n in range(len(linpol))])
+            terms.append(("laser", "linpol", self._project(v1, linpol, 1, 0, "packed")))
+            v2 = np.array([-1j * vc[n, 3] * one for n in range(len(linpol))])
+            terms.append(("laser", "linpol", self._project(v2, linpol, 0, 1, "packed")))
+        self.terms = terms
+        self.frequency, self.cycles = frequency, cycles
+        self.pulse_duration = 2 * np.pi / frequency * cycles
+        self.amplitude = E0 / frequency
+        self.S = self.bs.overlap_upper()
+        self.U, info = zpbtrf_upper(self.S, N, self.kd)
+        if info:
+            raise RuntimeError("overlap not positive definite (%d)" % info)
+        self.shape = (C, N, N)
+
+    def _w(self, prs, d):
+        """W[p, q] = w_q B_i(q) B_j^(d)(q) for i <= j, (-1)^d B_j B_i^(d) else."""
+        B, D = self._tab[0], self._tab[d]
+        i, j = prs[:, 0], prs[:, 1]
+        lo = np.where(i <= j, i, j)
+        hi = np.where(i <= j, j, i)
+        sign = np.where(i <= j, 1.0, (-1.0) ** d)
+        return sign[:, None] * B[lo] * D[hi] * self.bs.w[None, :]
+
+    def _project(self, data, angpairs, d1, d2, r2layout):
+        """data (P0, G, G) on the product grid -> (P0, packed pairs, r2 pairs)."""
+        W1 = self._w(self.packed, d1)
+        W2 = self._w(self.packed if r2layout == "packed" else self.bandnh, d2)
+        t = np.einsum("pab,qb->paq", np.asarray(data, dtype=np.complex128), W2)
+        return np.einsum("paq,ra->prq", t, W1)
+
+    def scaling(self, t):
+        return laser_velocity(t, self.amplitude, self.frequency, self.pulse_duration)
+
+    def ham(self):
+        out = []
+        for kind, pk, pot in self.terms:
+            r2 = ST_SIMP if pot.shape[2] == len(self.packed) else ST_BANDNH
+            pr = [self.pairs[pk].astype(np.int32), self.packed.astype(np.int32),
+                  self.packed.astype(np.int32) if r2 == ST_SIMP else None]
+            out.append((ST_SIMP, ST_SIMP, r2, pot, pr))
+        return Ham(self.shape, out, {1: (self.U, self.kd), 2: (self.U, self.kd)})
+
+    def term_scalings(self, times):
+        return np.array([[self.scaling(t) if kind == "laser" else 1.0 for kind, _, _ in self.terms]
+                         for t in times])

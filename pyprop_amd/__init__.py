@@ -15,5 +15,6 @@ from .potentials import PotentialType  # noqa: F401
 from .problem import InitialConditionType, WavefunctionFileFormat, Problem  # noqa: F401
 from .propagator import PropagatorBase, PamPropagator, ExpokitPropagator  # noqa: F401
 from .tensorpotential import BasisPropagator, TensorPotential  # noqa: F401
+from .coupled import CoupledIndex, DefaultCoupledIndexIterator  # noqa: F401
 
 del abs, max, min, round, sum  # keep the builtins for code that star-imports this package

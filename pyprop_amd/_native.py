@@ -27,6 +27,9 @@ EXPORTS = [
     "pp_band_cholesky_upper", "pp_overlap_create", "pp_overlap_create_chunked", "pp_overlap_factor", "pp_overlap_destroy",
     "pp_overlap_apply", "pp_tp_multiply", "pp_pamp_create", "pp_pamp_add_term", "pp_pamp_add_factored_term",
     "pp_lowrank_factor", "pp_pamp_propagate", "pp_pamp_diagnostics", "pp_pamp_set_mixed", "pp_pamp_set_pade_scaling", "pp_expokit_propagate", "pp_expokit_diagnostics", "pp_pamp_batch_propagate", "pp_pamp_lds_estimate", "pp_pamp_set_timing", "pp_pamp_timings", "pp_pamp_destroy",
+    "pp_pamp3_create", "pp_pamp3_add_kron_term", "pp_pamp3_add_multipole_term", "pp_pamp3_apply",
+    "pp_pamp3_propagate", "pp_pamp3_diagnostics", "pp_pamp3_set_timing", "pp_pamp3_kernel_time",
+    "pp_pamp3_multipole_flops", "pp_pamp3_destroy",
 ]
 
 _lib = None
@@ -78,6 +81,16 @@ def lib():
     L.pp_pamp_timings.argtypes = [P, ctypes.POINTER(ctypes.c_longlong), I, ip]
     L.pp_pamp_destroy.argtypes = [P]
     L.pp_pamp_lds_estimate.argtypes = [I, I, I, I, I, I]
+    L.pp_pamp3_create.argtypes = [I, I, I, I, P, ctypes.POINTER(P)]
+    L.pp_pamp3_add_kron_term.argtypes = [P, I, ip, dp, dp, dp, I]
+    L.pp_pamp3_add_multipole_term.argtypes = [P, I, ip, dp, dp, I]
+    L.pp_pamp3_apply.argtypes = [P, P, P, dp, I, P]
+    L.pp_pamp3_propagate.argtypes = [P, P, I, D, D, dp, P]
+    L.pp_pamp3_diagnostics.argtypes = [P, I, dp, P]
+    L.pp_pamp3_destroy.argtypes = [P]
+    L.pp_pamp3_set_timing.argtypes = [P, I]
+    L.pp_pamp3_kernel_time.argtypes = [P, dp, ctypes.POINTER(ctypes.c_long)]
+    L.pp_pamp3_multipole_flops.argtypes = [P, dp]
     for name in EXPORTS:
         getattr(L, name).restype = ctypes.c_int
     L.pp_pamp_lds_estimate.restype = ctypes.c_long

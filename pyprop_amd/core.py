@@ -88,6 +88,33 @@ class ReducedSphericalHarmonicRepresentation(RepresentationBase):
         return True
 
 
+class CoupledSphericalHarmonicRepresentation(RepresentationBase):
+    """Two-electron angular basis |(l1 l2) L M> (core/representation/
+    coupledspherical/coupledsphericalharmonicrepresentation.h); the grid is
+    the channel index, channels in the order of ``index_iterator``
+    (coupledrange.h:84-140)."""
+
+    def ApplyConfigSection(self, conf):
+        self.SetupRepresentation(conf.index_iterator)
+
+    def SetupRepresentation(self, indexIterator):
+        from .coupled import CoupledIndex
+        self.Channels = [CoupledIndex(*tuple(c)) for c in indexIterator]
+        if len(set(self.Channels)) != len(self.Channels):
+            raise ValueError("index_iterator yields a channel twice")
+
+    def GetFullShape(self):
+        return [len(self.Channels)]
+
+    def GetGlobalGrid(self, rank=0):
+        return np.arange(len(self.Channels), dtype=np.float64)
+
+    GetLocalGrid = GetGlobalGrid
+
+    def IsOrthogonalBasis(self):
+        return True
+
+
 class CombinedRepresentation(RepresentationBase):
     Rank = None
 
@@ -103,6 +130,14 @@ class CombinedRepresentation(RepresentationBase):
 
     def GetRepresentation(self, i):
         return self.Representations[i]
+
+    def RadialMatrices(self):
+        """Row-major band matrices (S, D1, D2, 1/r, 1/r^2) of the B-spline
+        basis shared by the radial ranks (rank-3 two-electron states)."""
+        if getattr(self, "_radial", None) is None:
+            from .helium import radial_matrices
+            self._radial = radial_matrices(self.Representations[1].GetBSplineObject())
+        return self._radial
 
     def GetFullShape(self):
         return [r.GetFullShape()[0] for r in self.Representations]
@@ -230,7 +265,8 @@ class _CoreNamespace(types.ModuleType):
 
 
 core = _CoreNamespace("core")
-for _name in ["BSplineRepresentation", "ReducedSphericalHarmonicRepresentation", "CombinedRepresentation_1",
+for _name in ["BSplineRepresentation", "ReducedSphericalHarmonicRepresentation",
+              "CoupledSphericalHarmonicRepresentation", "CombinedRepresentation_1",
               "CombinedRepresentation_2", "CombinedRepresentation_3", "Wavefunction"]:
     setattr(core, _name, globals()[_name])
 core.BSpline = _bs.BSplineBasis

@@ -1,0 +1,801 @@
+// pAMP propagation of two-electron states on MI355X: coupled angular
+// channels x two B-spline radial ranks, psi(c, i1, i2) with i2 contiguous.
+//
+//   psi~ = (U (x) U) psi;  fixed-m Arnoldi with classical Gram-Schmidt on
+//   H^ = (U (x) U)^-H (sum_t s_t(t_n) V_t) (U (x) U)^-1;  exp(-i dt H_m) e1
+//   by scaling + Pade(6);  psi = (U (x) U)^-1 |psi~| V c.
+//
+// Reference: pyprop/propagator/PamPropagator.py:50-75 and
+// core/krylov/pamp/pamp/pamp.h:271-350, :430-564, :613-669 (the step, as
+// for one radial rank), with the balanced matvec of
+// pyprop/tensorpotential/BasisPropagator.py:131-159 over the rank-3
+// potentials of examples/tensor/helium_stabilization/config.ini
+// (Simp_Simp_Simp one-electron terms, Simp_Simp_BandNH electron-electron
+// term, generated by TensorPotentialMultiply_* of
+// core/tensorpotential/tensorpotentialmultiply_generator.py).
+//
+// The state is 0.95 M amplitudes at the helium workload, far more than one
+// workgroup per fiber can keep resident, so this engine is a stream of
+// ordinary launches on one stream; the host issues a whole step without
+// synchronising (scalars stay on the device and later kernels read them).
+//
+// Operator, factored (no reference-layout tensor is formed):
+//  * Kronecker terms  sum_p a_p |c_p><c'_p| (x) A (x) B  with real banded
+//    A, B (the one-electron terms, the velocity-gauge dipole).  They are
+//    applied in two banded passes along i2 then i1 through intermediates:
+//    for a term whose B is shared by other terms the i2 pass is plain
+//    (X = y B^T per channel) and the angular mixing happens in the i1 pass;
+//    otherwise the mixing happens in the i2 pass and the i1 pass is plain.
+//  * Multipole terms  sum_p a_p |c_p><c'_p| (x) R  with a real 4-index
+//    radial tensor R[i1][d1][i2][d2] (the r<^l / r>^(l+1) term of one l).
+//    The angular mixing is applied first (one "mixed input" per term and
+//    16-channel chunk, channel-innermost), then one workgroup per (chunk,
+//    i1, half of i2) runs the 169-tap contraction of every term for 16
+//    channels at once: R slices and mixed input rows staged in LDS, each
+//    thread accumulating 2 channels x 4 consecutive i2 in registers so one
+//    R value feeds two channels and one input value feeds up to four
+//    outputs.
+#include "pp_common.h"
+#include <cstring>
+#include <cstdio>
+#include <cmath>
+#include <vector>
+#include <algorithm>
+
+#define PP_THREADS 256
+#include "pp_pade.h"
+
+#define P3_CHUNK 16        // channels per multipole chunk
+#define P3_HALF 64         // i2 per multipole workgroup
+#define P3_EE_THREADS 128  // (8 channel pairs) x (16 i2 quads)
+#define P3_RED_BLOCKS 480  // blocks of the Gram-Schmidt reductions
+
+struct p3_ent {            // one coupling of a banded pass or the mixing
+  int src, col, mat, slot; // source buffer, source channel, radial matrix, time slot (-1: 1.0)
+  double cre, cim;
+};
+
+struct pp_pamp3 {
+  int C, N, m, s, bw, reorth;
+  long n;
+  const pp_overlap *ov;
+  // host description
+  std::vector<std::vector<double>> mats;
+  struct kterm { std::vector<int> pr; std::vector<double> cf; int A, B, slot; };
+  std::vector<kterm> kterms;
+  struct mterm { std::vector<int> pr; std::vector<double> cf; double *R; int slot; };
+  std::vector<mterm> mterms;
+  int nslots;
+  bool dirty;
+  // device plan
+  double *dmats;
+  int K;                   // intermediates of the banded passes
+  int *rrow, *lrow, *mrow; // CSR row pointers (right: K*C, left: C, mix: nblk*16)
+  p3_ent *rent, *lent, *ment;
+  int nblk;                // multipole (term, chunk) blocks
+  int nchunk;
+  int *cb_ptr, *cb_blk;    // per chunk: its blocks
+  int *blk_term;
+  const double **dR;       // per multipole term: device R
+  // buffers
+  zd *Y, *X, *Z, *P, *V, *Ym;
+  double *part, *scal, *diag, *scal_dt;
+  zd *Hm, *cvec, *hcoef;
+  double *beta;
+  int *state;              // [0] broken, [1] mEff
+  int scal_cap, diag_cap;
+  // optional HIP-event timing of the multipole contraction launches
+  int timing;
+  std::vector<hipEvent_t> ev;   // pairs (before, after) of the current propagate
+  double ee_ms;
+  long ee_count;
+};
+
+// ------------------------------------------------------------------ banded passes
+// X[k][c][i1][i2] = sum_e coef_e * s_e * sum_d B_e[i2][d] * src[c'_e][i1][i2+d-s]
+__global__ void __launch_bounds__(128) p3_right(const double *mats, const int *row, const p3_ent *ent, const zd *y,
+                                                 zd *X, int C, int N, int s, int bw, const double *scal,
+                                                 const int *state)
+{
+  if (state[0]) return;
+  const int r = blockIdx.x / N, i1 = blockIdx.x % N;   // r = k*C + c
+  const long NN = (long)N * N;
+  for (int i2 = threadIdx.x; i2 < N; i2 += blockDim.x) {
+    zd acc = zmk(0.0, 0.0);
+    for (int e = row[r]; e < row[r + 1]; e++) {
+      const p3_ent E = ent[e];
+      const double sc = E.slot < 0 ? 1.0 : scal[E.slot];
+      const zd cf = zmk(E.cre * sc, E.cim * sc);
+      const double *Bm = mats + ((size_t)E.mat * N + i2) * bw;
+      const zd *src = y + E.col * NN + (long)i1 * N;
+      zd t = zmk(0.0, 0.0);
+      const int d0 = s - i2 > 0 ? s - i2 : 0, d1 = N - 1 - i2 + s < bw - 1 ? N - 1 - i2 + s : bw - 1;
+      for (int d = d0; d <= d1; d++) zfmar(t, Bm[d], src[i2 + d - s]);
+      zfma(acc, cf, t);
+    }
+    X[(long)r * NN + (long)i1 * N + i2] = acc;
+  }
+}
+
+// Z[c][i1][i2] = sum_e coef_e * s_e * sum_d A_e[i1][d] * X[k_e][c'_e][i1+d-s][i2]
+__global__ void __launch_bounds__(128) p3_left(const double *mats, const int *row, const p3_ent *ent, const zd *X,
+                                                zd *Z, int C, int N, int s, int bw, const double *scal,
+                                                const int *state)
+{
+  if (state[0]) return;
+  const int c = blockIdx.x / N, i1 = blockIdx.x % N;
+  const long NN = (long)N * N;
+  const int d0 = s - i1 > 0 ? s - i1 : 0, d1 = N - 1 - i1 + s < bw - 1 ? N - 1 - i1 + s : bw - 1;
+  for (int i2 = threadIdx.x; i2 < N; i2 += blockDim.x) {
+    zd acc = zmk(0.0, 0.0);
+    for (int e = row[c]; e < row[c + 1]; e++) {
+      const p3_ent E = ent[e];
+      const double sc = E.slot < 0 ? 1.0 : scal[E.slot];
+      const zd cf = zmk(E.cre * sc, E.cim * sc);
+      const double *Am = mats + ((size_t)E.mat * N + i1) * bw;
+      const zd *src = X + ((long)E.src * C + E.col) * NN + i2;
+      zd t = zmk(0.0, 0.0);
+      for (int d = d0; d <= d1; d++) zfmar(t, Am[d], src[(long)(i1 + d - s) * N]);
+      zfma(acc, cf, t);
+    }
+    Z[(long)c * NN + (long)i1 * N + i2] = acc;
+  }
+}
+
+// ------------------------------------------------------------------ multipole terms
+// Ym[b][j1][j2][cl] = sum_e coef_e * s_e * y[c'_e][j1][j2], row b*16 + cl
+__global__ void __launch_bounds__(256) p3_mix(const int *row, const p3_ent *ent, const zd *y, zd *Ym, int N,
+                                               const double *scal, const int *state)
+{
+  if (state[0]) return;
+  const int b = blockIdx.x / N, j1 = blockIdx.x % N;
+  const long NN = (long)N * N;
+  const int cl = threadIdx.x & (P3_CHUNK - 1);
+  const int r = b * P3_CHUNK + cl;
+  const int e0 = row[r], e1 = row[r + 1];
+  zd *out = Ym + ((long)b * NN + (long)j1 * N) * P3_CHUNK;
+  for (int j2 = threadIdx.x / P3_CHUNK; j2 < N; j2 += 256 / P3_CHUNK) {
+    zd acc = zmk(0.0, 0.0);
+    for (int e = e0; e < e1; e++) {
+      const p3_ent E = ent[e];
+      const double sc = E.slot < 0 ? 1.0 : scal[E.slot];
+      zfma(acc, zmk(E.cre * sc, E.cim * sc), y[E.col * NN + (long)j1 * N + j2]);
+    }
+    out[(long)j2 * P3_CHUNK + cl] = acc;
+  }
+}
+
+// Z[c][i1][i2] += sum_blocks sum_{d1,d2} R_t[i1][d1][i2][d2] Ym[b][i1+d1-s][i2+d2-s][cl]
+// workgroup: (chunk, i1, i2 half); thread: channel pair cp (8) x i2 quad q (16).
+template <int S>
+__global__ void __launch_bounds__(P3_EE_THREADS) p3_ee(const double *const *Rt, const int *blk_term, const int *cb_ptr,
+                                                       const int *cb_blk, const zd *Ym, zd *Z, int C, int N,
+                                                       int nhalf, int nchunk, const int *state)
+{
+  if (state[0]) return;
+  constexpr int BW = 2 * S + 1;
+  constexpr int WIN = P3_HALF + 2 * S;                  // staged input positions
+  __shared__ double Rs[P3_HALF * BW];                   // [i2 local][d2]
+  __shared__ zd Ys[WIN * P3_CHUNK];                     // [position][cl]
+  // blockIdx -> (group of 8 WGs on one XCD) so the chunks of one (i1, half)
+  // share the XCD's L2 for their R slices
+  const int g = blockIdx.x, xcd = g & 7, w = g >> 3;
+  const int pairIdx = (w / nchunk) * 8 + xcd;           // (i1, half) pair
+  const int chunk = w % nchunk;
+  if (pairIdx >= N * nhalf) return;
+  const int i1 = pairIdx / nhalf, half = pairIdx % nhalf;
+  const int i2base = half * P3_HALF;
+  const int nloc = N - i2base < P3_HALF ? N - i2base : P3_HALF;
+  const long NN = (long)N * N;
+  const int tid = threadIdx.x, cp = tid >> 4, q = tid & 15;
+  zd acc[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc[a][t] = zmk(0.0, 0.0);
+  for (int bi = cb_ptr[chunk]; bi < cb_ptr[chunk + 1]; bi++) {
+    const int b = cb_blk[bi];
+    const double *R = Rt[blk_term[b]];
+    const zd *Yb = Ym + (long)b * NN * P3_CHUNK;
+    for (int d1 = 0; d1 < BW; d1++) {
+      const int j1 = i1 + d1 - S;
+      if (j1 < 0 || j1 >= N) continue;      // uniform over the workgroup
+      __syncthreads();
+      const double *Rrow = R + (((size_t)i1 * BW + d1) * N + i2base) * BW;
+      for (int e = tid; e < nloc * BW; e += P3_EE_THREADS) Rs[e] = Rrow[e];
+      for (int e = nloc * BW + tid; e < P3_HALF * BW; e += P3_EE_THREADS) Rs[e] = 0.0;
+      const zd *Yrow = Yb + (long)j1 * N * P3_CHUNK;
+      for (int e = tid; e < WIN * P3_CHUNK; e += P3_EE_THREADS) {
+        const int p = i2base - S + e / P3_CHUNK;
+        Ys[e] = (p >= 0 && p < N) ? Yrow[(long)p * P3_CHUNK + (e & (P3_CHUNK - 1))] : zmk(0.0, 0.0);
+      }
+      __syncthreads();
+      // outputs i2 = 4q + t; input position of tap d2 is 4q + t + d2 (window base i2base - S)
+      const double *Rq = Rs + (4 * q) * BW;
+      const zd *Yq = Ys + (4 * q) * P3_CHUNK + 2 * cp;
+      zd w0[4], w1[4];
+#pragma unroll
+      for (int t = 0; t < 3; t++) { w0[t + 1] = Yq[t * P3_CHUNK]; w1[t + 1] = Yq[t * P3_CHUNK + 1]; }
+#pragma unroll
+      for (int d2 = 0; d2 < BW; d2++) {
+        // slide: window holds positions d2 .. d2+3
+#pragma unroll
+        for (int t = 0; t < 3; t++) { w0[t] = w0[t + 1]; w1[t] = w1[t + 1]; }
+        w0[3] = Yq[(d2 + 3) * P3_CHUNK];
+        w1[3] = Yq[(d2 + 3) * P3_CHUNK + 1];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const double r = Rq[t * BW + d2];
+          zfmar(acc[0][t], r, w0[t]);
+          zfmar(acc[1][t], r, w1[t]);
+        }
+      }
+    }
+  }
+  const int cbase = chunk * P3_CHUNK + 2 * cp;
+#pragma unroll
+  for (int a = 0; a < 2; a++) {
+    const int c = cbase + a;
+    if (c >= C) continue;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int i2l = 4 * q + t;
+      if (i2l >= nloc) continue;
+      zd *z = Z + (long)c * NN + (long)i1 * N + i2base + i2l;
+      *z = zadd(*z, acc[a][t]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ Gram-Schmidt
+// part[blk*pstr + 2k..] = partial <V_k, w> over this block's range, k < nk;
+// nk == 0: part[blk*pstr] = partial |w|^2.
+__global__ void __launch_bounds__(256) p3_dots(const zd *V, const zd *w, long n, int nk, double *part, int pstr,
+                                                const int *state, int skip_if_broken)
+{
+  if (skip_if_broken && state[0]) return;
+  __shared__ double red[8];
+  const long per = (n + gridDim.x - 1) / gridDim.x;
+  const long lo = blockIdx.x * per, hi = lo + per < n ? lo + per : n;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (nk == 0) {
+    double a = 0.0;
+    for (long i = lo + tid; i < hi; i += 256) { const zd v = w[i]; a = fma(v.x, v.x, fma(v.y, v.y, a)); }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    if (lane == 0) red[wv] = a;
+    __syncthreads();
+    if (tid == 0) part[(long)blockIdx.x * pstr] = (red[0] + red[1]) + (red[2] + red[3]);
+    return;
+  }
+  for (int k = wv; k < nk; k += 4) {
+    const zd *vk = V + (long)k * n;
+    double re = 0.0, im = 0.0;
+#pragma unroll 4
+    for (long i = lo + lane; i < hi; i += 64) {
+      const zd v = vk[i], x = w[i];
+      re = fma(v.x, x.x, re); re = fma(v.y, x.y, re);
+      im = fma(v.x, x.y, im); im = fma(-v.y, x.x, im);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { re += __shfl_xor(re, o, 64); im += __shfl_xor(im, o, 64); }
+    if (lane == 0) { part[(long)blockIdx.x * pstr + 2 * k] = re; part[(long)blockIdx.x * pstr + 2 * k + 1] = im; }
+  }
+}
+
+// Sum the partials in one fixed order.  mode 0: h_k = <V_k, w> for k <= j,
+// written to hcoef and column j of Hm.  mode 1: beta_{j+1} = |w| with the
+// breakdown test (pamp.h:305-312); mode 2: beta_0 = |psi~| (start of a step,
+// resets the breakdown state).
+__global__ void __launch_bounds__(256) p3_reduce(const double *part, int pstr, int nblk, int mode, int j, int m,
+                                                  zd *hcoef, zd *Hm, double *beta, int *state)
+{
+  const int tid = threadIdx.x;
+  if (mode != 2 && state[0]) return;
+  const int ncomp = mode == 0 ? 2 * (j + 1) : 1;
+  for (int c = tid; c < ncomp; c += 256) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int b = 0;
+    for (; b + 3 < nblk; b += 4) {
+      s0 += part[(long)b * pstr + c]; s1 += part[(long)(b + 1) * pstr + c];
+      s2 += part[(long)(b + 2) * pstr + c]; s3 += part[(long)(b + 3) * pstr + c];
+    }
+    for (; b < nblk; b++) s0 += part[(long)b * pstr + c];
+    const double v = (s0 + s1) + (s2 + s3);
+    if (mode == 0) {
+      ((double *)hcoef)[c] = v;   // h_k = (v[2k], v[2k+1])
+    } else {
+      const double bt = sqrt(v);
+      if (mode == 2) {
+        beta[0] = bt;
+        state[0] = 0;
+        state[1] = m;
+      } else {
+        const double tol = 1.4901161193847656e-08;   // sqrt(DBL_EPSILON), pamp.h:75
+        if (fabs(bt) < tol) { state[0] = 1; state[1] = j + 1; }
+        else { beta[j + 1] = bt; Hm[(j + 1) + j * m] = zmk(bt, 0.0); }
+      }
+    }
+  }
+  if (mode == 0) {
+    __syncthreads();
+    for (int k = tid; k <= j; k += 256) Hm[k + j * m] = hcoef[k];
+  }
+}
+
+// w -= sum_{k<nk} V_k h_k;  partial |w|^2 per block
+__global__ void __launch_bounds__(256) p3_project(const zd *V, zd *w, long n, int nk, const zd *hcoef, double *part,
+                                                   int pstr, const int *state)
+{
+  if (state[0]) return;
+  __shared__ zd h[64];
+  __shared__ double red[4];
+  const int tid = threadIdx.x;
+  for (int k = tid; k < nk; k += 256) h[k] = hcoef[k];
+  __syncthreads();
+  const long per = (n + gridDim.x - 1) / gridDim.x;
+  const long lo = blockIdx.x * per, hi = lo + per < n ? lo + per : n;
+  double a = 0.0;
+  for (long i = lo + tid; i < hi; i += 256) {
+    zd x = w[i];
+#pragma unroll 4
+    for (int k = 0; k < nk; k++) x = zsub(x, zmul(V[(long)k * n + i], h[k]));
+    w[i] = x;
+    a = fma(x.x, x.x, fma(x.y, x.y, a));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = a;
+  __syncthreads();
+  if (tid == 0) part[(long)blockIdx.x * pstr] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// dst1 = dst2 = src / beta[bi]
+__global__ void __launch_bounds__(256) p3_scale(const zd *src, zd *dst1, zd *dst2, long n, const double *beta, int bi,
+                                                 const int *state)
+{
+  if (state[0]) return;
+  const double inv = 1.0 / beta[bi];
+  for (long i = blockIdx.x * 256l + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const zd v = zscale(src[i], inv);
+    dst1[i] = v;
+    dst2[i] = v;
+  }
+}
+
+__global__ void __launch_bounds__(256) p3_zero(zd *a, long cnt)
+{
+  for (long i = blockIdx.x * 256l + threadIdx.x; i < cnt; i += (long)gridDim.x * 256) a[i] = zmk(0.0, 0.0);
+}
+
+__global__ void __launch_bounds__(PP_THREADS) p3_pade(int m, zd *Hm, double dt_re, double dt_im, zd *cvec, double *diag,
+                                                     const int *state, const double *beta, int normscale)
+{
+  pade_block(m, Hm, dt_re, dt_im, 0, cvec, diag, normscale);
+  if (threadIdx.x == 0) {
+    diag[1] = state[1];
+    diag[2] = beta[0];
+    diag[3] = 0.0;
+  }
+}
+
+// out = beta_0 sum_{k < mEff} c_k V_k
+__global__ void __launch_bounds__(256) p3_combine(const zd *V, const zd *cvec, const double *beta, const int *state,
+                                                   zd *out, long n)
+{
+  __shared__ zd c[64];
+  const int mEff = state[1];
+  for (int k = threadIdx.x; k < mEff; k += 256) c[k] = zld(cvec + k);
+  __syncthreads();
+  const double b0 = beta[0];
+  for (long i = blockIdx.x * 256l + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    zd acc = zmk(0.0, 0.0);
+    for (int k = 0; k < mEff; k++) zfma(acc, V[(long)k * n + i], c[k]);
+    out[i] = zscale(acc, b0);
+  }
+}
+
+// ------------------------------------------------------------------ host side
+extern "C" int pp_overlap_apply(const pp_overlap *ov, int op, int rank, int R, const int *shape, void *x,
+                                hipStream_t stream);
+
+static void p3_free_plan(pp_pamp3 *p)
+{
+  void *bufs[] = {p->dmats, p->rrow, p->lrow, p->mrow, p->rent, p->lent, p->ment, p->cb_ptr, p->cb_blk,
+                  p->blk_term, (void *)p->dR, p->X, p->Ym};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  p->dmats = nullptr; p->rrow = p->lrow = p->mrow = nullptr; p->rent = p->lent = p->ment = nullptr;
+  p->cb_ptr = p->cb_blk = p->blk_term = nullptr; p->dR = nullptr; p->X = nullptr; p->Ym = nullptr;
+}
+
+static void p3_collect(pp_pamp3 *p);
+
+extern "C" int pp_pamp3_create(int C, int N, int m, int reorth, const pp_overlap *ov, pp_pamp3 **out)
+{
+  if (!out || !ov || C <= 0 || N <= 0 || m < 1 || m > 48) return PP_EARG;
+  if (reorth) return PP_EUNSUP;
+  if (ov->N != N || ov->kd != 6) return PP_EUNSUP;   // order-7 B-splines (13 diagonals)
+  pp_pamp3 *p = new pp_pamp3();
+  p->C = C; p->N = N; p->m = m; p->reorth = 0; p->s = ov->kd; p->bw = 2 * ov->kd + 1;
+  p->n = (long)C * N * N; p->ov = ov;
+  p->nslots = 0; p->dirty = true;
+  p->dmats = nullptr; p->rrow = p->lrow = p->mrow = nullptr; p->rent = p->lent = p->ment = nullptr;
+  p->cb_ptr = p->cb_blk = p->blk_term = nullptr; p->dR = nullptr; p->X = nullptr; p->Ym = nullptr;
+  p->K = 0; p->nblk = 0; p->nchunk = (C + P3_CHUNK - 1) / P3_CHUNK;
+  p->scal = p->diag = nullptr; p->scal_cap = p->diag_cap = 0;
+  p->timing = 0; p->ee_ms = 0.0; p->ee_count = 0;
+  const size_t vb = sizeof(zd) * (size_t)p->n;
+  bool ok = hipMalloc(&p->Y, vb) == hipSuccess && hipMalloc(&p->Z, vb) == hipSuccess &&
+            hipMalloc(&p->P, vb) == hipSuccess && hipMalloc(&p->V, vb * (size_t)(m + 1)) == hipSuccess &&
+            hipMalloc(&p->part, sizeof(double) * P3_RED_BLOCKS * 128) == hipSuccess &&
+            hipMalloc(&p->Hm, sizeof(zd) * m * m) == hipSuccess && hipMalloc(&p->cvec, sizeof(zd) * m) == hipSuccess &&
+            hipMalloc(&p->hcoef, sizeof(zd) * 64) == hipSuccess && hipMalloc(&p->beta, sizeof(double) * 64) == hipSuccess &&
+            hipMalloc(&p->state, sizeof(int) * 4) == hipSuccess;
+  if (!ok) { delete p; return PP_EHIP; }
+  *out = p;
+  return PP_OK;
+}
+
+extern "C" int pp_pamp3_destroy(pp_pamp3 *p)
+{
+  if (!p) return PP_OK;
+  p3_collect(p);
+  p3_free_plan(p);
+  void *bufs[] = {p->Y, p->Z, p->P, p->V, p->part, p->Hm, p->cvec, p->hcoef, p->beta, p->state, p->scal, p->diag};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  for (auto &t : p->mterms)
+    if (t.R) (void)hipFree(t.R);
+  delete p;
+  return PP_OK;
+}
+
+static int p3_mat_id(pp_pamp3 *p, const double *M)
+{
+  const size_t cnt = (size_t)p->N * p->bw;
+  for (size_t i = 0; i < p->mats.size(); i++)
+    if (std::memcmp(p->mats[i].data(), M, cnt * sizeof(double)) == 0) return (int)i;
+  p->mats.emplace_back(M, M + cnt);
+  return (int)p->mats.size() - 1;
+}
+
+static int p3_check_pairs(const pp_pamp3 *p, int P, const int *pairs, const double *coef, int slot)
+{
+  if (P < 0 || (P > 0 && (!pairs || !coef)) || slot < -1 || slot >= 32) return PP_EARG;
+  for (int i = 0; i < 2 * P; i++)
+    if (pairs[i] < 0 || pairs[i] >= p->C) return PP_EARG;
+  return PP_OK;
+}
+
+extern "C" int pp_pamp3_add_kron_term(pp_pamp3 *p, int P, const int *pairs, const double *coef, const double *A,
+                                      const double *B, int slot)
+{
+  if (!p || !A || !B) return PP_EARG;
+  int rc = p3_check_pairs(p, P, pairs, coef, slot);
+  if (rc) return rc;
+  pp_pamp3::kterm t;
+  t.pr.assign(pairs, pairs + 2 * P);
+  t.cf.assign(coef, coef + 2 * P);
+  t.A = p3_mat_id(p, A);
+  t.B = p3_mat_id(p, B);
+  t.slot = slot;
+  p->kterms.push_back(t);
+  if (slot + 1 > p->nslots) p->nslots = slot + 1;
+  p->dirty = true;
+  return PP_OK;
+}
+
+extern "C" int pp_pamp3_add_multipole_term(pp_pamp3 *p, int P, const int *pairs, const double *coef, const double *R,
+                                           int slot)
+{
+  if (!p || !R) return PP_EARG;
+  int rc = p3_check_pairs(p, P, pairs, coef, slot);
+  if (rc) return rc;
+  const size_t cnt = (size_t)p->N * p->bw * p->N * p->bw;
+  pp_pamp3::mterm t;
+  t.pr.assign(pairs, pairs + 2 * P);
+  t.cf.assign(coef, coef + 2 * P);
+  t.slot = slot;
+  t.R = nullptr;
+  if (hipMalloc(&t.R, cnt * sizeof(double)) != hipSuccess) return PP_EHIP;
+  if (hipMemcpy(t.R, R, cnt * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { (void)hipFree(t.R); return PP_EHIP; }
+  p->mterms.push_back(t);
+  if (slot + 1 > p->nslots) p->nslots = slot + 1;
+  p->dirty = true;
+  return PP_OK;
+}
+
+template <class T>
+static int p3_upload(T **dst, const std::vector<T> &v)
+{
+  const size_t bytes = sizeof(T) * (v.empty() ? 1 : v.size());
+  if (hipMalloc(dst, bytes) != hipSuccess) return PP_EHIP;
+  if (!v.empty() && hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice) != hipSuccess) return PP_EHIP;
+  return PP_OK;
+}
+
+// Build the pass tables (see the file header): every Kronecker term goes to
+// the side whose matrix it shares with more terms.
+static int p3_build(pp_pamp3 *p)
+{
+  p3_free_plan(p);
+  const int C = p->C;
+  std::vector<int> cnt(p->mats.size(), 0);
+  for (auto &t : p->kterms) { cnt[t.A]++; cnt[t.B]++; }
+  // intermediates: ('L', B) plain i2 pass with matrix B, mixing in the i1 pass;
+  //                ('R', A) mixing in the i2 pass, plain i1 pass with matrix A
+  std::vector<std::pair<char, int>> inter;
+  auto find = [&](char kind, int mat) {
+    for (size_t i = 0; i < inter.size(); i++)
+      if (inter[i].first == kind && inter[i].second == mat) return (int)i;
+    inter.push_back({kind, mat});
+    return (int)inter.size() - 1;
+  };
+  std::vector<std::vector<p3_ent>> rrows, lrows(C);
+  for (auto &t : p->kterms) {
+    const bool left_mix = cnt[t.B] >= cnt[t.A];
+    const int k = find(left_mix ? 'L' : 'R', left_mix ? t.B : t.A);
+    if ((int)rrows.size() < (k + 1) * C) rrows.resize((size_t)(k + 1) * C);
+    const int P = (int)t.pr.size() / 2;
+    for (int q = 0; q < P; q++) {
+      const int row = t.pr[2 * q], col = t.pr[2 * q + 1];
+      if (left_mix) lrows[row].push_back({k, col, t.A, t.slot, t.cf[2 * q], t.cf[2 * q + 1]});
+      else rrows[(size_t)k * C + row].push_back({0, col, t.B, t.slot, t.cf[2 * q], t.cf[2 * q + 1]});
+    }
+  }
+  p->K = (int)inter.size();
+  rrows.resize((size_t)p->K * C);
+  for (int k = 0; k < p->K; k++) {
+    if (inter[k].first == 'L') {
+      for (int c = 0; c < C; c++) rrows[(size_t)k * C + c].push_back({0, c, inter[k].second, -1, 1.0, 0.0});
+    } else {
+      for (int c = 0; c < C; c++) lrows[c].push_back({k, c, inter[k].second, -1, 1.0, 0.0});
+    }
+  }
+  auto flatten = [](const std::vector<std::vector<p3_ent>> &rows, std::vector<int> &ptr, std::vector<p3_ent> &ent) {
+    ptr.assign(1, 0);
+    for (auto &r : rows) { ent.insert(ent.end(), r.begin(), r.end()); ptr.push_back((int)ent.size()); }
+  };
+  std::vector<int> rp, lp, mp;
+  std::vector<p3_ent> re, le, me;
+  flatten(rrows, rp, re);
+  flatten(lrows, lp, le);
+  // multipole blocks: (term, chunk) with any row in the chunk
+  std::vector<std::vector<p3_ent>> mrows;
+  std::vector<int> blk_term;
+  std::vector<std::vector<int>> chunk_blocks(p->nchunk);
+  for (size_t t = 0; t < p->mterms.size(); t++) {
+    auto &mt = p->mterms[t];
+    std::vector<std::vector<p3_ent>> rows(C);
+    const int P = (int)mt.pr.size() / 2;
+    for (int q = 0; q < P; q++)
+      rows[mt.pr[2 * q]].push_back({0, mt.pr[2 * q + 1], 0, mt.slot, mt.cf[2 * q], mt.cf[2 * q + 1]});
+    for (int ch = 0; ch < p->nchunk; ch++) {
+      bool any = false;
+      for (int cl = 0; cl < P3_CHUNK && ch * P3_CHUNK + cl < C; cl++) any = any || !rows[ch * P3_CHUNK + cl].empty();
+      if (!any) continue;
+      const int b = (int)blk_term.size();
+      blk_term.push_back((int)t);
+      chunk_blocks[ch].push_back(b);
+      for (int cl = 0; cl < P3_CHUNK; cl++) {
+        const int c = ch * P3_CHUNK + cl;
+        mrows.push_back(c < C ? rows[c] : std::vector<p3_ent>());
+      }
+    }
+  }
+  p->nblk = (int)blk_term.size();
+  flatten(mrows, mp, me);
+  std::vector<int> cbp(1, 0), cbb;
+  for (auto &v : chunk_blocks) { cbb.insert(cbb.end(), v.begin(), v.end()); cbp.push_back((int)cbb.size()); }
+  std::vector<double> mats;
+  for (auto &mm : p->mats) mats.insert(mats.end(), mm.begin(), mm.end());
+  std::vector<const double *> rptr;
+  for (auto &mt : p->mterms) rptr.push_back(mt.R);
+  int rc;
+  if ((rc = p3_upload(&p->dmats, mats)) || (rc = p3_upload(&p->rrow, rp)) || (rc = p3_upload(&p->rent, re)) ||
+      (rc = p3_upload(&p->lrow, lp)) || (rc = p3_upload(&p->lent, le)) || (rc = p3_upload(&p->mrow, mp)) ||
+      (rc = p3_upload(&p->ment, me)) || (rc = p3_upload(&p->cb_ptr, cbp)) || (rc = p3_upload(&p->cb_blk, cbb)) ||
+      (rc = p3_upload(&p->blk_term, blk_term)) || (rc = p3_upload(&p->dR, rptr)))
+    return rc;
+  const size_t vb = sizeof(zd) * (size_t)p->n;
+  if (p->K > 0 && hipMalloc(&p->X, vb * (size_t)p->K) != hipSuccess) return PP_EHIP;
+  if (p->nblk > 0 && hipMalloc(&p->Ym, sizeof(zd) * (size_t)p->nblk * p->N * p->N * P3_CHUNK) != hipSuccess)
+    return PP_EHIP;
+  p->dirty = false;
+  return PP_OK;
+}
+
+// z = H y (not balanced) with the time scalings at `scal` (device, nslots)
+static int p3_apply_raw(pp_pamp3 *p, const zd *y, zd *z, const double *scal, hipStream_t st)
+{
+  const int C = p->C, N = p->N;
+  if (p->K > 0) {
+    hipLaunchKernelGGL(p3_right, dim3(p->K * C * N), dim3(128), 0, st, p->dmats, p->rrow, p->rent, y, p->X, C, N, p->s,
+                       p->bw, scal, p->state);
+    hipLaunchKernelGGL(p3_left, dim3(C * N), dim3(128), 0, st, p->dmats, p->lrow, p->lent, p->X, z, C, N, p->s, p->bw,
+                       scal, p->state);
+  } else {
+    hipLaunchKernelGGL(p3_zero, dim3(1024), dim3(256), 0, st, z, p->n);
+  }
+  if (p->nblk > 0) {
+    hipLaunchKernelGGL(p3_mix, dim3(p->nblk * N), dim3(256), 0, st, p->mrow, p->ment, y, p->Ym, N, scal, p->state);
+    const int nhalf = (N + P3_HALF - 1) / P3_HALF;
+    const int pairs = N * nhalf;
+    const int groups = (pairs + 7) / 8;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (p->timing) {
+      if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return PP_EHIP;
+      (void)hipEventRecord(e0, st);
+    }
+    hipLaunchKernelGGL(p3_ee<6>, dim3(groups * 8 * p->nchunk), dim3(P3_EE_THREADS), 0, st, p->dR, p->blk_term,
+                       p->cb_ptr, p->cb_blk, p->Ym, z, C, N, nhalf, p->nchunk, p->state);
+    if (p->timing) {
+      (void)hipEventRecord(e1, st);
+      p->ev.push_back(e0);
+      p->ev.push_back(e1);
+    }
+  }
+  return hipGetLastError() == hipSuccess ? PP_OK : PP_EHIP;
+}
+
+static int p3_solve2(pp_pamp3 *p, int op, zd *x, hipStream_t st)
+{
+  const int shape[3] = {p->C, p->N, p->N};
+  int rc = pp_overlap_apply(p->ov, op, 2, 3, shape, x, st);
+  if (rc == PP_OK) rc = pp_overlap_apply(p->ov, op, 1, 3, shape, x, st);
+  return rc;
+}
+
+static int p3_ensure(pp_pamp3 *p, int nsteps, const double *scal)
+{
+  if (p->dirty) {
+    int rc = p3_build(p);
+    if (rc) return rc;
+  }
+  const int ns = p->nslots > 0 ? p->nslots : 1;
+  if (nsteps * ns > p->scal_cap) {
+    if (p->scal) (void)hipFree(p->scal);
+    p->scal_cap = nsteps * ns;
+    if (hipMalloc(&p->scal, sizeof(double) * p->scal_cap) != hipSuccess) { p->scal = nullptr; p->scal_cap = 0; return PP_EHIP; }
+  }
+  if (nsteps * 4 > p->diag_cap) {
+    if (p->diag) (void)hipFree(p->diag);
+    p->diag_cap = nsteps * 4;
+    if (hipMalloc(&p->diag, sizeof(double) * p->diag_cap) != hipSuccess) { p->diag = nullptr; p->diag_cap = 0; return PP_EHIP; }
+  }
+  if (p->nslots > 0) {
+    if (!scal) return PP_EARG;
+    if (hipMemcpy(p->scal, scal, sizeof(double) * nsteps * p->nslots, hipMemcpyHostToDevice) != hipSuccess) return PP_EHIP;
+  }
+  return PP_OK;
+}
+
+extern "C" int pp_pamp3_apply(pp_pamp3 *p, const void *src, void *dst, const double *scal, int balanced,
+                              hipStream_t stream)
+{
+  if (!p || !src || !dst) return PP_EARG;
+  int rc = p3_ensure(p, 1, scal);
+  if (rc) return rc;
+  int zero_state[4] = {0, p->m, 0, 0};
+  if (hipMemcpyAsync(p->state, zero_state, sizeof(zero_state), hipMemcpyHostToDevice, stream) != hipSuccess) return PP_EHIP;
+  if (hipMemcpyAsync(p->Y, src, sizeof(zd) * (size_t)p->n, hipMemcpyDeviceToDevice, stream) != hipSuccess) return PP_EHIP;
+  if (balanced && (rc = p3_solve2(p, 0, p->Y, stream))) return rc;
+  if ((rc = p3_apply_raw(p, p->Y, (zd *)dst, p->scal, stream))) return rc;
+  if (balanced && (rc = p3_solve2(p, 1, (zd *)dst, stream))) return rc;
+  return hipStreamSynchronize(stream) == hipSuccess ? PP_OK : PP_EHIP;
+}
+
+extern "C" int pp_pamp3_propagate(pp_pamp3 *p, void *psi, int nsteps, double dt_re, double dt_im, const double *scal,
+                                  hipStream_t stream)
+{
+  if (!p || !psi || nsteps < 0) return PP_EARG;
+  if (nsteps == 0) return PP_OK;
+  int rc = p3_ensure(p, nsteps, scal);
+  if (rc) return rc;
+  const long n = p->n;
+  const int m = p->m, G = P3_RED_BLOCKS, pstr = 128;
+  const size_t vb = sizeof(zd) * (size_t)n;
+  zd *ps = (zd *)psi;
+  const int gridv = 1024;
+  const size_t lds = sizeof(zd) * (size_t)(9 * m * m) + 64 * sizeof(double);
+  if (hipFuncSetAttribute((const void *)p3_pade, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+    return PP_EHIP;
+  for (int step = 0; step < nsteps; step++) {
+    const double *sc = p->scal + (size_t)step * (p->nslots > 0 ? p->nslots : 1);
+    hipLaunchKernelGGL(p3_zero, dim3(1), dim3(256), 0, stream, p->Hm, (long)m * m);
+    // psi~ = (U x U) psi
+    if (hipMemcpyAsync(p->P, ps, vb, hipMemcpyDeviceToDevice, stream) != hipSuccess) return PP_EHIP;
+    if ((rc = p3_solve2(p, 2 /* U x */, p->P, stream))) return rc;
+    hipLaunchKernelGGL(p3_dots, dim3(G), dim3(256), 0, stream, p->V, p->P, n, 0, p->part, pstr, p->state, 0);
+    hipLaunchKernelGGL(p3_reduce, dim3(1), dim3(256), 0, stream, p->part, pstr, G, 2, -1, m, p->hcoef, p->Hm, p->beta,
+                       p->state);
+    hipLaunchKernelGGL(p3_scale, dim3(gridv), dim3(256), 0, stream, p->P, p->V, p->Y, n, p->beta, 0, p->state);
+    for (int j = 0; j < m; j++) {
+      if ((rc = p3_solve2(p, 0, p->Y, stream))) return rc;
+      if ((rc = p3_apply_raw(p, p->Y, p->Z, sc, stream))) return rc;
+      if ((rc = p3_solve2(p, 1, p->Z, stream))) return rc;
+      hipLaunchKernelGGL(p3_dots, dim3(G), dim3(256), 0, stream, p->V, p->Z, n, j + 1, p->part, pstr, p->state, 1);
+      hipLaunchKernelGGL(p3_reduce, dim3(1), dim3(256), 0, stream, p->part, pstr, G, 0, j, m, p->hcoef, p->Hm,
+                         p->beta, p->state);
+      if (j == m - 1) break;   // the last residual is not used (pamp.h:419-426)
+      hipLaunchKernelGGL(p3_project, dim3(G), dim3(256), 0, stream, p->V, p->Z, n, j + 1, p->hcoef, p->part, pstr,
+                         p->state);
+      hipLaunchKernelGGL(p3_reduce, dim3(1), dim3(256), 0, stream, p->part, pstr, G, 1, j, m, p->hcoef, p->Hm,
+                         p->beta, p->state);
+      hipLaunchKernelGGL(p3_scale, dim3(gridv), dim3(256), 0, stream, p->Z, p->V + (size_t)(j + 1) * n, p->Y, n,
+                         p->beta, j + 1, p->state);
+    }
+    hipLaunchKernelGGL(p3_pade, dim3(1), dim3(PP_THREADS), lds, stream, m, p->Hm, dt_re, dt_im, p->cvec,
+                       p->diag + (size_t)step * 4, p->state, p->beta, 0);
+    hipLaunchKernelGGL(p3_combine, dim3(gridv), dim3(256), 0, stream, p->V, p->cvec, p->beta, p->state, ps, n);
+    if ((rc = p3_solve2(p, 0, ps, stream))) return rc;
+    if (hipGetLastError() != hipSuccess) return PP_EHIP;
+  }
+  return hipStreamSynchronize(stream) == hipSuccess ? PP_OK : PP_EHIP;
+}
+
+// Collect the event pairs recorded since the last call (the stream has been
+// synchronised by the caller).
+static void p3_collect(pp_pamp3 *p)
+{
+  for (size_t i = 0; i + 1 < p->ev.size(); i += 2) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]) == hipSuccess) {
+      p->ee_ms += ms;
+      p->ee_count++;
+    }
+    (void)hipEventDestroy(p->ev[i]);
+    (void)hipEventDestroy(p->ev[i + 1]);
+  }
+  p->ev.clear();
+}
+
+extern "C" int pp_pamp3_set_timing(pp_pamp3 *p, int enable)
+{
+  if (!p) return PP_EARG;
+  p3_collect(p);
+  p->timing = enable != 0;
+  p->ee_ms = 0.0;
+  p->ee_count = 0;
+  return PP_OK;
+}
+
+extern "C" int pp_pamp3_kernel_time(pp_pamp3 *p, double *ms_total, long *launches)
+{
+  if (!p || !ms_total || !launches) return PP_EARG;
+  if (hipDeviceSynchronize() != hipSuccess) return PP_EHIP;
+  p3_collect(p);
+  *ms_total = p->ee_ms;
+  *launches = p->ee_count;
+  return PP_OK;
+}
+
+// Algorithmic flops of one multipole contraction launch: per term, every
+// output channel with a coupling times every (i1, j1, i2, j2) inside both
+// bands, 4 flops per real x complex multiply-add.
+extern "C" int pp_pamp3_multipole_flops(pp_pamp3 *p, double *flops)
+{
+  if (!p || !flops) return PP_EARG;
+  const long N = p->N, s = p->s;
+  long bandpairs = 0;
+  for (long i = 0; i < N; i++) bandpairs += std::min(N - 1, i + s) - std::max(0l, i - s) + 1;
+  double f = 0.0;
+  for (auto &t : p->mterms) {
+    std::vector<char> rowset(p->C, 0);
+    for (size_t q = 0; q < t.pr.size(); q += 2) rowset[t.pr[q]] = 1;
+    long rows = 0;
+    for (char r : rowset) rows += r;
+    f += 4.0 * (double)rows * (double)bandpairs * (double)bandpairs;
+  }
+  *flops = f;
+  return PP_OK;
+}
+
+extern "C" int pp_pamp3_diagnostics(pp_pamp3 *p, int nsteps, double *diag, hipStream_t stream)
+{
+  if (!p || !diag || nsteps < 0 || nsteps * 4 > p->diag_cap) return PP_EARG;
+  if (hipMemcpyAsync(diag, p->diag, sizeof(double) * 4 * nsteps, hipMemcpyDeviceToHost, stream) != hipSuccess)
+    return PP_EHIP;
+  return hipStreamSynchronize(stream) == hipSuccess ? PP_OK : PP_EHIP;
+}

@@ -1,0 +1,367 @@
+"""Two-electron potentials in coupled spherical harmonics x two B-spline
+radial ranks, in factored form, and their mapping onto the rank-3 engine.
+
+The reference builds each potential of examples/tensor/helium_stabilization/
+config.ini as a full rank-3 tensor (angular pairs x radial pairs x radial
+pairs) by evaluating it on the product quadrature grid and projecting rank by
+rank (pyprop/tensorpotential/TensorPotentialGenerator.py:72-256 with the
+custom evaluators of coupledspherical.cpp, coupledvelocity.cpp and
+potential.cpp).  Every potential on this path is a sum of terms
+
+  sum_p a_p |c_p><c'_p|  (x)  A (x) B        (one-electron, dipole), or
+  sum_p a_p |c_p><c'_p|  (x)  R_l            (electron-electron, one per l),
+
+with banded radial matrices A, B (13 diagonals) and the 4-index multipole
+tensor R_l[i1, j1, i2, j2] = sum_{q1,q2} w_q1 B_i1 B_j1 (q1) w_q2 B_i2 B_j2 (q2)
+r<^l / r>^(l+1) on the same product grid.  Because the product quadrature
+factorises, projecting a separable grid function gives exactly the
+Kronecker product of the one-rank projections, so these factors carry the
+reference's numbers without forming its tensors (48 GB for the electron-
+electron term at 66 channels, 120 B-splines).  ``ReferenceLayout`` expands
+a potential to the reference layout for small cases.
+
+Radial band matrices are row-major: element (i, j) at [i, j - i + k - 1],
+with the reference's convention for i > j (the transposed integral times
+(-1)^d, bspline.cpp:380-392).
+"""
+import ctypes
+
+import numpy as np
+
+from . import coupled as cp
+from . import device as dev
+from . import _native as nat
+
+
+# ---------------------------------------------------------------- radial pieces
+def band_rowmajor(basis, f, d):
+    """<B_i| f d^d |B_j> as an (N, 2k-1) row-major band (real part)."""
+    k = basis.MaxSplineOrder
+    N = basis.NumberOfBSplines
+    s = k - 1
+    bw = 2 * s + 1
+    nh = np.real(basis.project_bandnh(f, d))        # BandNH: (i, j) at j*bw + s + i - j
+    out = np.zeros((N, bw))
+    for i in range(N):
+        for dd in range(bw):
+            j = i + dd - s
+            if 0 <= j < N:
+                out[i, dd] = nh[j * bw + s + i - j]
+    return out
+
+
+def radial_matrices(basis):
+    """S, D1, D2, 1/r, 1/r^2 as row-major bands."""
+    r = basis.QuadratureGridGlobal
+    one = np.ones_like(r)
+    return {"S": band_rowmajor(basis, one, 0), "D1": band_rowmajor(basis, one, 1),
+            "D2": band_rowmajor(basis, one, 2), "R1": band_rowmajor(basis, 1.0 / r, 0),
+            "R2": band_rowmajor(basis, 1.0 / (r * r), 0)}
+
+
+def multipole_tensor(basis, l):
+    """R_l[i1, d1, i2, d2] (j = i + d - (k-1)) of r<^l / r>^(l+1) on the
+    product quadrature grid (coupledspherical.cpp:92-110 evaluated, then
+    projected on both radial ranks)."""
+    k = basis.MaxSplineOrder
+    N = basis.NumberOfBSplines
+    s = k - 1
+    bw = 2 * s + 1
+    r = basis.QuadratureGridGlobal
+    w = basis.QuadratureWeightsGlobal
+    rmin = np.minimum.outer(r, r)
+    rmax = np.maximum.outer(r, r)
+    F = (rmin / rmax) ** l / rmax
+    Fw = F * w[:, None] * w[None, :]
+    M = basis._projector(0)                          # (G, N*bw) sparse, BandNH columns
+    T = np.asarray((M.T @ Fw))                       # (N*bw, G)
+    Rnh = np.asarray((M.T @ T.T)).T                  # (N*bw, N*bw), BandNH x BandNH
+    # BandNH index of (i, j) is j*bw + s + i - j; reorder to row-major (i, d)
+    idx = np.full((N, bw), -1, dtype=np.int64)
+    for i in range(N):
+        for dd in range(bw):
+            j = i + dd - s
+            if 0 <= j < N:
+                idx[i, dd] = j * bw + s + i - j
+    valid = idx >= 0
+    flat = np.where(valid, idx, 0).reshape(-1)
+    R = Rnh[np.ix_(flat, flat)].reshape(N, bw, N, bw)
+    R *= valid[:, :, None, None]
+    R *= valid[None, None, :, :]
+    return np.ascontiguousarray(R)
+
+
+# ---------------------------------------------------------------- factored potentials
+class KronTerm:
+    def __init__(self, pairs, coef, A, B):
+        self.pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        self.coef = np.ascontiguousarray(coef, dtype=np.complex128)
+        self.A, self.B = A, B
+
+
+class MultipoleTerm:
+    def __init__(self, pairs, coef, l, R):
+        self.pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        self.coef = np.ascontiguousarray(coef, dtype=np.complex128)
+        self.l = l
+        self.R = R
+
+
+class CoupledTensorPotential:
+    """A config-section potential on (coupled channels, B-spline, B-spline),
+    held as factored terms.  Mirrors TensorPotential's attributes
+    (TensorPotential.py:3-163): Name, IsTimeDependent, GetTimeScaling."""
+
+    def __init__(self, psi):
+        self.psi = psi
+        self.Name = None
+        self.IsTimeDependent = False
+        self.Terms = []
+
+    def ApplyConfigSection(self, conf):
+        repr = self.psi.GetRepresentation()
+        self.Name = conf.name
+        self.IsTimeDependent = hasattr(conf, "time_function")
+        if self.IsTimeDependent:
+            self.TimeFunction = lambda t: conf.time_function(conf, t)
+            self.OriginalTimeFunction = conf.time_function
+        ang = repr.GetRepresentation(0)
+        b1 = repr.GetRepresentation(1).GetBSplineObject()
+        b2 = repr.GetRepresentation(2).GetBSplineObject()
+        if b1 is not b2 and (b1.NumberOfBSplines != b2.NumberOfBSplines
+                             or not np.array_equal(b1.Knots, b2.Knots)):
+            raise NotImplementedError("both radial ranks must use the same B-spline basis")
+        self.Basis = b1
+        self.Channels = ang.Channels
+        g0 = conf.geometry0
+        for r in (1, 2):
+            g = getattr(conf, "geometry%i" % r).lower().strip()
+            if g not in ("banded-packed", "banded-nonhermitian"):
+                raise NotImplementedError("radial geometry %s is not on the rank-3 engine" % g)
+        self.Pairs = np.array(cp.basis_pairs(self.Channels, g0), dtype=np.int32).reshape(-1, 2)
+        self.GeometryNames = [g0, conf.geometry1, conf.geometry2]
+        self.Terms = _evaluate(conf, self.Channels, self.Pairs, self.Basis, repr.RadialMatrices())
+
+    def GetTimeScaling(self, t):
+        return float(np.real(self.TimeFunction(t))) if self.IsTimeDependent else 1.0
+
+    def SetupStep(self, timestep):
+        pass
+
+    def MultiplyPotential(self, srcPsi, destPsi, t, timestep):
+        raise NotImplementedError("two-electron potentials are applied by the rank-3 engine "
+                                  "(PamPropagator's Engine.apply), not one by one")
+
+    def CanConsolidate(self, other):
+        return False
+
+    def ReferenceLayout(self):
+        """The reference's tensor (angular pairs, radial pairs, radial pairs or
+        BandNH) for small cases: expands the factored terms."""
+        b = self.Basis
+        N = b.NumberOfBSplines
+        k = b.MaxSplineOrder
+        s = k - 1
+        bw = 2 * s + 1
+        if len(self.Pairs) * (N * bw) ** 2 > 2e7:
+            raise RuntimeError("reference layout of %s is too large to form" % self.Name)
+        r1 = _radial_pairs(self.GeometryNames[1], N, k)
+        r2 = _radial_pairs(self.GeometryNames[2], N, k)
+        out = np.zeros((len(self.Pairs), len(r1), len(r2)), dtype=np.complex128)
+        pindex = {tuple(p): n for n, p in enumerate(self.Pairs)}
+
+        def rm(M, prs):
+            i, j = prs[:, 0], prs[:, 1]
+            ok = np.abs(i - j) <= s
+            return np.where(ok, M[i, np.clip(j - i + s, 0, bw - 1)], 0.0)
+        for t in self.Terms:
+            if isinstance(t, KronTerm):
+                rad = np.multiply.outer(rm(t.A, r1), rm(t.B, r2))
+            else:
+                i1, j1 = r1[:, 0], r1[:, 1]
+                i2, j2 = r2[:, 0], r2[:, 1]
+                ok1 = np.abs(i1 - j1) <= s
+                ok2 = np.abs(i2 - j2) <= s
+                rad = t.R[i1[:, None], np.clip(j1 - i1 + s, 0, bw - 1)[:, None],
+                          i2[None, :], np.clip(j2 - i2 + s, 0, bw - 1)[None, :]]
+                rad = rad * ok1[:, None] * ok2[None, :]
+            for p, c in zip(t.pairs, t.coef):
+                out[pindex[tuple(p)]] += c * rad
+        return out
+
+
+def _radial_pairs(geom, N, k):
+    g = geom.lower().strip()
+    s = k - 1
+    if g == "banded-packed":
+        return np.array([(i, j) for i in range(N) for j in range(max(0, i - k + 1), min(i + k, N))], dtype=np.int64)
+    bw = 2 * s + 1
+    pr = np.zeros((N * bw, 2), dtype=np.int64)
+    pr[:, 1] = N - 1
+    for i in range(N):
+        for j in range(i - s, i + s + 1):
+            if 0 <= j < N:
+                pr[j * bw + s + i - j] = (i, j)
+    return pr
+
+
+def _diag(channels, pairs, f):
+    coef = np.array([f(channels[a]) for a, b in pairs], dtype=np.complex128)
+    return coef
+
+
+def _evaluate(conf, channels, pairs, basis, mats):
+    """Factored terms of one config section, by the section's classname."""
+    cls = conf.classname
+    S, D1, D2, R1, R2 = mats["S"], mats["D1"], mats["D2"], mats["R1"], mats["R2"]
+    ch = channels
+    diag_ok = all(a == b for a, b in pairs) if len(pairs) else True
+    if cls == "KineticEnergyPotential":
+        # -1/(2 mass) with differentiation on one radial rank (potential.cpp:7-35)
+        d1 = int(getattr(conf, "differentiation1", 0))
+        d2 = int(getattr(conf, "differentiation2", 0))
+        dm = {0: S, 1: D1, 2: D2}
+        c = np.full(len(pairs), -1.0 / (2.0 * conf.mass), dtype=np.complex128)
+        return [KronTerm(pairs, c, dm[d1], dm[d2])]
+    if cls == "OverlapPotential":
+        return [KronTerm(pairs, np.ones(len(pairs), dtype=np.complex128), S, S)]
+    if cls == "CoupledSphericalKineticEnergyEvaluator":
+        # l1(l1+1)/(2 m r1^2) + l2(l2+1)/(2 m r2^2) on matching channels
+        # (coupledspherical.cpp:120-205)
+        m = conf.mass
+        match = np.array([ch[a] == ch[b] for a, b in pairs])
+        c1 = np.array([ch[a].l1 * (ch[a].l1 + 1.0) / (2.0 * m) for a, b in pairs]) * match
+        c2 = np.array([ch[a].l2 * (ch[a].l2 + 1.0) / (2.0 * m) for a, b in pairs]) * match
+        return [KronTerm(pairs, c1, R2, S), KronTerm(pairs, c2, S, R2)]
+    if cls == "CoupledSphericalCoulombPotential":
+        # -Z/r1 - Z/r2 (potential.cpp:41-77)
+        if not diag_ok:
+            raise NotImplementedError("CoupledSphericalCoulombPotential on a non-diagonal angular geometry")
+        c = np.full(len(pairs), -float(conf.z), dtype=np.complex128)
+        return [KronTerm(pairs, c, R1, S), KronTerm(pairs, c, S, R1)]
+    if cls == "CustomPotentialEvaluationR12":
+        # sum_l coefficient(l) r<^l / r>^(l+1) (coupledspherical.cpp:16-114)
+        terms = []
+        lmax = 0
+        for a, b in pairs:
+            lo, hi = cp.multipole_range(ch[a], ch[b])
+            lmax = max(lmax, hi)
+        for l in range(lmax + 1):
+            sel, coef = [], []
+            for a, b in pairs:
+                lo, hi = cp.multipole_range(ch[a], ch[b])
+                if lo <= l <= hi:
+                    v = cp.ee_coefficient(ch[a], ch[b], l)
+                    if v != 0.0:
+                        sel.append((a, b))
+                        coef.append(v)
+            if sel:
+                terms.append(MultipoleTerm(sel, np.array(coef, dtype=np.complex128), l,
+                                           multipole_tensor(basis, l)))
+        return terms
+    if cls in ("CustomPotential_LaserVelocity", "CustomPotential_LaserVelocityDerivativeR1",
+               "CustomPotential_LaserVelocityDerivativeR2"):
+        cc = np.array([cp.velocity_couplings(ch[a], ch[b]) for a, b in pairs]).reshape(-1, 4)
+        mi = -1j
+        if cls == "CustomPotential_LaserVelocity":        # -i (c1/r1 + c2/r2)
+            return [KronTerm(pairs, mi * cc[:, 0], R1, S), KronTerm(pairs, mi * cc[:, 1], S, R1)]
+        if cls == "CustomPotential_LaserVelocityDerivativeR1":   # -i e1 d/dr1
+            if int(getattr(conf, "differentiation1", 0)) != 1:
+                raise NotImplementedError("LaserVelocityDerivativeR1 needs differentiation1 = 1")
+            return [KronTerm(pairs, mi * cc[:, 2], D1, S)]
+        if int(getattr(conf, "differentiation2", 0)) != 1:
+            raise NotImplementedError("LaserVelocityDerivativeR2 needs differentiation2 = 1")
+        return [KronTerm(pairs, mi * cc[:, 3], S, D1)]
+    raise NotImplementedError("potential class %s is not on the rank-3 path" % cls)
+
+
+# ---------------------------------------------------------------- engine
+class Pamp3Engine:
+    """pp_pamp3_*: pAMP steps of a (channels, N, N) state on the device."""
+
+    def __init__(self, C, N, m, overlap, reorth=False):
+        self.C, self.N, self.m = int(C), int(N), int(m)
+        self.overlap = overlap
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().pp_pamp3_create(self.C, self.N, self.m, int(bool(reorth)), overlap.h,
+                                            ctypes.byref(h)), "pp_pamp3_create")
+        self.h = h
+        self.nslots = 0
+        self._last_steps = 0
+
+    def __del__(self):
+        if getattr(self, "h", None) and nat._lib is not None:
+            nat._lib.pp_pamp3_destroy(self.h)
+            self.h = None
+
+    def add_kron(self, term, slot=-1):
+        coef = term.coef.view(np.float64)
+        A = np.ascontiguousarray(term.A, dtype=np.float64)
+        B = np.ascontiguousarray(term.B, dtype=np.float64)
+        nat.check(nat.lib().pp_pamp3_add_kron_term(self.h, len(term.pairs), nat.iptr(term.pairs), nat.dptr(coef),
+                                                   nat.dptr(A), nat.dptr(B), int(slot)), "pp_pamp3_add_kron_term")
+        self.nslots = max(self.nslots, slot + 1)
+
+    def add_multipole(self, term, slot=-1):
+        coef = term.coef.view(np.float64)
+        R = np.ascontiguousarray(term.R, dtype=np.float64)
+        nat.check(nat.lib().pp_pamp3_add_multipole_term(self.h, len(term.pairs), nat.iptr(term.pairs),
+                                                        nat.dptr(coef), nat.dptr(R), int(slot)),
+                  "pp_pamp3_add_multipole_term")
+        self.nslots = max(self.nslots, slot + 1)
+
+    def add_potential(self, pot, slot=-1):
+        for t in pot.Terms:
+            if isinstance(t, KronTerm):
+                self.add_kron(t, slot)
+            else:
+                self.add_multipole(t, slot)
+
+    def _check(self, x):
+        t = dev.torch()
+        if tuple(x.shape) != (self.C, self.N, self.N) or x.dtype != t.complex128 or not x.is_contiguous():
+            raise ValueError("expected a contiguous complex128 (%d, %d, %d) device tensor" % (self.C, self.N, self.N))
+
+    def _scal(self, scal, nsteps):
+        if self.nslots == 0:
+            return None
+        sc = np.ascontiguousarray(np.asarray(scal, dtype=np.float64).reshape(nsteps, self.nslots))
+        return sc
+
+    def apply(self, src, dst, scal=None, balanced=False):
+        """dst = H src (balanced: U^-H H U^-1 src); src, dst device (C, N, N)."""
+        self._check(src)
+        self._check(dst)
+        sc = self._scal(scal if scal is not None else np.ones(max(self.nslots, 1)), 1)
+        nat.check(nat.lib().pp_pamp3_apply(self.h, dev.ptr(src), dev.ptr(dst), nat.dptr(sc) if sc is not None else None,
+                                           int(bool(balanced)), dev.stream_ptr()), "pp_pamp3_apply")
+
+    def propagate(self, psi, nsteps, dt, scal=None):
+        self._check(psi)
+        dt = complex(dt)
+        sc = self._scal(scal, nsteps) if self.nslots else None
+        nat.check(nat.lib().pp_pamp3_propagate(self.h, dev.ptr(psi), int(nsteps), dt.real, dt.imag,
+                                               nat.dptr(sc) if sc is not None else None, dev.stream_ptr()),
+                  "pp_pamp3_propagate")
+        self._last_steps = int(nsteps)
+
+    def set_timing(self, enable=True):
+        nat.check(nat.lib().pp_pamp3_set_timing(self.h, int(bool(enable))), "pp_pamp3_set_timing")
+
+    def kernel_time(self):
+        """(summed ms, launches) of the timed multipole contraction launches."""
+        ms = ctypes.c_double()
+        cnt = ctypes.c_long()
+        nat.check(nat.lib().pp_pamp3_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(cnt)), "pp_pamp3_kernel_time")
+        return ms.value, cnt.value
+
+    def multipole_flops(self):
+        f = ctypes.c_double()
+        nat.check(nat.lib().pp_pamp3_multipole_flops(self.h, ctypes.byref(f)), "pp_pamp3_multipole_flops")
+        return f.value
+
+    def diagnostics(self):
+        n = self._last_steps
+        out = np.zeros(4 * max(n, 1))
+        nat.check(nat.lib().pp_pamp3_diagnostics(self.h, n, nat.dptr(out), dev.stream_ptr()), "pp_pamp3_diagnostics")
+        return out[:4 * n].reshape(n, 4)

@@ -105,6 +105,8 @@ class PamPropagator(PropagatorBase):
         rank 2, angular rank 0 (orthogonal), B-spline rank 1 with BandNH storage."""
         repr = self.psi.GetRepresentation()
         pots = self.BasePropagator.PotentialList
+        if self.Rank == 3:
+            return self._build_engine3(repr, pots)
         if self.Rank != 2 or repr.IsOrthogonalBasis(1) is not False or not repr.IsOrthogonalBasis(0):
             raise NotImplementedError("the pAMP HIP engine handles (angular, B-spline) rank-2 states")
         L, N = repr.GetFullShape()
@@ -125,7 +127,34 @@ class PamPropagator(PropagatorBase):
         self._pots = pots
         return eng
 
+    def _build_engine3(self, repr, pots):
+        """Two-electron states (coupled channels x B-spline x B-spline): the
+        factored potentials go to the rank-3 engine; each time-dependent
+        potential gets its own time slot."""
+        from .helium import CoupledTensorPotential, Pamp3Engine
+        if self.MatvecPrecision != "fp64":
+            raise NotImplementedError("matvec_precision = mixed is not on the rank-3 engine")
+        if not all(isinstance(p, CoupledTensorPotential) for p in pots):
+            raise NotImplementedError("rank-3 states run on the two-electron engine only")
+        C, N, N2 = repr.GetFullShape()
+        if N != N2:
+            raise NotImplementedError("both radial ranks must have the same size")
+        overlap = repr.GetRepresentation(1).GetOverlap()
+        eng = Pamp3Engine(C, N, self.BasisSize, overlap, reorth=self.DoubleOrthogonalization)
+        self._slots = []
+        for pot in pots:
+            slot = -1
+            if pot.IsTimeDependent:
+                slot = len(self._slots)
+                self._slots.append(pot)
+            eng.add_potential(pot, slot)
+        self._pots = pots
+        return eng
+
     def TimeScalings(self, times):
+        if self.Rank == 3:
+            return np.array([[p.GetTimeScaling(t) for p in self._slots] for t in times],
+                            dtype=np.float64).reshape(len(times), len(self._slots))
         return np.array([[p.GetTimeScaling(t) for p in self._pots] for t in times], dtype=np.float64)
 
     def MultiplyHamiltonian(self, srcPsi, destPsi, t, dt):

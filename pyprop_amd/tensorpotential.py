@@ -411,7 +411,14 @@ class BasisPropagator(PropagatorBase):
         self.ConsolidatePotentials()
 
     def GeneratePotential(self, configSection):
-        potential = TensorPotential(self.psi)
+        from .core import CoupledSphericalHarmonicRepresentation
+        repr = self.psi.GetRepresentation()
+        if self.Rank == 3 and isinstance(repr.GetRepresentation(0), CoupledSphericalHarmonicRepresentation):
+            # two-electron potentials are generated in factored form
+            from .helium import CoupledTensorPotential
+            potential = CoupledTensorPotential(self.psi)
+        else:
+            potential = TensorPotential(self.psi)
         configSection.Apply(potential)
         return potential
 

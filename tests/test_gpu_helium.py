@@ -1,0 +1,163 @@
+"""Two-electron (helium) HIP path through the C-ABI against the oracle:
+operator apply (plain and balanced), pAMP steps with the laser on, imaginary
+time to the ground state, and the multi-chunk / two-half launch shapes of the
+multipole kernel against a direct evaluation of the same factored terms."""
+import numpy as np
+import pytest
+
+import pyprop_amd as P
+from oracle import oracle as O
+from pyprop_amd import device as dev
+from pyprop_amd.examples import helium as HE
+from pyprop_amd.helium import KronTerm
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(lmax=1, Ls=(0, 1), nbp=6, rmax=8.0, frequency=2.0, cycles=8, E0=0.05)
+
+
+def _problem(name="helium_small", overrides=None, field_free=False):
+    conf = HE.load(name, overrides, field_free=field_free)
+    prob = P.Problem(conf)
+    prob.SetupStep()
+    return prob
+
+
+def _rand(shape, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def _oracle_scalings(h, s):
+    return np.array([s if kind == "laser" else 1.0 for kind, _, _ in h.terms])
+
+
+def test_helium_apply_matches_oracle():
+    print("helium apply, small (5 channels, N=9)", flush=True)
+    prob = _problem()
+    eng = prob.Propagator.Engine
+    h = O.Helium(**SMALL)
+    H = h.ham()
+    x = _rand(h.shape, 11)
+    s = 0.37
+    H.set_scaling(_oracle_scalings(h, s))
+    t = dev.torch()
+    xd = dev.to_device(x)
+    out = t.zeros_like(xd)
+    assert eng.nslots == 3          # the three laser sections, one time slot each
+    eng.apply(xd, out, scal=[s] * 3)
+    ref = H.potentials(x)
+    assert np.abs(out.cpu().numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
+    eng.apply(xd, out, scal=[s] * 3, balanced=True)
+    ref = H.balanced(x)
+    assert np.abs(out.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_helium_pamp_matches_oracle():
+    print("helium pAMP, small, 12 steps from t = 5", flush=True)
+    prob = _problem()
+    h = O.Helium(**SMALL)
+    psi0 = HE.start_state(prob)
+    psi0 = psi0 + 0.05 * _rand(psi0.shape, 5)
+    prob.SetWavefunction(psi0)
+    prob.psi.Normalize()
+    psi0 = prob.psi.GetData().cpu().numpy()
+    t0, n, dt = 5.0, 12, 0.01
+    prob.PropagatedTime = t0
+    prob.AdvanceSteps(n)
+    got = prob.psi.GetData().cpu().numpy()
+    times = O.step_times(n, dt, t0)
+    sc = np.array([_oracle_scalings(h, h.scaling(tt)) for tt in times])
+    ref = h.ham().propagate_pamp(psi0, n, dt, sc, 20)
+    assert np.linalg.norm(got - ref) <= 1e-11 * np.linalg.norm(ref)
+    assert abs(prob.psi.GetNorm() - 1.0) < 1e-11
+    d = prob.Propagator.GetDiagnostics()
+    assert d.shape == (n, 4) and np.all(d[:, 1] == 20)
+
+
+def test_helium_imaginary_time_ground_state():
+    print("helium imaginary time, small", flush=True)
+    h = O.Helium(**SMALL, laser=False)
+    H = h.ham()
+    n = int(np.prod(h.shape))
+    Hb = np.zeros((n, n), dtype=np.complex128)
+    for j in range(n):
+        e = np.zeros(n, dtype=np.complex128)
+        e[j] = 1.0
+        Hb[:, j] = H.balanced(e.reshape(h.shape)).reshape(-1)
+    E0 = np.linalg.eigvalsh(0.5 * (Hb + Hb.conj().T))[0]
+    psi, E = HE.ground_state("helium_small", steps=400, dt=0.05)
+    assert abs(E - E0) < 1e-8 * abs(E0)
+    # the state is the eigenvector: H psi = E0 S psi
+    prob = _problem(field_free=True)
+    t = dev.torch()
+    pd = dev.to_device(psi)
+    hp = t.zeros_like(pd)
+    prob.Propagator.Engine.apply(pd, hp)
+    sp = pd.clone()
+    prob.psi.GetRepresentation().Representations[1].GetOverlap().apply(5, 1, sp)
+    prob.psi.GetRepresentation().Representations[2].GetOverlap().apply(5, 2, sp)
+    r = (hp - E0 * sp).cpu().numpy()
+    assert np.linalg.norm(r) < 1e-6 * abs(E0) * np.linalg.norm(psi)
+
+
+def _band_apply_rows(M, x, axis):
+    """sum_j M[i, j] x[..., j, ...] along `axis` for a row-major band M."""
+    N, bw = M.shape
+    s = (bw - 1) // 2
+    xs = np.moveaxis(x, axis, -1)
+    out = np.zeros_like(xs)
+    for d in range(bw):
+        j = np.arange(N) + d - s
+        ok = (j >= 0) & (j < N)
+        out[..., ok] += M[ok, d] * xs[..., j[ok]]
+    return np.moveaxis(out, -1, axis)
+
+
+def _factored_apply(pots, y, scal):
+    """z = sum over terms, evaluated directly from the factored terms."""
+    C, N, _ = y.shape
+    z = np.zeros_like(y)
+    slot = 0
+    for pot in pots:
+        f = 1.0
+        if pot.IsTimeDependent:
+            f = scal[slot]
+            slot += 1
+        for t in pot.Terms:
+            src = sorted(set(int(b) for b in t.pairs[:, 1]))
+            if isinstance(t, KronTerm):
+                U = {b: _band_apply_rows(t.A, _band_apply_rows(t.B, y[b], 1), 0) for b in src}
+            else:
+                s = (t.R.shape[1] - 1) // 2
+                U = {}
+                for b in src:
+                    acc = np.zeros((N, N), dtype=np.complex128)
+                    yp = np.zeros((N + 2 * s, N + 2 * s), dtype=np.complex128)
+                    yp[s:s + N, s:s + N] = y[b]
+                    for d1 in range(2 * s + 1):
+                        for d2 in range(2 * s + 1):
+                            acc += t.R[:, d1, :, d2] * yp[d1:d1 + N, d2:d2 + N]
+                    U[b] = acc
+            for (a, b), c in zip(t.pairs, t.coef):
+                z[a] += f * c * U[int(b)]
+    return z
+
+
+def test_helium_apply_multichunk_shapes():
+    """34 channels (three 16-channel chunks, the last partial) and N = 70
+    (two i2 halves) against the factored terms evaluated in numpy."""
+    print("helium apply, 34 channels, N=70", flush=True)
+    over = {("AngularRepresentation", "index_iterator"): P.DefaultCoupledIndexIterator(lmax=3, L=[0, 1, 2, 3], M=[0]),
+            ("RadialRepresentation", "xsize"): 67, ("RadialRepresentation", "xmax"): 30.0}
+    prob = _problem(overrides=over)
+    C, N, _ = prob.psi.GetData().shape
+    assert (C, N) == (34, 70)
+    x = _rand((C, N, N), 21)
+    s = -0.61
+    t = dev.torch()
+    xd = dev.to_device(x)
+    out = t.zeros_like(xd)
+    prob.Propagator.Engine.apply(xd, out, scal=[s] * 3)
+    ref = _factored_apply(prob.Propagator.BasePropagator.PotentialList, x, [s] * 3)
+    assert np.abs(out.cpu().numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
