@@ -46,8 +46,6 @@
 #include "pp_pade.h"
 
 #define P3_CHUNK 16        // channels per multipole chunk
-#define P3_HALF 64         // i2 per multipole workgroup
-#define P3_EE_THREADS 128  // (8 channel pairs) x (16 i2 quads)
 #define P3_RED_BLOCKS 480  // blocks of the Gram-Schmidt reductions
 
 struct p3_ent {            // one coupling of a banded pass or the mixing
@@ -92,53 +90,93 @@ struct pp_pamp3 {
 };
 
 // ------------------------------------------------------------------ banded passes
-// X[k][c][i1][i2] = sum_e coef_e * s_e * sum_d B_e[i2][d] * src[c'_e][i1][i2+d-s]
+#define P3_RT 8            // rows per workgroup of the banded passes
+// X[k][c][i1][i2] = sum_e coef_e * s_e * sum_d B_e[i2][d] * y[c'_e][i1][i2+d-s]
+// workgroup: (row r = k*C + c, P3_RT rows i1); thread: i2.  The matrix row
+// B_e[i2][.] is loaded once per entry and used for all P3_RT rows.
+template <int S>
 __global__ void __launch_bounds__(128) p3_right(const double *mats, const int *row, const p3_ent *ent, const zd *y,
-                                                 zd *X, int C, int N, int s, int bw, const double *scal,
-                                                 const int *state)
+                                                 zd *X, int C, int N, const double *scal, const int *state)
 {
   if (state[0]) return;
-  const int r = blockIdx.x / N, i1 = blockIdx.x % N;   // r = k*C + c
+  constexpr int s = S, bw = 2 * S + 1;
+  const int ntile = (N + P3_RT - 1) / P3_RT;
+  const int r = blockIdx.x / ntile, i1a = (blockIdx.x % ntile) * P3_RT;   // r = k*C + c
   const long NN = (long)N * N;
   for (int i2 = threadIdx.x; i2 < N; i2 += blockDim.x) {
-    zd acc = zmk(0.0, 0.0);
+    zd acc[P3_RT];
+#pragma unroll
+    for (int rr = 0; rr < P3_RT; rr++) acc[rr] = zmk(0.0, 0.0);
+    const int d0 = s - i2 > 0 ? s - i2 : 0, d1 = N - 1 - i2 + s < bw - 1 ? N - 1 - i2 + s : bw - 1;
     for (int e = row[r]; e < row[r + 1]; e++) {
       const p3_ent E = ent[e];
       const double sc = E.slot < 0 ? 1.0 : scal[E.slot];
       const zd cf = zmk(E.cre * sc, E.cim * sc);
       const double *Bm = mats + ((size_t)E.mat * N + i2) * bw;
-      const zd *src = y + E.col * NN + (long)i1 * N;
-      zd t = zmk(0.0, 0.0);
-      const int d0 = s - i2 > 0 ? s - i2 : 0, d1 = N - 1 - i2 + s < bw - 1 ? N - 1 - i2 + s : bw - 1;
-      for (int d = d0; d <= d1; d++) zfmar(t, Bm[d], src[i2 + d - s]);
-      zfma(acc, cf, t);
+      double bv[bw];
+#pragma unroll
+      for (int d = 0; d < bw; d++) bv[d] = Bm[d];
+      const zd *src = y + E.col * NN + i2 - s;
+#pragma unroll
+      for (int rr = 0; rr < P3_RT; rr++) {
+        const int i1 = i1a + rr;
+        if (i1 >= N) break;
+        const zd *sr = src + (long)i1 * N;
+        zd t = zmk(0.0, 0.0);
+#pragma unroll
+        for (int d = 0; d < bw; d++)
+          if (d >= d0 && d <= d1) zfmar(t, bv[d], sr[d]);
+        zfma(acc[rr], cf, t);
+      }
     }
-    X[(long)r * NN + (long)i1 * N + i2] = acc;
+#pragma unroll
+    for (int rr = 0; rr < P3_RT; rr++)
+      if (i1a + rr < N) X[(long)r * NN + (long)(i1a + rr) * N + i2] = acc[rr];
   }
 }
 
 // Z[c][i1][i2] = sum_e coef_e * s_e * sum_d A_e[i1][d] * X[k_e][c'_e][i1+d-s][i2]
+// workgroup: (channel c, P3_RT rows i1); thread: i2.  Per entry the column
+// window X[.][i1a-s .. i1a+P3_RT-1+s][i2] is loaded once into registers.
+#define P3_LT 4            // rows per workgroup of the i1 pass
+template <int S>
 __global__ void __launch_bounds__(128) p3_left(const double *mats, const int *row, const p3_ent *ent, const zd *X,
-                                                zd *Z, int C, int N, int s, int bw, const double *scal,
-                                                const int *state)
+                                                zd *Z, int C, int N, const double *scal, const int *state)
 {
   if (state[0]) return;
-  const int c = blockIdx.x / N, i1 = blockIdx.x % N;
+  constexpr int s = S, bw = 2 * S + 1, WIN = P3_LT + 2 * S;
+  const int ntile = (N + P3_LT - 1) / P3_LT;
+  const int c = blockIdx.x / ntile, i1a = (blockIdx.x % ntile) * P3_LT;
   const long NN = (long)N * N;
-  const int d0 = s - i1 > 0 ? s - i1 : 0, d1 = N - 1 - i1 + s < bw - 1 ? N - 1 - i1 + s : bw - 1;
   for (int i2 = threadIdx.x; i2 < N; i2 += blockDim.x) {
-    zd acc = zmk(0.0, 0.0);
+    zd acc[P3_LT];
+#pragma unroll
+    for (int rr = 0; rr < P3_LT; rr++) acc[rr] = zmk(0.0, 0.0);
     for (int e = row[c]; e < row[c + 1]; e++) {
       const p3_ent E = ent[e];
       const double sc = E.slot < 0 ? 1.0 : scal[E.slot];
       const zd cf = zmk(E.cre * sc, E.cim * sc);
-      const double *Am = mats + ((size_t)E.mat * N + i1) * bw;
       const zd *src = X + ((long)E.src * C + E.col) * NN + i2;
-      zd t = zmk(0.0, 0.0);
-      for (int d = d0; d <= d1; d++) zfmar(t, Am[d], src[(long)(i1 + d - s) * N]);
-      zfma(acc, cf, t);
+      zd win[WIN];
+#pragma unroll
+      for (int w = 0; w < WIN; w++) {
+        const int j1 = i1a - s + w;
+        win[w] = (j1 >= 0 && j1 < N) ? src[(long)j1 * N] : zmk(0.0, 0.0);
+      }
+#pragma unroll
+      for (int rr = 0; rr < P3_LT; rr++) {
+        const int i1 = i1a + rr;
+        if (i1 >= N) break;
+        const double *Am = mats + ((size_t)E.mat * N + i1) * bw;   // uniform over the workgroup
+        zd t = zmk(0.0, 0.0);
+#pragma unroll
+        for (int d = 0; d < bw; d++) zfmar(t, Am[d], win[rr + d]);   // A is zero outside the matrix
+        zfma(acc[rr], cf, t);
+      }
     }
-    Z[(long)c * NN + (long)i1 * N + i2] = acc;
+#pragma unroll
+    for (int rr = 0; rr < P3_LT; rr++)
+      if (i1a + rr < N) Z[(long)c * NN + (long)(i1a + rr) * N + i2] = acc[rr];
   }
 }
 
@@ -165,8 +203,15 @@ __global__ void __launch_bounds__(256) p3_mix(const int *row, const p3_ent *ent,
   }
 }
 
+#define P3_HALF 64         // i2 per multipole workgroup
+#define P3_EE_THREADS 128  // (8 channel pairs) x (16 i2 quads)
+
 // Z[c][i1][i2] += sum_blocks sum_{d1,d2} R_t[i1][d1][i2][d2] Ym[b][i1+d1-s][i2+d2-s][cl]
 // workgroup: (chunk, i1, i2 half); thread: channel pair cp (8) x i2 quad q (16).
+// Many small workgroups (up to six per CU) keep enough loads in flight: the
+// stages are short (one R slice and one mixed input row), and a register-
+// blocked variant with three i1 rows per workgroup and the next stage
+// prefetched into registers measured slower (fewer resident waves).
 template <int S>
 __global__ void __launch_bounds__(P3_EE_THREADS) p3_ee(const double *const *Rt, const int *blk_term, const int *cb_ptr,
                                                        const int *cb_blk, const zd *Ym, zd *Z, int C, int N,
@@ -247,6 +292,102 @@ __global__ void __launch_bounds__(P3_EE_THREADS) p3_ee(const double *const *Rt, 
   }
 }
 
+// ------------------------------------------------------------------ overlap solves
+// Fiber tiles in LDS for both radial ranks.  A workgroup stages P3_FT
+// fibers (rank 2: P3_FT consecutive rows; rank 1: P3_FT consecutive columns
+// of one channel) with coalesced loads, the band of U and the reciprocals of
+// its diagonal, then one thread per fiber runs the substitution in LDS and
+// the tile is stored back coalesced.  OP 0: U x = b (back substitution,
+// ztbsv 'U','N'), OP 1: U^T x = b (forward; U is real so this is U^H, ztbsv
+// 'U','C'), OP 2: x := U x (ztbmv).  U(i, j) at Ub[j*(KD+1) + KD + i - j]
+// (overlapmatrix.cpp:22-32).
+#define P3_FT 32
+template <int KD, int OP>
+__global__ void __launch_bounds__(256) p3_ftile(const double *Ub, zd *x, int C, int N, int rank)
+{
+  extern __shared__ double p3_lds[];
+  double *uL = p3_lds;                                   // N*(KD+1) band
+  double *rd = uL + (size_t)N * (KD + 1);                // N reciprocals of the diagonal
+  const int toff = N * (KD + 1) + N + ((N * (KD + 1) + N) & 1);   // doubles before the tile (16-byte aligned)
+  zd *tile = (zd *)(p3_lds + toff);                      // fiber f, element i at f*(N+1) + i
+  const int ld = N + 1;
+  const long NN = (long)N * N;
+  for (int e = threadIdx.x; e < N * (KD + 1); e += 256) uL[e] = Ub[e];
+  for (int i = threadIdx.x; i < N; i += 256) rd[i] = 1.0 / Ub[(size_t)i * (KD + 1) + KD];
+  int nf;
+  zd *g;
+  long gstr, fstr;   // global element i of fiber f at g[f*fstr + i*gstr]
+  if (rank == 2) {
+    const long r0 = (long)blockIdx.x * P3_FT, nrow = (long)C * N;
+    nf = nrow - r0 < P3_FT ? (int)(nrow - r0) : P3_FT;
+    g = x + r0 * N; fstr = N; gstr = 1;
+    for (int e = threadIdx.x; e < nf * N; e += 256) tile[(e / N) * ld + e % N] = g[e];
+  } else {
+    const int ncol = (N + P3_FT - 1) / P3_FT;
+    const int c = blockIdx.x / ncol, i2a = (blockIdx.x % ncol) * P3_FT;
+    nf = N - i2a < P3_FT ? N - i2a : P3_FT;
+    g = x + (long)c * NN + i2a; fstr = 1; gstr = N;
+    for (int e = threadIdx.x; e < nf * N; e += 256) {
+      const int i = e / nf, f = e % nf;                   // consecutive threads: consecutive columns
+      tile[f * ld + i] = g[(long)i * N + f];
+    }
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < nf) {
+    zd *b = tile + t * ld;
+    auto U = [&](int i, int j) { return uL[(size_t)j * (KD + 1) + KD + i - j]; };
+    zd win[KD + 1];
+    if (OP == 0) {
+#pragma unroll
+      for (int d = 0; d < KD; d++) win[d] = zmk(0.0, 0.0);
+      for (int i = N - 1; i >= 0; i--) {
+        zd s = b[i];
+#pragma unroll
+        for (int d = KD; d >= 1; d--)
+          if (i + d < N) { const double u = U(i, i + d); s.x -= u * win[d - 1].x; s.y -= u * win[d - 1].y; }
+        const zd xi = zscale(s, rd[i]);
+        b[i] = xi;
+#pragma unroll
+        for (int d = KD - 1; d >= 1; d--) win[d] = win[d - 1];
+        win[0] = xi;
+      }
+    } else if (OP == 1) {
+#pragma unroll
+      for (int d = 0; d < KD; d++) win[d] = zmk(0.0, 0.0);
+      for (int i = 0; i < N; i++) {
+        zd s = b[i];
+#pragma unroll
+        for (int d = KD; d >= 1; d--)
+          if (i - d >= 0) { const double u = U(i - d, i); s.x -= u * win[d - 1].x; s.y -= u * win[d - 1].y; }
+        const zd xi = zscale(s, rd[i]);
+        b[i] = xi;
+#pragma unroll
+        for (int d = KD - 1; d >= 1; d--) win[d] = win[d - 1];
+        win[0] = xi;
+      }
+    } else {
+      for (int i = 0; i < N; i++) {
+        zd s = zmk(0.0, 0.0);
+#pragma unroll
+        for (int d = 0; d <= KD; d++)
+          if (i + d < N) { const double u = U(i, i + d); s.x += u * b[i + d].x; s.y += u * b[i + d].y; }
+        b[i] = s;                                        // b[i] is not read again
+      }
+    }
+  }
+  __syncthreads();
+  if (rank == 2) {
+    for (int e = threadIdx.x; e < nf * N; e += 256) g[e] = tile[(e / N) * ld + e % N];
+  } else {
+    for (int e = threadIdx.x; e < nf * N; e += 256) {
+      const int i = e / nf, f = e % nf;
+      g[(long)i * N + f] = tile[f * ld + i];
+    }
+  }
+  (void)fstr; (void)gstr;
+}
+
 // ------------------------------------------------------------------ Gram-Schmidt
 // part[blk*pstr + 2k..] = partial <V_k, w> over this block's range, k < nk;
 // nk == 0: part[blk*pstr] = partial |w|^2.
@@ -283,29 +424,28 @@ __global__ void __launch_bounds__(256) p3_dots(const zd *V, const zd *w, long n,
   }
 }
 
-// Sum the partials in one fixed order.  mode 0: h_k = <V_k, w> for k <= j,
-// written to hcoef and column j of Hm.  mode 1: beta_{j+1} = |w| with the
-// breakdown test (pamp.h:305-312); mode 2: beta_0 = |psi~| (start of a step,
-// resets the breakdown state).
-__global__ void __launch_bounds__(256) p3_reduce(const double *part, int pstr, int nblk, int mode, int j, int m,
-                                                  zd *hcoef, zd *Hm, double *beta, int *state)
+// Sum the partials in one fixed order: one wavefront per component, lane l
+// summing blocks l, l+64, ... and a fixed shuffle tree.  mode 0: h_k =
+// <V_k, w> for k <= j, written to hcoef and column j of Hm.  mode 1:
+// beta_{j+1} = |w| with the breakdown test (pamp.h:305-312); mode 2: beta_0
+// = |psi~| (start of a step, resets the breakdown state).
+__global__ void __launch_bounds__(1024) p3_reduce(const double *part, int pstr, int nblk, int mode, int j, int m,
+                                                   zd *hcoef, zd *Hm, double *beta, int *state)
 {
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (mode != 2 && state[0]) return;
+  __syncthreads();   // every thread has read state[0] before mode 2 rewrites it
   const int ncomp = mode == 0 ? 2 * (j + 1) : 1;
-  for (int c = tid; c < ncomp; c += 256) {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int b = 0;
-    for (; b + 3 < nblk; b += 4) {
-      s0 += part[(long)b * pstr + c]; s1 += part[(long)(b + 1) * pstr + c];
-      s2 += part[(long)(b + 2) * pstr + c]; s3 += part[(long)(b + 3) * pstr + c];
-    }
-    for (; b < nblk; b++) s0 += part[(long)b * pstr + c];
-    const double v = (s0 + s1) + (s2 + s3);
+  for (int c = wv; c < ncomp; c += 16) {
+    double a = 0.0;
+    for (int b = lane; b < nblk; b += 64) a += part[(long)b * pstr + c];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    if (lane != 0) continue;
     if (mode == 0) {
-      ((double *)hcoef)[c] = v;   // h_k = (v[2k], v[2k+1])
+      ((double *)hcoef)[c] = a;   // h_k = (v[2k], v[2k+1])
     } else {
-      const double bt = sqrt(v);
+      const double bt = sqrt(a);
       if (mode == 2) {
         beta[0] = bt;
         state[0] = 0;
@@ -319,7 +459,7 @@ __global__ void __launch_bounds__(256) p3_reduce(const double *part, int pstr, i
   }
   if (mode == 0) {
     __syncthreads();
-    for (int k = tid; k <= j; k += 256) Hm[k + j * m] = hcoef[k];
+    for (int k = tid; k <= j; k += 1024) Hm[k + j * m] = hcoef[k];
   }
 }
 
@@ -396,9 +536,6 @@ __global__ void __launch_bounds__(256) p3_combine(const zd *V, const zd *cvec, c
 }
 
 // ------------------------------------------------------------------ host side
-extern "C" int pp_overlap_apply(const pp_overlap *ov, int op, int rank, int R, const int *shape, void *x,
-                                hipStream_t stream);
-
 static void p3_free_plan(pp_pamp3 *p)
 {
   void *bufs[] = {p->dmats, p->rrow, p->lrow, p->mrow, p->rent, p->lent, p->ment, p->cb_ptr, p->cb_blk,
@@ -553,6 +690,40 @@ static int p3_build(pp_pamp3 *p)
       for (int c = 0; c < C; c++) lrows[c].push_back({k, c, inter[k].second, -1, 1.0, 0.0});
     }
   }
+  // Entries of one row with the same source, channel and time slot whose
+  // coefficients are all real (or all imaginary) become one entry with the
+  // summed band matrix (e.g. the kinetic, centrifugal and Coulomb bands of a
+  // channel, or the 1/r and d/dr dipole bands of a channel pair).
+  auto merge_row = [&](std::vector<p3_ent> &r) {
+    std::vector<p3_ent> out;
+    std::vector<char> used(r.size(), 0);
+    const size_t cnt = (size_t)p->N * p->bw;
+    for (size_t a = 0; a < r.size(); a++) {
+      if (used[a]) continue;
+      used[a] = 1;
+      const bool re = r[a].cim == 0.0, im = r[a].cre == 0.0;
+      std::vector<size_t> grp(1, a);
+      for (size_t b = a + 1; b < r.size() && (re || im); b++) {
+        if (used[b] || r[b].src != r[a].src || r[b].col != r[a].col || r[b].slot != r[a].slot) continue;
+        if ((re && r[b].cim == 0.0) || (!re && r[b].cre == 0.0)) { grp.push_back(b); used[b] = 1; }
+      }
+      if (grp.size() == 1) { out.push_back(r[a]); continue; }
+      std::vector<double> M(cnt, 0.0);
+      for (size_t gi : grp) {
+        const double f = re ? r[gi].cre : r[gi].cim;
+        const double *src = p->mats[r[gi].mat].data();
+        for (size_t i = 0; i < cnt; i++) M[i] += f * src[i];
+      }
+      p3_ent e = r[a];
+      e.mat = p3_mat_id(p, M.data());
+      e.cre = re ? 1.0 : 0.0;
+      e.cim = re ? 0.0 : 1.0;
+      out.push_back(e);
+    }
+    r.swap(out);
+  };
+  for (auto &r : rrows) merge_row(r);
+  for (auto &r : lrows) merge_row(r);
   auto flatten = [](const std::vector<std::vector<p3_ent>> &rows, std::vector<int> &ptr, std::vector<p3_ent> &ent) {
     ptr.assign(1, 0);
     for (auto &r : rows) { ent.insert(ent.end(), r.begin(), r.end()); ptr.push_back((int)ent.size()); }
@@ -611,10 +782,12 @@ static int p3_apply_raw(pp_pamp3 *p, const zd *y, zd *z, const double *scal, hip
 {
   const int C = p->C, N = p->N;
   if (p->K > 0) {
-    hipLaunchKernelGGL(p3_right, dim3(p->K * C * N), dim3(128), 0, st, p->dmats, p->rrow, p->rent, y, p->X, C, N, p->s,
-                       p->bw, scal, p->state);
-    hipLaunchKernelGGL(p3_left, dim3(C * N), dim3(128), 0, st, p->dmats, p->lrow, p->lent, p->X, z, C, N, p->s, p->bw,
-                       scal, p->state);
+    const int ntile = (N + P3_RT - 1) / P3_RT;
+    hipLaunchKernelGGL(p3_right<6>, dim3(p->K * C * ntile), dim3(128), 0, st, p->dmats, p->rrow, p->rent, y, p->X, C,
+                       N, scal, p->state);
+    const int ltile = (N + P3_LT - 1) / P3_LT;
+    hipLaunchKernelGGL(p3_left<6>, dim3(C * ltile), dim3(128), 0, st, p->dmats, p->lrow, p->lent, p->X, z, C, N, scal,
+                       p->state);
   } else {
     hipLaunchKernelGGL(p3_zero, dim3(1024), dim3(256), 0, st, z, p->n);
   }
@@ -635,16 +808,26 @@ static int p3_apply_raw(pp_pamp3 *p, const zd *y, zd *z, const double *scal, hip
       p->ev.push_back(e0);
       p->ev.push_back(e1);
     }
+
   }
   return hipGetLastError() == hipSuccess ? PP_OK : PP_EHIP;
 }
 
+// op 0: (U x U)^-1, 1: (U x U)^-H, 2: (U x U) on both radial ranks.
 static int p3_solve2(pp_pamp3 *p, int op, zd *x, hipStream_t st)
 {
-  const int shape[3] = {p->C, p->N, p->N};
-  int rc = pp_overlap_apply(p->ov, op, 2, 3, shape, x, st);
-  if (rc == PP_OK) rc = pp_overlap_apply(p->ov, op, 1, 3, shape, x, st);
-  return rc;
+  const int C = p->C, N = p->N;
+  const size_t lds = sizeof(double) * ((size_t)N * 7 + N + ((N * 7 + N) & 1)) + sizeof(zd) * (size_t)P3_FT * (N + 1);
+  const dim3 g2((unsigned)(((long)C * N + P3_FT - 1) / P3_FT)), g1((unsigned)(C * ((N + P3_FT - 1) / P3_FT)));
+  const void *fn[3] = {(const void *)p3_ftile<6, 0>, (const void *)p3_ftile<6, 1>, (const void *)p3_ftile<6, 2>};
+  if (hipFuncSetAttribute(fn[op], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return PP_EHIP;
+  for (int rank = 2; rank >= 1; rank--) {
+    const dim3 grid = rank == 2 ? g2 : g1;
+    if (op == 0) hipLaunchKernelGGL((p3_ftile<6, 0>), grid, dim3(256), lds, st, p->ov->Uband, x, C, N, rank);
+    else if (op == 1) hipLaunchKernelGGL((p3_ftile<6, 1>), grid, dim3(256), lds, st, p->ov->Uband, x, C, N, rank);
+    else hipLaunchKernelGGL((p3_ftile<6, 2>), grid, dim3(256), lds, st, p->ov->Uband, x, C, N, rank);
+  }
+  return hipGetLastError() == hipSuccess ? PP_OK : PP_EHIP;
 }
 
 static int p3_ensure(pp_pamp3 *p, int nsteps, const double *scal)
@@ -708,7 +891,7 @@ extern "C" int pp_pamp3_propagate(pp_pamp3 *p, void *psi, int nsteps, double dt_
     if (hipMemcpyAsync(p->P, ps, vb, hipMemcpyDeviceToDevice, stream) != hipSuccess) return PP_EHIP;
     if ((rc = p3_solve2(p, 2 /* U x */, p->P, stream))) return rc;
     hipLaunchKernelGGL(p3_dots, dim3(G), dim3(256), 0, stream, p->V, p->P, n, 0, p->part, pstr, p->state, 0);
-    hipLaunchKernelGGL(p3_reduce, dim3(1), dim3(256), 0, stream, p->part, pstr, G, 2, -1, m, p->hcoef, p->Hm, p->beta,
+    hipLaunchKernelGGL(p3_reduce, dim3(1), dim3(1024), 0, stream, p->part, pstr, G, 2, -1, m, p->hcoef, p->Hm, p->beta,
                        p->state);
     hipLaunchKernelGGL(p3_scale, dim3(gridv), dim3(256), 0, stream, p->P, p->V, p->Y, n, p->beta, 0, p->state);
     for (int j = 0; j < m; j++) {
@@ -716,12 +899,12 @@ extern "C" int pp_pamp3_propagate(pp_pamp3 *p, void *psi, int nsteps, double dt_
       if ((rc = p3_apply_raw(p, p->Y, p->Z, sc, stream))) return rc;
       if ((rc = p3_solve2(p, 1, p->Z, stream))) return rc;
       hipLaunchKernelGGL(p3_dots, dim3(G), dim3(256), 0, stream, p->V, p->Z, n, j + 1, p->part, pstr, p->state, 1);
-      hipLaunchKernelGGL(p3_reduce, dim3(1), dim3(256), 0, stream, p->part, pstr, G, 0, j, m, p->hcoef, p->Hm,
+      hipLaunchKernelGGL(p3_reduce, dim3(1), dim3(1024), 0, stream, p->part, pstr, G, 0, j, m, p->hcoef, p->Hm,
                          p->beta, p->state);
       if (j == m - 1) break;   // the last residual is not used (pamp.h:419-426)
       hipLaunchKernelGGL(p3_project, dim3(G), dim3(256), 0, stream, p->V, p->Z, n, j + 1, p->hcoef, p->part, pstr,
                          p->state);
-      hipLaunchKernelGGL(p3_reduce, dim3(1), dim3(256), 0, stream, p->part, pstr, G, 1, j, m, p->hcoef, p->Hm,
+      hipLaunchKernelGGL(p3_reduce, dim3(1), dim3(1024), 0, stream, p->part, pstr, G, 1, j, m, p->hcoef, p->Hm,
                          p->beta, p->state);
       hipLaunchKernelGGL(p3_scale, dim3(gridv), dim3(256), 0, stream, p->Z, p->V + (size_t)(j + 1) * n, p->Y, n,
                          p->beta, j + 1, p->state);

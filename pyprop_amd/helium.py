@@ -153,7 +153,23 @@ class CoupledTensorPotential:
                                   "(PamPropagator's Engine.apply), not one by one")
 
     def CanConsolidate(self, other):
-        return False
+        """Same rule as TensorPotential.CanConsolidate (TensorPotential.py:
+        128-163): same time dependence and time function, same storage and
+        the same basis pairs on every rank."""
+        if not isinstance(other, CoupledTensorPotential):
+            return False
+        if self.IsTimeDependent != other.IsTimeDependent:
+            return False
+        if self.IsTimeDependent and self.OriginalTimeFunction != other.OriginalTimeFunction:
+            return False
+        if [g.lower().strip() for g in self.GeometryNames[1:]] != [g.lower().strip() for g in other.GeometryNames[1:]]:
+            return False
+        return self.Pairs.shape == other.Pairs.shape and np.array_equal(self.Pairs, other.Pairs)
+
+    def ConsolidateWith(self, other):
+        """self += other (the factored terms are concatenated)."""
+        self.Terms = list(self.Terms) + list(other.Terms)
+        self.Name += "+" + other.Name
 
     def ReferenceLayout(self):
         """The reference's tensor (angular pairs, radial pairs, radial pairs or

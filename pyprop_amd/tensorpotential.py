@@ -396,6 +396,16 @@ class TensorPotential:
         return True
 
 
+def _consolidate(existingPot, pot):
+    """existingPot += pot (BasisPropagator.py:95-103); factored potentials
+    merge their term lists."""
+    if hasattr(existingPot, "ConsolidateWith"):
+        existingPot.ConsolidateWith(pot)
+    else:
+        existingPot.PotentialData[:] += pot.PotentialData
+        existingPot.Name += "+" + pot.Name
+
+
 # ---------------------------------------------------------------- BasisPropagator
 class BasisPropagator(PropagatorBase):
     """Applies tensor potentials in the basis (BasisPropagator.py:1-168)."""
@@ -429,8 +439,7 @@ class BasisPropagator(PropagatorBase):
                 pot = self.GeneratePotential(configSection)
                 for existingPot in self.PotentialList:
                     if existingPot.CanConsolidate(pot):
-                        existingPot.PotentialData[:] += pot.PotentialData
-                        existingPot.Name += "+" + pot.Name
+                        _consolidate(existingPot, pot)
                         pot = None
                         break
                 if pot is not None:
@@ -444,8 +453,7 @@ class BasisPropagator(PropagatorBase):
             cur = potentials[i]
             for other in list(potentials[i + 1:]):
                 if cur.CanConsolidate(other):
-                    cur.PotentialData[:] += other.PotentialData
-                    cur.Name += "+" + other.Name
+                    _consolidate(cur, other)
                     potentials.remove(other)
                     remove.append(other)
             i += 1

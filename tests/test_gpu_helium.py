@@ -44,11 +44,11 @@ def test_helium_apply_matches_oracle():
     t = dev.torch()
     xd = dev.to_device(x)
     out = t.zeros_like(xd)
-    assert eng.nslots == 3          # the three laser sections, one time slot each
-    eng.apply(xd, out, scal=[s] * 3)
+    assert eng.nslots == 1          # the three laser sections consolidate into one potential
+    eng.apply(xd, out, scal=[s])
     ref = H.potentials(x)
     assert np.abs(out.cpu().numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
-    eng.apply(xd, out, scal=[s] * 3, balanced=True)
+    eng.apply(xd, out, scal=[s], balanced=True)
     ref = H.balanced(x)
     assert np.abs(out.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
 
@@ -158,6 +158,6 @@ def test_helium_apply_multichunk_shapes():
     t = dev.torch()
     xd = dev.to_device(x)
     out = t.zeros_like(xd)
-    prob.Propagator.Engine.apply(xd, out, scal=[s] * 3)
-    ref = _factored_apply(prob.Propagator.BasePropagator.PotentialList, x, [s] * 3)
+    prob.Propagator.Engine.apply(xd, out, scal=[s])
+    ref = _factored_apply(prob.Propagator.BasePropagator.PotentialList, x, [s])
     assert np.abs(out.cpu().numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
