@@ -304,6 +304,36 @@ def run_helium(args):
     norm = prob.psi.GetNorm()
     pop_ground = abs(ground.InnerProduct(prob.psi)) ** 2
     diag = prob.Propagator.GetDiagnostics()
+
+    cpu = None
+    if not args.no_cpu and args.cpu_steps > 0:
+        # the oracle's factored restatement (the reference layout of the e-e
+        # term would be ~48 GB): same terms, same start state, channel-parallel
+        # operator apply (OpenMP), serial Gram-Schmidt / solves / Pade
+        from oracle import oracle as O
+        from pyprop_amd.helium import KronTerm
+        nsteps_cpu = min(args.cpu_steps, 2)
+        b = prob.psi.GetRepresentation().GetRepresentation(1).GetBSplineObject()
+        Ub, info = O.zpbtrf_upper(b.overlap_upper(), N, 6)
+        ham = O.Ham((C, N, N), [], {1: (Ub, 6), 2: (Ub, 6)})
+        dep = []
+        for pot in prob.Propagator.BasePropagator.PotentialList:
+            for t in pot.Terms:
+                if isinstance(t, KronTerm):
+                    ham.add_factored(t.pairs, t.coef, A=t.A, B=t.B)
+                else:
+                    ham.add_factored(t.pairs, t.coef, R=t.R)
+                dep.append(pot)
+        ts = O.step_times(nsteps_cpu, dt.real, args.warmup * dt.real)
+        sc = [[p.GetTimeScaling(tt) for p in dep] for tt in ts]
+        c0 = time.perf_counter()
+        ham.propagate_pamp(psi0, nsteps_cpu, dt.real, sc, m)
+        cpu_s = time.perf_counter() - c0
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = {"value": nsteps_cpu / cpu_s, "unit": "steps/s", "cores": threads, "kind": "port",
+               "sample": "%d pAMP steps of the same workload (C oracle, factored operator, complex128, "
+                         "operator apply OpenMP over %d threads, Gram-Schmidt / solves serial)" % (nsteps_cpu, threads)}
+
     line = {
         "metric": "Krylov steps/s + matvec HBM GB/s, FP64 vs mixed at <=1e-8 population error",
         "value": args.steps / wall,
@@ -325,7 +355,7 @@ def run_helium(args):
                      "launches": ee_launches,
                      "kernel_share_of_step": ee_ms / 1e3 / (ev0.elapsed_time(ev1) / 1e3),
                      "note": "FP64 vector FMA; MI355X FP64 matrix and vector peaks are equal"},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
         "checks": {"ground_energy": E0, "ground_state_seconds": gs_s, "imag_steps": args.imag_steps,
                    "norm_after": norm, "ground_population_after": pop_ground,
                    "krylov_error_estimate_last": float(diag[-1, 0]), "basis_used_last": int(diag[-1, 1])},

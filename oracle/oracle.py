@@ -62,6 +62,8 @@ def lib():
         L.orc_ham_create.argtypes = [ctypes.c_int, ip]
         L.orc_ham_set_overlap.argtypes = [P, ctypes.c_int, ctypes.c_int, dp]
         L.orc_ham_add_term.argtypes = [P, ip, ip, dp, ctypes.POINTER(ip)]
+        L.orc_ham_add_factored.argtypes = [P, ctypes.c_int, ctypes.c_int, ip, dp, dp, dp, dp, ctypes.c_int]
+        L.orc_ham_add_factored.restype = ctypes.c_int
         L.orc_ham_set_scaling.argtypes = [P, dp]
         L.orc_ham_destroy.argtypes = [P]
         L.orc_ham_balanced.argtypes = [P, dp, dp]
@@ -416,6 +418,32 @@ class Ham:
     def set_scaling(self, s):
         s = np.ascontiguousarray(s, dtype=np.float64)
         lib().orc_ham_set_scaling(self.h, _dp(s))
+
+    def add_factored(self, pairs, coef, A=None, B=None, R=None, s=6):
+        """Factored two-electron term on a (C, N, N) state: sum_p coef_p
+        |a_p><b_p| (x) A (x) B (row-major real bands) or (x) R[i1][d1][i2][d2].
+        Used where the reference layout cannot be formed (the helium
+        workload's CPU baseline); returns the term index."""
+        pr = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        cf = np.ascontiguousarray(coef, dtype=np.complex128)
+        kind = 1 if R is not None else 0
+        keep = [pr, cf]
+        a = b = r = None
+        if kind == 0:
+            a = np.ascontiguousarray(A, dtype=np.float64)
+            b = np.ascontiguousarray(B, dtype=np.float64)
+            keep += [a, b]
+        else:
+            r = np.ascontiguousarray(R, dtype=np.float64)
+            keep.append(r)
+        nul = ctypes.POINTER(ctypes.c_double)()
+        t = lib().orc_ham_add_factored(self.h, kind, len(pr), _ip(pr), _dp(cf.view(np.float64)),
+                                       _dp(a) if a is not None else nul, _dp(b) if b is not None else nul,
+                                       _dp(r) if r is not None else nul, int(s))
+        if t < 0:
+            raise RuntimeError("orc_ham_add_factored failed")
+        self.nterms += 1
+        return t
 
     def balanced(self, x):
         x = np.ascontiguousarray(x, dtype=np.complex128)

@@ -490,7 +490,20 @@ ORC_API int orc_tp_multiply(int R, const int *storage, const int *potShape, cons
 /* ------------------------------------------------------------------ */
 /* Hamiltonian: list of tensor potentials + overlap factors per rank.  */
 /* ------------------------------------------------------------------ */
-#define ORC_MAXTERMS 16
+#define ORC_MAXTERMS 32
+#define ORC_FACTORED (-1)        /* storage id marking a factored two-electron term */
+
+/* Factored two-electron term on a (C, N, N) state (no reference layout):  */
+/* sum_p coef_p |a_p><b_p| (x) A (x) B (kind 0, A, B real row-major bands,  */
+/* element (i, j) at i*bw + s + j - i) or (x) R (kind 1, R[i1][d1][i2][d2]). */
+/* Rows are kept in CSR order of a_p for the channel-parallel apply.        */
+typedef struct {
+    int kind, P, N, s;
+    int *rowptr, *col;           /* C+1, P */
+    zc *coef;                    /* P */
+    double *A, *B, *R;
+} orc_fterm;
+
 typedef struct {
     int R;
     int shape[3];
@@ -503,6 +516,7 @@ typedef struct {
     zc *pot[ORC_MAXTERMS];
     int *pairs[ORC_MAXTERMS][3];
     double scaling[ORC_MAXTERMS];
+    orc_fterm *ft[ORC_MAXTERMS];
     zc *tmp;
     long matvecs;
 } orc_ham;
@@ -547,6 +561,121 @@ ORC_API int orc_ham_add_term(orc_ham *h, const int *storage, const int *potShape
     return t;
 }
 
+/* Add a factored term (rank-3 states only); returns its term index, whose */
+/* scaling is set by orc_ham_set_scaling like any other term.              */
+ORC_API int orc_ham_add_factored(orc_ham *h, int kind, int P, const int *pairs, const double *coef,
+                                 const double *A, const double *B, const double *R, int s)
+{
+    if (h->nterms >= ORC_MAXTERMS || h->R != 3 || h->shape[1] != h->shape[2] || h->shape[0] <= 0) return -1;
+    const int C = h->shape[0], N = h->shape[1], bw = 2 * s + 1;
+    for (int p = 0; p < 2 * P; p++)
+        if (pairs[p] < 0 || pairs[p] >= C) return -1;
+    int t = h->nterms++;
+    orc_fterm *f = calloc(1, sizeof(*f));
+    f->kind = kind; f->P = P; f->N = N; f->s = s;
+    f->rowptr = calloc(C + 1, sizeof(int));
+    f->col = malloc(sizeof(int) * (P > 0 ? P : 1));
+    f->coef = malloc(sizeof(zc) * (P > 0 ? P : 1));
+    for (int p = 0; p < P; p++) f->rowptr[pairs[2 * p] + 1]++;
+    for (int c = 0; c < C; c++) f->rowptr[c + 1] += f->rowptr[c];
+    int *fill = calloc(C, sizeof(int));
+    for (int p = 0; p < P; p++) {
+        const int a = pairs[2 * p], q = f->rowptr[a] + fill[a]++;
+        f->col[q] = pairs[2 * p + 1];
+        f->coef[q] = coef[2 * p] + I * coef[2 * p + 1];
+    }
+    free(fill);
+    if (kind == 0) {
+        f->A = malloc(sizeof(double) * N * bw); memcpy(f->A, A, sizeof(double) * N * bw);
+        f->B = malloc(sizeof(double) * N * bw); memcpy(f->B, B, sizeof(double) * N * bw);
+    } else {
+        const size_t cnt = (size_t)N * bw * N * bw;
+        f->R = malloc(sizeof(double) * cnt); memcpy(f->R, R, sizeof(double) * cnt);
+    }
+    h->storage[t][0] = h->storage[t][1] = h->storage[t][2] = ORC_FACTORED;
+    h->pot[t] = NULL;
+    for (int r = 0; r < 3; r++) h->pairs[t][r] = NULL;
+    h->ft[t] = f;
+    h->scaling[t] = 1.0;
+    return t;
+}
+
+/* dst += sc * term(src) for a factored term, parallel over output channels */
+/* (each thread owns whole output channels, so the result does not depend   */
+/* on the thread count).                                                    */
+static void fterm_apply(const orc_fterm *f, int C, double sc, const zc *src, zc *dst)
+{
+    const int N = f->N, s = f->s, bw = 2 * s + 1;
+    const long NN = (long)N * N;
+    if (f->kind == 0) {
+        #pragma omp parallel
+        {
+            zc *W = malloc(sizeof(zc) * NN);
+            #pragma omp for schedule(dynamic)
+            for (int a = 0; a < C; a++) {
+                for (int q = f->rowptr[a]; q < f->rowptr[a + 1]; q++) {
+                    const zc *y = src + (long)f->col[q] * NN;
+                    for (int j1 = 0; j1 < N; j1++)          /* W = y B^T */
+                        for (int i2 = 0; i2 < N; i2++) {
+                            zc acc = 0.0;
+                            for (int d = 0; d < bw; d++) {
+                                const int j2 = i2 + d - s;
+                                if (j2 >= 0 && j2 < N) acc += f->B[(long)i2 * bw + d] * y[(long)j1 * N + j2];
+                            }
+                            W[(long)j1 * N + i2] = acc;
+                        }
+                    const zc cf = sc * f->coef[q];
+                    for (int i1 = 0; i1 < N; i1++)          /* dst += cf A W */
+                        for (int i2 = 0; i2 < N; i2++) {
+                            zc acc = 0.0;
+                            for (int d = 0; d < bw; d++) {
+                                const int j1 = i1 + d - s;
+                                if (j1 >= 0 && j1 < N) acc += f->A[(long)i1 * bw + d] * W[(long)j1 * N + i2];
+                            }
+                            dst[(long)a * NN + (long)i1 * N + i2] += cf * acc;
+                        }
+                }
+            }
+            free(W);
+        }
+        return;
+    }
+    /* multipole: U_b = R (.) y_b once per source channel, then the mixing */
+    char *need = calloc(C, 1);
+    for (int q = 0; q < f->rowptr[C]; q++) need[f->col[q]] = 1;
+    zc *Ub = malloc(sizeof(zc) * NN * C);
+    #pragma omp parallel for schedule(dynamic)
+    for (int b = 0; b < C; b++) {
+        if (!need[b]) continue;
+        const zc *y = src + (long)b * NN;
+        zc *u = Ub + (long)b * NN;
+        for (int i1 = 0; i1 < N; i1++)
+            for (int i2 = 0; i2 < N; i2++) {
+                zc acc = 0.0;
+                for (int d1 = 0; d1 < bw; d1++) {
+                    const int j1 = i1 + d1 - s;
+                    if (j1 < 0 || j1 >= N) continue;
+                    const double *Rr = f->R + (((long)i1 * bw + d1) * N + i2) * bw;
+                    for (int d2 = 0; d2 < bw; d2++) {
+                        const int j2 = i2 + d2 - s;
+                        if (j2 >= 0 && j2 < N) acc += Rr[d2] * y[(long)j1 * N + j2];
+                    }
+                }
+                u[(long)i1 * N + i2] = acc;
+            }
+    }
+    #pragma omp parallel for schedule(dynamic)
+    for (int a = 0; a < C; a++)
+        for (int q = f->rowptr[a]; q < f->rowptr[a + 1]; q++) {
+            const zc cf = sc * f->coef[q];
+            const zc *u = Ub + (long)f->col[q] * NN;
+            zc *d = dst + (long)a * NN;
+            for (long e = 0; e < NN; e++) d[e] += cf * u[e];
+        }
+    free(Ub);
+    free(need);
+}
+
 ORC_API void orc_ham_set_scaling(orc_ham *h, const double *s)
 {
     for (int t = 0; t < h->nterms; t++) h->scaling[t] = s[t];
@@ -558,6 +687,10 @@ ORC_API void orc_ham_destroy(orc_ham *h)
     for (int t = 0; t < h->nterms; t++) {
         free(h->pot[t]);
         for (int r = 0; r < h->R; r++) free(h->pairs[t][r]);
+        if (h->ft[t]) {
+            orc_fterm *f = h->ft[t];
+            free(f->rowptr); free(f->col); free(f->coef); free(f->A); free(f->B); free(f->R); free(f);
+        }
     }
     for (int r = 0; r < h->R; r++) free(h->U[r]);
     free(h->tmp);
@@ -568,10 +701,12 @@ ORC_API void orc_ham_destroy(orc_ham *h)
 static void ham_potentials(orc_ham *h, const zc *src, zc *dst)
 {
     memset(dst, 0, sizeof(zc) * h->n);
-    for (int t = 0; t < h->nterms; t++)
+    for (int t = 0; t < h->nterms; t++) {
+        if (h->ft[t]) { fterm_apply(h->ft[t], h->shape[0], h->scaling[t], src, dst); continue; }
         orc_tp_multiply(h->R, h->storage[t], h->potShape[t], (const double *)h->pot[t],
                         h->scaling[t], h->shape, (const double *)src, (double *)dst,
                         (const int *const *)h->pairs[t]);
+    }
 }
 
 /* MultiplyHamiltonianBalancedOverlap (BasisPropagator.py:131-159):    */

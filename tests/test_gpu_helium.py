@@ -161,3 +161,26 @@ def test_helium_apply_multichunk_shapes():
     prob.Propagator.Engine.apply(xd, out, scal=[s])
     ref = _factored_apply(prob.Propagator.BasePropagator.PotentialList, x, [s])
     assert np.abs(out.cpu().numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_helium_workload_size_properties():
+    """The helium workload itself (66 channels, N = 120, 950,400 amplitudes):
+    the imaginary-time ground state is stationary under field-free steps and
+    the laser steps conserve the norm."""
+    print("helium workload: ground state + 10 field-free + 10 laser steps", flush=True)
+    psi0, E0 = HE.ground_state("helium", steps=150, dt=0.05)
+    assert -2.91 < E0 < -2.89
+    prob = _problem("helium", field_free=True)
+    prob.SetWavefunction(psi0)
+    prob.psi.Normalize()
+    g = prob.psi.Copy()
+    prob.AdvanceSteps(10)
+    ov = abs(g.InnerProduct(prob.psi)) ** 2
+    assert ov > 1 - 1e-6
+    assert abs(prob.psi.GetNorm() - 1.0) < 1e-11
+    las = _problem("helium")
+    las.SetWavefunction(psi0)
+    las.psi.Normalize()
+    las.PropagatedTime = 10.0
+    las.AdvanceSteps(10)
+    assert abs(las.psi.GetNorm() - 1.0) < 1e-10

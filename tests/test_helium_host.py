@@ -212,3 +212,29 @@ def test_helium_energies_match_reference_table():
     assert abs(w[0] - (-2.903586)) < 1e-5
     assert abs(w[1] - (-2.175229)) < 2e-6
     assert abs(w[2] - (-2.145965)) < 2e-6
+
+
+def test_oracle_factored_terms_match_reference_layout():
+    """The oracle's factored two-electron apply (the helium workload's CPU
+    baseline) against its reference-layout tensors on the small case."""
+    conf, repr, bp = _assemble("helium_small")
+    h = O.Helium(lmax=1, Ls=(0, 1), nbp=6, rmax=8.0, frequency=2.0, cycles=8, E0=0.05)
+    Href = h.ham()
+    Hf = O.Ham(h.shape, [], {1: (h.U, h.kd), 2: (h.U, h.kd)})
+    dep = []
+    for pot in bp.PotentialList:
+        for t in pot.Terms:
+            if isinstance(t, KronTerm):
+                Hf.add_factored(t.pairs, t.coef, A=t.A, B=t.B)
+            else:
+                Hf.add_factored(t.pairs, t.coef, R=t.R)
+            dep.append(pot.IsTimeDependent)
+    s = 0.41
+    Hf.set_scaling(np.array([s if d else 1.0 for d in dep]))
+    Href.set_scaling(np.array([s if k == "laser" else 1.0 for k, _, _ in h.terms]))
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(h.shape) + 1j * rng.standard_normal(h.shape)
+    a, b = Hf.potentials(x), Href.potentials(x)
+    assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max()
+    a, b = Hf.balanced(x), Href.balanced(x)
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
