@@ -174,9 +174,16 @@ int pp_pamp3_add_kron_term(pp_pamp3 *p, int P, const int *pairs, const double *c
                            const double *B, int slot);
 /* A multipole term sum_p coef[p] |pairs[p,0]><pairs[p,1]| (x) R with the real
  * 4-index radial tensor R[i1][d1][i2][d2] (N x 13 x N x 13, j = i + d - 6):
- * one l of CustomPotentialEvaluationR12 (coupledspherical.cpp:16-114). */
+ * one l of CustomPotentialEvaluationR12 (coupledspherical.cpp:16-114).
+ * Pband, Qband (optional, both or neither): <B|r^l|B> and <B|r^-(l+1)|B> as
+ * row-major bands; with them and the radial supports set, the engine applies
+ * R as P (x) Q / Q (x) P wherever the supports of B_i1 and B_i2 are disjoint
+ * (exact for product quadrature) and the full R only on the near band. */
 int pp_pamp3_add_multipole_term(pp_pamp3 *p, int P, const int *pairs, const double *coef, const double *R,
-                                int slot);
+                                const double *Pband, const double *Qband, int slot);
+/* Support of every B-spline as an interval range [lo[i], hi[i]] of the
+ * breakpoint intervals (both non-decreasing in i). */
+int pp_pamp3_set_radial_supports(pp_pamp3 *p, const int *lo, const int *hi);
 /* dst = H src, or U^-H H U^-1 src with balanced != 0 (src and dst device,
  * scal: nslots time scalings on the host).  Synchronises the stream. */
 int pp_pamp3_apply(pp_pamp3 *p, const void *src, void *dst, const double *scal, int balanced, pp_stream stream);

@@ -29,6 +29,7 @@ EXPORTS = [
     "pp_lowrank_factor", "pp_pamp_propagate", "pp_pamp_diagnostics", "pp_pamp_set_mixed", "pp_pamp_set_pade_scaling", "pp_expokit_propagate", "pp_expokit_diagnostics", "pp_pamp_batch_propagate", "pp_pamp_lds_estimate", "pp_pamp_set_timing", "pp_pamp_timings", "pp_pamp_destroy",
     "pp_pamp3_create", "pp_pamp3_add_kron_term", "pp_pamp3_add_multipole_term", "pp_pamp3_apply",
     "pp_pamp3_propagate", "pp_pamp3_diagnostics", "pp_pamp3_set_timing", "pp_pamp3_kernel_time",
+    "pp_pamp3_set_radial_supports",
     "pp_pamp3_multipole_flops", "pp_pamp3_destroy",
 ]
 
@@ -83,7 +84,8 @@ def lib():
     L.pp_pamp_lds_estimate.argtypes = [I, I, I, I, I, I]
     L.pp_pamp3_create.argtypes = [I, I, I, I, P, ctypes.POINTER(P)]
     L.pp_pamp3_add_kron_term.argtypes = [P, I, ip, dp, dp, dp, I]
-    L.pp_pamp3_add_multipole_term.argtypes = [P, I, ip, dp, dp, I]
+    L.pp_pamp3_add_multipole_term.argtypes = [P, I, ip, dp, dp, dp, dp, I]
+    L.pp_pamp3_set_radial_supports.argtypes = [P, ip, ip]
     L.pp_pamp3_apply.argtypes = [P, P, P, dp, I, P]
     L.pp_pamp3_propagate.argtypes = [P, P, I, D, D, dp, P]
     L.pp_pamp3_diagnostics.argtypes = [P, I, dp, P]

@@ -61,7 +61,17 @@ struct pp_pamp3 {
   std::vector<std::vector<double>> mats;
   struct kterm { std::vector<int> pr; std::vector<double> cf; int A, B, slot; };
   std::vector<kterm> kterms;
-  struct mterm { std::vector<int> pr; std::vector<double> cf; double *R; int slot; };
+  struct mterm {
+    std::vector<int> pr; std::vector<double> cf; double *R; int slot;
+    std::vector<double> Rh, Pl, Ql;   // host copies for the near/far split (Pl, Ql empty: no split)
+    double *dRn, *dP, *dQ;
+  };
+  std::vector<int> sup_lo, sup_hi;     // quadrature-interval support of each B-spline
+  bool split;
+  int2 *nr;
+  std::vector<int2> nrh;               // near band per i1 (host copy, for the flop count)
+  const double **dRn, **dPt, **dQt;
+  zd *Tq, *Tp;
   std::vector<mterm> mterms;
   int nslots;
   bool dirty;
@@ -289,6 +299,216 @@ __global__ void __launch_bounds__(P3_EE_THREADS) p3_ee(const double *const *Rt, 
       zd *z = Z + (long)c * NN + (long)i1 * N + i2base + i2l;
       *z = zadd(*z, acc[a][t]);
     }
+  }
+}
+
+// ------------------------------------------------------------------ multipole terms, near/far split
+// Where the supports of B_i1 and B_i2 do not overlap (i2 > nhi[i1]: every
+// quadrature point of electron 1 lies below every point of electron 2;
+// i2 < nlo[i1]: above), r<^l / r>^(l+1) is a product on the whole product-
+// quadrature support and R_l[i1, j1, i2, j2] = P_l[i1, j1] Q_l[i2, j2] (resp.
+// Q_l[i1, j1] P_l[i2, j2]) exactly, with P_l = <B|r^l|B>, Q_l = <B|r^-(l+1)|B>.
+// Those outputs take two banded passes over the mixed input; only the near
+// band nlo[i1] <= i2 <= nhi[i1] (at most 2k-1 wide) needs the 169-tap
+// contraction, with R stored for that band only.
+#define P3_NW 16           // near-band width handled per i1 (>= 2k-1)
+#define P3_MAXB 32         // multipole blocks per chunk (terms l <= 31)
+
+// Tq[b][j1][i2][cl] = sum_d Q[i2][d] Ym[b][j1][i2+d-S][cl], Tp likewise with P
+template <int S>
+__global__ void __launch_bounds__(256) p3_farT(const double *const *Pt, const double *const *Qt, const int *blk_term,
+                                               const zd *Ym, zd *Tq, zd *Tp, int N, const int *state)
+{
+  if (state[0]) return;
+  constexpr int BW = 2 * S + 1;
+  extern __shared__ double p3_lds[];
+  zd *row = (zd *)p3_lds;                                // positions -S .. N+S-1, 16 channels each
+  const int b = blockIdx.x / N, j1 = blockIdx.x % N;
+  const long NN = (long)N * N;
+  const zd *Yr = Ym + ((long)b * NN + (long)j1 * N) * P3_CHUNK;
+  for (int e = threadIdx.x; e < (N + 2 * S) * P3_CHUNK; e += 256) {
+    const int pos = e / P3_CHUNK - S;
+    row[e] = (pos >= 0 && pos < N) ? Yr[(long)pos * P3_CHUNK + (e & (P3_CHUNK - 1))] : zmk(0.0, 0.0);
+  }
+  __syncthreads();
+  const double *P = Pt[blk_term[b]], *Q = Qt[blk_term[b]];
+  zd *oq = Tq + ((long)b * NN + (long)j1 * N) * P3_CHUNK, *op = Tp + ((long)b * NN + (long)j1 * N) * P3_CHUNK;
+  for (int e = threadIdx.x; e < N * P3_CHUNK; e += 256) {
+    const int i2 = e / P3_CHUNK, cl = e & (P3_CHUNK - 1);
+    const double *qr = Q + (size_t)i2 * BW, *pr = P + (size_t)i2 * BW;
+    zd tq = zmk(0.0, 0.0), tp = zmk(0.0, 0.0);
+#pragma unroll
+    for (int d = 0; d < BW; d++) {
+      const zd y = row[(i2 + d) * P3_CHUNK + cl];
+      zfmar(tq, qr[d], y);
+      zfmar(tp, pr[d], y);
+    }
+    oq[e] = tq;
+    op[e] = tp;
+  }
+}
+
+// Z[c][i1][i2] += sum_blocks [ i2 > nhi[i1]:  sum_d1 P[i1][d1] Tq[b][i1+d1-S][i2][cl]
+//                              i2 < nlo[i1]:  sum_d1 Q[i1][d1] Tp[b][i1+d1-S][i2][cl] ]
+// workgroup: (chunk, P3_RT rows i1, 16 columns i2); thread: (i2, channel lane).
+template <int S>
+__global__ void __launch_bounds__(256) p3_farL(const double *const *Pt, const double *const *Qt, const int *blk_term,
+                                               const int *cb_ptr, const int *cb_blk, const int2 *nr, const zd *Tq,
+                                               const zd *Tp, zd *Z, int C, int N, int nchunk, const int *state)
+{
+  if (state[0]) return;
+  constexpr int BW = 2 * S + 1, WIN = P3_RT + 2 * S;
+  const int ntile = (N + P3_RT - 1) / P3_RT, ngrp = (N + 15) / 16;
+  int g = blockIdx.x;
+  const int chunk = g % nchunk; g /= nchunk;
+  const int i1a = (g % ntile) * P3_RT; g /= ntile;
+  const int i2 = g * 16 + (threadIdx.x >> 4), cl = threadIdx.x & (P3_CHUNK - 1);
+  (void)ngrp;
+  const long NN = (long)N * N;
+  const bool col_ok = i2 < N;
+  bool below[P3_RT], above[P3_RT];
+#pragma unroll
+  for (int rr = 0; rr < P3_RT; rr++) {
+    const int i1 = i1a + rr;
+    const int2 r = i1 < N ? nr[i1] : make_int2(N, -1);
+    below[rr] = col_ok && i1 < N && i2 > r.y;
+    above[rr] = col_ok && i1 < N && i2 < r.x;
+  }
+  bool anyb = false, anya = false;
+#pragma unroll
+  for (int rr = 0; rr < P3_RT; rr++) { anyb = anyb || below[rr]; anya = anya || above[rr]; }
+  __shared__ const double *sP[P3_MAXB];
+  __shared__ const double *sQ[P3_MAXB];
+  __shared__ long sT[P3_MAXB];
+  const int b0 = cb_ptr[chunk], nb = cb_ptr[chunk + 1] - b0;
+  for (int k = threadIdx.x; k < nb; k += 256) {
+    const int b = cb_blk[b0 + k];
+    sP[k] = Pt[blk_term[b]];
+    sQ[k] = Qt[blk_term[b]];
+    sT[k] = (long)b * NN * P3_CHUNK;
+  }
+  __syncthreads();
+  zd acc[P3_RT];
+#pragma unroll
+  for (int rr = 0; rr < P3_RT; rr++) acc[rr] = zmk(0.0, 0.0);
+  for (int k = 0; k < nb; k++) {
+    const double *P = sP[k], *Q = sQ[k];
+    const long tb = sT[k];
+    zd win[WIN];
+    if (anyb) {
+#pragma unroll
+      for (int w = 0; w < WIN; w++) {
+        const int j1 = i1a - S + w;
+        win[w] = (j1 >= 0 && j1 < N && col_ok) ? Tq[tb + ((long)j1 * N + i2) * P3_CHUNK + cl] : zmk(0.0, 0.0);
+      }
+#pragma unroll
+      for (int rr = 0; rr < P3_RT; rr++) {
+        if (!below[rr]) continue;
+        const double *pr = P + (size_t)(i1a + rr) * BW;
+        zd t = zmk(0.0, 0.0);
+#pragma unroll
+        for (int d = 0; d < BW; d++) zfmar(t, pr[d], win[rr + d]);
+        acc[rr] = zadd(acc[rr], t);
+      }
+    }
+    if (anya) {
+#pragma unroll
+      for (int w = 0; w < WIN; w++) {
+        const int j1 = i1a - S + w;
+        win[w] = (j1 >= 0 && j1 < N && col_ok) ? Tp[tb + ((long)j1 * N + i2) * P3_CHUNK + cl] : zmk(0.0, 0.0);
+      }
+#pragma unroll
+      for (int rr = 0; rr < P3_RT; rr++) {
+        if (!above[rr]) continue;
+        const double *qr = Q + (size_t)(i1a + rr) * BW;
+        zd t = zmk(0.0, 0.0);
+#pragma unroll
+        for (int d = 0; d < BW; d++) zfmar(t, qr[d], win[rr + d]);
+        acc[rr] = zadd(acc[rr], t);
+      }
+    }
+  }
+  const int c = chunk * P3_CHUNK + cl;
+  if (c >= C || !col_ok) return;
+#pragma unroll
+  for (int rr = 0; rr < P3_RT; rr++) {
+    if (!(below[rr] || above[rr])) continue;
+    const long o = (long)c * NN + (long)(i1a + rr) * N + i2;
+    Z[o] = zadd(Z[o], acc[rr]);
+  }
+}
+
+// Z[c][i1][i2] += sum_blocks sum_{d1,d2} Rn_t[i1][d1][w][d2] Ym[b][i1+d1-S][i2+d2-S][cl]
+// for the near band i2 = nlo[i1] + w <= nhi[i1]; workgroup: (chunk, i1),
+// 1024 threads = P3_NG4 groups x (w, channel lane).  Group g takes the
+// blocks g, g + P3_NG4, ... and the groups' partial sums are added in LDS in
+// group order, so the result is deterministic.  The block list (input offset,
+// R slice) is staged in LDS first; two accumulators split the tap chain.
+#define P3_NG4 4
+template <int S>
+__global__ void __launch_bounds__(256 * P3_NG4) p3_near(const double *const *Rn, const int *blk_term,
+                                                        const int *cb_ptr, const int *cb_blk, const int2 *nr,
+                                                        const zd *Ym, zd *Z, int C, int N, int nchunk,
+                                                        const int *state)
+{
+  if (state[0]) return;
+  constexpr int BW = 2 * S + 1;
+  __shared__ const double *sR[P3_MAXB];
+  __shared__ long sY[P3_MAXB];
+  __shared__ zd part[P3_NG4][256];
+  const int chunk = blockIdx.x % nchunk, i1 = blockIdx.x / nchunk;
+  const long NN = (long)N * N;
+  const int b0 = cb_ptr[chunk], nb = cb_ptr[chunk + 1] - b0;
+  for (int k = threadIdx.x; k < nb; k += 256 * P3_NG4) {
+    const int b = cb_blk[b0 + k];
+    sR[k] = Rn[blk_term[b]] + (size_t)i1 * BW * P3_NW * BW;
+    sY[k] = (long)b * NN * P3_CHUNK;
+  }
+  __syncthreads();
+  const int grp = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const int w = t >> 4, cl = t & (P3_CHUNK - 1);
+  const int2 r = nr[i1];
+  const int i2 = r.x + w;
+  const bool ok = i2 <= r.y;
+  zd a0 = zmk(0.0, 0.0), a1 = zmk(0.0, 0.0);
+  if (ok) {
+    // taps outside 0 <= j2 < N have R = 0: load a clamped (valid) position
+    // instead of branching, so all 13 loads of a row issue together
+    int jc[BW];
+#pragma unroll
+    for (int d2 = 0; d2 < BW; d2++) {
+      const int j2 = i2 + d2 - S;
+      jc[d2] = j2 < 0 ? 0 : (j2 >= N ? N - 1 : j2);
+    }
+    for (int k = grp; k < nb; k += P3_NG4) {
+      const double *R = sR[k];
+      const zd *Yb = Ym + sY[k] + cl;
+      for (int d1 = 0; d1 < BW; d1++) {
+        const int j1 = i1 + d1 - S;
+        if (j1 < 0 || j1 >= N) continue;
+        const zd *Yr = Yb + (long)j1 * N * P3_CHUNK;
+        const double *Rr = R + ((size_t)d1 * P3_NW + w) * BW;
+        zd y[BW];
+#pragma unroll
+        for (int d2 = 0; d2 < BW; d2++) y[d2] = Yr[(long)jc[d2] * P3_CHUNK];
+#pragma unroll
+        for (int d2 = 0; d2 < BW; d2++) {
+          if (d2 & 1) zfmar(a1, Rr[d2], y[d2]);
+          else zfmar(a0, Rr[d2], y[d2]);
+        }
+      }
+    }
+  }
+  part[grp][t] = zadd(a0, a1);
+  __syncthreads();
+  if (grp != 0 || !ok) return;
+  zd v = part[0][t];
+#pragma unroll
+  for (int g = 1; g < P3_NG4; g++) v = zadd(v, part[g][t]);
+  const int c = chunk * P3_CHUNK + cl;
+  if (c < C) {
+    const long o = (long)c * NN + (long)i1 * N + i2;
+    Z[o] = zadd(Z[o], v);
   }
 }
 
@@ -539,9 +759,17 @@ __global__ void __launch_bounds__(256) p3_combine(const zd *V, const zd *cvec, c
 static void p3_free_plan(pp_pamp3 *p)
 {
   void *bufs[] = {p->dmats, p->rrow, p->lrow, p->mrow, p->rent, p->lent, p->ment, p->cb_ptr, p->cb_blk,
-                  p->blk_term, (void *)p->dR, p->X, p->Ym};
+                  p->blk_term, (void *)p->dR, p->X, p->Ym, p->nr, (void *)p->dRn, (void *)p->dPt, (void *)p->dQt,
+                  p->Tq, p->Tp};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
+  for (auto &t : p->mterms) {
+    void *tb[] = {t.dRn, t.dP, t.dQ};
+    for (void *b : tb)
+      if (b) (void)hipFree(b);
+    t.dRn = t.dP = t.dQ = nullptr;
+  }
+  p->nr = nullptr; p->dRn = p->dPt = p->dQt = nullptr; p->Tq = p->Tp = nullptr; p->split = false;
   p->dmats = nullptr; p->rrow = p->lrow = p->mrow = nullptr; p->rent = p->lent = p->ment = nullptr;
   p->cb_ptr = p->cb_blk = p->blk_term = nullptr; p->dR = nullptr; p->X = nullptr; p->Ym = nullptr;
 }
@@ -559,6 +787,7 @@ extern "C" int pp_pamp3_create(int C, int N, int m, int reorth, const pp_overlap
   p->nslots = 0; p->dirty = true;
   p->dmats = nullptr; p->rrow = p->lrow = p->mrow = nullptr; p->rent = p->lent = p->ment = nullptr;
   p->cb_ptr = p->cb_blk = p->blk_term = nullptr; p->dR = nullptr; p->X = nullptr; p->Ym = nullptr;
+  p->nr = nullptr; p->dRn = p->dPt = p->dQt = nullptr; p->Tq = p->Tp = nullptr; p->split = false;
   p->K = 0; p->nblk = 0; p->nchunk = (C + P3_CHUNK - 1) / P3_CHUNK;
   p->scal = p->diag = nullptr; p->scal_cap = p->diag_cap = 0;
   p->timing = 0; p->ee_ms = 0.0; p->ee_count = 0;
@@ -605,6 +834,17 @@ static int p3_check_pairs(const pp_pamp3 *p, int P, const int *pairs, const doub
   return PP_OK;
 }
 
+extern "C" int pp_pamp3_set_radial_supports(pp_pamp3 *p, const int *lo, const int *hi)
+{
+  if (!p || !lo || !hi) return PP_EARG;
+  for (int i = 0; i < p->N; i++)
+    if (lo[i] > hi[i] || (i > 0 && (lo[i] < lo[i - 1] || hi[i] < hi[i - 1]))) return PP_EARG;
+  p->sup_lo.assign(lo, lo + p->N);
+  p->sup_hi.assign(hi, hi + p->N);
+  p->dirty = true;
+  return PP_OK;
+}
+
 extern "C" int pp_pamp3_add_kron_term(pp_pamp3 *p, int P, const int *pairs, const double *coef, const double *A,
                                       const double *B, int slot)
 {
@@ -624,9 +864,9 @@ extern "C" int pp_pamp3_add_kron_term(pp_pamp3 *p, int P, const int *pairs, cons
 }
 
 extern "C" int pp_pamp3_add_multipole_term(pp_pamp3 *p, int P, const int *pairs, const double *coef, const double *R,
-                                           int slot)
+                                           const double *Pband, const double *Qband, int slot)
 {
-  if (!p || !R) return PP_EARG;
+  if (!p || !R || (!Pband) != (!Qband)) return PP_EARG;
   int rc = p3_check_pairs(p, P, pairs, coef, slot);
   if (rc) return rc;
   const size_t cnt = (size_t)p->N * p->bw * p->N * p->bw;
@@ -635,6 +875,12 @@ extern "C" int pp_pamp3_add_multipole_term(pp_pamp3 *p, int P, const int *pairs,
   t.cf.assign(coef, coef + 2 * P);
   t.slot = slot;
   t.R = nullptr;
+  t.dRn = t.dP = t.dQ = nullptr;
+  if (Pband) {
+    t.Rh.assign(R, R + cnt);
+    t.Pl.assign(Pband, Pband + (size_t)p->N * p->bw);
+    t.Ql.assign(Qband, Qband + (size_t)p->N * p->bw);
+  }
   if (hipMalloc(&t.R, cnt * sizeof(double)) != hipSuccess) return PP_EHIP;
   if (hipMemcpy(t.R, R, cnt * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { (void)hipFree(t.R); return PP_EHIP; }
   p->mterms.push_back(t);
@@ -773,6 +1019,45 @@ static int p3_build(pp_pamp3 *p)
   if (p->K > 0 && hipMalloc(&p->X, vb * (size_t)p->K) != hipSuccess) return PP_EHIP;
   if (p->nblk > 0 && hipMalloc(&p->Ym, sizeof(zd) * (size_t)p->nblk * p->N * p->N * P3_CHUNK) != hipSuccess)
     return PP_EHIP;
+  // near/far split when every multipole term brings P_l, Q_l and the supports are known
+  bool split = !p->mterms.empty() && (int)p->sup_lo.size() == p->N;
+  for (auto &t : p->mterms) split = split && !t.Pl.empty();
+  if (split) {
+    const int N = p->N, bw = p->bw;
+    std::vector<int2> nrh(N);
+    for (int i1 = 0; i1 < N; i1++) {
+      int lo = N, hi = -1;
+      for (int i2 = 0; i2 < N; i2++) {
+        const bool below = p->sup_hi[i1] < p->sup_lo[i2], above = p->sup_hi[i2] < p->sup_lo[i1];
+        if (!below && !above) { lo = std::min(lo, i2); hi = std::max(hi, i2); }
+      }
+      if (hi - lo + 1 > P3_NW || hi < lo) return PP_EUNSUP;
+      for (int i2 = lo; i2 <= hi; i2++)   // the near set must be contiguous
+        if (p->sup_hi[i1] < p->sup_lo[i2] || p->sup_hi[i2] < p->sup_lo[i1]) return PP_EUNSUP;
+      nrh[i1] = make_int2(lo, hi);
+    }
+    std::vector<const double *> rn, pp, qq;
+    for (auto &t : p->mterms) {
+      std::vector<double> Rn((size_t)N * bw * P3_NW * bw, 0.0);
+      for (int i1 = 0; i1 < N; i1++)
+        for (int d1 = 0; d1 < bw; d1++)
+          for (int w = 0; w <= nrh[i1].y - nrh[i1].x; w++)
+            for (int d2 = 0; d2 < bw; d2++)
+              Rn[(((size_t)i1 * bw + d1) * P3_NW + w) * bw + d2] =
+                  t.Rh[(((size_t)i1 * bw + d1) * N + nrh[i1].x + w) * bw + d2];
+      if ((rc = p3_upload(&t.dRn, Rn)) || (rc = p3_upload(&t.dP, t.Pl)) || (rc = p3_upload(&t.dQ, t.Ql))) return rc;
+      rn.push_back(t.dRn); pp.push_back(t.dP); qq.push_back(t.dQ);
+    }
+    for (int ch = 0; ch < p->nchunk; ch++)
+      if (cbp[ch + 1] - cbp[ch] > P3_MAXB) return PP_EUNSUP;
+    p->nrh = nrh;
+    if ((rc = p3_upload(&p->nr, nrh)) || (rc = p3_upload(&p->dRn, rn)) || (rc = p3_upload(&p->dPt, pp)) ||
+        (rc = p3_upload(&p->dQt, qq)))
+      return rc;
+    const size_t tb = sizeof(zd) * (size_t)p->nblk * N * N * P3_CHUNK;
+    if (hipMalloc(&p->Tq, tb) != hipSuccess || hipMalloc(&p->Tp, tb) != hipSuccess) return PP_EHIP;
+    p->split = true;
+  }
   p->dirty = false;
   return PP_OK;
 }
@@ -792,7 +1077,6 @@ static int p3_apply_raw(pp_pamp3 *p, const zd *y, zd *z, const double *scal, hip
     hipLaunchKernelGGL(p3_zero, dim3(1024), dim3(256), 0, st, z, p->n);
   }
   if (p->nblk > 0) {
-    hipLaunchKernelGGL(p3_mix, dim3(p->nblk * N), dim3(256), 0, st, p->mrow, p->ment, y, p->Ym, N, scal, p->state);
     const int nhalf = (N + P3_HALF - 1) / P3_HALF;
     const int pairs = N * nhalf;
     const int groups = (pairs + 7) / 8;
@@ -801,8 +1085,20 @@ static int p3_apply_raw(pp_pamp3 *p, const zd *y, zd *z, const double *scal, hip
       if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return PP_EHIP;
       (void)hipEventRecord(e0, st);
     }
-    hipLaunchKernelGGL(p3_ee<6>, dim3(groups * 8 * p->nchunk), dim3(P3_EE_THREADS), 0, st, p->dR, p->blk_term,
-                       p->cb_ptr, p->cb_blk, p->Ym, z, C, N, nhalf, p->nchunk, p->state);
+    hipLaunchKernelGGL(p3_mix, dim3(p->nblk * N), dim3(256), 0, st, p->mrow, p->ment, y, p->Ym, N, scal, p->state);
+    if (p->split) {
+      const size_t lds = sizeof(zd) * (size_t)(N + 12) * P3_CHUNK;
+      hipLaunchKernelGGL(p3_farT<6>, dim3(p->nblk * N), dim3(256), lds, st, p->dPt, p->dQt, p->blk_term, p->Ym,
+                         p->Tq, p->Tp, N, p->state);
+      hipLaunchKernelGGL(p3_near<6>, dim3(p->nchunk * N), dim3(256 * P3_NG4), 0, st, p->dRn, p->blk_term, p->cb_ptr,
+                         p->cb_blk, p->nr, p->Ym, z, C, N, p->nchunk, p->state);
+      const int ntile = (N + P3_RT - 1) / P3_RT, ngrp = (N + 15) / 16;
+      hipLaunchKernelGGL(p3_farL<6>, dim3(p->nchunk * ntile * ngrp), dim3(256), 0, st, p->dPt, p->dQt, p->blk_term,
+                         p->cb_ptr, p->cb_blk, p->nr, p->Tq, p->Tp, z, C, N, p->nchunk, p->state);
+    } else {
+      hipLaunchKernelGGL(p3_ee<6>, dim3(groups * 8 * p->nchunk), dim3(P3_EE_THREADS), 0, st, p->dR, p->blk_term,
+                         p->cb_ptr, p->cb_blk, p->Ym, z, C, N, nhalf, p->nchunk, p->state);
+    }
     if (p->timing) {
       (void)hipEventRecord(e1, st);
       p->ev.push_back(e0);
@@ -963,13 +1259,32 @@ extern "C" int pp_pamp3_multipole_flops(pp_pamp3 *p, double *flops)
   const long N = p->N, s = p->s;
   long bandpairs = 0;
   for (long i = 0; i < N; i++) bandpairs += std::min(N - 1, i + s) - std::max(0l, i - s) + 1;
+  if (p->dirty) {
+    int rc = p3_build(p);
+    if (rc) return rc;
+  }
+  const double NN = (double)N * N;
   double f = 0.0;
   for (auto &t : p->mterms) {
     std::vector<char> rowset(p->C, 0);
     for (size_t q = 0; q < t.pr.size(); q += 2) rowset[t.pr[q]] = 1;
     long rows = 0;
     for (char r : rowset) rows += r;
-    f += 4.0 * (double)rows * (double)bandpairs * (double)bandpairs;
+    f += 8.0 * (double)(t.pr.size() / 2) * NN;   // channel mixing, complex x complex
+    if (!p->split) {
+      f += 4.0 * (double)rows * (double)bandpairs * (double)bandpairs;
+      continue;
+    }
+    // near band: every (j1, j2) inside both bands; far field: two i2 passes
+    // (Q and P) over every row and one i1 pass per far output
+    double near = 0.0, far = 0.0;
+    for (long i1 = 0; i1 < N; i1++) {
+      const long t1 = std::min(N - 1, i1 + s) - std::max(0l, i1 - s) + 1;
+      for (long i2 = p->nrh[i1].x; i2 <= p->nrh[i1].y; i2++)
+        near += (double)t1 * (double)(std::min(N - 1, i2 + s) - std::max(0l, i2 - s) + 1);
+      far += (double)(N - (p->nrh[i1].y - p->nrh[i1].x + 1)) * (double)t1;
+    }
+    f += 4.0 * (double)rows * (near + 2.0 * NN * (2 * s + 1) + far);
   }
   *flops = f;
   return PP_OK;

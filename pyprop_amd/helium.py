@@ -59,6 +59,22 @@ def radial_matrices(basis):
             "R2": band_rowmajor(basis, 1.0 / (r * r), 0)}
 
 
+def radial_supports(basis):
+    """(lo, hi): the first and last breakpoint interval on which each
+    B-spline is nonzero (from the basis tables at the quadrature points)."""
+    k = basis.MaxSplineOrder
+    N = basis.NumberOfBSplines
+    iv = np.arange(len(basis.QuadratureGridGlobal)) // basis.nq
+    lo = np.full(N, np.iinfo(np.int32).max, dtype=np.int64)
+    hi = np.full(N, -1, dtype=np.int64)
+    for r in range(k):
+        i = basis.Span + r
+        ok = (i >= 0) & (i < N) & (basis.B[:, r] != 0.0)
+        np.minimum.at(lo, i[ok], iv[ok])
+        np.maximum.at(hi, i[ok], iv[ok])
+    return lo.astype(np.int32), hi.astype(np.int32)
+
+
 def multipole_tensor(basis, l):
     """R_l[i1, d1, i2, d2] (j = i + d - (k-1)) of r<^l / r>^(l+1) on the
     product quadrature grid (coupledspherical.cpp:92-110 evaluated, then
@@ -100,11 +116,15 @@ class KronTerm:
 
 
 class MultipoleTerm:
-    def __init__(self, pairs, coef, l, R):
+    """R: the 4-index tensor; P, Q: <B|r^l|B> and <B|r^-(l+1)|B>, the two
+    factors of R wherever the radial supports do not overlap."""
+
+    def __init__(self, pairs, coef, l, R, P=None, Q=None):
         self.pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
         self.coef = np.ascontiguousarray(coef, dtype=np.complex128)
         self.l = l
         self.R = R
+        self.P, self.Q = P, Q
 
 
 class CoupledTensorPotential:
@@ -272,8 +292,10 @@ def _evaluate(conf, channels, pairs, basis, mats):
                         sel.append((a, b))
                         coef.append(v)
             if sel:
+                r = basis.QuadratureGridGlobal
                 terms.append(MultipoleTerm(sel, np.array(coef, dtype=np.complex128), l,
-                                           multipole_tensor(basis, l)))
+                                           multipole_tensor(basis, l), band_rowmajor(basis, r ** l, 0),
+                                           band_rowmajor(basis, r ** (-(l + 1.0)), 0)))
         return terms
     if cls in ("CustomPotential_LaserVelocity", "CustomPotential_LaserVelocityDerivativeR1",
                "CustomPotential_LaserVelocityDerivativeR2"):
@@ -321,10 +343,25 @@ class Pamp3Engine:
     def add_multipole(self, term, slot=-1):
         coef = term.coef.view(np.float64)
         R = np.ascontiguousarray(term.R, dtype=np.float64)
+        nul = ctypes.POINTER(ctypes.c_double)()
+        P = Q = None
+        if term.P is not None:
+            P = np.ascontiguousarray(term.P, dtype=np.float64)
+            Q = np.ascontiguousarray(term.Q, dtype=np.float64)
         nat.check(nat.lib().pp_pamp3_add_multipole_term(self.h, len(term.pairs), nat.iptr(term.pairs),
-                                                        nat.dptr(coef), nat.dptr(R), int(slot)),
+                                                        nat.dptr(coef), nat.dptr(R),
+                                                        nat.dptr(P) if P is not None else nul,
+                                                        nat.dptr(Q) if Q is not None else nul, int(slot)),
                   "pp_pamp3_add_multipole_term")
         self.nslots = max(self.nslots, slot + 1)
+
+    def set_radial_supports(self, lo, hi):
+        """Enables the near/far split of the multipole terms."""
+        lo = np.ascontiguousarray(lo, dtype=np.int32)
+        hi = np.ascontiguousarray(hi, dtype=np.int32)
+        self._sup = (lo, hi)
+        nat.check(nat.lib().pp_pamp3_set_radial_supports(self.h, nat.iptr(lo), nat.iptr(hi)),
+                  "pp_pamp3_set_radial_supports")
 
     def add_potential(self, pot, slot=-1):
         for t in pot.Terms:

@@ -131,7 +131,7 @@ class PamPropagator(PropagatorBase):
         """Two-electron states (coupled channels x B-spline x B-spline): the
         factored potentials go to the rank-3 engine; each time-dependent
         potential gets its own time slot."""
-        from .helium import CoupledTensorPotential, Pamp3Engine
+        from .helium import CoupledTensorPotential, Pamp3Engine, radial_supports
         if self.MatvecPrecision != "fp64":
             raise NotImplementedError("matvec_precision = mixed is not on the rank-3 engine")
         if not all(isinstance(p, CoupledTensorPotential) for p in pots):
@@ -141,6 +141,7 @@ class PamPropagator(PropagatorBase):
             raise NotImplementedError("both radial ranks must have the same size")
         overlap = repr.GetRepresentation(1).GetOverlap()
         eng = Pamp3Engine(C, N, self.BasisSize, overlap, reorth=self.DoubleOrthogonalization)
+        eng.set_radial_supports(*radial_supports(repr.GetRepresentation(1).GetBSplineObject()))
         self._slots = []
         for pot in pots:
             slot = -1

@@ -238,3 +238,31 @@ def test_oracle_factored_terms_match_reference_layout():
     assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max()
     a, b = Hf.balanced(x), Href.balanced(x)
     assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
+def test_multipole_far_field_factorises():
+    """Where the supports of B_i1 and B_i2 are disjoint, R_l = P_l (x) Q_l
+    (below) or Q_l (x) P_l (above) to rounding: the product quadrature of
+    r<^l / r>^(l+1) factorises there.  This is what lets the engine apply
+    the far field by banded passes."""
+    from pyprop_amd import helium as HL
+    over = {("RadialRepresentation", "xsize"): 37, ("RadialRepresentation", "xmax"): 30.0}
+    conf = HE.load("helium_small", over)
+    b = CreateRepresentation(conf).GetRepresentation(1).GetBSplineObject()
+    N = b.NumberOfBSplines
+    lo, hi = HL.radial_supports(b)
+    below = hi[:, None] < lo[None, :]
+    above = hi[None, :] < lo[:, None]
+    assert below.sum() > 0.6 * N * N / 2 and not np.any(below & above)
+    r = b.QuadratureGridGlobal
+    for l in (0, 1, 4, 10):
+        R = HL.multipole_tensor(b, l)
+        P = HL.band_rowmajor(b, r ** l, 0)
+        Q = HL.band_rowmajor(b, r ** (-(l + 1.0)), 0)
+        fb = np.einsum("ad,ce->adce", P, Q)           # P[i1,d1] Q[i2,d2]
+        fa = np.einsum("ad,ce->adce", Q, P)
+        mb = below[:, None, :, None]
+        ma = above[:, None, :, None]
+        scale = np.abs(R).max()
+        assert np.abs(np.where(mb, R - fb, 0.0)).max() <= 1e-14 * scale
+        assert np.abs(np.where(ma, R - fa, 0.0)).max() <= 1e-14 * scale
