@@ -348,6 +348,76 @@ __global__ void __launch_bounds__(256) p3_farT(const double *const *Pt, const do
   }
 }
 
+// Mixing fused with the far field's i2 pass: workgroup (block b, row j1)
+// forms the mixed input row Ym[b][j1][.][.] in LDS (written out for the
+// near band), then Tq/Tp of that row from LDS.
+#define P3_MIXE 12         // mixing entries per row held in registers
+template <int S>
+__global__ void __launch_bounds__(256) p3_mixT(const int *row, const p3_ent *ent, const zd *y, zd *Ym,
+                                               const double *const *Pt, const double *const *Qt, const int *blk_term,
+                                               const int2 *nr, zd *Tq, zd *Tp, int N, const double *scal,
+                                               const int *state)
+{
+  if (state[0]) return;
+  constexpr int BW = 2 * S + 1;
+  extern __shared__ double p3_lds[];
+  zd *rowb = (zd *)p3_lds;                               // positions -S .. N+S-1, 16 channels each
+  const int b = blockIdx.x / N, j1 = blockIdx.x % N;
+  const long NN = (long)N * N;
+  const int cl = threadIdx.x & (P3_CHUNK - 1);
+  const int r = b * P3_CHUNK + cl;
+  const int e0 = row[r], ne = row[r + 1] - e0;
+  // this lane's couplings: source row offset and scaled coefficient
+  long src[P3_MIXE];
+  zd cf[P3_MIXE];
+#pragma unroll
+  for (int q = 0; q < P3_MIXE; q++) {
+    src[q] = 0;
+    cf[q] = zmk(0.0, 0.0);
+    if (q < ne) {
+      const p3_ent E = ent[e0 + q];
+      const double sc = E.slot < 0 ? 1.0 : scal[E.slot];
+      src[q] = E.col * NN + (long)j1 * N;
+      cf[q] = zmk(E.cre * sc, E.cim * sc);
+    }
+  }
+  // the near band reads this row only at positions nlo(j1-S)-S .. nhi(j1+S)+S
+  const int wlo = nr[j1 - S > 0 ? j1 - S : 0].x - S, whi = nr[j1 + S < N - 1 ? j1 + S : N - 1].y + S;
+  zd *out = Ym + ((long)b * NN + (long)j1 * N) * P3_CHUNK;
+  for (int e = threadIdx.x; e < (N + 2 * S) * P3_CHUNK; e += 256) {
+    const int pos = e / P3_CHUNK - S;
+    zd acc = zmk(0.0, 0.0);
+    if (pos >= 0 && pos < N) {
+#pragma unroll
+      for (int q = 0; q < P3_MIXE; q++)
+        if (q < ne) zfma(acc, cf[q], y[src[q] + pos]);
+      for (int q = P3_MIXE; q < ne; q++) {               // rows with more couplings
+        const p3_ent E = ent[e0 + q];
+        const double sc = E.slot < 0 ? 1.0 : scal[E.slot];
+        zfma(acc, zmk(E.cre * sc, E.cim * sc), y[E.col * NN + (long)j1 * N + pos]);
+      }
+      if (pos >= wlo && pos <= whi) out[(long)pos * P3_CHUNK + cl] = acc;
+    }
+    rowb[e] = acc;
+  }
+  __syncthreads();
+  const double *P = Pt[blk_term[b]], *Q = Qt[blk_term[b]];
+  zd *oq = Tq + ((long)b * NN + (long)j1 * N) * P3_CHUNK, *op = Tp + ((long)b * NN + (long)j1 * N) * P3_CHUNK;
+  for (int e = threadIdx.x; e < N * P3_CHUNK; e += 256) {
+    const int i2 = e / P3_CHUNK, c2 = e & (P3_CHUNK - 1);
+    const double *qr = Q + (size_t)i2 * BW, *pr = P + (size_t)i2 * BW;
+    zd tq = zmk(0.0, 0.0), tp = zmk(0.0, 0.0);
+#pragma unroll
+    for (int d = 0; d < BW; d++) {
+      const zd v = rowb[(i2 + d) * P3_CHUNK + c2];
+      zfmar(tq, qr[d], v);
+      zfmar(tp, pr[d], v);
+    }
+    oq[e] = tq;
+    op[e] = tp;
+  }
+}
+
 // Z[c][i1][i2] += sum_blocks [ i2 > nhi[i1]:  sum_d1 P[i1][d1] Tq[b][i1+d1-S][i2][cl]
 //                              i2 < nlo[i1]:  sum_d1 Q[i1][d1] Tp[b][i1+d1-S][i2][cl] ]
 // workgroup: (chunk, P3_RT rows i1, 16 columns i2); thread: (i2, channel lane).
@@ -472,17 +542,18 @@ __global__ void __launch_bounds__(256 * P3_NG4) p3_near(const double *const *Rn,
   const bool ok = i2 <= r.y;
   zd a0 = zmk(0.0, 0.0), a1 = zmk(0.0, 0.0);
   if (ok) {
-    // taps outside 0 <= j2 < N have R = 0: load a clamped (valid) position
-    // instead of branching, so all 13 loads of a row issue together
-    int jc[BW];
+    // taps outside 0 <= j2 < N are not loaded (select, no branch), so all 13
+    // loads of a row issue together; the mixed input is only written where
+    // the near band reads it
+    bool jv[BW];
 #pragma unroll
     for (int d2 = 0; d2 < BW; d2++) {
       const int j2 = i2 + d2 - S;
-      jc[d2] = j2 < 0 ? 0 : (j2 >= N ? N - 1 : j2);
+      jv[d2] = j2 >= 0 && j2 < N;
     }
     for (int k = grp; k < nb; k += P3_NG4) {
       const double *R = sR[k];
-      const zd *Yb = Ym + sY[k] + cl;
+      const zd *Yb = Ym + sY[k] + cl + (long)(i2 - S) * P3_CHUNK;
       for (int d1 = 0; d1 < BW; d1++) {
         const int j1 = i1 + d1 - S;
         if (j1 < 0 || j1 >= N) continue;
@@ -490,7 +561,7 @@ __global__ void __launch_bounds__(256 * P3_NG4) p3_near(const double *const *Rn,
         const double *Rr = R + ((size_t)d1 * P3_NW + w) * BW;
         zd y[BW];
 #pragma unroll
-        for (int d2 = 0; d2 < BW; d2++) y[d2] = Yr[(long)jc[d2] * P3_CHUNK];
+        for (int d2 = 0; d2 < BW; d2++) y[d2] = jv[d2] ? Yr[(long)d2 * P3_CHUNK] : zmk(0.0, 0.0);
 #pragma unroll
         for (int d2 = 0; d2 < BW; d2++) {
           if (d2 & 1) zfmar(a1, Rr[d2], y[d2]);
@@ -1085,17 +1156,17 @@ static int p3_apply_raw(pp_pamp3 *p, const zd *y, zd *z, const double *scal, hip
       if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return PP_EHIP;
       (void)hipEventRecord(e0, st);
     }
-    hipLaunchKernelGGL(p3_mix, dim3(p->nblk * N), dim3(256), 0, st, p->mrow, p->ment, y, p->Ym, N, scal, p->state);
     if (p->split) {
       const size_t lds = sizeof(zd) * (size_t)(N + 12) * P3_CHUNK;
-      hipLaunchKernelGGL(p3_farT<6>, dim3(p->nblk * N), dim3(256), lds, st, p->dPt, p->dQt, p->blk_term, p->Ym,
-                         p->Tq, p->Tp, N, p->state);
+      hipLaunchKernelGGL(p3_mixT<6>, dim3(p->nblk * N), dim3(256), lds, st, p->mrow, p->ment, y, p->Ym, p->dPt,
+                         p->dQt, p->blk_term, p->nr, p->Tq, p->Tp, N, scal, p->state);
       hipLaunchKernelGGL(p3_near<6>, dim3(p->nchunk * N), dim3(256 * P3_NG4), 0, st, p->dRn, p->blk_term, p->cb_ptr,
                          p->cb_blk, p->nr, p->Ym, z, C, N, p->nchunk, p->state);
       const int ntile = (N + P3_RT - 1) / P3_RT, ngrp = (N + 15) / 16;
       hipLaunchKernelGGL(p3_farL<6>, dim3(p->nchunk * ntile * ngrp), dim3(256), 0, st, p->dPt, p->dQt, p->blk_term,
                          p->cb_ptr, p->cb_blk, p->nr, p->Tq, p->Tp, z, C, N, p->nchunk, p->state);
     } else {
+      hipLaunchKernelGGL(p3_mix, dim3(p->nblk * N), dim3(256), 0, st, p->mrow, p->ment, y, p->Ym, N, scal, p->state);
       hipLaunchKernelGGL(p3_ee<6>, dim3(groups * 8 * p->nchunk), dim3(P3_EE_THREADS), 0, st, p->dR, p->blk_term,
                          p->cb_ptr, p->cb_blk, p->Ym, z, C, N, nhalf, p->nchunk, p->state);
     }
