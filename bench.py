@@ -255,9 +255,9 @@ def run_adjoint(args):
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (matrix and vector peaks are equal)
 
 # HBM-side bytes of one multipole stage (the three kernels' FETCH_SIZE +
-# WRITE_SIZE per matvec, scaled by a 512 MiB copy), from the counter passes
-# recorded in profiles/multipole.helium.pmc.md; None until measured.
-HE_PMC_TRAFFIC = {}
+# WRITE_SIZE per matvec, FETCH scaled x2 and WRITE x1 by a 512 MiB copy),
+# from the counter passes recorded in profiles/multipole.helium.md.
+HE_PMC_TRAFFIC = {"helium": 1629.9e6}
 
 
 def run_helium(args):
