@@ -147,6 +147,15 @@ int pp_pamp_set_pade_scaling(pp_pamp *p, int mode);
  * sums); Gram-Schmidt, the overlap solves, the exchanged U^-1 w and the Pade
  * exponential stay FP64.  0 restores the all-FP64 drop-in. */
 int pp_pamp_set_mixed(pp_pamp *p, unsigned term_mask);
+/* Launch shape of pp_pamp_propagate (no reference counterpart; the
+ * reference runs one fiber loop per MPI rank): q workgroups share each
+ * angular fiber, each owning a contiguous run of the overlap solver's
+ * chunks.  0 (default) chooses q so that L*q workgroups fill the CUs (at
+ * most 8, at least 2 chunks each; 1 with re-orthogonalisation), 1 keeps one
+ * workgroup per fiber, q >= 2 forces q (PP_EUNSUP above the chunk count).
+ * Results agree with q = 1 to rounding (partial sums are grouped
+ * differently); batch and Expokit launches are not affected. */
+int pp_pamp_set_fiber_split(pp_pamp *p, int q);
 /* Profiling hook (no reference counterpart; the reference keeps per-phase
  * Timers in pAMP, pamp.h:41,125,295-349): record wall-clock stamps (100 MHz)
  * of workgroup 0 around every grid barrier of step 0 of later launches. */

@@ -89,18 +89,20 @@ struct pp_nostamp {
   __device__ void operator()() const {}
 };
 
-// Stage 1: out_c = U_cc^-1 in_c (UPPER) or U_cc^-T in_c for every chunk c;
-// 8-column blocks that are structurally zero for a row are skipped (upper:
-// j < i, lower: j > i).  The blocks are read-only and stream from L2.
+// Stage 1: out_c = U_cc^-1 in_c (UPPER) or U_cc^-T in_c for the nch chunks
+// c0 <= c < c0+nch (all chunks by default); in/out hold those chunks' rows
+// only (row c*B+i at (c-c0)*B+i).  8-column blocks that are structurally
+// zero for a row are skipped (upper: j < i, lower: j > i).  The blocks are
+// read-only and stream from L2.
 template <bool UPPER, int NT>
-__device__ inline void pp_solve_blocks(const pp_band_plan &pl, const zd *in, zd *out)
+__device__ inline void pp_solve_blocks(const pp_band_plan &pl, const zd *in, zd *out, int c0 = 0, int nch = -1)
 {
-  const int tid = threadIdx.x, B = pl.B, lb = pl.lb, npad = pl.P << lb;
+  const int tid = threadIdx.x, B = pl.B, lb = pl.lb, nloc = (nch < 0 ? pl.P : nch) << lb;
   const double *Bm = UPPER ? pl.binv : pl.binvT;
-  for (int row = tid; row < npad; row += NT) {
-    const int c = row >> lb, i = row & (B - 1);
+  for (int row = tid; row < nloc; row += NT) {
+    const int c = c0 + (row >> lb), i = row & (B - 1);
     const double *col = Bm + ((size_t)c << (2 * lb)) + i;
-    const zd *bb = in + (c << lb);
+    const zd *bb = in + ((row >> lb) << lb);
     zd a0 = zmk(0.0, 0.0), a1 = zmk(0.0, 0.0);
     for (int j0 = 0; j0 < B; j0 += 8) {
       if (UPPER ? (j0 + 7 < i) : (j0 > i)) continue;
@@ -117,18 +119,20 @@ __device__ inline void pp_solve_blocks(const pp_band_plan &pl, const zd *in, zd 
 
 // Stage 2 (wavefront 0 only): boundary unknowns of every chunk from the
 // chunk couplings, t_c = y_c[0:kd] - M_c t_{c+1} (UPPER) or
-// t_c = y_c[B-kd:B] - M'_c t_{c-1}, by a log-depth scan (lane = chunk);
-// tb[c*KD + k] receives the neighbour's t (t_{c+1} resp. t_{c-1}).
-template <int KD, bool UPPER>
-__device__ inline void pp_solve_scan(const pp_band_plan &pl, const double *nl, const zd *y, zd *tb)
+// t_c = y_c[B-kd:B] - M'_c t_{c-1}, by a log-depth scan (lane = chunk).
+// `src(c, k)` returns chunk c's boundary value k (y_c[k], resp. y_c[B-kd+k]);
+// tb[(c-c0)*KD + k] receives the neighbour's t (t_{c+1} resp. t_{c-1}) for
+// c0 <= c < c1.
+template <int KD, bool UPPER, class Src>
+__device__ inline void pp_solve_scan_src(const pp_band_plan &pl, const double *nl, Src src, zd *tb, int c0, int c1)
 {
-  const int P = pl.P, B = pl.B, lb = pl.lb;
+  const int P = pl.P;
   const int c = threadIdx.x;
   const bool act = c < P;
   const double *Nl = UPPER ? nl : nl + pl.levels * KD * KD * P;
   zd a[KD];
 #pragma unroll
-  for (int k = 0; k < KD; k++) a[k] = act ? y[(c << lb) + (UPPER ? k : B - KD + k)] : zmk(0.0, 0.0);
+  for (int k = 0; k < KD; k++) a[k] = act ? src(c, k) : zmk(0.0, 0.0);
   for (int l = 0; l < pl.levels; l++) {
     const int s = 1 << l;
     zd an[KD];
@@ -154,21 +158,30 @@ __device__ inline void pp_solve_scan(const pp_band_plan &pl, const double *nl, c
     zd t;
     if (UPPER) { t.x = __shfl_down(a[k].x, 1, 64); t.y = __shfl_down(a[k].y, 1, 64); if (c + 1 >= P) t = zmk(0.0, 0.0); }
     else { t.x = __shfl_up(a[k].x, 1, 64); t.y = __shfl_up(a[k].y, 1, 64); if (c == 0) t = zmk(0.0, 0.0); }
-    if (act) tb[c * KD + k] = t;
+    if (c >= c0 && c < c1) tb[(c - c0) * KD + k] = t;
   }
 }
 
-// Stage 3: x_c = y_c - G_c t_{c+1} (UPPER) or y_c - G'_c t_{c-1}, in place.
-template <int KD, bool UPPER, int NT>
-__device__ inline void pp_solve_fixup(const pp_band_plan &pl, const zd *tb, zd *out)
+// Stage 2 over a whole fiber whose stage-1 output y (all P*B rows) is in LDS.
+template <int KD, bool UPPER>
+__device__ inline void pp_solve_scan(const pp_band_plan &pl, const double *nl, const zd *y, zd *tb)
 {
-  const int npad = pl.P << pl.lb, lb = pl.lb;
+  const int lb = pl.lb, off = UPPER ? 0 : pl.B - KD;
+  pp_solve_scan_src<KD, UPPER>(pl, nl, [&](int c, int k) { return y[(c << lb) + off + k]; }, tb, 0, pl.P);
+}
+
+// Stage 3: x_c = y_c - G_c t_{c+1} (UPPER) or y_c - G'_c t_{c-1}, in place,
+// for the nch chunks from c0 (all by default; out and tb hold those only).
+template <int KD, bool UPPER, int NT>
+__device__ inline void pp_solve_fixup(const pp_band_plan &pl, const zd *tb, zd *out, int c0 = 0, int nch = -1)
+{
+  const int npad = pl.P << pl.lb, lb = pl.lb, nloc = (nch < 0 ? pl.P : nch) << lb, r0 = c0 << lb;
   const double *Gm = UPPER ? pl.G : pl.GT;
-  for (int row = threadIdx.x; row < npad; row += NT) {
+  for (int row = threadIdx.x; row < nloc; row += NT) {
     const zd *t = tb + (row >> lb) * KD;
     zd v = out[row];
 #pragma unroll
-    for (int k = 0; k < KD; k++) zfmar(v, -Gm[(size_t)k * npad + row], t[k]);
+    for (int k = 0; k < KD; k++) zfmar(v, -Gm[(size_t)k * npad + r0 + row], t[k]);
     out[row] = v;
   }
 }

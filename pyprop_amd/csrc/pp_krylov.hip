@@ -86,6 +86,9 @@ struct pp_pamp_args {
   double *diag;          // nsteps x 4
   pp_barrier *bar;
   long long *tlog;
+  int Q;                 // split kernel: workgroups per fiber
+  zd *sb;                // split kernel: L x 2 x P*KD chunk boundary values (exchanged)
+  unsigned *subw;        // split kernel: L x Q arrival words of the per-fiber barriers
 };
 
 struct pp_pamp {
@@ -106,21 +109,22 @@ struct pp_pamp {
   int diag_cap, scal_cap, pstr;
   long long *tlog;
   int timing;
+  int fsplit;                              // workgroups per fiber: 0 automatic, 1 none, >= 2 fixed
+  size_t bar_words, part_cap, sb_cap;      // allocated arrival words, partial-sum doubles, boundary values
+  zd *sb;
 };
 
-// Barrier over the n co-resident workgroups of one instance (member `me`):
-// every workgroup publishes the
-// number of barriers it has reached in its own arrival word (the words follow
-// the pp_barrier header) and wavefront 0 polls all words until each has
-// reached the same count, so a barrier costs one store plus the polling
-// round trips.  Data crossing it is written/read with agent-scope atomics
+// Barrier over n co-resident workgroups (member `me`; a whole instance, or
+// the workgroups sharing one fiber in the split kernel): every workgroup
+// publishes the number of barriers it has reached in its own arrival word
+// (word[me]) and wavefront 0 polls the n words until each has reached the
+// same count, so a barrier costs one store plus the polling round trips.  Data crossing it is written/read with agent-scope atomics
 // (above); the workgroup-scope fences order this workgroup's accesses around
 // the arrival.  A bounded spin turns a lost arrival into an error flag
-// instead of a hang.  `ep` counts the barriers of this launch (identical in
-// every workgroup, since all execute the same barrier sequence).
-__device__ inline void grid_barrier(pp_barrier *b, unsigned &ep, int n, int me)
+// instead of a hang (*err).  `ep` counts the barriers of this launch
+// (identical in every member, since all execute the same barrier sequence).
+__device__ inline void group_barrier(unsigned *word, unsigned *err, unsigned &ep, int n, int me)
 {
-  unsigned *word = (unsigned *)(b + 1);
   const int tid = threadIdx.x;
   ep++;
   __syncthreads();
@@ -137,13 +141,18 @@ __device__ inline void grid_barrier(pp_barrier *b, unsigned &ep, int n, int me)
       if (__all(ok)) break;
       __builtin_amdgcn_s_sleep(1);
       if (++spins > (1l << 24)) {
-        if (tid == 0) __hip_atomic_store(&b->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   }
   __syncthreads();
+}
+
+__device__ inline void grid_barrier(pp_barrier *b, unsigned &ep, int n, int me)
+{
+  group_barrier((unsigned *)(b + 1), &b->err, ep, n, me);
 }
 
 // ------------------------------------------------------------------ reductions
@@ -193,17 +202,18 @@ __device__ inline double block_sum(double v, double *wave_lds)
   return r;
 }
 
-// out[2k], out[2k+1] = <V_k, W> over one fiber for k < nk; with extra_norm
-// also out[2nk] = |W|^2.  Wave w owns k = w, w+4, ...; each lane streams a
-// strided slice so the loads of one k are independent and pipeline.
-__device__ inline void fiber_dots(const zd *V, const zd *W, int nk, int N, bool extra_norm, double *out)
+// out[2k], out[2k+1] = <V_k, W> over n rows for k < nk (V_k at V + k*ldv);
+// with extra_norm also out[2nk] = |W|^2.  Wave w owns k = w, w+4, ...; each
+// lane streams a strided slice so the loads of one k are independent and
+// pipeline.
+__device__ inline void fiber_dots(const zd *V, int ldv, const zd *W, int nk, int N, bool extra_norm, double *out)
 {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int ntot = nk + (extra_norm ? 1 : 0);
   for (int k = wave; k < ntot; k += PP_THREADS / 64) {
     double re = 0.0, im = 0.0;
     if (k < nk) {
-      const zd *vk = V + (size_t)k * N;
+      const zd *vk = V + (size_t)k * ldv;
 #pragma unroll PP_VUNROLL
       for (int i = lane; i < N; i += 64) {
         zd v = vk[i], w = W[i];
@@ -291,6 +301,123 @@ __device__ __attribute__((always_inline)) inline void pp_load_persistent(const p
   __syncthreads();
 }
 
+// Coefficients of the combo table times the terms' time factors at t_step
+// and the factor fac (rewritten before every operator apply).
+__device__ inline void pp_step_coeffs(const pp_pamp_args &A, const pp_lds_map &M, int step, zd fac)
+{
+  for (int e = threadIdx.x; e < A.ncombo * PP_MAXENT; e += PP_THREADS) {
+    M.cur_c[e] = zmul(M.cmb_c[e], zscale(fac, A.scal[(size_t)step * A.nterms + M.cmb_t[e / PP_MAXENT]]));
+    M.cur_q[e] = M.cmb_q[e];
+    if (e % PP_MAXENT == 0) M.cur_n[e / PP_MAXENT] = M.cmb_n[e / PP_MAXENT];
+  }
+}
+
+// X[i - r0] = sum_k B_k (sum_q c_kq Y_q) at rows r0 <= i < r0 + nloc (zero
+// for rows >= N), with row j of staged source fiber q at
+// Ys[q * ystr + j - ybase] for every |j - i| <= KD inside [0, N).  Bands
+// diagonal-major, all loads independent; the combo table and coefficients
+// are in LDS (pp_step_coeffs).
+template <int KD, bool MIXED>
+__device__ __attribute__((always_inline)) inline void pp_band_rows(const pp_pamp_args &A, const pp_lds_map &M, const zd *Ys,
+                                                                   int ystr, int ybase, int r0, int nloc, zd *X)
+{
+  const int N = A.N, tid = threadIdx.x;
+  const zd *cur_c = M.cur_c;
+  const int *cur_q = M.cur_q, *cur_n = M.cur_n;
+  if (MIXED) for (int i = r0 + tid; i < r0 + nloc; i += PP_THREADS) {
+    zd acc = zmk(0.0, 0.0);
+    if (i < N) {
+      const bool interior = i >= KD && i + KD < N;
+      int k = 0;
+      for (int t = 0; t < A.nterms; t++) {
+        const bool f32 = A.term[t].fp32 != 0;
+        for (int r = 0; r < A.term[t].rank; r++, k++) {
+          const int ne = cur_n[k];
+          if (ne == 0) continue;
+          if (f32) {
+            const float *Br = A.term[t].bands32 + (size_t)r * (2 * KD + 1) * N;
+            float bv[2 * KD + 1];
+#pragma unroll
+            for (int d = -KD; d <= KD; d++)
+              bv[d + KD] = (interior || (i + d >= 0 && i + d < N)) ? Br[(size_t)(d + KD) * N + i] : 0.f;
+#pragma unroll
+            for (int q = 0; q < PP_MAXENT; q++) {
+              if (q < ne) {
+                const zd *Yq = Ys + cur_q[k * PP_MAXENT + q] * ystr;
+                float zr = 0.f, zi = 0.f;
+#pragma unroll
+                for (int d = -KD; d <= KD; d++) {
+                  const int j = i + d;
+                  const zd yv = (interior || (j >= 0 && j < N)) ? Yq[j - ybase] : zmk(0.0, 0.0);
+                  zr = fmaf(bv[d + KD], (float)yv.x, zr);
+                  zi = fmaf(bv[d + KD], (float)yv.y, zi);
+                }
+                zfma(acc, cur_c[k * PP_MAXENT + q], zmk((double)zr, (double)zi));
+              }
+            }
+          } else {
+            const double *Br = A.term[t].bands + (size_t)r * (2 * KD + 1) * N;
+            double bv[2 * KD + 1];
+#pragma unroll
+            for (int d = -KD; d <= KD; d++)
+              bv[d + KD] = (interior || (i + d >= 0 && i + d < N)) ? Br[(size_t)(d + KD) * N + i] : 0.0;
+#pragma unroll
+            for (int q = 0; q < PP_MAXENT; q++) {
+              if (q < ne) {
+                const zd *Yq = Ys + cur_q[k * PP_MAXENT + q] * ystr;
+                zd z = zmk(0.0, 0.0);
+#pragma unroll
+                for (int d = -KD; d <= KD; d++) {
+                  const int j = i + d;
+                  const zd yv = (interior || (j >= 0 && j < N)) ? Yq[j - ybase] : zmk(0.0, 0.0);
+                  zfmar(z, bv[d + KD], yv);
+                }
+                zfma(acc, cur_c[k * PP_MAXENT + q], z);
+              }
+            }
+          }
+        }
+      }
+    }
+    X[i - r0] = acc;
+  }
+  else for (int i = r0 + tid; i < r0 + nloc; i += PP_THREADS) {
+    zd acc = zmk(0.0, 0.0);
+    if (i < N) {
+      const bool interior = i >= KD && i + KD < N;
+      int k = 0;
+      for (int t = 0; t < A.nterms; t++) {
+        const double *Bt = A.term[t].bands;
+        for (int r = 0; r < A.term[t].rank; r++, k++) {
+          const int ne = cur_n[k];
+          if (ne == 0) continue;
+          const double *Br = Bt + (size_t)r * (2 * KD + 1) * N;
+          double bv[2 * KD + 1];
+#pragma unroll
+          for (int d = -KD; d <= KD; d++)
+            bv[d + KD] = (interior || (i + d >= 0 && i + d < N)) ? Br[(size_t)(d + KD) * N + i] : 0.0;
+#pragma unroll
+          for (int q = 0; q < PP_MAXENT; q++) {
+            if (q < ne) {
+              const zd cq = cur_c[k * PP_MAXENT + q];
+              const zd *Yq = Ys + cur_q[k * PP_MAXENT + q] * ystr;
+              zd z = zmk(0.0, 0.0);
+#pragma unroll
+              for (int d = -KD; d <= KD; d++) {
+                const int j = i + d;
+                const zd yv = (interior || (j >= 0 && j < N)) ? Yq[j - ybase] : zmk(0.0, 0.0);
+                zfmar(z, bv[d + KD], yv);
+              }
+              zfma(acc, cq, z);
+            }
+          }
+        }
+      }
+    }
+    X[i - r0] = acc;
+  }
+}
+
 // W := fac * U^-H (sum_t s_t(t_step) V_t) y for fiber a, with y
 // the exchanged U^-1 v of every fiber (A.y) staged for this row's couplings.
 // The balanced-overlap matvec of BasisPropagator.py:131-159.  `stamp` is a
@@ -299,122 +426,19 @@ template <int KD, bool MIXED, class Stamp>
 __device__ __attribute__((always_inline)) inline void pp_matvec(const pp_pamp_args &A, const pp_lds_map &M, int a, int step, zd fac,
                                                                 Stamp stamp)
 {
-  const int N = A.N, tid = threadIdx.x, padN = A.padN;
-  const pp_band_plan &pl = A.plan;
-  zd *Ys = M.Ys, *X = M.X, *W = M.W, *tb = M.tb, *cur_c = M.cur_c;
-  int *cur_q = M.cur_q, *cur_n = M.cur_n;
-  const zd *cmb_c = M.cmb_c;
-  const int *cmb_q = M.cmb_q, *cmb_n = M.cmb_n, *cmb_t = M.cmb_t;
-  const double *nl = M.nl;
-        // stage the source fibers U^-1 w of this row's couplings
-        for (int q = 0; q < A.nslots; q++) {
-          const int col = A.slotcols[a * A.nslots + q];
-          if (col >= 0)
-            for (int i = tid; i < N; i += PP_THREADS) Ys[q * N + i] = zld(A.y + (size_t)col * N + i);
-        }
-        // coefficients times the terms' time factors at t_n and the factor fac
-        for (int e = tid; e < A.ncombo * PP_MAXENT; e += PP_THREADS) {
-          cur_c[e] = zmul(cmb_c[e], zscale(fac, A.scal[(size_t)step * A.nterms + cmb_t[e / PP_MAXENT]]));
-          cur_q[e] = cmb_q[e];
-          if (e % PP_MAXENT == 0) cur_n[e / PP_MAXENT] = cmb_n[e / PP_MAXENT];
-        }
-        __syncthreads();
-        // X = sum_k B_k (sum_q c_kq Ys_q): bands diagonal-major, all loads independent
-        if (MIXED) for (int i = tid; i < padN; i += PP_THREADS) {
-          zd acc = zmk(0.0, 0.0);
-          if (i < N) {
-            const bool interior = i >= KD && i + KD < N;
-            int k = 0;
-            for (int t = 0; t < A.nterms; t++) {
-              const bool f32 = A.term[t].fp32 != 0;
-              for (int r = 0; r < A.term[t].rank; r++, k++) {
-                const int ne = cur_n[k];
-                if (ne == 0) continue;
-                if (f32) {
-                  const float *Br = A.term[t].bands32 + (size_t)r * (2 * KD + 1) * N;
-                  float bv[2 * KD + 1];
-#pragma unroll
-                  for (int d = -KD; d <= KD; d++)
-                    bv[d + KD] = (interior || (i + d >= 0 && i + d < N)) ? Br[(size_t)(d + KD) * N + i] : 0.f;
-#pragma unroll
-                  for (int q = 0; q < PP_MAXENT; q++) {
-                    if (q < ne) {
-                      const zd *Yq = Ys + cur_q[k * PP_MAXENT + q] * N;
-                      float zr = 0.f, zi = 0.f;
-#pragma unroll
-                      for (int d = -KD; d <= KD; d++) {
-                        const int j = i + d;
-                        const zd yv = (interior || (j >= 0 && j < N)) ? Yq[j] : zmk(0.0, 0.0);
-                        zr = fmaf(bv[d + KD], (float)yv.x, zr);
-                        zi = fmaf(bv[d + KD], (float)yv.y, zi);
-                      }
-                      zfma(acc, cur_c[k * PP_MAXENT + q], zmk((double)zr, (double)zi));
-                    }
-                  }
-                } else {
-                  const double *Br = A.term[t].bands + (size_t)r * (2 * KD + 1) * N;
-                  double bv[2 * KD + 1];
-#pragma unroll
-                  for (int d = -KD; d <= KD; d++)
-                    bv[d + KD] = (interior || (i + d >= 0 && i + d < N)) ? Br[(size_t)(d + KD) * N + i] : 0.0;
-#pragma unroll
-                  for (int q = 0; q < PP_MAXENT; q++) {
-                    if (q < ne) {
-                      const zd *Yq = Ys + cur_q[k * PP_MAXENT + q] * N;
-                      zd z = zmk(0.0, 0.0);
-#pragma unroll
-                      for (int d = -KD; d <= KD; d++) {
-                        const int j = i + d;
-                        const zd yv = (interior || (j >= 0 && j < N)) ? Yq[j] : zmk(0.0, 0.0);
-                        zfmar(z, bv[d + KD], yv);
-                      }
-                      zfma(acc, cur_c[k * PP_MAXENT + q], z);
-                    }
-                  }
-                }
-              }
-            }
-          }
-          X[i] = acc;
-        }
-        else for (int i = tid; i < padN; i += PP_THREADS) {
-          zd acc = zmk(0.0, 0.0);
-          if (i < N) {
-            const bool interior = i >= KD && i + KD < N;
-            int k = 0;
-            for (int t = 0; t < A.nterms; t++) {
-              const double *Bt = A.term[t].bands;
-              for (int r = 0; r < A.term[t].rank; r++, k++) {
-                const int ne = cur_n[k];
-                if (ne == 0) continue;
-                const double *Br = Bt + (size_t)r * (2 * KD + 1) * N;
-                double bv[2 * KD + 1];
-#pragma unroll
-                for (int d = -KD; d <= KD; d++)
-                  bv[d + KD] = (interior || (i + d >= 0 && i + d < N)) ? Br[(size_t)(d + KD) * N + i] : 0.0;
-#pragma unroll
-                for (int q = 0; q < PP_MAXENT; q++) {
-                  if (q < ne) {
-                    const zd cq = cur_c[k * PP_MAXENT + q];
-                    const zd *Yq = Ys + cur_q[k * PP_MAXENT + q] * N;
-                    zd z = zmk(0.0, 0.0);
-#pragma unroll
-                    for (int d = -KD; d <= KD; d++) {
-                      const int j = i + d;
-                      const zd yv = (interior || (j >= 0 && j < N)) ? Yq[j] : zmk(0.0, 0.0);
-                      zfmar(z, bv[d + KD], yv);
-                    }
-                    zfma(acc, cq, z);
-                  }
-                }
-              }
-            }
-          }
-          X[i] = acc;
-        }
-        __syncthreads();
-        stamp();
-        pp_band_solve<KD, false, PP_SOLVE_THREADS>(pl, nl, X, W, tb, stamp);
+  const int N = A.N, tid = threadIdx.x;
+  // stage the source fibers U^-1 w of this row's couplings
+  for (int q = 0; q < A.nslots; q++) {
+    const int col = A.slotcols[a * A.nslots + q];
+    if (col >= 0)
+      for (int i = tid; i < N; i += PP_THREADS) M.Ys[q * N + i] = zld(A.y + (size_t)col * N + i);
+  }
+  pp_step_coeffs(A, M, step, fac);
+  __syncthreads();
+  pp_band_rows<KD, MIXED>(A, M, M.Ys, N, 0, 0, A.padN, M.X);
+  __syncthreads();
+  stamp();
+  pp_band_solve<KD, false, PP_SOLVE_THREADS>(A.plan, M.nl, M.X, M.W, M.tb, stamp);
 }
 
 // MIXED: terms flagged fp32 are applied in complex FP32 (FP32 radial bands,
@@ -487,7 +511,7 @@ __device__ __attribute__((always_inline)) inline void pp_pamp_body(const pp_pamp
         }
         pp_matvec<KD, MIXED>(A, M, a, step, zmk(inv, 0.0), [&]() { stamp(step); });
         stamp(step);
-        fiber_dots(Va, W, nk, N, A.reorth != 0, hred);
+        fiber_dots(Va, N, W, nk, N, A.reorth != 0, hred);
         for (int c = tid; c < ncomp; c += PP_THREADS) xst(mypart + c, hred[c]);
         if (A.reorth && tid == 0) xst(mypart + ONRM, hred[ncomp]);
       }
@@ -539,7 +563,7 @@ __device__ __attribute__((always_inline)) inline void pp_pamp_body(const pp_pamp
             else stop = true;
           }
           if (doit) {
-            fiber_dots(Va, W, nk, N, false, hred);
+            fiber_dots(Va, N, W, nk, N, false, hred);
             for (int c = tid; c < ncomp; c += PP_THREADS) xst(mypart + c, hred[c]);
           }
           grid_barrier(A.bar, ep, L, a);
@@ -628,6 +652,256 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_batch_kernel(const pp_pamp
   const int2 w = map[blockIdx.x];
   const pp_pamp_args A = args[w.x];
   pp_pamp_body<KD, MIXED>(A, w.y);
+}
+
+// ------------------------------------------------------------------ fiber split
+// Split form of the step kernel for problems with fewer fibers than CUs:
+// Q workgroups share each fiber, workgroup (a, q) owning the solver chunks
+// c0 = q*P/Q .. (q+1)*P/Q - 1 of fiber a (rows r0 = c0*B onwards), so every
+// CU streams its share of the Krylov basis.  A band solve becomes: stage 1
+// on the own chunks; their boundary values to the fiber's exchange slot
+// (sb, double-buffered by solve parity); a barrier among the fiber's Q
+// workgroups; the log-depth scan over all P chunks, repeated by each of
+// them; stage 3 on the own chunks.  The operator apply stages rows
+// r0-KD .. r1+KD-1 of the coupled fibers; inner products and norms are
+// partial over own rows and summed over all L*Q workgroups in one fixed
+// order.  Same algorithm and barrier discipline as pp_pamp_body, without
+// re-orthogonalisation (the launcher keeps that form on pp_pamp_kernel).
+
+// Workgroup b -> (fiber a, part q).  Workgroups are dispatched round-robin
+// over the 8 XCDs, so when 8 divides the grid the parts of one fiber (and
+// neighbouring fibers) are given blocks b = x + 8s with one x: they share
+// one XCD's L2 for the per-fiber barrier and the boundary exchange.
+__device__ inline void pp_split_map(int b, int L, int Q, int &a, int &q)
+{
+  const int nw = L * Q;
+  const int idx = (nw & 7) == 0 ? (b & 7) * (nw >> 3) + (b >> 3) : b;
+  a = idx / Q;
+  q = idx - a * Q;
+}
+
+// out[c] = sum_{w < nw} part[w*pstr + off + c] for c < ncomp, in one fixed
+// order (identical in every workgroup): G = a power of two <= 64 lanes per
+// component each sum the partials w = g, g+G, ... in order, then a
+// butterfly over the G lanes.
+__device__ inline void grid_partial_sum_wide(const double *part, int pstr, int off, int ncomp, int nw,
+                                             double *out_lds)
+{
+  const int tid = threadIdx.x;
+  int G = 64;
+  while (G > 1 && G * ncomp > PP_THREADS) G >>= 1;
+  const int c = tid / G, g = tid - c * G;
+  double s = 0.0;
+  if (c < ncomp) {
+#pragma unroll 8
+    for (int w = g; w < nw; w += G) s += xld(part + (size_t)w * pstr + off + c);
+  }
+  for (int o = G >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (c < ncomp && g == 0) out_lds[c] = s;
+  __syncthreads();
+}
+
+// x = U^-1 b (UPPER) or U^-T b over the own chunks c0 .. c0+nch-1 of one
+// fiber (in/out: those chunks' rows, LDS; tb: nch*KD complex).  All Q
+// workgroups of the fiber call it in the same sequence.  Ends with a
+// __syncthreads.
+template <int KD, bool UPPER>
+__device__ __attribute__((always_inline)) inline void pp_split_solve(const pp_pamp_args &A, const double *nl, const zd *in,
+                                                                     zd *out, zd *tb, int c0, int nch, zd *sbuf,
+                                                                     unsigned *fw, unsigned &sep, int q)
+{
+  const pp_band_plan &pl = A.plan;
+  const int tid = threadIdx.x, lb = pl.lb, off = UPPER ? 0 : pl.B - KD;
+  if (tid < PP_SOLVE_THREADS) pp_solve_blocks<UPPER, PP_SOLVE_THREADS>(pl, in, out, c0, nch);
+  __syncthreads();
+  for (int e = tid; e < nch * KD; e += PP_THREADS) {
+    const int cl = e / KD, k = e - cl * KD;
+    zst(sbuf + (size_t)(c0 + cl) * KD + k, out[(cl << lb) + off + k]);
+  }
+  group_barrier(fw, &A.bar->err, sep, A.Q, q);
+  if (tid < 64)
+    pp_solve_scan_src<KD, UPPER>(pl, nl, [&](int c, int k) { return zld(sbuf + (size_t)c * KD + k); }, tb, c0, c0 + nch);
+  __syncthreads();
+  if (tid < PP_SOLVE_THREADS) pp_solve_fixup<KD, UPPER, PP_SOLVE_THREADS>(pl, tb, out, c0, nch);
+  __syncthreads();
+}
+
+// LDS doubles of the split kernel's per-workgroup buffers above the
+// persistent scan matrices and combo table (maxch = chunks per workgroup,
+// rounded up): W, X (maxch*B each), tb, the staged coupled rows, the
+// reduction scratch and exp(-i dt H) e1.
+__host__ __device__ inline int pp_split_doubles(int nslots, int m, int kd, int B, int maxch)
+{
+  const int R = maxch * B;
+  return 2 * (2 * R + maxch * kd + nslots * (R + 2 * kd)) + 16 + (2 * m + 4) + 2 * m;
+}
+
+template <int KD, bool MIXED>
+__global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args A)
+{
+  extern __shared__ double lds_d[];
+  const int N = A.N, L = A.L, m = A.m, Q = A.Q, tid = threadIdx.x;
+  const pp_band_plan &pl = A.plan;
+  const int P = pl.P, lb = pl.lb;
+  int a, q;
+  pp_split_map(blockIdx.x, L, Q, a, q);
+  const int c0 = q * P / Q, nch = (q + 1) * P / Q - c0, maxch = (P + Q - 1) / Q;
+  const int r0 = c0 << lb, nloc = nch << lb, R = maxch << lb;
+  const int nown = (r0 + nloc < N ? r0 + nloc : N) - r0;   // own rows below N (>= 1: only the last chunk pads)
+  const int ystr = R + 2 * KD;
+  pp_lds_map M = pp_map_lds<KD>(A, lds_d);
+  M.W = (zd *)(lds_d + A.nlevEven + PP_CMB_DOUBLES);
+  M.X = M.W + R;
+  M.tb = M.X + R;
+  M.Ys = M.tb + maxch * KD;
+  M.Z = M.Ys;
+  M.wsum = (double *)(M.Ys + A.nslots * ystr);
+  M.hred = M.wsum + 16;
+  M.cv = (zd *)(M.hred + (2 * m + 4));
+  zd *W = M.W, *X = M.X, *tb = M.tb, *Ys = M.Ys, *Z = M.Z, *cv = M.cv;
+  double *wsum = M.wsum, *hred = M.hred;
+  const double tol = 1.4901161193847656e-08;           // sqrt(DBL_EPSILON), pamp.h:75
+  zd *Va = A.V + (size_t)a * (m + 1) * N;
+  const int nwg = L * Q, wid = a * Q + q;
+  double *mypart = A.part + (size_t)wid * A.pstr;
+  unsigned *fw = A.subw + (size_t)a * Q;
+  zd *sbase = A.sb + (size_t)a * 2 * P * KD;
+  const bool lead = a == 0 && q == 0;
+  const int NRM = 2 * m;
+  unsigned ep = 0, sep = 0;   // grid and per-fiber barriers passed
+  int nsolve = 0;             // split solves done (selects the boundary buffer)
+  auto sbuf = [&]() { return sbase + (size_t)((nsolve++) & 1) * P * KD; };
+  int tcount = 0;
+  auto stamp = [&](int step) {
+    if (A.tlog && step == 0 && lead && tid == 0 && tcount < PP_TLOG_MAX) A.tlog[tcount++] = (long long)wall_clock64();
+  };
+  pp_load_persistent(A, M, a);
+
+  for (int step = 0; step < A.nsteps; step++) {
+    // ---- psi~ = U psi on the own rows (psi rows up to r0+nloc+KD-1, written
+    // by the neighbouring part at the end of the previous step) ; y = U^-1 psi~
+    if (step > 0) group_barrier(fw, &A.bar->err, sep, Q, q);
+    const int phi = r0 + nloc + KD < N ? r0 + nloc + KD : N;
+    for (int j = r0 + tid; j < phi; j += PP_THREADS) Ys[j - r0] = zld(A.psi + (size_t)a * N + j);
+    __syncthreads();
+    double nrm = 0.0;
+    for (int i = tid; i < nloc; i += PP_THREADS) {
+      const int gi = r0 + i;
+      zd acc = zmk(0.0, 0.0);
+      if (gi < N) {
+        const int j1 = gi + KD < N - 1 ? gi + KD : N - 1;
+        for (int j = gi; j <= j1; j++) zfmar(acc, A.Uband[(size_t)j * (KD + 1) + KD + gi - j], Ys[j - r0]);
+      }
+      W[i] = acc;
+      X[i] = acc;
+      nrm += zabs2(acc);
+    }
+    nrm = block_sum(nrm, wsum);
+    if (tid == 0) xst(mypart + NRM, nrm);
+    pp_split_solve<KD, true>(A, M.nl, X, Z, tb, c0, nch, sbuf(), fw, sep, q);
+    for (int i = tid; i < nown; i += PP_THREADS) zst(A.y + (size_t)a * N + r0 + i, Z[i]);
+    stamp(step);
+    grid_barrier(A.bar, ep, nwg, wid);
+    stamp(step);
+
+    double beta0 = 0.0, beta = 0.0;
+    int mEff = m;
+    bool broken = false;
+    for (int j = 0; j < m; j++) {
+      // ---------------- phase A: beta_j, v_j = w/beta, w' = U^-H H U^-1 v_j, <v_k, w'>
+      const int nk = j + 1, ncomp = 2 * nk;
+      grid_partial_sum_wide(A.part, A.pstr, NRM, 1, nwg, hred);
+      beta = sqrt(hred[0]);
+      if (j == 0) beta0 = beta;
+      else if (!broken) {
+        if (fabs(beta) < tol) { broken = true; mEff = j; }   // happy breakdown (pamp.h:305-312)
+        else if (lead && tid == 0) A.Hm[j + (j - 1) * m] = zmk(beta, 0.0);
+      }
+      if (!broken) {
+        const double inv = 1.0 / beta;
+        for (int i = tid; i < nown; i += PP_THREADS) Va[(size_t)j * N + r0 + i] = zscale(W[i], inv);
+        // coupled fibers' U^-1 w, rows r0-KD .. r0+nloc+KD-1 inside [0, N)
+        const int ybase = r0 - KD, ylo = ybase > 0 ? ybase : 0;
+        for (int sl = 0; sl < A.nslots; sl++) {
+          const int col = A.slotcols[a * A.nslots + sl];
+          if (col >= 0)
+            for (int jj = ylo + tid; jj < phi; jj += PP_THREADS) Ys[sl * ystr + jj - ybase] = zld(A.y + (size_t)col * N + jj);
+        }
+        pp_step_coeffs(A, M, step, zmk(inv, 0.0));
+        __syncthreads();
+        pp_band_rows<KD, MIXED>(A, M, Ys, ystr, ybase, r0, nloc, X);
+        __syncthreads();
+        stamp(step);
+        pp_split_solve<KD, false>(A, M.nl, X, W, tb, c0, nch, sbuf(), fw, sep, q);
+        stamp(step);
+        fiber_dots(Va + r0, N, W, nk, nown, false, hred);
+        for (int c = tid; c < ncomp; c += PP_THREADS) xst(mypart + c, hred[c]);
+      }
+      stamp(step);
+      grid_barrier(A.bar, ep, nwg, wid);
+      stamp(step);
+
+      // ---------------- phase B: h = sum of partials ; w' -= V h ; |w'|^2 ; y = U^-1 w'
+      if (!broken) {
+        grid_partial_sum_wide(A.part, A.pstr, 0, ncomp, nwg, hred);
+        if (lead && tid <= j) A.Hm[tid + j * m] = zmk(hred[2 * tid], hred[2 * tid + 1]);
+      }
+      if (j == m - 1) break;   // the last residual is not used (pamp.h:419-426)
+      if (!broken) {
+        double v = 0.0;
+        for (int i = tid; i < nloc; i += PP_THREADS) {
+          zd w = zmk(0.0, 0.0);
+          if (i < nown) {
+            w = W[i];
+#pragma unroll PP_VUNROLL
+            for (int k = 0; k <= j; k++) w = zsub(w, zmul(Va[(size_t)k * N + r0 + i], zmk(hred[2 * k], hred[2 * k + 1])));
+          }
+          W[i] = w;
+          X[i] = w;
+          v += zabs2(w);
+        }
+        v = block_sum(v, wsum);
+        if (tid == 0) xst(mypart + NRM, v);
+        stamp(step);
+        pp_split_solve<KD, true>(A, M.nl, X, Z, tb, c0, nch, sbuf(), fw, sep, q);
+        for (int i = tid; i < nown; i += PP_THREADS) zst(A.y + (size_t)a * N + r0 + i, Z[i]);
+      }
+      stamp(step);
+      grid_barrier(A.bar, ep, nwg, wid);
+      stamp(step);
+    }
+
+    // ---------------- exponentiate on the lead workgroup, broadcast exp(-i dt H) e1
+    if (lead) {
+      pade_block(m, A.Hm, A.dt_re, A.dt_im, A.nlevEven + PP_CMB_DOUBLES, A.cvec, A.diag + (size_t)step * 4,
+                 A.normscale);
+      if (tid == 0) {
+        A.diag[(size_t)step * 4 + 1] = mEff;
+        A.diag[(size_t)step * 4 + 2] = beta0;
+        A.diag[(size_t)step * 4 + 3] = 0;
+      }
+      __syncthreads();
+      for (int e = tid; e < m * m; e += PP_THREADS) A.Hm[e] = zmk(0.0, 0.0);
+    }
+    stamp(step);
+    grid_barrier(A.bar, ep, nwg, wid);
+    stamp(step);
+    // ---------------- psi~ = |psi~| V c ; psi = U^-1 psi~ on the own rows
+    for (int k = tid; k < m; k += PP_THREADS) cv[k] = zld(A.cvec + k);
+    __syncthreads();
+    for (int i = tid; i < nloc; i += PP_THREADS) {
+      zd acc = zmk(0.0, 0.0);
+      if (i < nown)
+        for (int k = 0; k < mEff; k++) zfma(acc, Va[(size_t)k * N + r0 + i], cv[k]);
+      X[i] = zscale(acc, beta0);
+    }
+    __syncthreads();
+    pp_split_solve<KD, true>(A, M.nl, X, Z, tb, c0, nch, sbuf(), fw, sep, q);
+    for (int i = tid; i < nown; i += PP_THREADS) zst(A.psi + (size_t)a * N + r0 + i, Z[i]);
+    __syncthreads();
+    stamp(step);
+  }
+  if (A.tlog && lead && tid == 0) A.tlog[PP_TLOG_MAX] = tcount;
 }
 
 // ------------------------------------------------------------------ Expokit
@@ -896,7 +1170,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_expokit_kernel(pp_pamp_args A, 
         nmult++;
         pp_matvec<KD, MIXED>(A, M, a, step, fac, pp_nostamp());
         // first pass: h1 = V^H z
-        fiber_dots(Va, W, j, N, false, hred);
+        fiber_dots(Va, N, W, j, N, false, hred);
         for (int c = tid; c < 2 * j; c += PP_THREADS) xst(mypart + D1 + c, hred[c]);
         grid_barrier(A.bar, ep, L, a);
         grid_partial_sum(A.part, A.pstr, D1, 2 * j, L, hred, scr);
@@ -908,7 +1182,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_expokit_kernel(pp_pamp_args A, 
         }
         __syncthreads();
         // second pass: h2 = V^H z', |z'|^2
-        fiber_dots(Va, W, j, N, true, hred);
+        fiber_dots(Va, N, W, j, N, true, hred);
         for (int c = tid; c <= 2 * j; c += PP_THREADS) xst(mypart + D2 + c, hred[c]);
         grid_barrier(A.bar, ep, L, a);
         grid_partial_sum(A.part, A.pstr, D2, 2 * j + 1, L, hred, scr);
@@ -1022,6 +1296,8 @@ extern "C" int pp_pamp_create(int L, int N, int m, int reorth, const pp_overlap 
   p->normscale = 0;
   p->part = p->diag = p->scal = nullptr;
   p->bar = nullptr; p->tlog = nullptr; p->timing = 0;
+  p->fsplit = 0; p->sb = nullptr; p->sb_cap = 0;
+  p->bar_words = (size_t)L; p->part_cap = (size_t)L * p->pstr;
   p->diag_cap = p->scal_cap = 0;
   for (int t = 0; t < PP_MAXTERMS; t++) p->termbuf[t] = nullptr;
   PP_HIP(hipMalloc(&p->V, sizeof(zd) * (size_t)L * (m + 1) * N));
@@ -1333,7 +1609,7 @@ static int pamp_prepare(pp_pamp *p, void *psi, int nsteps, double dt_re, double 
     PP_HIP(hipMalloc(&p->diag, sizeof(double) * 6 * (size_t)nsteps));
   }
   PP_HIP(hipMemcpyAsync(p->scal, scal, sizeof(double) * nsteps * p->nterms, hipMemcpyHostToDevice, stream));
-  PP_HIP(hipMemsetAsync(p->bar, 0, sizeof(pp_barrier) + sizeof(unsigned) * (size_t)p->L, stream));
+  PP_HIP(hipMemsetAsync(p->bar, 0, sizeof(pp_barrier) + sizeof(unsigned) * p->bar_words, stream));
   memset(&A, 0, sizeof(A));
   A.L = p->L; A.N = p->N; A.m = p->m; A.s = p->s; A.bw = p->bw; A.nterms = p->nterms;
   A.nsteps = nsteps; A.reorth = p->reorth; A.nslots = p->nslots; A.normscale = p->normscale;
@@ -1367,6 +1643,69 @@ static int pamp_launch(const void *fn, int L, void **args, size_t lds, hipStream
   return err == hipSuccess ? PP_OK : PP_EHIP;
 }
 
+// Workgroups per fiber for pp_pamp_propagate: as many as keep one workgroup
+// per CU (at most 8) and at least two solver chunks per workgroup; 1 (the
+// one-workgroup-per-fiber kernel) with re-orthogonalisation or when that
+// leaves fewer than 2.  A fixed request must leave every workgroup a chunk.
+static int pamp_split_factor(const pp_pamp *p, int &Q)
+{
+  const int P = p->ov->plan.P;
+  Q = 1;
+  if (p->reorth || p->fsplit == 1) return PP_OK;
+  if (p->fsplit >= 2) {
+    if (p->fsplit > P) return PP_EUNSUP;
+    Q = p->fsplit;
+    return PP_OK;
+  }
+  int dev = 0, cus = 0;
+  PP_HIP(hipGetDevice(&dev));
+  PP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  int q = cus / p->L;
+  if (q > 8) q = 8;
+  if (q > P / 2) q = P / 2;
+  Q = q >= 2 ? q : 1;
+  return PP_OK;
+}
+
+// LDS bytes of one split-kernel workgroup (the lead one also runs the Pade
+// exponential in the same space).
+static size_t pamp_split_lds_bytes(const pp_pamp *p, int Q)
+{
+  const pp_band_plan &pl = p->ov->plan;
+  const size_t nle = (size_t)((pl.nlev + 1) & ~1) + PP_CMB_DOUBLES;
+  const int maxch = (pl.P + Q - 1) / Q;
+  const size_t fib = sizeof(double) * (nle + (size_t)pp_split_doubles(p->nslots, p->m, p->ov->kd, pl.B, maxch)) + 64;
+  const size_t pade = sizeof(double) * nle + sizeof(zd) * (size_t)(9 * p->m * p->m) + sizeof(double) * (PP_THREADS / 64) + 64;
+  return fib > pade ? fib : pade;
+}
+
+// Buffers of the split kernel: partial sums of L*Q workgroups, grid and
+// per-fiber arrival words, the chunk boundary exchange.
+static int pamp_split_buffers(pp_pamp *p, int Q)
+{
+  const size_t nw = (size_t)p->L * Q;
+  if (nw * p->pstr > p->part_cap) {
+    (void)hipFree(p->part);
+    p->part = nullptr;
+    PP_HIP(hipMalloc(&p->part, sizeof(double) * nw * p->pstr));
+    p->part_cap = nw * p->pstr;
+  }
+  if (2 * nw > p->bar_words) {
+    (void)hipFree(p->bar);
+    p->bar = nullptr;
+    PP_HIP(hipMalloc(&p->bar, sizeof(pp_barrier) + sizeof(unsigned) * 2 * nw));
+    p->bar_words = 2 * nw;
+  }
+  const size_t nsb = (size_t)p->L * 2 * p->ov->plan.P * p->ov->kd;
+  if (nsb > p->sb_cap) {
+    if (p->sb) (void)hipFree(p->sb);
+    p->sb = nullptr;
+    PP_HIP(hipMalloc(&p->sb, sizeof(zd) * nsb));
+    p->sb_cap = nsb;
+  }
+  return PP_OK;
+}
+
 // Propagate psi (device, L x N complex128) through nsteps steps of length dt.
 // scal: host array nsteps x nterms with each term's time factor at t_n.
 extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re, double dt_im,
@@ -1375,16 +1714,42 @@ extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re
   if (!p || !psi || nsteps < 0 || (!scal && p->nterms > 0)) return PP_EARG;
   if (nsteps == 0) return PP_OK;
   if (p->nterms == 0) return PP_EARG;
-  pp_pamp_args A;
-  int rc = pamp_prepare(p, psi, nsteps, dt_re, dt_im, scal, stream, A);
+  int Q = 1;
+  int rc = pamp_split_factor(p, Q);
   if (rc) return rc;
-  const size_t lds = pamp_lds_bytes(p, A.padN);
+  if (Q > 1 && (rc = pamp_split_buffers(p, Q))) return rc;
+  pp_pamp_args A;
+  rc = pamp_prepare(p, psi, nsteps, dt_re, dt_im, scal, stream, A);
+  if (rc) return rc;
   void *args[] = {&A};
   rc = PP_EUNSUP;
+  if (Q > 1) {
+    A.Q = Q;
+    A.sb = p->sb;
+    A.subw = (unsigned *)(p->bar + 1) + (size_t)p->L * Q;
+    const size_t lds = pamp_split_lds_bytes(p, Q);
+    PP_KD_SWITCH(p->ov->kd, {
+      rc = pamp_launch(p->mixed ? (const void *)pp_pamp_split_kernel<KD_, true>
+                                : (const void *)pp_pamp_split_kernel<KD_, false>,
+                       p->L * Q, args, lds, stream); })
+    return rc;
+  }
+  A.Q = 1;
+  const size_t lds = pamp_lds_bytes(p, A.padN);
   PP_KD_SWITCH(p->ov->kd, {
     rc = pamp_launch(p->mixed ? (const void *)pp_pamp_kernel<KD_, true> : (const void *)pp_pamp_kernel<KD_, false>,
                      p->L, args, lds, stream); })
   return rc;
+}
+
+// Workgroups per fiber of pp_pamp_propagate: 0 automatic, 1 one workgroup
+// per fiber, q >= 2 that many (at most the solver's chunk count).  No
+// reference counterpart: a launch-shape control of this implementation.
+extern "C" int pp_pamp_set_fiber_split(pp_pamp *p, int q)
+{
+  if (!p || q < 0 || q > 64) return PP_EARG;
+  p->fsplit = q;
+  return PP_OK;
 }
 
 // Many independent instances (each a pp_pamp with its own operator, overlap
@@ -1581,6 +1946,7 @@ extern "C" int pp_pamp_destroy(pp_pamp *p)
   if (p->diag) (void)hipFree(p->diag);
   if (p->scal) (void)hipFree(p->scal);
   if (p->bar) (void)hipFree(p->bar);
+  if (p->sb) (void)hipFree(p->sb);
   delete p;
   return PP_OK;
 }

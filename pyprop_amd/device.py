@@ -181,6 +181,11 @@ class PampEngine:
         """0: largest element (pAMP); 1: 1-norm, accurate and smooth."""
         nat.check(nat.lib().pp_pamp_set_pade_scaling(self.h, int(mode)), "pp_pamp_set_pade_scaling")
 
+    def set_fiber_split(self, q=0):
+        """Workgroups per angular fiber in propagate: 0 automatic (fill the
+        CUs), 1 one per fiber, q >= 2 fixed."""
+        nat.check(nat.lib().pp_pamp_set_fiber_split(self.h, int(q)), "pp_pamp_set_fiber_split")
+
     def set_timing(self, enable=True):
         nat.check(nat.lib().pp_pamp_set_timing(self.h, int(bool(enable))), "pp_pamp_set_timing")
 

@@ -106,6 +106,45 @@ def test_pamp_mixed_close_to_oracle(lmax, nbp, m, nsteps, dt):
     assert abs(h.norm(got) - h.norm(ref)) < 1e-8
 
 
+@pytest.mark.parametrize("lmax,nbp,m,q,mixed", [
+    (63, 120, 24, 0, False), (63, 120, 24, 2, False), (10, 100, 20, 3, False), (10, 100, 20, 7, False),
+    (31, 300, 16, 8, False), (63, 120, 24, 0, True)])
+def test_pamp_fiber_split_matches_one_workgroup_per_fiber(lmax, nbp, m, q, mixed):
+    """Several workgroups per fiber (automatic and forced counts, including
+    ones that do not divide the chunk count) against one workgroup per fiber
+    on the same inputs: only the grouping of the partial sums differs, so the
+    results agree to rounding (FP64: 1e-12; mixed: 1e-9, since a different
+    last bit of an FP64 input can round its FP32 conversion the other way);
+    the norm is conserved and the Krylov diagnostics match."""
+    from pyprop_amd import device as D
+    h = _hydrogen(lmax=lmax, nbp=nbp, rmax=float(nbp))
+    psi0 = h.ground_state()
+    psi0[1] = 0.1 * psi0[0]
+    nsteps, dt = 6, 0.01
+    ts = O.step_times(nsteps, dt, t0=10.0)
+    sc = np.array([[1.0, h.scaling(t)] for t in ts])
+    out, diags = [], []
+    for split in (1, q):
+        ov = D.Overlap(h.S, h.N, h.kd)
+        eng = D.PampEngine(h.L, h.N, m, ov)
+        eng.add_term("Diag", D.to_device(h.H0))
+        eng.add_term("Simp", D.to_device(h.Hint), pairs=h.pairs)
+        if mixed:
+            eng.set_mixed([1])
+        eng.set_fiber_split(split)
+        print("fiber split", lmax, nbp, m, "q", split, "mixed", mixed, flush=True)
+        psi = D.to_device(psi0)
+        eng.propagate(psi, nsteps, dt, sc)
+        out.append(psi.cpu().numpy())
+        diags.append(eng.diagnostics())
+    err = np.abs(out[1] - out[0]).max() / np.abs(out[0]).max()
+    print("split vs one workgroup per fiber, rel diff", err)
+    assert err < (1e-9 if mixed else 1e-12)
+    assert abs(h.norm(out[1]) - h.norm(out[0])) < (1e-10 if mixed else 1e-12)
+    assert np.array_equal(diags[0][:, 1], diags[1][:, 1])
+    assert np.allclose(diags[0][:, 2], diags[1][:, 2], rtol=1e-13, atol=0)
+
+
 @pytest.mark.parametrize("lmax,nbp,m,nsteps,dt,anorm", [(6, 40, 12, 3, 0.05, 1e6), (10, 100, 20, 3, 0.01, 1e6),
                                                          (10, 100, 16, 4, 0.02, 50.0)])
 def test_expokit_persistent_matches_oracle(lmax, nbp, m, nsteps, dt, anorm):
