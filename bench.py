@@ -24,11 +24,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM-side bytes per step of pp_pamp_kernel on each workload, read by the
-# FETCH_SIZE and WRITE_SIZE counter passes recorded in
+# HBM-side bytes per step of the pAMP step kernel (its fiber-split form, 4
+# workgroups per fiber) on each workload, read by the FETCH_SIZE and
+# WRITE_SIZE counter passes recorded in
 # profiles/pp_pamp_kernel.hydrogen_large.pmc.md (FETCH_SIZE scaled x2 and
 # WRITE_SIZE x1 by a 512 MiB device copy of known size).
-PMC_TRAFFIC_PER_STEP = {"hydrogen_large": {"fetch": 79.2e6, "write": 59.8e6}}
+PMC_TRAFFIC_PER_STEP = {"hydrogen_large": {"fetch": 60.1e6, "write": 78.0e6}}
 
 
 def step_bytes(n, m, N, s_mv=16, s_gs=16):
@@ -277,33 +278,41 @@ def run_helium(args):
     gs_s = time.perf_counter() - t0
     print("bench: ground state E0 = %.10f after %d imaginary-time steps (%.1f s)" % (E0, args.imag_steps, gs_s),
           file=sys.stderr, flush=True)
-    conf = HE.load(args.workload)
-    prob = pyprop_amd.Problem(conf)
-    prob.SetupStep()
-    prob.SetWavefunction(psi0)
-    prob.psi.Normalize()
-    eng = prob.Propagator.Engine
+    def run_form(mixed):
+        over = {("Propagation", "matvec_precision"): "mixed"} if mixed else None
+        prob = pyprop_amd.Problem(HE.load(args.workload, over))
+        prob.SetupStep()
+        prob.SetWavefunction(psi0)
+        prob.psi.Normalize()
+        eng = prob.Propagator.Engine
+        ground = prob.psi.Copy()
+        prob.AdvanceSteps(args.warmup)
+        torch.cuda.synchronize()
+        print("bench: %s form warm-up done, timing %d steps" % ("mixed" if mixed else "fp64", args.steps),
+              file=sys.stderr, flush=True)
+        eng.set_timing(True)
+        stream = torch.cuda.current_stream()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        ev0.record(stream)
+        prob.AdvanceSteps(args.steps)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        ee_ms, ee_launches = eng.kernel_time()
+        eng.set_timing(False)
+        return dict(prob=prob, eng=eng, ground=ground, wall=wall, ee_ms=ee_ms, ee_launches=ee_launches,
+                    step_ms=ev0.elapsed_time(ev1))
+
+    fp = run_form(False)
+    prob, eng, ground, wall = fp["prob"], fp["eng"], fp["ground"], fp["wall"]
+    ee_ms, ee_launches = fp["ee_ms"], fp["ee_launches"]
     C, N, _ = prob.psi.GetData().shape
     n = C * N * N
     m = prob.Propagator.BasisSize
     dt = prob.TimeStep
-    ground = prob.psi.Copy()
-    prob.AdvanceSteps(args.warmup)
-    torch.cuda.synchronize()
-    print("bench: warm-up done, timing %d steps" % args.steps, file=sys.stderr, flush=True)
-    eng.set_timing(True)
-    stream = torch.cuda.current_stream()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    w0 = time.perf_counter()
-    ev0.record(stream)
-    prob.AdvanceSteps(args.steps)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - w0
-    ee_ms, ee_launches = eng.kernel_time()
-    eng.set_timing(False)
     flops = eng.multipole_flops()
     ee_avg_s = ee_ms / max(ee_launches, 1) / 1e3
     # algorithmic bytes of one multipole stage (mixing, near band, far field):
@@ -319,6 +328,25 @@ def run_helium(args):
     norm = prob.psi.GetNorm()
     pop_ground = abs(ground.InnerProduct(prob.psi)) ** 2
     diag = prob.Propagator.GetDiagnostics()
+
+    mixed = None
+    if not args.no_mixed:
+        mx = run_form(True)
+        mprob = mx["prob"]
+        pop_mixed = abs(ground.InnerProduct(mprob.psi)) ** 2
+        d = prob.psi.Copy()
+        d.GetData().sub_(mprob.psi.GetData())
+        mixed = {"value": args.steps / mx["wall"], "unit": "steps/s", "ms_per_step": 1e3 * mx["wall"] / args.steps,
+                 "speedup_vs_fp64": wall / mx["wall"],
+                 "multipole_stage_ms_avg": mx["ee_ms"] / max(mx["ee_launches"], 1),
+                 "ground_population_fp64": pop_ground, "ground_population_mixed": pop_mixed,
+                 "population_error": abs(pop_ground - pop_mixed),
+                 "norm_fp64": norm, "norm_mixed": mprob.psi.GetNorm(),
+                 "psi_diff_snorm": d.GetNorm(),
+                 "steps_compared": args.warmup + args.steps,
+                 "dtype": "multipole terms FP32 (complex64 intermediates); one-electron and laser terms, "
+                          "Gram-Schmidt, overlap solves, Pade FP64"}
+        del mx, mprob, d
 
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
@@ -370,13 +398,14 @@ def run_helium(args):
                      "kernel": "multipole stage (p3_mixT + p3_near + p3_farL), one launch each per matvec",
                      "algorithmic_bytes_per_launch": ee_bytes, "kernel_ms_avg": 1e3 * ee_avg_s,
                      "launches": ee_launches,
-                     "kernel_share_of_step": ee_ms / 1e3 / (ev0.elapsed_time(ev1) / 1e3),
+                     "kernel_share_of_step": ee_ms / fp["step_ms"],
                      "algorithmic_flops_per_launch": flops,
                      "fp64_tflops": flops / ee_avg_s / 1e12, "fp64_peak_tflops": FP64_PEAK_TFLOPS},
         "cpu_baseline": cpu,
         "checks": {"ground_energy": E0, "ground_state_seconds": gs_s, "imag_steps": args.imag_steps,
                    "norm_after": norm, "ground_population_after": pop_ground,
                    "krylov_error_estimate_last": float(diag[-1, 0]), "basis_used_last": int(diag[-1, 1])},
+        "mixed": mixed,
     }
     print(json.dumps(line))
 
@@ -480,6 +509,10 @@ def main():
                  "psi_diff_snorm": dnorm, "steps_compared": args.warmup + args.steps,
                  "dtype": "c64 operator apply, c128 Gram-Schmidt / overlap solves / Pade"}
 
+    # launch shape picked by the library: q = min(8, CUs // L, chunks // 2)
+    # workgroups per fiber (q >= 2: the fiber-split kernel)
+    q = min(8, torch.cuda.get_device_properties(0).multi_processor_count // L, ((N + 31) // 32) // 2)
+    kernel_name = ("pp_pamp_split_kernel (%d workgroups per fiber)" % q) if q >= 2 else "pp_pamp_kernel"
     Bstep = step_bytes(n, m, N)
     pmc = PMC_TRAFFIC_PER_STEP.get(args.workload)
     traffic = None if pmc is None else (pmc["fetch"] + pmc["write"]) * args.steps
@@ -519,7 +552,8 @@ def main():
                    "dt": dt.real, "form": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "pp_pamp_kernel", "algorithmic_bytes_per_step": Bstep,
+                     "kernel": kernel_name,
+                     "algorithmic_bytes_per_step": Bstep,
                      "kernel_ms_per_step": 1e3 * kernel_s / args.steps,
                      "launch": "one launch of %d steps; traffic from counter passes" % args.steps},
         "cpu_baseline": cpu,
