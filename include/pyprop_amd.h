@@ -204,6 +204,15 @@ int pp_pamp3_propagate(pp_pamp3 *p, void *psi, int nsteps, double dt_re, double 
 /* Per step of the last pp_pamp3_propagate (host, nsteps x 4): |last Krylov
  * coefficient|, basis size used, |psi~|, 0. */
 int pp_pamp3_diagnostics(pp_pamp3 *p, int nsteps, double *diag, pp_stream stream);
+/* Mixed form of the rank-3 engine (no reference counterpart: the reference
+ * is complex128 throughout, helium_stabilization/config.ini): with mixed != 0
+ * the multipole terms (the near/far split stage) run in FP32 with complex64
+ * intermediates and FP32 radial tables, each output added once into the
+ * FP64 result; the one-electron and laser terms, Gram-Schmidt, the overlap
+ * solves and the Pade exponential stay FP64.  Needs the split (radial
+ * supports and P_l, Q_l of every multipole term): later calls return
+ * PP_EUNSUP otherwise. */
+int pp_pamp3_set_mixed(pp_pamp3 *p, int mixed);
 /* Measurement hooks (no reference counterpart): with timing enabled, every
  * multipole contraction launch of later calls is bracketed by HIP events on
  * its stream; pp_pamp3_kernel_time returns their summed duration (ms) and

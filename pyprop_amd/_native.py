@@ -29,7 +29,7 @@ EXPORTS = [
     "pp_lowrank_factor", "pp_pamp_propagate", "pp_pamp_diagnostics", "pp_pamp_set_mixed", "pp_pamp_set_fiber_split", "pp_pamp_set_pade_scaling", "pp_expokit_propagate", "pp_expokit_diagnostics", "pp_pamp_batch_propagate", "pp_pamp_lds_estimate", "pp_pamp_set_timing", "pp_pamp_timings", "pp_pamp_destroy",
     "pp_pamp3_create", "pp_pamp3_add_kron_term", "pp_pamp3_add_multipole_term", "pp_pamp3_apply",
     "pp_pamp3_propagate", "pp_pamp3_diagnostics", "pp_pamp3_set_timing", "pp_pamp3_kernel_time",
-    "pp_pamp3_set_radial_supports",
+    "pp_pamp3_set_radial_supports", "pp_pamp3_set_mixed",
     "pp_pamp3_multipole_flops", "pp_pamp3_destroy",
 ]
 
@@ -92,6 +92,7 @@ def lib():
     L.pp_pamp3_diagnostics.argtypes = [P, I, dp, P]
     L.pp_pamp3_destroy.argtypes = [P]
     L.pp_pamp3_set_timing.argtypes = [P, I]
+    L.pp_pamp3_set_mixed.argtypes = [P, I]
     L.pp_pamp3_kernel_time.argtypes = [P, dp, ctypes.POINTER(ctypes.c_long)]
     L.pp_pamp3_multipole_flops.argtypes = [P, dp]
     for name in EXPORTS:

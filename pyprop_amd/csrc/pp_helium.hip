@@ -53,6 +53,30 @@ struct p3_ent {            // one coupling of a banded pass or the mixing
   double cre, cim;
 };
 
+// Working precision of the split multipole stage: the FP64 form keeps the
+// mixed input, the far-field intermediates and the radial tables in double;
+// the mixed form (pp_pamp3_set_mixed) stores them as complex64 / float and
+// accumulates in FP32, adding each output into the FP64 result once.
+template <class T> struct p3_cx;
+template <> struct p3_cx<double> { typedef double2 type; };
+template <> struct p3_cx<float> { typedef float2 type; };
+template <class Z> __device__ inline Z p3_czero();
+template <> __device__ inline double2 p3_czero<double2>() { return make_double2(0.0, 0.0); }
+template <> __device__ inline float2 p3_czero<float2>() { return make_float2(0.f, 0.f); }
+template <class Z> __device__ inline Z p3_cfrom(zd v);
+template <> __device__ inline double2 p3_cfrom<double2>(zd v) { return v; }
+template <> __device__ inline float2 p3_cfrom<float2>(zd v) { return make_float2((float)v.x, (float)v.y); }
+__device__ inline zd p3_ctod(double2 v) { return v; }
+__device__ inline zd p3_ctod(float2 v) { return zmk((double)v.x, (double)v.y); }
+__device__ inline double2 p3_cadd(double2 a, double2 b) { return zadd(a, b); }
+__device__ inline float2 p3_cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline void p3_cfmar(double2 &a, double s, double2 c) { zfmar(a, s, c); }
+__device__ inline void p3_cfmar(float2 &a, float s, float2 c)
+{
+  a.x = fmaf(s, c.x, a.x);
+  a.y = fmaf(s, c.y, a.y);
+}
+
 struct pp_pamp3 {
   int C, N, m, s, bw, reorth;
   long n;
@@ -65,12 +89,15 @@ struct pp_pamp3 {
     std::vector<int> pr; std::vector<double> cf; double *R; int slot;
     std::vector<double> Rh, Pl, Ql;   // host copies for the near/far split (Pl, Ql empty: no split)
     double *dRn, *dP, *dQ;
+    float *fRn, *fP, *fQ;             // the same in FP32 (mixed form)
   };
   std::vector<int> sup_lo, sup_hi;     // quadrature-interval support of each B-spline
   bool split;
   int2 *nr;
   std::vector<int2> nrh;               // near band per i1 (host copy, for the flop count)
   const double **dRn, **dPt, **dQt;
+  const float **fRnt, **fPt, **fQt;    // per-term FP32 tables of the mixed form
+  int mixed;                           // multipole stage in FP32 (complex64 intermediates)
   zd *Tq, *Tp;
   std::vector<mterm> mterms;
   int nslots;
@@ -314,54 +341,22 @@ __global__ void __launch_bounds__(P3_EE_THREADS) p3_ee(const double *const *Rt, 
 #define P3_NW 16           // near-band width handled per i1 (>= 2k-1)
 #define P3_MAXB 32         // multipole blocks per chunk (terms l <= 31)
 
-// Tq[b][j1][i2][cl] = sum_d Q[i2][d] Ym[b][j1][i2+d-S][cl], Tp likewise with P
-template <int S>
-__global__ void __launch_bounds__(256) p3_farT(const double *const *Pt, const double *const *Qt, const int *blk_term,
-                                               const zd *Ym, zd *Tq, zd *Tp, int N, const int *state)
-{
-  if (state[0]) return;
-  constexpr int BW = 2 * S + 1;
-  extern __shared__ double p3_lds[];
-  zd *row = (zd *)p3_lds;                                // positions -S .. N+S-1, 16 channels each
-  const int b = blockIdx.x / N, j1 = blockIdx.x % N;
-  const long NN = (long)N * N;
-  const zd *Yr = Ym + ((long)b * NN + (long)j1 * N) * P3_CHUNK;
-  for (int e = threadIdx.x; e < (N + 2 * S) * P3_CHUNK; e += 256) {
-    const int pos = e / P3_CHUNK - S;
-    row[e] = (pos >= 0 && pos < N) ? Yr[(long)pos * P3_CHUNK + (e & (P3_CHUNK - 1))] : zmk(0.0, 0.0);
-  }
-  __syncthreads();
-  const double *P = Pt[blk_term[b]], *Q = Qt[blk_term[b]];
-  zd *oq = Tq + ((long)b * NN + (long)j1 * N) * P3_CHUNK, *op = Tp + ((long)b * NN + (long)j1 * N) * P3_CHUNK;
-  for (int e = threadIdx.x; e < N * P3_CHUNK; e += 256) {
-    const int i2 = e / P3_CHUNK, cl = e & (P3_CHUNK - 1);
-    const double *qr = Q + (size_t)i2 * BW, *pr = P + (size_t)i2 * BW;
-    zd tq = zmk(0.0, 0.0), tp = zmk(0.0, 0.0);
-#pragma unroll
-    for (int d = 0; d < BW; d++) {
-      const zd y = row[(i2 + d) * P3_CHUNK + cl];
-      zfmar(tq, qr[d], y);
-      zfmar(tp, pr[d], y);
-    }
-    oq[e] = tq;
-    op[e] = tp;
-  }
-}
-
 // Mixing fused with the far field's i2 pass: workgroup (block b, row j1)
 // forms the mixed input row Ym[b][j1][.][.] in LDS (written out for the
 // near band), then Tq/Tp of that row from LDS.
 #define P3_MIXE 12         // mixing entries per row held in registers
-template <int S>
-__global__ void __launch_bounds__(256) p3_mixT(const int *row, const p3_ent *ent, const zd *y, zd *Ym,
-                                               const double *const *Pt, const double *const *Qt, const int *blk_term,
-                                               const int2 *nr, zd *Tq, zd *Tp, int N, const double *scal,
+template <int S, class T>
+__global__ void __launch_bounds__(256) p3_mixT(const int *row, const p3_ent *ent, const zd *y,
+                                               typename p3_cx<T>::type *Ym, const T *const *Pt, const T *const *Qt,
+                                               const int *blk_term, const int2 *nr, typename p3_cx<T>::type *Tq,
+                                               typename p3_cx<T>::type *Tp, int N, const double *scal,
                                                const int *state)
 {
+  typedef typename p3_cx<T>::type Z;
   if (state[0]) return;
   constexpr int BW = 2 * S + 1;
   extern __shared__ double p3_lds[];
-  zd *rowb = (zd *)p3_lds;                               // positions -S .. N+S-1, 16 channels each
+  Z *rowb = (Z *)p3_lds;                                 // positions -S .. N+S-1, 16 channels each
   const int b = blockIdx.x / N, j1 = blockIdx.x % N;
   const long NN = (long)N * N;
   const int cl = threadIdx.x & (P3_CHUNK - 1);
@@ -383,7 +378,7 @@ __global__ void __launch_bounds__(256) p3_mixT(const int *row, const p3_ent *ent
   }
   // the near band reads this row only at positions nlo(j1-S)-S .. nhi(j1+S)+S
   const int wlo = nr[j1 - S > 0 ? j1 - S : 0].x - S, whi = nr[j1 + S < N - 1 ? j1 + S : N - 1].y + S;
-  zd *out = Ym + ((long)b * NN + (long)j1 * N) * P3_CHUNK;
+  Z *out = Ym + ((long)b * NN + (long)j1 * N) * P3_CHUNK;
   for (int e = threadIdx.x; e < (N + 2 * S) * P3_CHUNK; e += 256) {
     const int pos = e / P3_CHUNK - S;
     zd acc = zmk(0.0, 0.0);
@@ -396,22 +391,22 @@ __global__ void __launch_bounds__(256) p3_mixT(const int *row, const p3_ent *ent
         const double sc = E.slot < 0 ? 1.0 : scal[E.slot];
         zfma(acc, zmk(E.cre * sc, E.cim * sc), y[E.col * NN + (long)j1 * N + pos]);
       }
-      if (pos >= wlo && pos <= whi) out[(long)pos * P3_CHUNK + cl] = acc;
+      if (pos >= wlo && pos <= whi) out[(long)pos * P3_CHUNK + cl] = p3_cfrom<Z>(acc);
     }
-    rowb[e] = acc;
+    rowb[e] = p3_cfrom<Z>(acc);
   }
   __syncthreads();
-  const double *P = Pt[blk_term[b]], *Q = Qt[blk_term[b]];
-  zd *oq = Tq + ((long)b * NN + (long)j1 * N) * P3_CHUNK, *op = Tp + ((long)b * NN + (long)j1 * N) * P3_CHUNK;
+  const T *P = Pt[blk_term[b]], *Q = Qt[blk_term[b]];
+  Z *oq = Tq + ((long)b * NN + (long)j1 * N) * P3_CHUNK, *op = Tp + ((long)b * NN + (long)j1 * N) * P3_CHUNK;
   for (int e = threadIdx.x; e < N * P3_CHUNK; e += 256) {
     const int i2 = e / P3_CHUNK, c2 = e & (P3_CHUNK - 1);
-    const double *qr = Q + (size_t)i2 * BW, *pr = P + (size_t)i2 * BW;
-    zd tq = zmk(0.0, 0.0), tp = zmk(0.0, 0.0);
+    const T *qr = Q + (size_t)i2 * BW, *pr = P + (size_t)i2 * BW;
+    Z tq = p3_czero<Z>(), tp = p3_czero<Z>();
 #pragma unroll
     for (int d = 0; d < BW; d++) {
-      const zd v = rowb[(i2 + d) * P3_CHUNK + c2];
-      zfmar(tq, qr[d], v);
-      zfmar(tp, pr[d], v);
+      const Z v = rowb[(i2 + d) * P3_CHUNK + c2];
+      p3_cfmar(tq, qr[d], v);
+      p3_cfmar(tp, pr[d], v);
     }
     oq[e] = tq;
     op[e] = tp;
@@ -421,11 +416,13 @@ __global__ void __launch_bounds__(256) p3_mixT(const int *row, const p3_ent *ent
 // Z[c][i1][i2] += sum_blocks [ i2 > nhi[i1]:  sum_d1 P[i1][d1] Tq[b][i1+d1-S][i2][cl]
 //                              i2 < nlo[i1]:  sum_d1 Q[i1][d1] Tp[b][i1+d1-S][i2][cl] ]
 // workgroup: (chunk, P3_RT rows i1, 16 columns i2); thread: (i2, channel lane).
-template <int S>
-__global__ void __launch_bounds__(256) p3_farL(const double *const *Pt, const double *const *Qt, const int *blk_term,
-                                               const int *cb_ptr, const int *cb_blk, const int2 *nr, const zd *Tq,
-                                               const zd *Tp, zd *Z, int C, int N, int nchunk, const int *state)
+template <int S, class T>
+__global__ void __launch_bounds__(256) p3_farL(const T *const *Pt, const T *const *Qt, const int *blk_term,
+                                               const int *cb_ptr, const int *cb_blk, const int2 *nr,
+                                               const typename p3_cx<T>::type *Tq, const typename p3_cx<T>::type *Tp,
+                                               zd *Z, int C, int N, int nchunk, const int *state)
 {
+  typedef typename p3_cx<T>::type ZT;
   if (state[0]) return;
   constexpr int BW = 2 * S + 1, WIN = P3_RT + 2 * S;
   const int ntile = (N + P3_RT - 1) / P3_RT, ngrp = (N + 15) / 16;
@@ -447,8 +444,8 @@ __global__ void __launch_bounds__(256) p3_farL(const double *const *Pt, const do
   bool anyb = false, anya = false;
 #pragma unroll
   for (int rr = 0; rr < P3_RT; rr++) { anyb = anyb || below[rr]; anya = anya || above[rr]; }
-  __shared__ const double *sP[P3_MAXB];
-  __shared__ const double *sQ[P3_MAXB];
+  __shared__ const T *sP[P3_MAXB];
+  __shared__ const T *sQ[P3_MAXB];
   __shared__ long sT[P3_MAXB];
   const int b0 = cb_ptr[chunk], nb = cb_ptr[chunk + 1] - b0;
   for (int k = threadIdx.x; k < nb; k += 256) {
@@ -458,43 +455,43 @@ __global__ void __launch_bounds__(256) p3_farL(const double *const *Pt, const do
     sT[k] = (long)b * NN * P3_CHUNK;
   }
   __syncthreads();
-  zd acc[P3_RT];
+  ZT acc[P3_RT];
 #pragma unroll
-  for (int rr = 0; rr < P3_RT; rr++) acc[rr] = zmk(0.0, 0.0);
+  for (int rr = 0; rr < P3_RT; rr++) acc[rr] = p3_czero<ZT>();
   for (int k = 0; k < nb; k++) {
-    const double *P = sP[k], *Q = sQ[k];
+    const T *P = sP[k], *Q = sQ[k];
     const long tb = sT[k];
-    zd win[WIN];
+    ZT win[WIN];
     if (anyb) {
 #pragma unroll
       for (int w = 0; w < WIN; w++) {
         const int j1 = i1a - S + w;
-        win[w] = (j1 >= 0 && j1 < N && col_ok) ? Tq[tb + ((long)j1 * N + i2) * P3_CHUNK + cl] : zmk(0.0, 0.0);
+        win[w] = (j1 >= 0 && j1 < N && col_ok) ? Tq[tb + ((long)j1 * N + i2) * P3_CHUNK + cl] : p3_czero<ZT>();
       }
 #pragma unroll
       for (int rr = 0; rr < P3_RT; rr++) {
         if (!below[rr]) continue;
-        const double *pr = P + (size_t)(i1a + rr) * BW;
-        zd t = zmk(0.0, 0.0);
+        const T *pr = P + (size_t)(i1a + rr) * BW;
+        ZT t = p3_czero<ZT>();
 #pragma unroll
-        for (int d = 0; d < BW; d++) zfmar(t, pr[d], win[rr + d]);
-        acc[rr] = zadd(acc[rr], t);
+        for (int d = 0; d < BW; d++) p3_cfmar(t, pr[d], win[rr + d]);
+        acc[rr] = p3_cadd(acc[rr], t);
       }
     }
     if (anya) {
 #pragma unroll
       for (int w = 0; w < WIN; w++) {
         const int j1 = i1a - S + w;
-        win[w] = (j1 >= 0 && j1 < N && col_ok) ? Tp[tb + ((long)j1 * N + i2) * P3_CHUNK + cl] : zmk(0.0, 0.0);
+        win[w] = (j1 >= 0 && j1 < N && col_ok) ? Tp[tb + ((long)j1 * N + i2) * P3_CHUNK + cl] : p3_czero<ZT>();
       }
 #pragma unroll
       for (int rr = 0; rr < P3_RT; rr++) {
         if (!above[rr]) continue;
-        const double *qr = Q + (size_t)(i1a + rr) * BW;
-        zd t = zmk(0.0, 0.0);
+        const T *qr = Q + (size_t)(i1a + rr) * BW;
+        ZT t = p3_czero<ZT>();
 #pragma unroll
-        for (int d = 0; d < BW; d++) zfmar(t, qr[d], win[rr + d]);
-        acc[rr] = zadd(acc[rr], t);
+        for (int d = 0; d < BW; d++) p3_cfmar(t, qr[d], win[rr + d]);
+        acc[rr] = p3_cadd(acc[rr], t);
       }
     }
   }
@@ -504,7 +501,7 @@ __global__ void __launch_bounds__(256) p3_farL(const double *const *Pt, const do
   for (int rr = 0; rr < P3_RT; rr++) {
     if (!(below[rr] || above[rr])) continue;
     const long o = (long)c * NN + (long)(i1a + rr) * N + i2;
-    Z[o] = zadd(Z[o], acc[rr]);
+    Z[o] = zadd(Z[o], p3_ctod(acc[rr]));
   }
 }
 
@@ -515,17 +512,18 @@ __global__ void __launch_bounds__(256) p3_farL(const double *const *Pt, const do
 // group order, so the result is deterministic.  The block list (input offset,
 // R slice) is staged in LDS first; two accumulators split the tap chain.
 #define P3_NG4 4
-template <int S>
-__global__ void __launch_bounds__(256 * P3_NG4) p3_near(const double *const *Rn, const int *blk_term,
+template <int S, class T>
+__global__ void __launch_bounds__(256 * P3_NG4) p3_near(const T *const *Rn, const int *blk_term,
                                                         const int *cb_ptr, const int *cb_blk, const int2 *nr,
-                                                        const zd *Ym, zd *Z, int C, int N, int nchunk,
-                                                        const int *state)
+                                                        const typename p3_cx<T>::type *Ym, zd *Z, int C, int N,
+                                                        int nchunk, const int *state)
 {
+  typedef typename p3_cx<T>::type ZT;
   if (state[0]) return;
   constexpr int BW = 2 * S + 1;
-  __shared__ const double *sR[P3_MAXB];
+  __shared__ const T *sR[P3_MAXB];
   __shared__ long sY[P3_MAXB];
-  __shared__ zd part[P3_NG4][256];
+  __shared__ ZT part[P3_NG4][256];
   const int chunk = blockIdx.x % nchunk, i1 = blockIdx.x / nchunk;
   const long NN = (long)N * N;
   const int b0 = cb_ptr[chunk], nb = cb_ptr[chunk + 1] - b0;
@@ -540,7 +538,7 @@ __global__ void __launch_bounds__(256 * P3_NG4) p3_near(const double *const *Rn,
   const int2 r = nr[i1];
   const int i2 = r.x + w;
   const bool ok = i2 <= r.y;
-  zd a0 = zmk(0.0, 0.0), a1 = zmk(0.0, 0.0);
+  ZT a0 = p3_czero<ZT>(), a1 = p3_czero<ZT>();
   if (ok) {
     // taps outside 0 <= j2 < N are not loaded (select, no branch), so all 13
     // loads of a row issue together; the mixed input is only written where
@@ -552,34 +550,34 @@ __global__ void __launch_bounds__(256 * P3_NG4) p3_near(const double *const *Rn,
       jv[d2] = j2 >= 0 && j2 < N;
     }
     for (int k = grp; k < nb; k += P3_NG4) {
-      const double *R = sR[k];
-      const zd *Yb = Ym + sY[k] + cl + (long)(i2 - S) * P3_CHUNK;
+      const T *R = sR[k];
+      const ZT *Yb = Ym + sY[k] + cl + (long)(i2 - S) * P3_CHUNK;
       for (int d1 = 0; d1 < BW; d1++) {
         const int j1 = i1 + d1 - S;
         if (j1 < 0 || j1 >= N) continue;
-        const zd *Yr = Yb + (long)j1 * N * P3_CHUNK;
-        const double *Rr = R + ((size_t)d1 * P3_NW + w) * BW;
-        zd y[BW];
+        const ZT *Yr = Yb + (long)j1 * N * P3_CHUNK;
+        const T *Rr = R + ((size_t)d1 * P3_NW + w) * BW;
+        ZT y[BW];
 #pragma unroll
-        for (int d2 = 0; d2 < BW; d2++) y[d2] = jv[d2] ? Yr[(long)d2 * P3_CHUNK] : zmk(0.0, 0.0);
+        for (int d2 = 0; d2 < BW; d2++) y[d2] = jv[d2] ? Yr[(long)d2 * P3_CHUNK] : p3_czero<ZT>();
 #pragma unroll
         for (int d2 = 0; d2 < BW; d2++) {
-          if (d2 & 1) zfmar(a1, Rr[d2], y[d2]);
-          else zfmar(a0, Rr[d2], y[d2]);
+          if (d2 & 1) p3_cfmar(a1, Rr[d2], y[d2]);
+          else p3_cfmar(a0, Rr[d2], y[d2]);
         }
       }
     }
   }
-  part[grp][t] = zadd(a0, a1);
+  part[grp][t] = p3_cadd(a0, a1);
   __syncthreads();
   if (grp != 0 || !ok) return;
-  zd v = part[0][t];
+  ZT v = part[0][t];
 #pragma unroll
-  for (int g = 1; g < P3_NG4; g++) v = zadd(v, part[g][t]);
+  for (int g = 1; g < P3_NG4; g++) v = p3_cadd(v, part[g][t]);
   const int c = chunk * P3_CHUNK + cl;
   if (c < C) {
     const long o = (long)c * NN + (long)i1 * N + i2;
-    Z[o] = zadd(Z[o], v);
+    Z[o] = zadd(Z[o], p3_ctod(v));
   }
 }
 
@@ -831,16 +829,18 @@ static void p3_free_plan(pp_pamp3 *p)
 {
   void *bufs[] = {p->dmats, p->rrow, p->lrow, p->mrow, p->rent, p->lent, p->ment, p->cb_ptr, p->cb_blk,
                   p->blk_term, (void *)p->dR, p->X, p->Ym, p->nr, (void *)p->dRn, (void *)p->dPt, (void *)p->dQt,
-                  p->Tq, p->Tp};
+                  (void *)p->fRnt, (void *)p->fPt, (void *)p->fQt, p->Tq, p->Tp};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &t : p->mterms) {
-    void *tb[] = {t.dRn, t.dP, t.dQ};
+    void *tb[] = {t.dRn, t.dP, t.dQ, t.fRn, t.fP, t.fQ};
     for (void *b : tb)
       if (b) (void)hipFree(b);
     t.dRn = t.dP = t.dQ = nullptr;
+    t.fRn = t.fP = t.fQ = nullptr;
   }
   p->nr = nullptr; p->dRn = p->dPt = p->dQt = nullptr; p->Tq = p->Tp = nullptr; p->split = false;
+  p->fRnt = p->fPt = p->fQt = nullptr;
   p->dmats = nullptr; p->rrow = p->lrow = p->mrow = nullptr; p->rent = p->lent = p->ment = nullptr;
   p->cb_ptr = p->cb_blk = p->blk_term = nullptr; p->dR = nullptr; p->X = nullptr; p->Ym = nullptr;
 }
@@ -859,6 +859,7 @@ extern "C" int pp_pamp3_create(int C, int N, int m, int reorth, const pp_overlap
   p->dmats = nullptr; p->rrow = p->lrow = p->mrow = nullptr; p->rent = p->lent = p->ment = nullptr;
   p->cb_ptr = p->cb_blk = p->blk_term = nullptr; p->dR = nullptr; p->X = nullptr; p->Ym = nullptr;
   p->nr = nullptr; p->dRn = p->dPt = p->dQt = nullptr; p->Tq = p->Tp = nullptr; p->split = false;
+  p->fRnt = p->fPt = p->fQt = nullptr; p->mixed = 0;
   p->K = 0; p->nblk = 0; p->nchunk = (C + P3_CHUNK - 1) / P3_CHUNK;
   p->scal = p->diag = nullptr; p->scal_cap = p->diag_cap = 0;
   p->timing = 0; p->ee_ms = 0.0; p->ee_count = 0;
@@ -947,6 +948,7 @@ extern "C" int pp_pamp3_add_multipole_term(pp_pamp3 *p, int P, const int *pairs,
   t.slot = slot;
   t.R = nullptr;
   t.dRn = t.dP = t.dQ = nullptr;
+  t.fRn = t.fP = t.fQ = nullptr;
   if (Pband) {
     t.Rh.assign(R, R + cnt);
     t.Pl.assign(Pband, Pband + (size_t)p->N * p->bw);
@@ -1108,6 +1110,8 @@ static int p3_build(pp_pamp3 *p)
       nrh[i1] = make_int2(lo, hi);
     }
     std::vector<const double *> rn, pp, qq;
+    std::vector<const float *> frn, fpp, fqq;
+    auto tof = [](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
     for (auto &t : p->mterms) {
       std::vector<double> Rn((size_t)N * bw * P3_NW * bw, 0.0);
       for (int i1 = 0; i1 < N; i1++)
@@ -1116,14 +1120,18 @@ static int p3_build(pp_pamp3 *p)
             for (int d2 = 0; d2 < bw; d2++)
               Rn[(((size_t)i1 * bw + d1) * P3_NW + w) * bw + d2] =
                   t.Rh[(((size_t)i1 * bw + d1) * N + nrh[i1].x + w) * bw + d2];
-      if ((rc = p3_upload(&t.dRn, Rn)) || (rc = p3_upload(&t.dP, t.Pl)) || (rc = p3_upload(&t.dQ, t.Ql))) return rc;
+      if ((rc = p3_upload(&t.dRn, Rn)) || (rc = p3_upload(&t.dP, t.Pl)) || (rc = p3_upload(&t.dQ, t.Ql)) ||
+          (rc = p3_upload(&t.fRn, tof(Rn))) || (rc = p3_upload(&t.fP, tof(t.Pl))) || (rc = p3_upload(&t.fQ, tof(t.Ql))))
+        return rc;
       rn.push_back(t.dRn); pp.push_back(t.dP); qq.push_back(t.dQ);
+      frn.push_back(t.fRn); fpp.push_back(t.fP); fqq.push_back(t.fQ);
     }
     for (int ch = 0; ch < p->nchunk; ch++)
       if (cbp[ch + 1] - cbp[ch] > P3_MAXB) return PP_EUNSUP;
     p->nrh = nrh;
     if ((rc = p3_upload(&p->nr, nrh)) || (rc = p3_upload(&p->dRn, rn)) || (rc = p3_upload(&p->dPt, pp)) ||
-        (rc = p3_upload(&p->dQt, qq)))
+        (rc = p3_upload(&p->dQt, qq)) || (rc = p3_upload(&p->fRnt, frn)) || (rc = p3_upload(&p->fPt, fpp)) ||
+        (rc = p3_upload(&p->fQt, fqq)))
       return rc;
     const size_t tb = sizeof(zd) * (size_t)p->nblk * N * N * P3_CHUNK;
     if (hipMalloc(&p->Tq, tb) != hipSuccess || hipMalloc(&p->Tp, tb) != hipSuccess) return PP_EHIP;
@@ -1156,15 +1164,25 @@ static int p3_apply_raw(pp_pamp3 *p, const zd *y, zd *z, const double *scal, hip
       if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return PP_EHIP;
       (void)hipEventRecord(e0, st);
     }
-    if (p->split) {
+    const int ntile = (N + P3_RT - 1) / P3_RT, ngrp = (N + 15) / 16;
+    if (p->split && p->mixed) {
+      // FP32 stage: the complex64 intermediates use the first half of the FP64 buffers
+      float2 *Ym = (float2 *)p->Ym, *Tq = (float2 *)p->Tq, *Tp = (float2 *)p->Tp;
+      const size_t lds = sizeof(float2) * (size_t)(N + 12) * P3_CHUNK;
+      hipLaunchKernelGGL((p3_mixT<6, float>), dim3(p->nblk * N), dim3(256), lds, st, p->mrow, p->ment, y, Ym, p->fPt,
+                         p->fQt, p->blk_term, p->nr, Tq, Tp, N, scal, p->state);
+      hipLaunchKernelGGL((p3_near<6, float>), dim3(p->nchunk * N), dim3(256 * P3_NG4), 0, st, p->fRnt, p->blk_term,
+                         p->cb_ptr, p->cb_blk, p->nr, Ym, z, C, N, p->nchunk, p->state);
+      hipLaunchKernelGGL((p3_farL<6, float>), dim3(p->nchunk * ntile * ngrp), dim3(256), 0, st, p->fPt, p->fQt,
+                         p->blk_term, p->cb_ptr, p->cb_blk, p->nr, Tq, Tp, z, C, N, p->nchunk, p->state);
+    } else if (p->split) {
       const size_t lds = sizeof(zd) * (size_t)(N + 12) * P3_CHUNK;
-      hipLaunchKernelGGL(p3_mixT<6>, dim3(p->nblk * N), dim3(256), lds, st, p->mrow, p->ment, y, p->Ym, p->dPt,
-                         p->dQt, p->blk_term, p->nr, p->Tq, p->Tp, N, scal, p->state);
-      hipLaunchKernelGGL(p3_near<6>, dim3(p->nchunk * N), dim3(256 * P3_NG4), 0, st, p->dRn, p->blk_term, p->cb_ptr,
-                         p->cb_blk, p->nr, p->Ym, z, C, N, p->nchunk, p->state);
-      const int ntile = (N + P3_RT - 1) / P3_RT, ngrp = (N + 15) / 16;
-      hipLaunchKernelGGL(p3_farL<6>, dim3(p->nchunk * ntile * ngrp), dim3(256), 0, st, p->dPt, p->dQt, p->blk_term,
-                         p->cb_ptr, p->cb_blk, p->nr, p->Tq, p->Tp, z, C, N, p->nchunk, p->state);
+      hipLaunchKernelGGL((p3_mixT<6, double>), dim3(p->nblk * N), dim3(256), lds, st, p->mrow, p->ment, y, p->Ym,
+                         p->dPt, p->dQt, p->blk_term, p->nr, p->Tq, p->Tp, N, scal, p->state);
+      hipLaunchKernelGGL((p3_near<6, double>), dim3(p->nchunk * N), dim3(256 * P3_NG4), 0, st, p->dRn, p->blk_term,
+                         p->cb_ptr, p->cb_blk, p->nr, p->Ym, z, C, N, p->nchunk, p->state);
+      hipLaunchKernelGGL((p3_farL<6, double>), dim3(p->nchunk * ntile * ngrp), dim3(256), 0, st, p->dPt, p->dQt,
+                         p->blk_term, p->cb_ptr, p->cb_blk, p->nr, p->Tq, p->Tp, z, C, N, p->nchunk, p->state);
     } else {
       hipLaunchKernelGGL(p3_mix, dim3(p->nblk * N), dim3(256), 0, st, p->mrow, p->ment, y, p->Ym, N, scal, p->state);
       hipLaunchKernelGGL(p3_ee<6>, dim3(groups * 8 * p->nchunk), dim3(P3_EE_THREADS), 0, st, p->dR, p->blk_term,
@@ -1203,6 +1221,8 @@ static int p3_ensure(pp_pamp3 *p, int nsteps, const double *scal)
     int rc = p3_build(p);
     if (rc) return rc;
   }
+  // the mixed form needs the near/far split (radial supports and P_l, Q_l of every multipole term)
+  if (p->mixed && p->nblk > 0 && !p->split) return PP_EUNSUP;
   const int ns = p->nslots > 0 ? p->nslots : 1;
   if (nsteps * ns > p->scal_cap) {
     if (p->scal) (void)hipFree(p->scal);
@@ -1299,6 +1319,13 @@ static void p3_collect(pp_pamp3 *p)
     (void)hipEventDestroy(p->ev[i + 1]);
   }
   p->ev.clear();
+}
+
+extern "C" int pp_pamp3_set_mixed(pp_pamp3 *p, int mixed)
+{
+  if (!p) return PP_EARG;
+  p->mixed = mixed ? 1 : 0;
+  return PP_OK;
 }
 
 extern "C" int pp_pamp3_set_timing(pp_pamp3 *p, int enable)

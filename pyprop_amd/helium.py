@@ -401,6 +401,11 @@ class Pamp3Engine:
     def set_timing(self, enable=True):
         nat.check(nat.lib().pp_pamp3_set_timing(self.h, int(bool(enable))), "pp_pamp3_set_timing")
 
+    def set_mixed(self, enable=True):
+        """Mixed form: the multipole terms in FP32 with complex64
+        intermediates (everything else FP64); needs the near/far split."""
+        nat.check(nat.lib().pp_pamp3_set_mixed(self.h, int(bool(enable))), "pp_pamp3_set_mixed")
+
     def kernel_time(self):
         """(summed ms, launches) of the timed multipole contraction launches."""
         ms = ctypes.c_double()

@@ -82,9 +82,10 @@ class PamPropagator(PropagatorBase):
         if hasattr(section, "krylov_double_orthogonalization"):
             self.DoubleOrthogonalization = bool(section.krylov_double_orthogonalization)
         # extension key (no reference counterpart): "fp64" (default, the
-        # drop-in) or "mixed" (time-dependent potentials applied in complex
-        # FP32; the static Hamiltonian, Gram-Schmidt, overlap solves and Pade
-        # stay FP64)
+        # drop-in) or "mixed" (one-electron states: time-dependent potentials
+        # applied in complex FP32; two-electron states: the electron-electron
+        # multipole terms in FP32; the rest of the Hamiltonian, Gram-Schmidt,
+        # overlap solves and Pade stay FP64)
         if hasattr(section, "matvec_precision"):
             prec = str(section.matvec_precision).lower()
             if prec not in ("fp64", "mixed"):
@@ -132,8 +133,6 @@ class PamPropagator(PropagatorBase):
         factored potentials go to the rank-3 engine; each time-dependent
         potential gets its own time slot."""
         from .helium import CoupledTensorPotential, Pamp3Engine, radial_supports
-        if self.MatvecPrecision != "fp64":
-            raise NotImplementedError("matvec_precision = mixed is not on the rank-3 engine")
         if not all(isinstance(p, CoupledTensorPotential) for p in pots):
             raise NotImplementedError("rank-3 states run on the two-electron engine only")
         C, N, N2 = repr.GetFullShape()
@@ -149,6 +148,11 @@ class PamPropagator(PropagatorBase):
                 slot = len(self._slots)
                 self._slots.append(pot)
             eng.add_potential(pot, slot)
+        if self.MatvecPrecision == "mixed":
+            # two-electron form: the electron-electron multipole terms (the
+            # bulk of the matvec) in FP32; the one-electron and laser terms
+            # keep FP64
+            eng.set_mixed(True)
         self._pots = pots
         return eng
 
