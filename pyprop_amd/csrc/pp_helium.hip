@@ -398,18 +398,27 @@ __global__ void __launch_bounds__(256) p3_mixT(const int *row, const p3_ent *ent
   __syncthreads();
   const T *P = Pt[blk_term[b]], *Q = Qt[blk_term[b]];
   Z *oq = Tq + ((long)b * NN + (long)j1 * N) * P3_CHUNK, *op = Tp + ((long)b * NN + (long)j1 * N) * P3_CHUNK;
+  // p3_farL reads row j1 of Tq only for outputs i1 <= j1 + S with
+  // i2 > nhi[i1] (so i2 > nhi[j1 - S], nhi non-decreasing) and of Tp only for
+  // i1 >= j1 - S with i2 < nlo[i1] (so i2 < nlo[j1 + S]): the rest of each row
+  // is neither computed nor stored.
+  const int qlo = nr[j1 - S > 0 ? j1 - S : 0].y, phi = nr[j1 + S < N - 1 ? j1 + S : N - 1].x;
   for (int e = threadIdx.x; e < N * P3_CHUNK; e += 256) {
     const int i2 = e / P3_CHUNK, c2 = e & (P3_CHUNK - 1);
-    const T *qr = Q + (size_t)i2 * BW, *pr = P + (size_t)i2 * BW;
-    Z tq = p3_czero<Z>(), tp = p3_czero<Z>();
+    if (i2 > qlo) {
+      const T *qr = Q + (size_t)i2 * BW;
+      Z tq = p3_czero<Z>();
 #pragma unroll
-    for (int d = 0; d < BW; d++) {
-      const Z v = rowb[(i2 + d) * P3_CHUNK + c2];
-      p3_cfmar(tq, qr[d], v);
-      p3_cfmar(tp, pr[d], v);
+      for (int d = 0; d < BW; d++) p3_cfmar(tq, qr[d], rowb[(i2 + d) * P3_CHUNK + c2]);
+      oq[e] = tq;
     }
-    oq[e] = tq;
-    op[e] = tp;
+    if (i2 < phi) {
+      const T *pr = P + (size_t)i2 * BW;
+      Z tp = p3_czero<Z>();
+#pragma unroll
+      for (int d = 0; d < BW; d++) p3_cfmar(tp, pr[d], rowb[(i2 + d) * P3_CHUNK + c2]);
+      op[e] = tp;
+    }
   }
 }
 
