@@ -258,7 +258,7 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (matrix and vector peaks are equal
 # HBM-side bytes of one multipole stage (the three kernels' FETCH_SIZE +
 # WRITE_SIZE per matvec, FETCH scaled x2 and WRITE x1 by a 512 MiB copy),
 # from the counter passes recorded in profiles/multipole.helium.md.
-HE_PMC_TRAFFIC = {"helium": 1629.9e6}
+HE_PMC_TRAFFIC = {"helium": 1392.4e6, "helium_mixed": 683.1e6}
 
 
 def run_helium(args):
@@ -339,6 +339,7 @@ def run_helium(args):
         mixed = {"value": args.steps / mx["wall"], "unit": "steps/s", "ms_per_step": 1e3 * mx["wall"] / args.steps,
                  "speedup_vs_fp64": wall / mx["wall"],
                  "multipole_stage_ms_avg": mx["ee_ms"] / max(mx["ee_launches"], 1),
+                 "multipole_stage_traffic": HE_PMC_TRAFFIC.get(args.workload + "_mixed"),
                  "ground_population_fp64": pop_ground, "ground_population_mixed": pop_mixed,
                  "population_error": abs(pop_ground - pop_mixed),
                  "norm_fp64": norm, "norm_mixed": mprob.psi.GetNorm(),
