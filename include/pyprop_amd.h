@@ -21,6 +21,7 @@ extern "C" {
 
 typedef struct pp_overlap pp_overlap;
 typedef struct pp_pamp pp_pamp;
+typedef struct pp_batch_graph pp_batch_graph;
 typedef struct pp_pamp3 pp_pamp3;
 typedef void *pp_stream;  /* hipStream_t */
 
@@ -130,6 +131,18 @@ int pp_expokit_diagnostics(pp_pamp *p, int nsteps, double *stats, pp_stream stre
  * stream.  Engines must share the B-spline order and the precision form. */
 int pp_pamp_batch_propagate(int ninst, pp_pamp *const *inst, void *const *psi, int nsteps, const double *dt,
                             const double *const *scal, pp_stream stream);
+/* The same batch recorded once as a HIP graph (no reference counterpart):
+ * per pack a kernel node that clears the barrier words and the batch kernel
+ * as a cooperative kernel node, chained in order.  The graph keeps psi[i] and
+ * the engines' buffers; each pp_pamp_batch_graph_launch uploads the replay's
+ * time factors (scal[i]: host nsteps x nterms_i) and replays all packs on the
+ * stream, advancing every psi[i] by nsteps steps.  A launch returns PP_EARG
+ * if an engine changed since recording (terms, precision, Pade scaling, or a
+ * launch that reallocated its buffers); destroy and record again then. */
+int pp_pamp_batch_graph_create(int ninst, pp_pamp *const *inst, void *const *psi, int nsteps, const double *dt,
+                               pp_batch_graph **out);
+int pp_pamp_batch_graph_launch(pp_batch_graph *g, const double *const *scal, pp_stream stream);
+int pp_pamp_batch_graph_destroy(pp_batch_graph *g);
 /* LDS bytes per workgroup a pAMP launch of an (L, N) problem with Krylov
  * dimension m, nslots coupled fibers per row, kd super-diagonals and solver
  * chunk length B needs (-1: invalid B).  No reference counterpart: the batch

@@ -94,6 +94,17 @@ class Batch:
             if p.Propagator.RenormalizeActive:
                 raise NotImplementedError("renormalisation between steps is not batched")
         self.Problems = list(problems)
+        self.Graph = None
+
+    def CaptureGraph(self, n):
+        """Record the packed launches of an n-step advance as one HIP graph;
+        later AdvanceSteps(n) calls replay it (other step counts launch
+        directly).  Changing an engine afterwards (terms, precision) makes the
+        replay raise; record again then."""
+        self.Graph = None
+        self.Graph = dev.BatchGraph([p.Propagator.Engine for p in self.Problems],
+                                    [p.psi.GetData() for p in self.Problems], n,
+                                    [p.TimeStep for p in self.Problems])
 
     def AdvanceSteps(self, n):
         if n <= 0:
@@ -108,7 +119,10 @@ class Batch:
             psis.append(p.psi.GetData())
             dts.append(p.TimeStep)
             scs.append(p.Propagator.TimeScalings(times))
-        dev.pamp_batch_propagate(engines, psis, n, dts, scs)
+        if self.Graph is not None and self.Graph.nsteps == n:
+            self.Graph.launch(scs, psis)
+        else:
+            dev.pamp_batch_propagate(engines, psis, n, dts, scs)
 
     def GetDiagnostics(self):
         return [p.Propagator.GetDiagnostics() for p in self.Problems]

@@ -112,6 +112,7 @@ struct pp_pamp {
   int fsplit;                              // workgroups per fiber: 0 automatic, 1 none, >= 2 fixed
   size_t bar_words, part_cap, sb_cap;      // allocated arrival words, partial-sum doubles, boundary values
   zd *sb;
+  unsigned gen;                            // bumped whenever a buffer or setting a recorded launch uses changes
 };
 
 // Barrier over n co-resident workgroups (member `me`; a whole instance, or
@@ -1294,6 +1295,7 @@ extern "C" int pp_pamp_create(int L, int N, int m, int reorth, const pp_overlap 
   p->V = p->y = p->Hm = p->cvec = nullptr;
   p->mixed = 0;
   p->normscale = 0;
+  p->gen = 0;
   p->part = p->diag = p->scal = nullptr;
   p->bar = nullptr; p->tlog = nullptr; p->timing = 0;
   p->fsplit = 0; p->sb = nullptr; p->sb_cap = 0;
@@ -1385,6 +1387,7 @@ static int pamp_add_factored(pp_pamp *p, int angular_storage, int P0, const int 
   p->bw = bw;
   p->s = s;
   p->dirty = true;
+  p->gen++;
   return PP_OK;
 }
 
@@ -1582,6 +1585,7 @@ static int pamp_upload_slots(pp_pamp *p)
   for (int a = 0; a < p->L; a++)
     for (size_t q = 0; q < p->rowcols[a].size(); q++) sc[(size_t)a * ns + q] = p->rowcols[a][q];
   if (p->slotcols) (void)hipFree(p->slotcols);
+  p->gen++;
   PP_HIP(hipMalloc(&p->slotcols, sc.size() * sizeof(int)));
   PP_HIP(hipMemcpy(p->slotcols, sc.data(), sc.size() * sizeof(int), hipMemcpyHostToDevice));
   p->nslots = ns;
@@ -1589,10 +1593,9 @@ static int pamp_upload_slots(pp_pamp *p)
   return PP_OK;
 }
 
-// Common launch set-up of the step kernels: uploads the time factors,
-// clears the barrier words and fills the kernel arguments.
-static int pamp_prepare(pp_pamp *p, void *psi, int nsteps, double dt_re, double dt_im, const double *scal,
-                        hipStream_t stream, pp_pamp_args &A)
+// Launch set-up of the step kernels without stream work: buffers for
+// nsteps steps and the kernel arguments.
+static int pamp_args(pp_pamp *p, void *psi, int nsteps, double dt_re, double dt_im, pp_pamp_args &A)
 {
   int rc = pamp_upload_slots(p);
   if (rc) return rc;
@@ -1600,16 +1603,16 @@ static int pamp_prepare(pp_pamp *p, void *psi, int nsteps, double dt_re, double 
   if (p->s != ov->kd) return PP_EUNSUP;   // radial bands must have the overlap's bandwidth
   if (nsteps * p->nterms > p->scal_cap) {
     if (p->scal) (void)hipFree(p->scal);
+    p->gen++;
     p->scal_cap = nsteps * p->nterms;
     PP_HIP(hipMalloc(&p->scal, sizeof(double) * p->scal_cap));
   }
   if (nsteps > p->diag_cap) {
     if (p->diag) (void)hipFree(p->diag);
+    p->gen++;
     p->diag_cap = nsteps;
     PP_HIP(hipMalloc(&p->diag, sizeof(double) * 6 * (size_t)nsteps));
   }
-  PP_HIP(hipMemcpyAsync(p->scal, scal, sizeof(double) * nsteps * p->nterms, hipMemcpyHostToDevice, stream));
-  PP_HIP(hipMemsetAsync(p->bar, 0, sizeof(pp_barrier) + sizeof(unsigned) * p->bar_words, stream));
   memset(&A, 0, sizeof(A));
   A.L = p->L; A.N = p->N; A.m = p->m; A.s = p->s; A.bw = p->bw; A.nterms = p->nterms;
   A.nsteps = nsteps; A.reorth = p->reorth; A.nslots = p->nslots; A.normscale = p->normscale;
@@ -1632,6 +1635,18 @@ static int pamp_prepare(pp_pamp *p, void *psi, int nsteps, double dt_re, double 
   A.psi = (zd *)psi; A.V = p->V; A.y = p->y; A.part = p->part; A.pstr = p->pstr;
   A.Hm = p->Hm; A.cvec = p->cvec; A.diag = p->diag; A.bar = p->bar;
   A.tlog = p->timing ? p->tlog : nullptr;
+  return PP_OK;
+}
+
+// Common launch set-up of the step kernels: pamp_args, then the time
+// factors uploaded and the barrier words cleared on the stream.
+static int pamp_prepare(pp_pamp *p, void *psi, int nsteps, double dt_re, double dt_im, const double *scal,
+                        hipStream_t stream, pp_pamp_args &A)
+{
+  int rc = pamp_args(p, psi, nsteps, dt_re, dt_im, A);
+  if (rc) return rc;
+  PP_HIP(hipMemcpyAsync(p->scal, scal, sizeof(double) * nsteps * p->nterms, hipMemcpyHostToDevice, stream));
+  PP_HIP(hipMemsetAsync(p->bar, 0, sizeof(pp_barrier) + sizeof(unsigned) * p->bar_words, stream));
   return PP_OK;
 }
 
@@ -1685,12 +1700,14 @@ static int pamp_split_buffers(pp_pamp *p, int Q)
 {
   const size_t nw = (size_t)p->L * Q;
   if (nw * p->pstr > p->part_cap) {
+    p->gen++;
     (void)hipFree(p->part);
     p->part = nullptr;
     PP_HIP(hipMalloc(&p->part, sizeof(double) * nw * p->pstr));
     p->part_cap = nw * p->pstr;
   }
   if (2 * nw > p->bar_words) {
+    p->gen++;
     (void)hipFree(p->bar);
     p->bar = nullptr;
     PP_HIP(hipMalloc(&p->bar, sizeof(pp_barrier) + sizeof(unsigned) * 2 * nw));
@@ -1758,26 +1775,31 @@ extern "C" int pp_pamp_set_fiber_split(pp_pamp *p, int q)
 // a batch larger than the GPU holds at once becomes several launches on the
 // stream.  dt: ninst x 2 (re, im); scal[i]: host nsteps x nterms_i.  All
 // instances need the same B-spline order and precision form.
-extern "C" int pp_pamp_batch_propagate(int ninst, pp_pamp *const *inst, void *const *psi, int nsteps,
-                                       const double *dt, const double *const *scal, hipStream_t stream)
+struct pp_batch_pack {
+  int i0, i1, wgs;         // instances [i0, i1), workgroups
+  size_t lds;              // dynamic LDS of the launch (the group's largest)
+};
+
+// Arguments of every instance and the packing of the batch (no stream work).
+static int batch_plan(int ninst, pp_pamp *const *inst, void *const *psi, int nsteps, const double *dt,
+                      std::vector<pp_pamp_args> &args, const void *&fn, std::vector<pp_batch_pack> &packs)
 {
-  if (ninst < 0 || (ninst > 0 && (!inst || !psi || !dt || !scal)) || nsteps < 0) return PP_EARG;
-  if (ninst == 0 || nsteps == 0) return PP_OK;
-  const int kd = inst[0]->ov->kd;
-  const bool mixed = inst[0]->mixed != 0;
+  if (ninst <= 0 || !inst || !psi || !dt || nsteps <= 0) return PP_EARG;
+  const int kd = inst[0] ? inst[0]->ov->kd : 0;
+  const bool mixed = inst[0] && inst[0]->mixed != 0;
   for (int i = 0; i < ninst; i++) {
-    if (!inst[i] || !psi[i] || inst[i]->nterms == 0 || !scal[i]) return PP_EARG;
+    if (!inst[i] || !psi[i] || inst[i]->nterms == 0) return PP_EARG;
     if (inst[i]->ov->kd != kd || (inst[i]->mixed != 0) != mixed) return PP_EUNSUP;
   }
-  std::vector<pp_pamp_args> args(ninst);
+  args.assign(ninst, pp_pamp_args());
   std::vector<size_t> lds(ninst);
   for (int i = 0; i < ninst; i++) {
-    int rc = pamp_prepare(inst[i], psi[i], nsteps, dt[2 * i], dt[2 * i + 1], scal[i], stream, args[i]);
+    int rc = pamp_args(inst[i], psi[i], nsteps, dt[2 * i], dt[2 * i + 1], args[i]);
     if (rc) return rc;
     lds[i] = pamp_lds_bytes(inst[i], args[i].padN);
     if (lds[i] > 160 * 1024) return PP_EUNSUP;
   }
-  const void *fn = nullptr;
+  fn = nullptr;
   PP_KD_SWITCH(kd, { fn = mixed ? (const void *)pp_pamp_batch_kernel<KD_, true> : (const void *)pp_pamp_batch_kernel<KD_, false>; })
   if (!fn) return PP_EUNSUP;
   int dev = 0, ncu = 0, reg_per_cu = 0;
@@ -1786,6 +1808,7 @@ extern "C" int pp_pamp_batch_propagate(int ninst, pp_pamp *const *inst, void *co
   // workgroups per CU: the register limit from the runtime, the LDS limit
   // (160 KB per CU) from the group's largest allocation
   PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&reg_per_cu, fn, PP_THREADS, 0));
+  packs.clear();
   int i0 = 0;
   while (i0 < ninst) {
     // grow the group while all its workgroups fit on the GPU at once
@@ -1800,23 +1823,179 @@ extern "C" int pp_pamp_batch_propagate(int ninst, pp_pamp *const *inst, void *co
       i1++;
     }
     if (i1 == i0) return PP_EUNSUP;   // one instance alone does not fit
-    std::vector<int2> map;
-    map.reserve(wgs);
-    for (int i = i0; i < i1; i++)
-      for (int a = 0; a < inst[i]->L; a++) map.push_back(make_int2(i - i0, a));
-    pp_pamp_args *dargs = nullptr;
-    int2 *dmap = nullptr;
-    PP_HIP(hipMallocAsync((void **)&dargs, sizeof(pp_pamp_args) * (i1 - i0), stream));
-    PP_HIP(hipMallocAsync((void **)&dmap, sizeof(int2) * map.size(), stream));
-    PP_HIP(hipMemcpyAsync(dargs, args.data() + i0, sizeof(pp_pamp_args) * (i1 - i0), hipMemcpyHostToDevice, stream));
-    PP_HIP(hipMemcpyAsync(dmap, map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice, stream));
-    PP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lmax));
-    void *kargs[] = {&dargs, &dmap};
-    PP_HIP(hipLaunchCooperativeKernel(fn, dim3(wgs), dim3(PP_THREADS), kargs, (unsigned)lmax, stream));
-    PP_HIP(hipFreeAsync(dargs, stream));
-    PP_HIP(hipFreeAsync(dmap, stream));
+    packs.push_back({i0, i1, wgs, lmax});
     i0 = i1;
   }
+  return PP_OK;
+}
+
+// Workgroup -> (instance within the pack, fiber) map of one pack.
+static std::vector<int2> batch_map(pp_pamp *const *inst, const pp_batch_pack &pk)
+{
+  std::vector<int2> map;
+  map.reserve(pk.wgs);
+  for (int i = pk.i0; i < pk.i1; i++)
+    for (int a = 0; a < inst[i]->L; a++) map.push_back(make_int2(i - pk.i0, a));
+  return map;
+}
+
+extern "C" int pp_pamp_batch_propagate(int ninst, pp_pamp *const *inst, void *const *psi, int nsteps,
+                                       const double *dt, const double *const *scal, hipStream_t stream)
+{
+  if (ninst < 0 || (ninst > 0 && (!inst || !psi || !dt || !scal)) || nsteps < 0) return PP_EARG;
+  if (ninst == 0 || nsteps == 0) return PP_OK;
+  for (int i = 0; i < ninst; i++)
+    if (!scal[i]) return PP_EARG;
+  std::vector<pp_pamp_args> args;
+  std::vector<pp_batch_pack> packs;
+  const void *fn = nullptr;
+  int rc = batch_plan(ninst, inst, psi, nsteps, dt, args, fn, packs);
+  if (rc) return rc;
+  for (int i = 0; i < ninst; i++) {
+    pp_pamp *p = inst[i];
+    PP_HIP(hipMemcpyAsync(p->scal, scal[i], sizeof(double) * nsteps * p->nterms, hipMemcpyHostToDevice, stream));
+    PP_HIP(hipMemsetAsync(p->bar, 0, sizeof(pp_barrier) + sizeof(unsigned) * p->bar_words, stream));
+  }
+  for (const pp_batch_pack &pk : packs) {
+    const std::vector<int2> map = batch_map(inst, pk);
+    pp_pamp_args *dargs = nullptr;
+    int2 *dmap = nullptr;
+    PP_HIP(hipMallocAsync((void **)&dargs, sizeof(pp_pamp_args) * (pk.i1 - pk.i0), stream));
+    PP_HIP(hipMallocAsync((void **)&dmap, sizeof(int2) * map.size(), stream));
+    PP_HIP(hipMemcpyAsync(dargs, args.data() + pk.i0, sizeof(pp_pamp_args) * (pk.i1 - pk.i0), hipMemcpyHostToDevice,
+                          stream));
+    PP_HIP(hipMemcpyAsync(dmap, map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice, stream));
+    PP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pk.lds));
+    void *kargs[] = {&dargs, &dmap};
+    PP_HIP(hipLaunchCooperativeKernel(fn, dim3(pk.wgs), dim3(PP_THREADS), kargs, (unsigned)pk.lds, stream));
+    PP_HIP(hipFreeAsync(dargs, stream));
+    PP_HIP(hipFreeAsync(dmap, stream));
+  }
+  return PP_OK;
+}
+
+// ------------------------------------------------------------------ batch graph
+// The packed launches of pp_pamp_batch_propagate recorded once as a HIP
+// graph: per pack, a kernel node that clears the pack's barrier words (a
+// kernel, not a memset node) and the batch kernel as a cooperative kernel
+// node, the packs chained in order.  The graph holds the instances' buffers
+// and psi pointers; a replay reads the time factors from the instances'
+// device buffers, which pp_pamp_batch_graph_launch refreshes first.
+__global__ void __launch_bounds__(256) pp_batch_bar_reset(const pp_pamp_args *args, const int *nwords, int ninst)
+{
+  for (int i = blockIdx.x; i < ninst; i += gridDim.x) {
+    unsigned *w = (unsigned *)args[i].bar;
+    for (int e = threadIdx.x; e < nwords[i]; e += 256) w[e] = 0u;
+  }
+}
+
+struct pp_batch_graph {
+  int ninst, nsteps;
+  std::vector<pp_pamp *> inst;
+  std::vector<unsigned> gen;   // each instance's generation when recorded
+  std::vector<void *> dbuf;    // device argument blocks, maps and word counts
+  const void *fn;              // the batch kernel and its largest dynamic LDS
+  size_t lds;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+extern "C" int pp_pamp_batch_graph_destroy(pp_batch_graph *g)
+{
+  if (!g) return PP_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  for (void *b : g->dbuf)
+    if (b) (void)hipFree(b);
+  delete g;
+  return PP_OK;
+}
+
+extern "C" int pp_pamp_batch_graph_create(int ninst, pp_pamp *const *inst, void *const *psi, int nsteps,
+                                          const double *dt, pp_batch_graph **out)
+{
+  if (!out) return PP_EARG;
+  *out = nullptr;
+  std::vector<pp_pamp_args> args;
+  std::vector<pp_batch_pack> packs;
+  const void *fn = nullptr;
+  int rc = batch_plan(ninst, inst, psi, nsteps, dt, args, fn, packs);
+  if (rc) return rc;
+  pp_batch_graph *g = new pp_batch_graph();
+  g->ninst = ninst;
+  g->nsteps = nsteps;
+  g->graph = nullptr;
+  g->exec = nullptr;
+  g->fn = fn;
+  g->lds = 0;
+  for (const pp_batch_pack &pk : packs) g->lds = std::max(g->lds, pk.lds);
+  g->inst.assign(inst, inst + ninst);
+  for (int i = 0; i < ninst; i++) g->gen.push_back(inst[i]->gen);
+  auto fail = [&](int code) { pp_pamp_batch_graph_destroy(g); return code; };
+  auto upload = [&](const void *src, size_t bytes, void **dst) {
+    if (hipMalloc(dst, bytes) != hipSuccess) return false;
+    g->dbuf.push_back(*dst);
+    return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  if (hipGraphCreate(&g->graph, 0) != hipSuccess) return fail(PP_EHIP);
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g->lds) != hipSuccess)
+    return fail(PP_EHIP);
+  hipGraphNode_t prev = nullptr;
+  for (const pp_batch_pack &pk : packs) {
+    const int np = pk.i1 - pk.i0;
+    const std::vector<int2> map = batch_map(inst, pk);
+    std::vector<int> nwords(np);
+    for (int i = 0; i < np; i++)
+      nwords[i] = (int)((sizeof(pp_barrier) + sizeof(unsigned) * inst[pk.i0 + i]->bar_words) / sizeof(unsigned));
+    void *dargs = nullptr, *dmap = nullptr, *dnw = nullptr;
+    if (!upload(args.data() + pk.i0, sizeof(pp_pamp_args) * np, &dargs) ||
+        !upload(map.data(), sizeof(int2) * map.size(), &dmap) || !upload(nwords.data(), sizeof(int) * np, &dnw))
+      return fail(PP_EHIP);
+    hipKernelNodeParams rp = {};
+    void *rargs[] = {&dargs, &dnw, (void *)&np};
+    rp.func = (void *)pp_batch_bar_reset;
+    rp.gridDim = dim3(np < 1024 ? np : 1024);
+    rp.blockDim = dim3(256);
+    rp.sharedMemBytes = 0;
+    rp.kernelParams = rargs;
+    hipGraphNode_t rn = nullptr, kn = nullptr;
+    if (hipGraphAddKernelNode(&rn, g->graph, prev ? &prev : nullptr, prev ? 1 : 0, &rp) != hipSuccess)
+      return fail(PP_EHIP);
+    hipKernelNodeParams kp = {};
+    void *kargs[] = {&dargs, &dmap};
+    kp.func = (void *)fn;
+    kp.gridDim = dim3(pk.wgs);
+    kp.blockDim = dim3(PP_THREADS);
+    kp.sharedMemBytes = (unsigned)pk.lds;
+    kp.kernelParams = kargs;
+    if (hipGraphAddKernelNode(&kn, g->graph, &rn, 1, &kp) != hipSuccess) return fail(PP_EHIP);
+    hipKernelNodeAttrValue coop = {};
+    coop.cooperative = 1;
+    if (hipGraphKernelNodeSetAttribute(kn, hipKernelNodeAttributeCooperative, &coop) != hipSuccess)
+      return fail(PP_EUNSUP);
+    prev = kn;
+  }
+  if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) return fail(PP_EHIP);
+  *out = g;
+  return PP_OK;
+}
+
+// One replay: every instance advances the recorded nsteps steps from its
+// current psi.  scal[i]: host nsteps x nterms_i time factors of the replay.
+// PP_EARG if an instance changed since the graph was recorded (new terms,
+// precision or scaling mode, or a launch that grew its buffers).
+extern "C" int pp_pamp_batch_graph_launch(pp_batch_graph *g, const double *const *scal, hipStream_t stream)
+{
+  if (!g || !g->exec || !scal) return PP_EARG;
+  for (int i = 0; i < g->ninst; i++)
+    if (!scal[i] || g->inst[i]->gen != g->gen[i] || g->inst[i]->dirty) return PP_EARG;
+  for (int i = 0; i < g->ninst; i++) {
+    pp_pamp *p = g->inst[i];
+    PP_HIP(hipMemcpyAsync(p->scal, scal[i], sizeof(double) * g->nsteps * p->nterms, hipMemcpyHostToDevice, stream));
+  }
+  // the stream form sets this attribute per launch; restore the graph's largest
+  PP_HIP(hipFuncSetAttribute(g->fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g->lds));
+  PP_HIP(hipGraphLaunch(g->exec, stream));
   return PP_OK;
 }
 
@@ -1892,6 +2071,7 @@ extern "C" int pp_pamp_set_pade_scaling(pp_pamp *p, int mode)
 {
   if (!p || mode < 0 || mode > 1) return PP_EARG;
   p->normscale = mode;
+  p->gen++;
   return PP_OK;
 }
 
@@ -1903,6 +2083,7 @@ extern "C" int pp_pamp_set_mixed(pp_pamp *p, unsigned term_mask)
   if (!p) return PP_EARG;
   if (term_mask >> PP_MAXTERMS) return PP_EARG;
   p->mixed = term_mask;
+  p->gen++;
   return PP_OK;
 }
 

@@ -203,6 +203,43 @@ class PampEngine:
             self.h = None
 
 
+class BatchGraph:
+    """The packed launches of a batch recorded as one HIP graph
+    (pp_pamp_batch_graph_*): launch(scalings) advances every psi by the
+    recorded nsteps.  The graph keeps the psi tensors and engines alive and
+    refuses a replay if a psi tensor was replaced."""
+
+    def __init__(self, engines, psis, nsteps, dts):
+        n = len(engines)
+        if not (len(psis) == len(dts) == n) or n == 0:
+            raise ValueError("engines, psis and dts must have the same non-zero length")
+        self.engines, self.psis, self.nsteps = list(engines), list(psis), int(nsteps)
+        self._ptrs = [ptr(p).value for p in psis]
+        H = (ctypes.c_void_p * n)(*[e.h.value for e in engines])
+        Pp = (ctypes.c_void_p * n)(*self._ptrs)
+        dt = np.ascontiguousarray([[complex(d).real, complex(d).imag] for d in dts], dtype=np.float64)
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().pp_pamp_batch_graph_create(n, H, Pp, self.nsteps, nat.dptr(dt), ctypes.byref(h)),
+                  "pp_pamp_batch_graph_create")
+        self.h = h
+
+    def launch(self, scalings, psis=None):
+        if psis is not None and [ptr(p).value for p in psis] != self._ptrs:
+            raise RuntimeError("a wavefunction buffer changed since the graph was recorded")
+        n = len(self.engines)
+        keep = [np.ascontiguousarray(np.asarray(sc, dtype=np.float64).reshape(self.nsteps, len(e.terms)))
+                for e, sc in zip(self.engines, scalings)]
+        S = (ctypes.POINTER(ctypes.c_double) * n)(*[nat.dptr(k) for k in keep])
+        nat.check(nat.lib().pp_pamp_batch_graph_launch(self.h, S, stream_ptr()), "pp_pamp_batch_graph_launch")
+        for e in self.engines:
+            e._last = self.nsteps
+
+    def __del__(self):
+        if getattr(self, "h", None) and nat._lib is not None:
+            nat._lib.pp_pamp_batch_graph_destroy(self.h)
+            self.h = None
+
+
 def pamp_batch_propagate(engines, psis, nsteps, dts, scalings):
     """nsteps steps of many independent engines in packed cooperative
     launches (pp_pamp_batch_propagate).  scalings[i]: (nsteps, nterms_i)."""

@@ -246,6 +246,53 @@ def test_batch_matches_single_instances():
         assert np.all(d[:, 1] == 12)
 
 
+def test_batch_graph_replays_match_direct_launches():
+    """The batch recorded as one HIP graph (barrier-reset kernel nodes and
+    cooperative batch-kernel nodes, two packs) replayed three times, with
+    eager work and synchronisation between the replays: every replay equals
+    the direct packed launches on the same inputs bitwise (same kernels,
+    same arguments); a replay after an engine changed is refused."""
+    from pyprop_amd import device as D
+    from pyprop_amd import _native as nat
+    t = D.torch()
+    rng = np.random.default_rng(7)
+    engines, psa, psb, dts, hs = [], [], [], [], []
+    nsteps, dt = 3, 0.02
+    for i in range(30):
+        lmax = int(rng.integers(6, 13))
+        nbp = int(rng.integers(30, 60))
+        h = O.Hydrogen(rmax=float(nbp), nbp=nbp, order=7, lmax=lmax, frequency=float(rng.uniform(0.1, 0.5)),
+                       cycles=4, E0=float(rng.uniform(0.02, 0.1)))
+        psi0 = h.ground_state()
+        psi0[1] = 0.1 * psi0[0]
+        ov = D.Overlap(h.S, h.N, h.kd)
+        eng = D.PampEngine(h.L, h.N, 12, ov)
+        eng.add_term("Diag", D.to_device(h.H0))
+        eng.add_term("Simp", D.to_device(h.Hint), pairs=h.pairs)
+        engines.append(eng)
+        psa.append(D.to_device(psi0))
+        psb.append(D.to_device(psi0))
+        dts.append(dt)
+        hs.append(h)
+    print("batch graph: 30 instances, %d workgroups" % sum(e.L for e in engines), flush=True)
+    graph = D.BatchGraph(engines, psb, nsteps, dts)
+    for r in range(3):
+        ts = O.step_times(nsteps, dt, t0=3.0 + r * nsteps * dt)
+        scs = [np.array([[1.0, h.scaling(tt)] for tt in ts]) for h in hs]
+        D.pamp_batch_propagate(engines, psa, nsteps, dts, scs)
+        t.cuda.synchronize()
+        graph.launch(scs, psb)
+        t.cuda.synchronize()
+        norms = [float(t.linalg.vector_norm(p).item()) for p in psb]   # eager work between replays
+        assert all(np.isfinite(norms))
+        for a, b in zip(psa, psb):
+            assert t.equal(a, b), "replay %d differs from the direct launches" % r
+        print("  replay %d equals the direct launches" % r, flush=True)
+    engines[0].set_mixed([1])
+    with pytest.raises(nat.NativeError):
+        graph.launch(scs, psb)
+
+
 def _adjoint_problem(K, seed=3):
     import pyprop_amd
     from pyprop_amd.examples import hydrogen as H
