@@ -236,6 +236,25 @@ int pp_pamp3_kernel_time(pp_pamp3 *p, double *ms_total, long *launches);
 int pp_pamp3_multipole_flops(pp_pamp3 *p, double *flops);
 int pp_pamp3_destroy(pp_pamp3 *p);
 
+/* Krylov-basis vector work of the eigenvalue solver (PiramSolver), whose
+ * reference does it with BLAS on the host inside pIRAM
+ * (core/krylov/piram/piram/piram.h:320-445 Arnoldi step and
+ * orthogonalisation, :540-640 restart).  Basis V: k vectors of n complex128,
+ * vector j at V + j*n.
+ * pp_gram_schmidt: classical Gram-Schmidt of w against V_0..V_{k-1}, passes
+ * times (1-3; deterministic reductions), h (host, 2k doubles re/im) = the
+ * summed coefficients, *wnorm = |w| afterwards; synchronises the stream.
+ * pp_basis_combine: out[p] = sum_j V_j Q[j, p], p < np, Q host column-major
+ * m x np complex128; out must not overlap V. */
+int pp_gram_schmidt(const void *V, long n, int k, void *w, int passes, double *h, double *wnorm, pp_stream stream);
+int pp_basis_combine(const void *V, long n, int m, const double *Q, int np, void *out, pp_stream stream);
+/* Per-fiber banded LU solve of the shift-invert preconditioner (the device
+ * side of pyprop's BSplinePreconditioner.Solve,
+ * examples/tensor/helium_stabilization_sae/example.py:275-318): x_a := A_a^-1
+ * x_a for nfib fibers of length N, A_a factored by LAPACK zgbtrf (lub: nfib
+ * x N x (2kl+ku+1) complex128 column bands, piv: nfib x N 0-based). */
+int pp_band_lu_solve(int nfib, int N, int kl, int ku, const void *lub, const int *piv, void *x, pp_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,8 +83,18 @@ def lib():
                                  ctypes.c_int, ip]
         L.orc_propagate_expokit.argtypes = [P, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int,
                                             ctypes.c_double, ctypes.c_double, ip]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel regions (independent outputs
+    only, so results do not depend on n); returns the previous count."""
+    prev = lib().orc_get_threads()
+    lib().orc_set_threads(int(n))
+    return prev
 
 
 def _dp(a):

@@ -1,6 +1,8 @@
 /*
  * pyprop_oracle.c -- CPU restatement of pyprop's tensor-potential Krylov
- * propagation path, in complex128, single thread, no third-party BLAS.
+ * propagation path, in complex128, no third-party BLAS.  OpenMP runs only
+ * over independent outputs (angular rows, fibers, whole dot products,
+ * elements), so every thread count gives bitwise the result of one thread.
  *
  * TEST INFRASTRUCTURE ONLY.  This file is the checker: it is linked by the
  * tests, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and by
@@ -18,6 +20,7 @@
  *   Expokit zgexpv / zgpadm .... core/krylov/expokit/f2c/zgexpv.cpp:32-586, zgpadm.cpp:28-285
  */
 #include <complex.h>
+#include <omp.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -336,6 +339,8 @@ ORC_API void orc_band_apply(int op, int rank, int R, const int *shape, int kd,
     for (int i = 0; i < rank; i++) pre *= shape[i];
     for (int i = rank + 1; i < R; i++) post *= shape[i];
     int N = shape[rank];
+    /* fibers are independent: any thread count gives the same result */
+    #pragma omp parallel for collapse(2) schedule(static)
     for (long a = 0; a < pre; a++)
         for (long c = 0; c < post; c++) {
             zc *f = x + a * N * post + c;
@@ -468,7 +473,23 @@ ORC_API int orc_tp_multiply(int R, const int *storage, const int *potShape, cons
         potStride[r] = potStride[r + 1] * potShape[r + 1];
     }
     orc_entry dummy = { 0, 0, 0, 0 };
-    for (int a = 0; a < ne[0]; a++) {
+    /* rank-0 entries grouped by destination row in their original order:  */
+    /* each thread owns whole rows, so every destination element sums its   */
+    /* contributions in the serial order (result independent of threads)   */
+    int nrow0 = shape[0];
+    int *rp = calloc((size_t)nrow0 + 1, sizeof(int)), *idx = malloc(sizeof(int) * (ne[0] > 0 ? ne[0] : 1));
+    for (int a = 0; a < ne[0]; a++) rp[E[0][a].row + 1]++;
+    for (int r0 = 0; r0 < nrow0; r0++) rp[r0 + 1] += rp[r0];
+    {
+        int *fill = malloc(sizeof(int) * (nrow0 > 0 ? nrow0 : 1));
+        for (int r0 = 0; r0 < nrow0; r0++) fill[r0] = rp[r0];
+        for (int a = 0; a < ne[0]; a++) idx[fill[E[0][a].row]++] = a;
+        free(fill);
+    }
+    #pragma omp parallel for schedule(dynamic)
+    for (int r0 = 0; r0 < nrow0; r0++)
+    for (int q = rp[r0]; q < rp[r0 + 1]; q++) {
+        const int a = idx[q];
         orc_entry e0 = E[0][a];
         for (int b = 0; b < (R > 1 ? ne[1] : 1); b++) {
             orc_entry e1 = R > 1 ? E[1][b] : dummy;
@@ -483,6 +504,8 @@ ORC_API int orc_tp_multiply(int R, const int *storage, const int *potShape, cons
             }
         }
     }
+    free(rp);
+    free(idx);
     for (int r = 0; r < R; r++) free(E[r]);
     return 0;
 }
@@ -697,6 +720,10 @@ ORC_API void orc_ham_destroy(orc_ham *h)
     free(h);
 }
 
+/* threads of the OpenMP regions (timing of the all-core CPU baseline) */
+ORC_API void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+ORC_API int orc_get_threads(void) { return omp_get_max_threads(); }
+
 /* sum of tensor potentials (Propagator.py:61-66 after dest = 0) */
 static void ham_potentials(orc_ham *h, const zc *src, zc *dst)
 {
@@ -889,7 +916,22 @@ static zc zdotc(long n, const zc *a, const zc *b)
     return s;
 }
 
-static double vnorm(long n, const zc *a) { return sqrt(creal(zdotc(n, a, a))); }
+/* |a| as 64 fixed blocks summed in order: the same result on any thread count */
+static double vnorm(long n, const zc *a)
+{
+    enum { NB = 64 };
+    double part[NB];
+    #pragma omp parallel for schedule(static)
+    for (int b = 0; b < NB; b++) {
+        const long i0 = n * b / NB, i1 = n * (b + 1) / NB;
+        double s = 0.0;
+        for (long i = i0; i < i1; i++) s += creal(a[i]) * creal(a[i]) + cimag(a[i]) * cimag(a[i]);
+        part[b] = s;
+    }
+    double s = 0.0;
+    for (int b = 0; b < NB; b++) s += part[b];
+    return sqrt(s);
+}
 
 ORC_API orc_pamp *orc_pamp_create(long n, int m, int reorth)
 {
@@ -922,6 +964,7 @@ ORC_API int orc_pamp_step(orc_pamp *p, orc_ham *h, double *psid, double dt_re, d
     double tol = sqrt(DBL_EPSILON);
     memcpy(r, psi, sizeof(zc) * n);
     double norm = vnorm(n, r);
+    #pragma omp parallel for schedule(static)
     for (long i = 0; i < n; i++) r[i] *= 1.0 / norm;
     memset(V, 0, sizeof(zc) * n * m);
     memset(H, 0, sizeof(zc) * m * m);
@@ -939,11 +982,16 @@ ORC_API int orc_pamp_step(orc_pamp *p, orc_ham *h, double *psid, double dt_re, d
         double beta = vnorm(n, r);
         if (fabs(beta) < tol) { p->happy = 1; break; }
         zc *vj1 = V + (size_t)(j + 1) * n;
+        #pragma omp parallel for schedule(static)
         for (long i = 0; i < n; i++) { r[i] *= 1.0 / beta; vj1[i] = r[i]; }
         H[(j + 1) + j * m] = beta;
         orc_ham_balanced(h, (double *)vj1, (double *)r);
         double origNorm = vnorm(n, r);
+        /* one whole dot per thread and independent elements: the result */
+        /* does not depend on the thread count                           */
+        #pragma omp parallel for schedule(dynamic)
         for (int k = 0; k <= j + 1; k++) ov[k] = zdotc(n, V + (size_t)k * n, r);
+        #pragma omp parallel for schedule(static)
         for (long i = 0; i < n; i++) {
             zc t = 0.0;
             for (int k = 0; k <= j + 1; k++) t += V[(size_t)k * n + i] * ov[k];
@@ -976,6 +1024,7 @@ ORC_API int orc_pamp_step(orc_pamp *p, orc_ham *h, double *psid, double dt_re, d
     if (info) return info;
     /* psi = norm * V * exp(H) e1 (:657-662) */
     zc *v = p->Hexp;   /* first column */
+    #pragma omp parallel for schedule(static)
     for (long i = 0; i < n; i++) {
         zc t = 0.0;
         for (int k = 0; k < m; k++) t += V[(size_t)k * n + i] * v[k];

@@ -16,5 +16,6 @@ from .problem import InitialConditionType, WavefunctionFileFormat, Problem  # no
 from .propagator import PropagatorBase, PamPropagator, ExpokitPropagator  # noqa: F401
 from .tensorpotential import BasisPropagator, TensorPotential  # noqa: F401
 from .coupled import CoupledIndex, DefaultCoupledIndexIterator  # noqa: F401
+from .solver import PiramSolver, GMRESShiftInvertSolver, BSplinePreconditioner  # noqa: F401
 
 del abs, max, min, round, sum  # keep the builtins for code that star-imports this package

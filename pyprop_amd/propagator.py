@@ -110,6 +110,9 @@ class PamPropagator(PropagatorBase):
             return self._build_engine3(repr, pots)
         if self.Rank != 2 or repr.IsOrthogonalBasis(1) is not False or not repr.IsOrthogonalBasis(0):
             raise NotImplementedError("the pAMP HIP engine handles (angular, B-spline) rank-2 states")
+        # a coupling potential without angular pairs (the dipole term at
+        # lmax = 0) contributes nothing and is left out of the engine
+        pots = [p for p in pots if p.PotentialData.shape[0] > 0]
         L, N = repr.GetFullShape()
         overlap = repr.GetRepresentation(1).GetOverlap()
         eng = dev.PampEngine(L, N, self.BasisSize, overlap, reorth=self.DoubleOrthogonalization)

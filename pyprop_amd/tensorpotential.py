@@ -367,6 +367,8 @@ class TensorPotential:
         return float(np.real(self.TimeFunction(t))) if self.IsTimeDependent else 1.0
 
     def MultiplyPotential(self, srcPsi, destPsi, t, timestep):
+        if self.PotentialData.size == 0:   # no pairs (the dipole term at lmax = 0): adds nothing
+            return
         argList = [self.DeviceData(), self.GetTimeScaling(t), srcPsi.GetData(), destPsi.GetData()]
         for g in self.GeometryList:
             argList += g.GetMultiplyArguments(srcPsi)

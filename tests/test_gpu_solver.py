@@ -1,0 +1,160 @@
+"""Eigenvalue solver on the device (PiramSolver through the [Arpack] config
+section) against dense generalized eigenproblems of the same operator:
+the lowest eigenvalues of (H0, S) from scipy.linalg.eigh per angular block,
+the eigenvector against the dense one, and the vector kernels
+(pp_gram_schmidt, pp_basis_combine) against numpy."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+import pyprop_amd as P
+from pyprop_amd import _native as nat
+from pyprop_amd import device as dev
+from pyprop_amd.examples import hydrogen as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _dense_levels(prob):
+    """All eigenvalues of the field-free (H0_l, S) blocks, sorted."""
+    E = []
+    L = prob.psi.GetRepresentation().GetFullShape()[0]
+    for l in range(L):
+        S = prob.psi.GetRepresentation().GetRepresentation(1).GetBSplineObject().overlap_dense()
+        N = S.shape[0]
+        Hl = np.zeros((N, N), dtype=np.complex128)
+        for pot in prob.Propagator.BasePropagator.PotentialList:
+            if pot.IsTimeDependent or pot.StorageIds() != ["Diag", "BandNH"]:
+                continue
+            Hl += H._block(pot, l, N, pot.PotentialData.shape[1] // N)
+        E.extend(scipy.linalg.eigh(Hl, S, eigvals_only=True))
+    return np.sort(np.array(E))
+
+
+def test_vector_kernels_match_numpy():
+    print("pp_gram_schmidt / pp_basis_combine, n = 5000, k = 7", flush=True)
+    t = dev.torch()
+    rng = np.random.default_rng(3)
+    n, k = 5000, 7
+    V0 = rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))
+    Q, _ = np.linalg.qr(V0.T)
+    V0 = Q.T.copy()                                     # orthonormal rows
+    w0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    V = dev.to_device(V0)
+    w = dev.to_device(w0)
+    h = np.zeros(2 * k)
+    nrm = ctypes.c_double()
+    nat.check(nat.lib().pp_gram_schmidt(dev.ptr(V), n, k, dev.ptr(w), 2, nat.dptr(h), ctypes.byref(nrm),
+                                        dev.stream_ptr()), "pp_gram_schmidt")
+    href = V0.conj() @ w0
+    wref = w0 - V0.T @ href
+    assert np.abs((h[0::2] + 1j * h[1::2]) - href).max() < 1e-12 * np.abs(href).max()
+    assert np.abs(w.cpu().numpy() - wref).max() < 1e-12 * np.abs(wref).max()
+    assert abs(nrm.value - np.linalg.norm(wref)) < 1e-12 * np.linalg.norm(wref)
+    C = rng.standard_normal((k, 3)) + 1j * rng.standard_normal((k, 3))
+    out = t.empty((3, n), dtype=t.complex128, device=V.device)
+    Cf = np.asfortranarray(C)
+    nat.check(nat.lib().pp_basis_combine(dev.ptr(V), n, k, Cf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 3,
+                                         dev.ptr(out), dev.stream_ptr()), "pp_basis_combine")
+    ref = (V0.T @ C).T
+    assert np.abs(out.cpu().numpy() - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("maxl,count,basis", [(0, 4, 12), (2, 3, 16)])
+def test_piram_lowest_eigenvalues_match_dense(maxl, count, basis):
+    """Field-free hydrogen (t = 0: the laser term vanishes), order-7
+    B-splines on r <= 30: the lowest `count` eigenvalues to 1e-9 relative
+    (the convergence test asks 1e-10 of each Ritz pair's error bound), and
+    the ground-state eigenvector equal to the dense one up to phase (S-norm
+    overlap 1 - 1e-9)."""
+    print("pIRAM: maxl %d, %d eigenvalues, basis %d" % (maxl, count, basis), flush=True)
+    over = {("AngularRepresentation", "maxl"): maxl, ("RadialRepresentation", "xsize"): 30,
+            ("RadialRepresentation", "xmax"): 30.0, ("Arpack", "krylov_eigenvalue_count"): count,
+            ("Arpack", "krylov_basis_size"): basis}
+    prob = P.Problem(H.load("hydrogen_small", over))
+    prob.SetupStep()
+    solver = P.PiramSolver(prob)
+    assert solver.Solve(), solver.GetConvergenceEstimates()
+    E = solver.GetEigenvalues()
+    ref = _dense_levels(prob)[:count]
+    print("  eigenvalues", E, "dense", ref, "restarts", solver.RestartCount, "matvecs", solver.OperatorCount,
+          flush=True)
+    assert np.abs(E - ref).max() <= 1e-9 * np.abs(ref).max()
+    assert np.all(solver.GetConvergenceEstimates() <= 0.0)
+    # ground state: S-normalised, overlap with the dense 1s state
+    psi = prob.psi.Copy()
+    solver.SetEigenvector(psi, 0)
+    g, _ = H.bound_state(prob, 0, 0)
+    ref_psi = prob.psi.Copy()
+    ref_psi.GetData().copy_(dev.to_device(g))
+    ov = abs(ref_psi.InnerProduct(psi))
+    assert abs(psi.GetNorm() - 1.0) < 1e-12
+    assert ov > 1.0 - 1e-9, ov
+
+
+def test_piram_rejects_small_basis():
+    over = {("Arpack", "krylov_eigenvalue_count"): 6, ("Arpack", "krylov_basis_size"): 10}
+    prob = P.Problem(H.load("hydrogen_small", over))
+    prob.SetupStep()
+    with pytest.raises(RuntimeError):
+        P.PiramSolver(prob)
+
+
+def test_gmres_shift_invert_solves_shifted_system():
+    """GMRESShiftInvertSolver.InverseIterations: x = (S^-1 H - shift)^-1 b
+    on the field-free operator (t = 0), preconditioned by the per-fiber
+    banded LU of H0 - shift S (BSplinePreconditioner): against a dense
+    solve of (H - shift S) x = S b, relative 1e-10."""
+    print("GMRES shift-invert: maxl 2, N=33, shift -0.13", flush=True)
+    over = {("AngularRepresentation", "maxl"): 2, ("RadialRepresentation", "xsize"): 30,
+            ("RadialRepresentation", "xmax"): 30.0}
+    prob = P.Problem(H.load("hydrogen_small", over))
+    prob.SetupStep()
+    gm = P.GMRESShiftInvertSolver(prob)
+    rng = np.random.default_rng(9)
+    L, N = prob.psi.GetData().shape
+    b = rng.standard_normal((L, N)) + 1j * rng.standard_normal((L, N))
+    src = prob.psi.Copy()
+    src.GetData().copy_(dev.to_device(b))
+    dst = prob.psi.Copy()
+    gm.InverseIterations(src, dst, 0.0, 0.0)
+    x = dst.GetData().cpu().numpy()
+    S = prob.psi.GetRepresentation().GetRepresentation(1).GetBSplineObject().overlap_dense()
+    ref = np.zeros_like(b)
+    for l in range(L):
+        Hl = np.zeros((N, N), dtype=np.complex128)
+        for pot in prob.Propagator.BasePropagator.PotentialList:
+            if pot.IsTimeDependent or pot.StorageIds() != ["Diag", "BandNH"]:
+                continue
+            Hl += H._block(pot, l, N, pot.PotentialData.shape[1] // N)
+        ref[l] = np.linalg.solve(Hl - gm.Shift * S, S @ b[l])
+    err = np.abs(x - ref).max() / np.abs(ref).max()
+    print("  rel err %.3e, GMRES iterations %d" % (err, gm.Counters["GMRES iterations"]), flush=True)
+    assert err < 1e-10
+
+
+def test_piram_shift_invert_finds_levels_near_shift():
+    """pIRAM with inverse_iterations and GMRESShiftInvertSolver as its
+    matrix_vector_func (the reference's FindEigenvaluesNearShift,
+    examples/tensor/helium_stabilization_sae/eigenvalues.py:1-25): the two
+    eigenvalues nearest shift = -0.13 (2s, 2p) from 1/lambda + shift, against
+    the dense levels, relative 1e-9."""
+    print("pIRAM shift-invert: maxl 2, shift -0.13, 2 eigenvalues", flush=True)
+    over = {("AngularRepresentation", "maxl"): 2, ("RadialRepresentation", "xsize"): 30,
+            ("RadialRepresentation", "xmax"): 30.0, ("Arpack", "krylov_eigenvalue_count"): 2,
+            ("Arpack", "krylov_basis_size"): 8}
+    prob = P.Problem(H.load("hydrogen_small", over))
+    prob.SetupStep()
+    gm = P.GMRESShiftInvertSolver(prob)
+    prob.Config.Arpack.inverse_iterations = True
+    prob.Config.Arpack.matrix_vector_func = gm.InverseIterations
+    solver = P.PiramSolver(prob)
+    assert solver.Solve()
+    lam = solver.Eigenvalues[:2]
+    E = np.sort((1.0 / lam + gm.Shift).real)
+    dense = _dense_levels(prob)
+    ref = np.sort(dense[np.argsort(np.abs(dense - gm.Shift))[:2]])
+    print("  E", E, "dense", ref, "restarts", solver.RestartCount, flush=True)
+    assert np.abs(E - ref).max() <= 1e-9 * np.abs(ref).max()

@@ -274,3 +274,24 @@ def test_expokit_propagation_agrees_with_pamp(small):
     a = ham.propagate_pamp(psi0, 10, 0.02, sc, 20)
     b, st = ham.propagate_expokit(psi0, 10, 0.02, sc, 20, tol=0.0, anorm=1e6)
     assert np.abs(a - b).max() < 1e-9
+
+
+def test_oracle_result_independent_of_thread_count():
+    """The oracle's OpenMP regions run over independent outputs only (angular
+    rows of the tensor multiply, fibers of the overlap solves, whole dot
+    products, elements): 3 pAMP steps of a small hydrogen problem with the
+    laser on are bitwise equal on 1 and 4 threads."""
+    h = O.Hydrogen(rmax=30.0, nbp=30, order=7, lmax=5, frequency=0.3, cycles=6, E0=0.1)
+    psi0 = h.ground_state()
+    psi0[1] = 0.2 * psi0[0]
+    ts = O.step_times(3, 0.01, t0=4.0)
+    sc = [[1.0, h.scaling(t)] for t in ts]
+    out = []
+    prev = O.set_threads(1)
+    try:
+        for nt in (1, 4):
+            O.set_threads(nt)
+            out.append(h.ham().propagate_pamp(psi0, 3, 0.01, sc, 12))
+    finally:
+        O.set_threads(prev)
+    assert np.array_equal(out[0], out[1])
