@@ -128,6 +128,10 @@ __device__ inline void group_barrier(unsigned *word, unsigned *err, unsigned &ep
 {
   const int tid = threadIdx.x;
   ep++;
+  // every wave waits until its own stores (the exchanged values, written
+  // with agent-scope atomics) are acknowledged before the workgroup barrier:
+  // the arrival word below must not become visible before them
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (tid < 64) {
     if (tid == 0) {
