@@ -516,75 +516,80 @@ __global__ void __launch_bounds__(256) p3_farL(const T *const *Pt, const T *cons
 
 // Z[c][i1][i2] += sum_blocks sum_{d1,d2} Rn_t[i1][d1][w][d2] Ym[b][i1+d1-S][i2+d2-S][cl]
 // for the near band i2 = nlo[i1] + w <= nhi[i1]; workgroup: (chunk, i1),
-// 1024 threads = P3_NG4 groups x (w, channel lane).  Group g takes the
-// blocks g, g + P3_NG4, ... and the groups' partial sums are added in LDS in
-// group order, so the result is deterministic.  The block list (input offset,
-// R slice) is staged in LDS first; two accumulators split the tap chain.
+// 256 threads = P3_NG4 groups x 4 w-quads x 16 channel lanes.  A lane
+// computes the 4 outputs w = 4 wq .. 4 wq + 3 of its channel, so each input
+// row is loaded once per lane as 16 consecutive positions shared by the 4
+// outputs' 13-tap windows (a quarter of the loads of one output per lane).
+// Group g takes the blocks g, g + P3_NG4, ... and the groups' partial sums
+// are added in LDS in group order, so the result is deterministic.  Lanes
+// whose outputs fall past the band accumulate values that are never stored.
 #define P3_NG4 4
 template <int S, class T>
-__global__ void __launch_bounds__(256 * P3_NG4) p3_near(const T *const *Rn, const int *blk_term,
-                                                        const int *cb_ptr, const int *cb_blk, const int2 *nr,
-                                                        const typename p3_cx<T>::type *Ym, zd *Z, int C, int N,
-                                                        int nchunk, const int *state)
+__global__ void __launch_bounds__(256) p3_near(const T *const *Rn, const int *blk_term, const int *cb_ptr,
+                                               const int *cb_blk, const int2 *nr,
+                                               const typename p3_cx<T>::type *Ym, zd *Z, int C, int N, int nchunk,
+                                               const int *state)
 {
   typedef typename p3_cx<T>::type ZT;
   if (state[0]) return;
-  constexpr int BW = 2 * S + 1;
+  constexpr int BW = 2 * S + 1, WIN = 4 + 2 * S;
   __shared__ const T *sR[P3_MAXB];
   __shared__ long sY[P3_MAXB];
-  __shared__ ZT part[P3_NG4][256];
+  __shared__ ZT part[P3_NG4][64 * 4];
   const int chunk = blockIdx.x % nchunk, i1 = blockIdx.x / nchunk;
   const long NN = (long)N * N;
   const int b0 = cb_ptr[chunk], nb = cb_ptr[chunk + 1] - b0;
-  for (int k = threadIdx.x; k < nb; k += 256 * P3_NG4) {
+  for (int k = threadIdx.x; k < nb; k += 256) {
     const int b = cb_blk[b0 + k];
     sR[k] = Rn[blk_term[b]] + (size_t)i1 * BW * P3_NW * BW;
     sY[k] = (long)b * NN * P3_CHUNK;
   }
   __syncthreads();
-  const int grp = threadIdx.x >> 8, t = threadIdx.x & 255;
-  const int w = t >> 4, cl = t & (P3_CHUNK - 1);
+  const int grp = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wq = lane >> 4, cl = lane & (P3_CHUNK - 1);
   const int2 r = nr[i1];
-  const int i2 = r.x + w;
-  const bool ok = i2 <= r.y;
-  ZT a0 = p3_czero<ZT>(), a1 = p3_czero<ZT>();
-  if (ok) {
-    // taps outside 0 <= j2 < N are not loaded (select, no branch), so all 13
-    // loads of a row issue together; the mixed input is only written where
-    // the near band reads it
-    bool jv[BW];
+  const int w0 = 4 * wq, p0 = r.x + w0 - S;     // first input position of this lane's window
+  const bool any = r.x + w0 <= r.y;
+  ZT acc[4];
 #pragma unroll
-    for (int d2 = 0; d2 < BW; d2++) {
-      const int j2 = i2 + d2 - S;
-      jv[d2] = j2 >= 0 && j2 < N;
-    }
+  for (int t = 0; t < 4; t++) acc[t] = p3_czero<ZT>();
+  if (any) {
+    bool pv[WIN];
+#pragma unroll
+    for (int p = 0; p < WIN; p++) pv[p] = p0 + p >= 0 && p0 + p < N;
     for (int k = grp; k < nb; k += P3_NG4) {
-      const T *R = sR[k];
-      const ZT *Yb = Ym + sY[k] + cl + (long)(i2 - S) * P3_CHUNK;
+      const T *R = sR[k] + (size_t)w0 * BW;
+      const ZT *Yb = Ym + sY[k] + cl + (long)p0 * P3_CHUNK;
       for (int d1 = 0; d1 < BW; d1++) {
         const int j1 = i1 + d1 - S;
         if (j1 < 0 || j1 >= N) continue;
         const ZT *Yr = Yb + (long)j1 * N * P3_CHUNK;
-        const T *Rr = R + ((size_t)d1 * P3_NW + w) * BW;
-        ZT y[BW];
+        ZT y[WIN];
 #pragma unroll
-        for (int d2 = 0; d2 < BW; d2++) y[d2] = jv[d2] ? Yr[(long)d2 * P3_CHUNK] : p3_czero<ZT>();
+        for (int p = 0; p < WIN; p++) y[p] = pv[p] ? Yr[(long)p * P3_CHUNK] : p3_czero<ZT>();
+        const T *Rr = R + (size_t)d1 * P3_NW * BW;
 #pragma unroll
-        for (int d2 = 0; d2 < BW; d2++) {
-          if (d2 & 1) p3_cfmar(a1, Rr[d2], y[d2]);
-          else p3_cfmar(a0, Rr[d2], y[d2]);
+        for (int t = 0; t < 4; t++) {
+          ZT a = p3_czero<ZT>();
+#pragma unroll
+          for (int d2 = 0; d2 < BW; d2++) p3_cfmar(a, Rr[t * BW + d2], y[t + d2]);
+          acc[t] = p3_cadd(acc[t], a);
         }
       }
     }
   }
-  part[grp][t] = p3_cadd(a0, a1);
-  __syncthreads();
-  if (grp != 0 || !ok) return;
-  ZT v = part[0][t];
 #pragma unroll
-  for (int g = 1; g < P3_NG4; g++) v = p3_cadd(v, part[g][t]);
+  for (int t = 0; t < 4; t++) part[grp][lane * 4 + t] = acc[t];
+  __syncthreads();
+  if (grp != 0) return;
   const int c = chunk * P3_CHUNK + cl;
-  if (c < C) {
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int i2 = r.x + w0 + t;
+    if (c >= C || i2 > r.y) continue;
+    ZT v = part[0][lane * 4 + t];
+#pragma unroll
+    for (int g = 1; g < P3_NG4; g++) v = p3_cadd(v, part[g][lane * 4 + t]);
     const long o = (long)c * NN + (long)i1 * N + i2;
     Z[o] = zadd(Z[o], p3_ctod(v));
   }
@@ -1180,7 +1185,7 @@ static int p3_apply_raw(pp_pamp3 *p, const zd *y, zd *z, const double *scal, hip
       const size_t lds = sizeof(float2) * (size_t)(N + 12) * P3_CHUNK;
       hipLaunchKernelGGL((p3_mixT<6, float>), dim3(p->nblk * N), dim3(256), lds, st, p->mrow, p->ment, y, Ym, p->fPt,
                          p->fQt, p->blk_term, p->nr, Tq, Tp, N, scal, p->state);
-      hipLaunchKernelGGL((p3_near<6, float>), dim3(p->nchunk * N), dim3(256 * P3_NG4), 0, st, p->fRnt, p->blk_term,
+      hipLaunchKernelGGL((p3_near<6, float>), dim3(p->nchunk * N), dim3(256), 0, st, p->fRnt, p->blk_term,
                          p->cb_ptr, p->cb_blk, p->nr, Ym, z, C, N, p->nchunk, p->state);
       hipLaunchKernelGGL((p3_farL<6, float>), dim3(p->nchunk * ntile * ngrp), dim3(256), 0, st, p->fPt, p->fQt,
                          p->blk_term, p->cb_ptr, p->cb_blk, p->nr, Tq, Tp, z, C, N, p->nchunk, p->state);
@@ -1188,7 +1193,7 @@ static int p3_apply_raw(pp_pamp3 *p, const zd *y, zd *z, const double *scal, hip
       const size_t lds = sizeof(zd) * (size_t)(N + 12) * P3_CHUNK;
       hipLaunchKernelGGL((p3_mixT<6, double>), dim3(p->nblk * N), dim3(256), lds, st, p->mrow, p->ment, y, p->Ym,
                          p->dPt, p->dQt, p->blk_term, p->nr, p->Tq, p->Tp, N, scal, p->state);
-      hipLaunchKernelGGL((p3_near<6, double>), dim3(p->nchunk * N), dim3(256 * P3_NG4), 0, st, p->dRn, p->blk_term,
+      hipLaunchKernelGGL((p3_near<6, double>), dim3(p->nchunk * N), dim3(256), 0, st, p->dRn, p->blk_term,
                          p->cb_ptr, p->cb_blk, p->nr, p->Ym, z, C, N, p->nchunk, p->state);
       hipLaunchKernelGGL((p3_farL<6, double>), dim3(p->nchunk * ntile * ngrp), dim3(256), 0, st, p->dPt, p->dQt,
                          p->blk_term, p->cb_ptr, p->cb_blk, p->nr, p->Tq, p->Tp, z, C, N, p->nchunk, p->state);
