@@ -262,7 +262,7 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (matrix and vector peaks are equal
 # HBM-side bytes of one multipole stage (the three kernels' FETCH_SIZE +
 # WRITE_SIZE per matvec, FETCH scaled x2 and WRITE x1 by a 512 MiB copy),
 # from the counter passes recorded in profiles/multipole.helium.md.
-HE_PMC_TRAFFIC = {"helium": 1392.4e6, "helium_mixed": 683.1e6}
+HE_PMC_TRAFFIC = {"helium": 1492.2e6, "helium_mixed": 757.7e6}
 
 
 def run_helium(args):
