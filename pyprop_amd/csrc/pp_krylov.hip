@@ -89,6 +89,7 @@ struct pp_pamp_args {
   int Q;                 // split kernel: workgroups per fiber
   zd *sb;                // split kernel: L x 2 x P*KD chunk boundary values (exchanged)
   unsigned *subw;        // split kernel: L x Q arrival words of the per-fiber barriers
+  int vcache;            // split kernel: Krylov vectors whose own rows are also kept in LDS
 };
 
 struct pp_pamp {
@@ -232,6 +233,39 @@ __device__ inline void fiber_dots(const zd *V, int ldv, const zd *W, int nk, int
     re = wave_sum(re);
     im = wave_sum(im);
     if (lane == 0) { out[2 * k] = re; if (k < nk) out[2 * k + 1] = im; }
+  }
+  __syncthreads();
+}
+
+// fiber_dots over one workgroup's rows of the Krylov vectors with the first
+// kc vectors read from their LDS copy (stride ldl) and the rest from global
+// memory (stride ldg): the same values in the same order as fiber_dots.
+__device__ __attribute__((always_inline)) inline void split_dots(const zd *Vl, int ldl, int kc, const zd *Vg, int ldg,
+                                                                 const zd *W, int nk, int N, double *out)
+{
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  for (int k = wave; k < nk; k += PP_THREADS / 64) {
+    double re = 0.0, im = 0.0;
+    if (k < kc) {
+      const zd *vk = Vl + (size_t)k * ldl;
+#pragma unroll PP_VUNROLL
+      for (int i = lane; i < N; i += 64) {
+        zd v = vk[i], w = W[i];
+        re = fma(v.x, w.x, re); re = fma(v.y, w.y, re);
+        im = fma(v.x, w.y, im); im = fma(-v.y, w.x, im);
+      }
+    } else {
+      const zd *vk = Vg + (size_t)k * ldg;
+#pragma unroll PP_VUNROLL
+      for (int i = lane; i < N; i += 64) {
+        zd v = vk[i], w = W[i];
+        re = fma(v.x, w.x, re); re = fma(v.y, w.y, re);
+        im = fma(v.x, w.y, im); im = fma(-v.y, w.x, im);
+      }
+    }
+    re = wave_sum(re);
+    im = wave_sum(im);
+    if (lane == 0) { out[2 * k] = re; out[2 * k + 1] = im; }
   }
   __syncthreads();
 }
@@ -764,6 +798,12 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
   M.hred = M.wsum + 16;
   M.cv = (zd *)(M.hred + (2 * m + 4));
   zd *W = M.W, *X = M.X, *tb = M.tb, *Ys = M.Ys, *Z = M.Z, *cv = M.cv;
+  // own rows of the first KC Krylov vectors, kept in LDS as well (stride R)
+  // for the inner products and the projection; the Pade exponential of the
+  // lead workgroup reuses this space, so the final combination reads them
+  // back from global memory
+  zd *Vl = cv + m;
+  const int KC = A.vcache;
   double *wsum = M.wsum, *hred = M.hred;
   const double tol = 1.4901161193847656e-08;           // sqrt(DBL_EPSILON), pamp.h:75
   zd *Va = A.V + (size_t)a * (m + 1) * N;
@@ -824,7 +864,11 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
       }
       if (!broken) {
         const double inv = 1.0 / beta;
-        for (int i = tid; i < nown; i += PP_THREADS) Va[(size_t)j * N + r0 + i] = zscale(W[i], inv);
+        for (int i = tid; i < nown; i += PP_THREADS) {
+          const zd v = zscale(W[i], inv);
+          Va[(size_t)j * N + r0 + i] = v;
+          if (j < KC) Vl[(size_t)j * R + i] = v;
+        }
         // coupled fibers' U^-1 w, rows r0-KD .. r0+nloc+KD-1 inside [0, N)
         const int ybase = r0 - KD, ylo = ybase > 0 ? ybase : 0;
         for (int sl = 0; sl < A.nslots; sl++) {
@@ -839,7 +883,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
         stamp(step);
         pp_split_solve<KD, false>(A, M.nl, X, W, tb, c0, nch, sbuf(), fw, sep, q);
         stamp(step);
-        fiber_dots(Va + r0, N, W, nk, nown, false, hred);
+        split_dots(Vl, R, KC, Va + r0, N, W, nk, nown, hred);
         for (int c = tid; c < ncomp; c += PP_THREADS) xst(mypart + c, hred[c]);
       }
       stamp(step);
@@ -858,8 +902,11 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
           zd w = zmk(0.0, 0.0);
           if (i < nown) {
             w = W[i];
+            const int kl = j + 1 < KC ? j + 1 : KC;   // LDS copies first, same k order
 #pragma unroll PP_VUNROLL
-            for (int k = 0; k <= j; k++) w = zsub(w, zmul(Va[(size_t)k * N + r0 + i], zmk(hred[2 * k], hred[2 * k + 1])));
+            for (int k = 0; k < kl; k++) w = zsub(w, zmul(Vl[(size_t)k * R + i], zmk(hred[2 * k], hred[2 * k + 1])));
+#pragma unroll PP_VUNROLL
+            for (int k = kl; k <= j; k++) w = zsub(w, zmul(Va[(size_t)k * N + r0 + i], zmk(hred[2 * k], hred[2 * k + 1])));
           }
           W[i] = w;
           X[i] = w;
@@ -1698,6 +1745,24 @@ static size_t pamp_split_lds_bytes(const pp_pamp *p, int Q)
   return fib > pade ? fib : pade;
 }
 
+// Krylov vectors whose own rows the split kernel also keeps in LDS: as many
+// as fit beside its per-workgroup buffers within 160 KB (the LDS the lead's
+// Pade exponential needs is allocated anyway); the LDS bytes to allocate.
+static int pamp_split_vcache(const pp_pamp *p, int Q, size_t &lds)
+{
+  const pp_band_plan &pl = p->ov->plan;
+  const size_t nle = (size_t)((pl.nlev + 1) & ~1) + PP_CMB_DOUBLES;
+  const int maxch = (pl.P + Q - 1) / Q;
+  const size_t R = (size_t)maxch << pl.lb;
+  const size_t fib = sizeof(double) * (nle + (size_t)pp_split_doubles(p->nslots, p->m, p->ov->kd, pl.B, maxch)) + 64;
+  lds = pamp_split_lds_bytes(p, Q);
+  const size_t cap = 160 * 1024, per = sizeof(zd) * R;
+  long kc = fib < cap ? (long)((cap - fib) / per) : 0;
+  if (kc > p->m) kc = p->m;
+  if (fib + per * (size_t)kc > lds) lds = fib + per * (size_t)kc;
+  return (int)kc;
+}
+
 // Buffers of the split kernel: partial sums of L*Q workgroups, grid and
 // per-fiber arrival words, the chunk boundary exchange.
 static int pamp_split_buffers(pp_pamp *p, int Q)
@@ -1748,7 +1813,8 @@ extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re
     A.Q = Q;
     A.sb = p->sb;
     A.subw = (unsigned *)(p->bar + 1) + (size_t)p->L * Q;
-    const size_t lds = pamp_split_lds_bytes(p, Q);
+    size_t lds = 0;
+    A.vcache = pamp_split_vcache(p, Q, lds);
     PP_KD_SWITCH(p->ov->kd, {
       rc = pamp_launch(p->mixed ? (const void *)pp_pamp_split_kernel<KD_, true>
                                 : (const void *)pp_pamp_split_kernel<KD_, false>,
