@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE counter passes recorded in
 # profiles/pp_pamp_kernel.hydrogen_large.pmc.md (FETCH_SIZE scaled x2 and
 # WRITE_SIZE x1 by a 512 MiB device copy of known size).
-PMC_TRAFFIC_PER_STEP = {"hydrogen_large": {"fetch": 60.1e6, "write": 78.0e6}}
+PMC_TRAFFIC_PER_STEP = {"hydrogen_large": {"fetch": 58.8e6, "write": 78.0e6}}
 
 
 def step_bytes(n, m, N, s_mv=16, s_gs=16):
