@@ -644,11 +644,24 @@ __global__ void __launch_bounds__(256) p3_ftile(const double *Ub, zd *x, int C, 
     if (OP == 0) {
 #pragma unroll
       for (int d = 0; d < KD; d++) win[d] = zmk(0.0, 0.0);
-      for (int i = N - 1; i >= 0; i--) {
+      // the last KD rows see the end of the band; the rest take all KD taps
+      // with no bounds tests (same terms in the same order)
+      int i = N - 1;
+      for (; i >= 0 && i + KD >= N; i--) {
         zd s = b[i];
 #pragma unroll
         for (int d = KD; d >= 1; d--)
           if (i + d < N) { const double u = U(i, i + d); s.x -= u * win[d - 1].x; s.y -= u * win[d - 1].y; }
+        const zd xi = zscale(s, rd[i]);
+        b[i] = xi;
+#pragma unroll
+        for (int d = KD - 1; d >= 1; d--) win[d] = win[d - 1];
+        win[0] = xi;
+      }
+      for (; i >= 0; i--) {
+        zd s = b[i];
+#pragma unroll
+        for (int d = KD; d >= 1; d--) { const double u = U(i, i + d); s.x -= u * win[d - 1].x; s.y -= u * win[d - 1].y; }
         const zd xi = zscale(s, rd[i]);
         b[i] = xi;
 #pragma unroll
