@@ -855,6 +855,17 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
     for (int j = 0; j < m; j++) {
       // ---------------- phase A: beta_j, v_j = w/beta, w' = U^-H H U^-1 v_j, <v_k, w'>
       const int nk = j + 1, ncomp = 2 * nk;
+      // coupled fibers' U^-1 w, rows r0-KD .. r0+nloc+KD-1 inside [0, N): they
+      // do not depend on beta (1/beta goes into the coefficients), so their
+      // loads are issued before the norm reduction and the two round trips
+      // overlap (unused if this iteration breaks down)
+      const int ybase = r0 - KD, ylo = ybase > 0 ? ybase : 0;
+      if (!broken)
+        for (int sl = 0; sl < A.nslots; sl++) {
+          const int col = A.slotcols[a * A.nslots + sl];
+          if (col >= 0)
+            for (int jj = ylo + tid; jj < phi; jj += PP_THREADS) Ys[sl * ystr + jj - ybase] = zld(A.y + (size_t)col * N + jj);
+        }
       grid_partial_sum_wide(A.part, A.pstr, NRM, 1, nwg, hred);
       beta = sqrt(hred[0]);
       if (j == 0) beta0 = beta;
@@ -868,13 +879,6 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
           const zd v = zscale(W[i], inv);
           Va[(size_t)j * N + r0 + i] = v;
           if (j < KC) Vl[(size_t)j * R + i] = v;
-        }
-        // coupled fibers' U^-1 w, rows r0-KD .. r0+nloc+KD-1 inside [0, N)
-        const int ybase = r0 - KD, ylo = ybase > 0 ? ybase : 0;
-        for (int sl = 0; sl < A.nslots; sl++) {
-          const int col = A.slotcols[a * A.nslots + sl];
-          if (col >= 0)
-            for (int jj = ylo + tid; jj < phi; jj += PP_THREADS) Ys[sl * ystr + jj - ybase] = zld(A.y + (size_t)col * N + jj);
         }
         pp_step_coeffs(A, M, step, zmk(inv, 0.0));
         __syncthreads();
