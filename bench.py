@@ -367,8 +367,9 @@ def run_helium(args):
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
         # the oracle's factored restatement (the reference layout of the e-e
-        # term would be ~48 GB): same terms, same start state, channel-parallel
-        # operator apply (OpenMP), serial Gram-Schmidt / solves / Pade
+        # term would be ~48 GB): same terms, same start state; OpenMP over
+        # output channels (apply), fibers (solves), whole dot products and
+        # elements (Gram-Schmidt), serial Pade
         from oracle import oracle as O
         from pyprop_amd.helium import KronTerm
         nsteps_cpu = min(args.cpu_steps, 2)
@@ -385,13 +386,18 @@ def run_helium(args):
                 dep.append(pot)
         ts = O.step_times(nsteps_cpu, dt.real, args.warmup * dt.real)
         sc = [[p.GetTimeScaling(tt) for p in dep] for tt in ts]
-        c0 = time.perf_counter()
-        ham.propagate_pamp(psi0, nsteps_cpu, dt.real, sc, m)
-        cpu_s = time.perf_counter() - c0
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        prev = O.set_threads(threads)
+        try:
+            c0 = time.perf_counter()
+            ham.propagate_pamp(psi0, nsteps_cpu, dt.real, sc, m)
+            cpu_s = time.perf_counter() - c0
+        finally:
+            O.set_threads(prev)
         cpu = {"value": nsteps_cpu / cpu_s, "unit": "steps/s", "cores": threads, "kind": "port",
-               "sample": "%d pAMP steps of the same workload (C oracle, factored operator, complex128, "
-                         "operator apply OpenMP over %d threads, Gram-Schmidt / solves serial)" % (nsteps_cpu, threads)}
+               "sample": "%d pAMP steps of the same workload (C oracle, factored operator, complex128; OpenMP over "
+                         "output channels, fibers, whole dot products and elements on %d threads, bitwise equal to "
+                         "1 thread; Pade serial)" % (nsteps_cpu, threads)}
 
     line = {
         "metric": "Krylov steps/s + matvec HBM GB/s, FP64 vs mixed at <=1e-8 population error",
