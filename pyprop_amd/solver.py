@@ -34,6 +34,21 @@ from . import _native as nat
 from . import device as dev
 
 
+def _balanced_operator(prop):
+    """dst = U^-H H(t) U^-1 src for the problem's propagator: one-electron
+    states through BasisPropagator.MultiplyHamiltonianBalancedOverlap (the
+    HIP overlap and tensor kernels); two-electron states through the rank-3
+    engine's balanced apply, whose potentials are applied only as a whole."""
+    if prop.psi.GetRank() == 3:
+        eng = prop.Propagator.Engine
+
+        def apply3(src, dst, t, dt):
+            sc = prop.Propagator.TimeScalings([t])[0]
+            eng.apply(src.GetData(), dst.GetData(), scal=sc if len(sc) else None, balanced=True)
+        return apply3
+    return prop.Propagator.BasePropagator.MultiplyHamiltonianBalancedOverlap
+
+
 class PiramSolver:
     """Implicitly restarted Arnoldi for the lowest eigenvalues of the
     problem's Hamiltonian (pyprop/solver/PiramSolver.py)."""
@@ -65,7 +80,7 @@ class PiramSolver:
         if func:
             self.ApplyMatrix = func
         else:
-            self.ApplyMatrix = prop.Propagator.BasePropagator.MultiplyHamiltonianBalancedOverlap
+            self.ApplyMatrix = _balanced_operator(prop)
         self.TempPsi = prop.psi.Copy()
         self.SrcPsi = prop.psi.Copy()
         self.Shape = tuple(prop.psi.GetData().shape)
@@ -350,6 +365,8 @@ class GMRESShiftInvertSolver:
     def __init__(self, prop):
         self.BaseProblem = prop
         self.Rank = prop.psi.GetRank()
+        if self.Rank != 2:
+            raise NotImplementedError("GMRESShiftInvertSolver on the device handles (angular, B-spline) rank-2 states")
         self.Config = prop.Config.GMRES
         self.psi = prop.psi
         self.Shift = float(self.Config.shift)

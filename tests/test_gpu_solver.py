@@ -158,3 +158,30 @@ def test_piram_shift_invert_finds_levels_near_shift():
     ref = np.sort(dense[np.argsort(np.abs(dense - gm.Shift))[:2]])
     print("  E", E, "dense", ref, "restarts", solver.RestartCount, flush=True)
     assert np.abs(E - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+def test_piram_helium_ground_level_matches_dense():
+    """Rank-3 states through the same solver: the lowest eigenvalue of the
+    small field-free helium problem (5 coupled channels, N = 9) against the
+    dense balanced operator built column by column from the oracle,
+    relative 1e-9."""
+    from oracle import oracle as O
+    from pyprop_amd.examples import helium as HE
+    print("pIRAM: helium small, lowest level", flush=True)
+    small = dict(lmax=1, Ls=(0, 1), nbp=6, rmax=8.0, frequency=2.0, cycles=8, E0=0.05)
+    h = O.Helium(**small, laser=False)
+    Hm = h.ham()
+    n = int(np.prod(h.shape))
+    Hb = np.zeros((n, n), dtype=np.complex128)
+    for j in range(n):
+        e = np.zeros(n, dtype=np.complex128)
+        e[j] = 1.0
+        Hb[:, j] = Hm.balanced(e.reshape(h.shape)).reshape(-1)
+    E0 = np.linalg.eigvalsh(0.5 * (Hb + Hb.conj().T))[0]
+    prob = P.Problem(HE.load("helium_small", None, field_free=True))
+    prob.SetupStep()
+    solver = P.PiramSolver(prob)
+    assert solver.Solve(), solver.GetConvergenceEstimates()
+    E = solver.GetEigenvalues()[0]
+    print("  E0 %.12f dense %.12f restarts %d" % (E, E0, solver.RestartCount), flush=True)
+    assert abs(E - E0) <= 1e-9 * abs(E0)
