@@ -1105,8 +1105,19 @@ static int p3_build(pp_pamp3 *p)
   flatten(mrows, mp, me);
   std::vector<int> cbp(1, 0), cbb;
   for (auto &v : chunk_blocks) { cbb.insert(cbb.end(), v.begin(), v.end()); cbp.push_back((int)cbb.size()); }
+  // band rows are uploaded with the entries outside the matrix (j < 0 or
+  // j >= N) set to 0: the i1 pass reads them against its zero-padded window
+  // instead of testing every tap, whatever bands the caller handed over
   std::vector<double> mats;
-  for (auto &mm : p->mats) mats.insert(mats.end(), mm.begin(), mm.end());
+  for (auto &mm : p->mats) {
+    const size_t base = mats.size();
+    mats.insert(mats.end(), mm.begin(), mm.end());
+    for (int i = 0; i < p->N; i++)
+      for (int d = 0; d < p->bw; d++) {
+        const int j = i + d - p->s;
+        if (j < 0 || j >= p->N) mats[base + (size_t)i * p->bw + d] = 0.0;
+      }
+  }
   std::vector<const double *> rptr;
   for (auto &mt : p->mterms) rptr.push_back(mt.R);
   int rc;
