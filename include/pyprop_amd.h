@@ -145,8 +145,10 @@ int pp_pamp_batch_graph_launch(pp_batch_graph *g, const double *const *scal, pp_
 int pp_pamp_batch_graph_destroy(pp_batch_graph *g);
 /* LDS bytes per workgroup a pAMP launch of an (L, N) problem with Krylov
  * dimension m, nslots coupled fibers per row, kd super-diagonals and solver
- * chunk length B needs (-1: invalid B).  No reference counterpart: the batch
- * autotuner uses it to keep two workgroups per CU. */
+ * chunk length B needs (B = 0: the default chunk of pp_overlap_create_chunked;
+ * -1: invalid B).  No reference counterpart: the batch autotuner uses it to
+ * keep two workgroups per CU, and the Expokit host side to pick a chunk whose
+ * kernel fits 160 KB of LDS. */
 long pp_pamp_lds_estimate(int L, int N, int m, int nslots, int kd, int B);
 /* Pade scaling of later pp_pamp_propagate launches (no reference
  * counterpart): 0 (default) scales A = -i dt H_m by its largest element as

@@ -1622,6 +1622,7 @@ static size_t pamp_lds_bytes(const pp_pamp *p, int padN)
 // counterpart: the batch autotuner uses it to keep two workgroups per CU.
 extern "C" long pp_pamp_lds_estimate(int L, int N, int m, int nslots, int kd, int B)
 {
+  if (B == 0) B = pp_plan_chunk(N, kd);
   if (L < 1 || N < 1 || m < 1 || nslots < 1 || kd < 1 || B < 8 || B < kd || (B & (B - 1))) return -1;
   const int P = (N + B - 1) / B;
   if (P > 64) return -1;

@@ -10,6 +10,7 @@ launch; the time functions are sampled on the host at t_n for each step
 """
 import numpy as np
 
+from . import _native as nat
 from . import device as dev
 
 
@@ -114,7 +115,7 @@ class PamPropagator(PropagatorBase):
         # lmax = 0) contributes nothing and is left out of the engine
         pots = [p for p in pots if p.PotentialData.shape[0] > 0]
         L, N = repr.GetFullShape()
-        overlap = repr.GetRepresentation(1).GetOverlap()
+        overlap = self._engine_overlap(repr, L, N)
         eng = dev.PampEngine(L, N, self.BasisSize, overlap, reorth=self.DoubleOrthogonalization)
         for pot in pots:
             ids = pot.StorageIds()
@@ -130,6 +131,11 @@ class PamPropagator(PropagatorBase):
             eng.set_mixed([i for i, p in enumerate(pots) if p.IsTimeDependent])
         self._pots = pots
         return eng
+
+    def _engine_overlap(self, repr, L, N):
+        """The overlap factor (with its chunked solver plan) the engine uses:
+        the representation's own."""
+        return repr.GetRepresentation(1).GetOverlap()
 
     def _build_engine3(self, repr, pots):
         """Two-electron states (coupled channels x B-spline x B-spline): the
@@ -182,6 +188,27 @@ class PamPropagator(PropagatorBase):
         return self.Engine.diagnostics()
 
 
+def expokit_chunk(L, N, m, kd, chunk):
+    """Solver chunk length for an Expokit engine over an (L, N) problem with
+    Krylov dimension m: None when the overlap's own plan (`chunk`, 0 = the
+    default) leaves room in 160 KB of LDS for zgexpv's Hessenberg matrix
+    and exponential workspace (16 ((m+2)^2 + m+2) bytes), else the shortest
+    longer chunk that does."""
+    lib = nat.lib()
+    hm = 16 * ((m + 2) * (m + 2) + m + 2)
+
+    def fits(B):
+        est = lib.pp_pamp_lds_estimate(L, N, m, 3, kd, B)
+        return est > 0 and est + hm <= 160 * 1024
+
+    if fits(chunk):
+        return None
+    for B in (32, 64, 128, 256):
+        if B > chunk and fits(B):
+            return B
+    return None
+
+
 class ExpokitPropagator(PamPropagator):
     """Expokit zgexpv propagator (pyprop/propagator/KrylovPropagator.py:1-60,
     core/krylov/expokit/expokitpropagator.cpp:9-103): config keys
@@ -207,6 +234,19 @@ class ExpokitPropagator(PamPropagator):
             if prec not in ("fp64", "mixed"):
                 raise ValueError("matvec_precision must be 'fp64' or 'mixed', got %r" % prec)
             self.MatvecPrecision = prec
+
+    def _engine_overlap(self, repr, L, N):
+        """zgexpv keeps its (m+2) x (m+2) Hessenberg matrix in LDS beside
+        the pAMP buffers of every fiber; when the representation's solver
+        plan leaves too little of the 160 KB, the engine gets its own plan
+        with longer chunks (fewer scan matrices): same factor U, so the
+        results agree to rounding."""
+        ov = repr.GetRepresentation(1).GetOverlap()
+        B = expokit_chunk(L, N, self.BasisSize, ov.kd, ov.chunk)
+        if B is None:
+            return ov
+        b = repr.GetRepresentation(1).GetBSplineObject()
+        return dev.Overlap(b.overlap_upper(), N, ov.kd, chunk=B)
 
     def AdvanceSteps(self, times, dt):
         sc = self.TimeScalings(times)

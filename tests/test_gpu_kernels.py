@@ -229,6 +229,44 @@ def test_expokit_persistent_matches_oracle(lmax, nbp, m, nsteps, dt, anorm):
     assert err < 1e-11
 
 
+def test_expokit_bench_shape_matches_oracle():
+    """Expokit at the hydrogen_large shape (L = 64, N = 800, m = 24, dt =
+    0.005), where the default solver plan leaves no LDS for zgexpv's
+    workspace and the engine takes the longer chunk expokit_chunk picks:
+    same substep control as the oracle's zgexpv and psi within 1e-11 of
+    max|psi| after 3 steps."""
+    from pyprop_amd import device as D
+    from pyprop_amd.propagator import expokit_chunk
+    h = O.Hydrogen(rmax=400.0, nbp=797, order=7, lmax=63, frequency=0.5, cycles=10, E0=0.1)
+    m, nsteps, dt = 24, 3, 0.005
+    B = expokit_chunk(h.L, h.N, m, h.kd, 0)
+    print("expokit hydrogen_large shape: L %d N %d m %d chunk %s" % (h.L, h.N, m, B), flush=True)
+    assert B is not None
+    psi0 = h.ground_state()
+    psi0[1] = 0.1 * psi0[0]
+    ts = O.step_times(nsteps, dt, t0=10.0)
+    sc = np.array([[1.0, h.scaling(t)] for t in ts])
+    ham = h.ham()
+    ref = psi0.copy()
+    ref_stats = []
+    for s in range(nsteps):
+        ref, st = ham.propagate_expokit(ref, 1, dt, sc[s:s + 1], m, 0.0, 1e6)
+        ref_stats.append(st.copy())
+    ov = D.Overlap(h.S, h.N, h.kd, chunk=B)
+    eng = D.PampEngine(h.L, h.N, m, ov)
+    eng.add_term("Diag", D.to_device(h.H0))
+    eng.add_term("Simp", D.to_device(h.Hint), pairs=h.pairs)
+    psi = D.to_device(psi0)
+    eng.propagate_expokit(psi, nsteps, dt, sc, 0.0, 1e6)
+    got = psi.cpu().numpy()
+    st = eng.expokit_stats()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print("  rel err", err, "gpu stats", st[-1], "oracle", ref_stats[-1], flush=True)
+    for s in range(nsteps):
+        assert list(st[s, :5].astype(int)) == list(ref_stats[s]), (s, st[s], ref_stats[s])
+    assert err < 1e-11
+
+
 def test_expokit_imaginary_time_matches_oracle():
     """dt = -i tau: w = exp(-tau H^) psi~ (imaginary-time zgexpv) against the
     oracle's zgexpv with imag_time, wrapped in U / U^-1 by hand."""

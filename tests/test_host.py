@@ -122,3 +122,20 @@ def test_lowrank_factor_finds_exact_rank_of_batch_operators():
             rc = lib.pp_lowrank_factor(P0, M, nat.dptr(V.view(np.float64)), mr, ctypes.byref(rk), nat.dptr(coef),
                                        nat.dptr(bands), ctypes.byref(err))
             assert rc == 0 and rk.value == 2 and err.value < 1e-13, (d, pot.StorageIds(), rk.value, err.value)
+
+
+def test_expokit_chunk_fits_lds():
+    """The Expokit engine's solver chunk: the default plan where zgexpv's
+    Hessenberg workspace still fits 160 KB of LDS (small shapes), a longer
+    chunk at the hydrogen_large shape (L = 64, N = 800, m = 24, kd = 6),
+    where the default B = 32 plan needs 157,856 B before the 11,232 B of
+    zgexpv workspace."""
+    from pyprop_amd import _native as nat
+    from pyprop_amd.propagator import expokit_chunk
+    lib = nat.lib()
+    assert lib.pp_pamp_lds_estimate(64, 800, 24, 3, 6, 0) == lib.pp_pamp_lds_estimate(64, 800, 24, 3, 6, 32)
+    assert expokit_chunk(11, 103, 20, 6, 0) is None
+    B = expokit_chunk(64, 800, 24, 6, 0)
+    assert B == 64
+    assert lib.pp_pamp_lds_estimate(64, 800, 24, 3, 6, B) + 16 * (26 * 26 + 26) <= 160 * 1024
+    assert lib.pp_pamp_lds_estimate(64, 800, 24, 3, 6, 32) + 16 * (26 * 26 + 26) > 160 * 1024
