@@ -13,7 +13,9 @@ operator assembly the reference performs in Python:
   (example.py:220-230);
 * the 1s start state by inverse iteration on the l=0 block (the reference
   offers shift-invert Arnoldi, pyprop/solver/PiramSolver.py:36-69; for a
-  single block this is the same fixed point).
+  single block this is the same fixed point);
+* the Cayley (Crank-Nicolson) step with the reference's single-cycle GMRES
+  (`gmres_solve`, `cayley_step`; numpy, dense operators on small shapes).
 """
 import ctypes
 import os
@@ -501,6 +503,89 @@ class Ham:
         if rc:
             raise RuntimeError("oracle pAMP failed (%d)" % rc)
         return psi
+
+
+def gmres_solve(apply, b, basis_size, tol=1.4901161193847656e-08, double_orth=True, max_orth=3):
+    """x with apply(x) ~ b by gmres::GMRES<cplx>::SolveVector
+    (core/krylov/gmres/gmres.h:430-500): zero start, one cycle (no restart)
+    of at most basis_size Arnoldi vectors, Euclidean inner products (the
+    BLAS integration, gmreswrapper.cpp:91-96); the initial step
+    orthogonalises once (:257-290); later steps by classical Gram-Schmidt
+    with the 0.717 re-orthogonalisation rule, at most max_orth extra passes
+    (:293-400); tolerance sqrt(eps) unless set, floored at eps^(2/3)
+    (:67, :221-225: krylov_tolerance = 0 gives eps^(2/3)); the loop stops at
+    happy breakdown (|beta| < tol) or when the least-squares residual
+    ||H y - ||b|| e1|| is <= tol (an absolute residual, :459-490).
+    Returns (x, error estimate, operator count)."""
+    b = np.asarray(b, dtype=np.complex128).reshape(-1)
+    tol = max(tol, np.finfo(float).eps ** (2.0 / 3.0))     # default sqrt(eps) (gmres.h:67)
+    m = int(basis_size)
+    n = b.size
+    V = np.zeros((m, n), dtype=np.complex128)
+    Hm = np.zeros((m + 1, m), dtype=np.complex128)       # Hm[i, j] = h_ij
+    bnorm = np.linalg.norm(b)
+    V[0] = b / bnorm
+    r = apply(V[0])
+    nops = 1
+    alpha = np.vdot(V[0], r)
+    r = r - alpha * V[0]
+    Hm[0, 0] = alpha
+    beta = np.linalg.norm(r)
+    happy = beta < tol
+    Hm[1, 0] = beta
+    cur = 0
+    y = np.array([bnorm / alpha]) if happy else np.zeros(1, dtype=np.complex128)
+    err = beta
+    while cur < m - 1 and not happy and abs(err) > tol:
+        j = cur
+        V[j + 1] = r / Hm[j + 1, j]
+        r = apply(V[j + 1])
+        nops += 1
+        orig = np.linalg.norm(r)
+        h = V[:j + 2].conj() @ r
+        r = r - V[:j + 2].T @ h
+        res = np.linalg.norm(r)
+        count = 0
+        while double_orth and res < 0.717 * orig:
+            count += 1
+            if count > max_orth:
+                break
+            h2 = V[:j + 2].conj() @ r
+            r = r - V[:j + 2].T @ h2
+            h = h + h2
+            orig, res = res, np.linalg.norm(r)
+        Hm[:j + 2, j + 1] = h
+        beta = np.linalg.norm(r)
+        happy = beta < tol
+        Hm[j + 2, j + 1] = beta
+        cur += 1
+        k = cur + 1
+        e1 = np.zeros(k + 1, dtype=np.complex128)
+        e1[0] = bnorm
+        y, *_ = np.linalg.lstsq(Hm[:k + 1, :k], e1, rcond=None)
+        err = np.linalg.norm(Hm[:k + 1, :k] @ y - e1)
+    k = cur + 1
+    return V[:k].T @ y[:k], err, nops
+
+
+def cayley_step(apply_h, S_dense, M_solve, psi, dt, basis_size, tol=1.4901161193847656e-08):
+    """One CayleyPropagator.AdvanceStep (pyprop/propagator/CayleyPropagator.py:73-140):
+    rhs = M^-1 (S - i dt/2 H) psi, then psi' solves M^-1 (S + i dt/2 H) psi'
+    = rhs by gmres_solve; H = apply_h(x) is the raw (non-balanced)
+    Hamiltonian at t + |dt/2|, S the overlap applied along the B-spline rank
+    (S_dense: N x N), M_solve the left preconditioner (identity when None).
+    psi is (L, N).  Returns (psi', error estimate, operator count)."""
+    psi = np.asarray(psi, dtype=np.complex128)
+    shape = psi.shape
+    prec = M_solve if M_solve is not None else (lambda x: x)
+
+    def sph(x, f):
+        x = x.reshape(shape)
+        return (x @ S_dense.T + f * apply_h(x)).reshape(-1)
+
+    rhs = prec(sph(psi, -0.5j * dt).reshape(shape)).reshape(-1)
+    x, err, nops = gmres_solve(lambda v: prec(sph(v, 0.5j * dt).reshape(shape)).reshape(-1), rhs, basis_size, tol)
+    return x.reshape(shape), err, nops
 
 
 def zgpadm(A, t, ideg=6):

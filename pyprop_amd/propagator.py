@@ -256,3 +256,104 @@ class ExpokitPropagator(PamPropagator):
         """Per step of the last launch: matvecs, exponentials, substeps,
         rejected substeps, basis size of the last substep, zgexpv iflag."""
         return self.Engine.expokit_stats()
+
+
+class CayleyPropagator(PropagatorBase):
+    """Crank-Nicolson step (S + i dt/2 H) psi(t+dt) = (S - i dt/2 H) psi(t),
+    H at the midpoint t + |dt/2|, solved by the reference's single-cycle
+    GMRES with an optional left preconditioner
+    (pyprop/propagator/CayleyPropagator.py:1-140): the [Propagation] keys
+    base_propagator, preconditioner, krylov_basis_size, krylov_tolerance
+    and krylov_double_orthogonalization keep their meaning.  psi(t) is the
+    GMRES right-hand side's source and psi(t+dt) its solution (zero start).
+    Rank-2 states apply H through the base propagator's tensor-potential
+    kernels; rank-3 (two-electron) states through the rank-3 engine's raw
+    apply (no preconditioner: BSplinePreconditioner covers rank-2 states)."""
+
+    def __init__(self, psi):
+        PropagatorBase.__init__(self, psi)
+        from .solver import GmresSolver
+        self.Rank = psi.GetRank()
+        self.Solver = GmresSolver(psi)
+        self.Preconditioner = None
+        self.LastErrorEstimate = 0.0
+
+    def ApplyConfig(self, config):
+        PropagatorBase.ApplyConfig(self, config)
+        self.BasePropagator = config.Propagation.base_propagator(self.psi)
+        config.Apply(self.BasePropagator)
+        self.Solver.ApplyConfigSection(config.Propagation)
+        name = config.Propagation.get("preconditioner", None)
+        if name:
+            sec = config.GetSection(name)
+            self.Preconditioner = sec.type(self.psi)
+            sec.Apply(self.Preconditioner)
+
+    def ApplyConfigSection(self, configSection):
+        PropagatorBase.ApplyConfigSection(self, configSection)
+        configSection.Apply(self.BasePropagator)
+        self.Solver.ApplyConfigSection(configSection)
+
+    def SetupStep(self, dt):
+        self.BasePropagator.SetupStep(dt)
+        self.Solver.Setup(self.psi)
+        self.TempPsi = self.psi.Copy()
+        repr = self.psi.GetRepresentation()
+        self.IsOrthogonalBasis = all(repr.IsOrthogonalBasis(i) for i in range(self.Rank))
+        self._engine3 = None
+        if self.Rank == 3:
+            if self.Preconditioner is not None:
+                raise NotImplementedError("the Cayley preconditioner on the HIP path covers rank-2 states")
+            # raw H of the two-electron engine (its Krylov buffers are sized
+            # by the GMRES basis but unused here)
+            helper = PamPropagator(self.psi)
+            helper.BasePropagator = self.BasePropagator
+            helper.BasisSize = max(2, self.Solver.BasisSize)
+            self._engine3 = helper._build_engine3(repr, self.BasePropagator.PotentialList)
+            self._helper3 = helper
+        elif self.Rank != 2:
+            raise NotImplementedError("CayleyPropagator on the HIP path handles rank-2 and rank-3 states")
+        if self.Preconditioner is not None:
+            self.Preconditioner.SetHamiltonianScaling(1.0j * dt / 2.0)
+            self.Preconditioner.SetOverlapScaling(1.0)
+            self.Preconditioner.Setup(self)
+        self.TempPsiMultiply = self.psi.Copy()
+
+    def MultiplyHamiltonian(self, srcPsi, destPsi, t, dt):
+        self.BasePropagator.MultiplyHamiltonian(srcPsi, destPsi, t, dt)
+
+    def _raw_hamiltonian(self, srcPsi, destPsi, t, dt):
+        if self._engine3 is not None:
+            sc = self._helper3.TimeScalings([t])[0]
+            self._engine3.apply(srcPsi.GetData(), destPsi.GetData(), sc, balanced=False)
+        else:
+            self.BasePropagator.MultiplyHamiltonianNoOverlap(srcPsi, destPsi, t, dt)
+
+    def MultiplySpH(self, sourcePsi, destPsi, hFactor, t, dt):
+        """destPsi = (S + hFactor H(t)) sourcePsi (CayleyPropagator.py:117-134)."""
+        tmp = self.TempPsiMultiply
+        tmp.GetData().copy_(sourcePsi.GetData())
+        if not self.IsOrthogonalBasis:
+            tmp.GetRepresentation().MultiplyOverlap(tmp)
+        self._raw_hamiltonian(sourcePsi, destPsi, t, dt)
+        destPsi.GetData().mul_(hFactor).add_(tmp.GetData())
+
+    def SolverCallback(self, sourcePsi, destPsi, t, dt):
+        self.MultiplySpH(sourcePsi, destPsi, 1.0j * dt / 2.0, t, dt)
+        if self.Preconditioner is not None:
+            self.Preconditioner.Solve(destPsi)
+
+    def AdvanceStep(self, t, dt):
+        tm = t + abs(dt / 2.0)
+        self.TempPsi.Clear()
+        self.MultiplySpH(self.psi, self.TempPsi, -1.0j * dt / 2.0, tm, dt)
+        if self.Preconditioner is not None:
+            self.Preconditioner.Solve(self.TempPsi)
+        self.Solver.Solve(lambda s, d: self.SolverCallback(s, d, tm, dt), self.TempPsi, self.psi, False)
+        self.LastErrorEstimate = self.Solver.GetErrorEstimate()
+        if self.LastErrorEstimate > 1e-4:
+            print("Error = %s" % self.LastErrorEstimate)
+
+    def AdvanceSteps(self, times, dt):
+        for t in times:
+            self.AdvanceStep(t, dt)

@@ -479,3 +479,156 @@ class GMRESShiftInvertSolver:
     def PrintStatistics(self):
         print("Statistics for GMRESShiftInvertSolver")
         print(self.Counters)
+
+
+class GmresSolver:
+    """The reference's GMRES wrapper (core.krylov_GmresWrapper,
+    core/krylov/gmres/gmreswrapper.cpp:52-111 over gmres::GMRES<cplx>,
+    gmres.h:207-500) on device vectors: zero start, one cycle of at most
+    krylov_basis_size Arnoldi vectors, Euclidean inner products, classical
+    Gram-Schmidt with the 0.717 re-orthogonalisation rule (at most 3 extra
+    passes; krylov_double_orthogonalization, on by default), tolerance
+    floored at eps^(2/3), stop at happy breakdown or when the least-squares
+    residual is below the tolerance.  Arnoldi vectors, Gram-Schmidt and the
+    solution update run on the device (pp_gram_schmidt, pp_basis_combine);
+    the (k+1) x k least-squares problem on the host."""
+
+    def __init__(self, psi=None):
+        self.BasisSize = 20
+        self.Tolerance = float(np.sqrt(np.finfo(float).eps))
+        self.PerformDoubleOrthogonalization = True
+        self.MaxOrthogonalizationCount = 3
+        self.ErrorEstimate = 0.0
+        self.ErrorEstimateList = np.zeros(0)
+        self.HessenbergMatrix = np.zeros((0, 0), dtype=np.complex128)
+        self.OperatorCount = 0
+        self.OrthogonalizationCount = 0
+        self.V = None
+
+    def ApplyConfigSection(self, conf):
+        self.BasisSize = int(conf.krylov_basis_size)
+        if hasattr(conf, "krylov_tolerance"):
+            self.Tolerance = float(conf.krylov_tolerance)
+        if hasattr(conf, "krylov_double_orthogonalization"):
+            self.PerformDoubleOrthogonalization = bool(conf.krylov_double_orthogonalization)
+
+    def Setup(self, psi):
+        if self.BasisSize < 1:
+            raise ValueError("krylov_basis_size must be at least 1")
+        self.Tolerance = max(self.Tolerance, np.finfo(float).eps ** (2.0 / 3.0))
+        self.Shape = tuple(psi.GetData().shape)
+        self.MatrixSize = int(np.prod(self.Shape))
+        t = dev.torch()
+        self.V = t.zeros((self.BasisSize, self.MatrixSize), dtype=t.complex128, device=psi.GetData().device)
+        self.Residual = t.zeros(self.MatrixSize, dtype=t.complex128, device=psi.GetData().device)
+        self.ErrorEstimateList = np.zeros(self.BasisSize)
+        self.ResetStatistics()
+
+    def ResetStatistics(self):
+        self.OperatorCount = 0
+        self.OrthogonalizationCount = 0
+
+    def _gs(self, k, w):
+        """w -= V_k V_k^H w (one pass); returns (V_k^H w, |w|) after the pass."""
+        h = np.zeros(2 * max(k, 1))
+        nrm = ctypes.c_double(0.0)
+        nat.check(nat.lib().pp_gram_schmidt(dev.ptr(self.V), self.MatrixSize, int(k), dev.ptr(w), 1, nat.dptr(h),
+                                            ctypes.byref(nrm), dev.stream_ptr()), "pp_gram_schmidt")
+        return h[0:2 * k:2] + 1j * h[1:2 * k:2], nrm.value
+
+    def _apply(self, src, dst):
+        """ApplyOperator (gmreswrapper.cpp:17-50): the callback sees the two
+        wavefunctions of the Solve call holding the input / output buffers."""
+        self.OperatorCount += 1
+        a, b = self._psi, self._tmp
+        da, db = a.GetData(), b.GetData()
+        a.SetData(src.view(self.Shape))
+        b.SetData(dst.view(self.Shape))
+        dst.zero_()
+        try:
+            self._callback(a, b)
+        finally:
+            a.SetData(da)
+            b.SetData(db)
+
+    def Solve(self, callback, psi, tempPsi, usePypropIntegration=False):
+        """tempPsi := x with A x = psi, A applied by callback(src, dst)."""
+        if usePypropIntegration:
+            raise NotImplementedError("GMRES with the S inner product is not on the HIP path")
+        if self.V is None:
+            raise RuntimeError("GmresSolver.Setup was not called")
+        self._callback, self._psi, self._tmp = callback, psi, tempPsi
+        m, V, r = self.BasisSize, self.V, self.Residual
+        tol = self.Tolerance
+        Hm = np.zeros((m + 1, m), dtype=np.complex128)
+        self.ErrorEstimateList[:] = 0.0
+        r.copy_(psi.GetData().reshape(-1))
+        _, bnorm = self._gs(0, r)
+        sol = tempPsi.GetData().reshape(-1)
+        if bnorm == 0.0:
+            sol.zero_()
+            self.ErrorEstimate = 0.0
+            self.HessenbergMatrix = Hm
+            return
+        r.mul_(1.0 / bnorm)
+        V[0].copy_(r)
+        self._apply(V[0], r)
+        h, beta = self._gs(1, r)
+        Hm[0, 0], Hm[1, 0] = h[0], beta
+        happy = beta < tol
+        y = np.array([bnorm / h[0]]) if happy else np.zeros(1, dtype=np.complex128)
+        cur = 0
+        err = beta
+        self.ErrorEstimateList[0] = err
+        while cur < m - 1 and not happy and abs(err) > tol:
+            j = cur
+            V[j + 1].copy_(r).mul_(complex(1.0 / Hm[j + 1, j]))
+            self._apply(V[j + 1], r)
+            _, orig = self._gs(0, r)
+            h, res = self._gs(j + 2, r)
+            count = 0
+            while self.PerformDoubleOrthogonalization and res < 0.717 * orig:
+                count += 1
+                if count > self.MaxOrthogonalizationCount:
+                    break
+                self.OrthogonalizationCount += 1
+                h2, res2 = self._gs(j + 2, r)
+                h = h + h2
+                orig, res = res, res2
+            Hm[:j + 2, j + 1] = h
+            beta = res
+            happy = beta < tol
+            Hm[j + 2, j + 1] = beta
+            cur += 1
+            k = cur + 1
+            e1 = np.zeros(k + 1, dtype=np.complex128)
+            e1[0] = bnorm
+            y, *_ = np.linalg.lstsq(Hm[:k + 1, :k], e1, rcond=None)
+            err = float(np.linalg.norm(Hm[:k + 1, :k] @ y - e1))
+            self.ErrorEstimateList[cur] = err
+        k = cur + 1
+        Yf = np.asfortranarray(np.asarray(y[:k], dtype=np.complex128).reshape(k, 1))
+        nat.check(nat.lib().pp_basis_combine(dev.ptr(V), self.MatrixSize, k,
+                                             Yf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 1, dev.ptr(sol),
+                                             dev.stream_ptr()), "pp_basis_combine")
+        self.ErrorEstimate = err
+        self.HessenbergMatrix = Hm[:k + 1, :k].copy()
+
+    def GetErrorEstimate(self):
+        return abs(self.ErrorEstimate)
+
+    def GetErrorEstimateList(self):
+        return self.ErrorEstimateList
+
+    def GetHessenbergMatrix(self):
+        return self.HessenbergMatrix
+
+    def GetOperatorCount(self):
+        return self.OperatorCount
+
+    def GetOrthogonalizationCount(self):
+        return self.OrthogonalizationCount
+
+    def EstimateMemoryUsage(self, matrixSize, basisSize):
+        """MiB of the Arnoldi workspace (gmres.h:197-203)."""
+        return ((matrixSize * (basisSize + 3.0)) + basisSize * (3 * basisSize + 7.0)) * 16 / (1024.0 * 1024.0)

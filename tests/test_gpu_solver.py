@@ -185,3 +185,55 @@ def test_piram_helium_ground_level_matches_dense():
     E = solver.GetEigenvalues()[0]
     print("  E0 %.12f dense %.12f restarts %d" % (E, E0, solver.RestartCount), flush=True)
     assert abs(E - E0) <= 1e-9 * abs(E0)
+
+
+@pytest.mark.parametrize("precondition", [True, False])
+def test_cayley_steps_match_oracle(precondition):
+    """CayleyPropagator (propagator = CayleyPropagator, GMRES basis 20,
+    tolerance 0 -> eps^(2/3), optionally the RadialPreconditioner block
+    solve) for 4 laser steps from t = 150 against the oracle's cayley_step
+    with the reference's GMRES on the same operator: psi within 1e-10 of
+    max|psi| and the same operator count per step.  Preconditioned, GMRES
+    converges in 6 operator calls and the S-norm is kept to 1e-9; without
+    it, 20 Arnoldi vectors leave a residual near 1e-3 (the reference then
+    prints its "Error =" line) and both sides stop at the same truncated
+    solution."""
+    from oracle import oracle as O
+    print("Cayley: maxl 5, N 40, preconditioner %s" % precondition, flush=True)
+    over = {("AngularRepresentation", "maxl"): 5, ("RadialRepresentation", "xsize"): 37,
+            ("RadialRepresentation", "xmax"): 40.0, ("Propagation", "timestep"): 0.05}
+    conf = H.load("hydrogen_small", over)
+    conf.Propagation.propagator = P.CayleyPropagator
+    conf.Propagation.preconditioner = "RadialPreconditioner" if precondition else None
+    prob = P.Problem(conf)
+    prob.SetupStep()
+    psi0 = H.ground_state(prob)
+    psi0[1] = 0.2 * psi0[0]
+    prob.SetWavefunction(psi0)
+    t0, dt, K = 150.0, 0.05, 4
+    prob.PropagatedTime = t0
+    h = O.Hydrogen(rmax=40.0, nbp=37, order=7, lmax=5, frequency=0.057, cycles=6, E0=0.05)
+    assert prob.psi.GetData().shape == h.shape
+    ham = h.ham()
+    S = h.overlap_dense()
+    L = h.shape[0]
+    Ml = [S + 0.5j * dt * h.band_to_dense(h.H0[l]) for l in range(L)]
+    msolve = (lambda x: np.stack([np.linalg.solve(Ml[l], x[l]) for l in range(L)])) if precondition else None
+    ref = psi0.copy()
+    n0 = prob.psi.GetNorm()
+    solver = prob.Propagator.Solver
+    for k in range(K):
+        t = t0 + k * dt
+        ham.set_scaling([1.0, h.scaling(t + dt / 2)])
+        ref, err_ref, nops_ref = O.cayley_step(ham.potentials, S, msolve, ref, dt, 20, 0.0)
+        before = solver.GetOperatorCount()
+        prob.AdvanceStep()
+        nops = solver.GetOperatorCount() - before
+        got = prob.psi.GetData().cpu().numpy()
+        rel = np.abs(got - ref).max() / np.abs(ref).max()
+        print("  step %d: rel diff %.3e, operator calls %d (oracle %d), error estimate %.3e (oracle %.3e)"
+              % (k, rel, nops, nops_ref, solver.GetErrorEstimate(), err_ref), flush=True)
+        assert nops == nops_ref
+        assert rel < 1e-10
+    if precondition:
+        assert abs(prob.psi.GetNorm() - n0) < 1e-9

@@ -295,3 +295,64 @@ def test_oracle_result_independent_of_thread_count():
     finally:
         O.set_threads(prev)
     assert np.array_equal(out[0], out[1])
+
+
+def test_gmres_restatement_against_dense_solves():
+    """gmres_solve: converged (basis = n) it equals numpy's solve; truncated,
+    its error estimate equals the true residual ||A x - b|| (the least-squares
+    residual of the Hessenberg problem is that residual in exact arithmetic,
+    so a transposed Hessenberg or a dropped projection term fails here) and
+    decreases with the basis size (GMRES minimises the residual over nested
+    Krylov spaces); A = 2I ends after one operator call with x = b / 2."""
+    rng = np.random.default_rng(4)
+    n = 60
+    R = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = np.eye(n) + 0.9 * R / np.linalg.norm(R, 2)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x, err, nops = O.gmres_solve(lambda v: A @ v, b, n, 1e-13)
+    ref = np.linalg.solve(A, b)
+    assert np.abs(x - ref).max() < 1e-10 * np.abs(ref).max()
+    assert err <= 1e-10 * np.linalg.norm(b) and nops <= n
+    errs = []
+    for m in (4, 8, 16):
+        x, err, nops = O.gmres_solve(lambda v: A @ v, b, m, 1e-13)
+        assert nops == m
+        assert abs(err - np.linalg.norm(A @ x - b)) < 1e-12 * np.linalg.norm(b)
+        errs.append(err)
+    assert errs[0] > errs[1] > errs[2]
+    x, err, nops = O.gmres_solve(lambda v: 2.0 * v, b, 10)
+    assert nops == 1 and np.abs(x - b / 2).max() < 1e-15 * np.abs(b).max()
+
+
+def test_cayley_step_against_dense_crank_nicolson(small):
+    """cayley_step with a full basis equals the dense Crank-Nicolson step
+    (S + i dt/2 H) psi' = (S - i dt/2 H) psi, with or without the block
+    preconditioner M_l = S + i dt/2 H0_l, and keeps the S-norm (H is
+    Hermitian, so the Cayley map is S-unitary)."""
+    h = small
+    L, N = h.shape
+    S = h.overlap_dense()
+    ham = h.ham()
+    t, dt = 3.0, 0.05
+    ham.set_scaling([1.0, h.scaling(t + dt / 2)])
+    rng = np.random.default_rng(5)
+    psi = rng.standard_normal((L, N)) + 1j * rng.standard_normal((L, N))
+    Hd = np.zeros((L * N, L * N), dtype=np.complex128)
+    for c in range(L * N):
+        e = np.zeros(L * N, dtype=np.complex128)
+        e[c] = 1.0
+        Hd[:, c] = ham.potentials(e.reshape(L, N)).reshape(-1)
+    assert np.abs(Hd - Hd.conj().T).max() < 1e-10 * np.abs(Hd).max()
+    Sb = np.kron(np.eye(L), S)
+    ref = np.linalg.solve(Sb + 0.5j * dt * Hd, (Sb - 0.5j * dt * Hd) @ psi.reshape(-1)).reshape(L, N)
+    Ml = [S + 0.5j * dt * h.band_to_dense(h.H0[l]) for l in range(L)]
+
+    def msolve(x):
+        return np.stack([np.linalg.solve(Ml[l], x[l]) for l in range(L)])
+
+    for prec in (None, msolve):
+        out, err, nops = O.cayley_step(ham.potentials, S, prec, psi, dt, L * N, 1e-14)
+        assert np.abs(out - ref).max() < 1e-9 * np.abs(ref).max(), prec
+        n0 = np.vdot(psi.reshape(-1), Sb @ psi.reshape(-1)).real
+        n1 = np.vdot(out.reshape(-1), Sb @ out.reshape(-1)).real
+        assert abs(n1 - n0) < 1e-9 * n0
