@@ -167,6 +167,66 @@ __global__ void __launch_bounds__(64) pv_gbtrs(int nfib, int N, int kl, int ku, 
   }
 }
 
+// The same substitutions with one wavefront per fiber: the fiber's vector
+// lives in LDS and the band factor is staged through LDS 64 rows at a time
+// (coalesced loads), so each step of the recurrence waits on LDS rather than
+// on a global-memory round trip.  Per step the pivot swap and the kl (forward)
+// or kd (backward) updates run on separate lanes.
+#define PV_GB_ROWS 64
+__global__ void __launch_bounds__(64) pv_gbtrs_lds(int N, int kl, int ku, const zd *lub, const int *piv, zd *x)
+{
+  extern __shared__ zd pv_lds[];
+  const int a = blockIdx.x, lane = threadIdx.x;
+  const int ldab = 2 * kl + ku + 1, kd = kl + ku;
+  zd *bs = pv_lds;                              // N
+  zd *As = bs + N;                              // PV_GB_ROWS columns of ldab
+  int *ps = (int *)(As + PV_GB_ROWS * ldab);    // PV_GB_ROWS pivots
+  const zd *A = lub + (long)a * ldab * N;
+  const int *p = piv + (long)a * N;
+  zd *b = x + (long)a * N;
+  for (int i = lane; i < N; i += 64) bs[i] = b[i];
+  for (int j0 = 0; j0 < N - 1; j0 += PV_GB_ROWS) {
+    const int nr = min(PV_GB_ROWS, N - 1 - j0);
+    __syncthreads();
+    for (int e = lane; e < nr * ldab; e += 64) As[e] = A[(long)j0 * ldab + e];
+    for (int e = lane; e < nr; e += 64) ps[e] = p[j0 + e];
+    __syncthreads();
+    for (int r = 0; r < nr; r++) {
+      const int j = j0 + r;
+      const int lm = min(kl, N - 1 - j);
+      const int l = ps[r];
+      const zd bl = bs[l], bj = bs[j];
+      zd v = zmk(0.0, 0.0);
+      if (lane >= 1 && lane <= lm) v = (j + lane == l) ? bj : bs[j + lane];
+      __syncthreads();
+      if (lane == 0) bs[j] = bl;
+      else if (lane <= lm) bs[j + lane] = zsub(v, zmul(As[r * ldab + kd + lane], bl));
+      __syncthreads();
+    }
+  }
+  for (int jh = N - 1; jh >= 0; jh -= PV_GB_ROWS) {
+    const int jl = jh - PV_GB_ROWS + 1 > 0 ? jh - PV_GB_ROWS + 1 : 0;
+    __syncthreads();
+    for (int e = lane; e < (jh - jl + 1) * ldab; e += 64) As[e] = A[(long)jl * ldab + e];
+    __syncthreads();
+    for (int j = jh; j >= jl; j--) {
+      const zd *col = As + (j - jl) * ldab;
+      const zd d = col[kd];
+      const double dn = zabs2(d);
+      const zd bj = zmul(bs[j], zmk(d.x / dn, -d.y / dn));
+      const int i = j - lane;
+      zd v = zmk(0.0, 0.0);
+      if (lane >= 1 && lane <= kd && i >= 0) v = bs[i];
+      __syncthreads();
+      if (lane == 0) bs[j] = bj;
+      else if (lane <= kd && i >= 0) bs[i] = zsub(v, zmul(col[kd - lane], bj));
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  for (int i = lane; i < N; i += 64) b[i] = bs[i];
+}
+
 // Solve A_a x_a = b_a in place for nfib fibers of length N (x: nfib x N
 // complex128, device); lub: nfib band LU factors (device, nfib x N x ldab
 // complex128), piv: nfib x N 0-based pivots (device).  Asynchronous.
@@ -174,7 +234,12 @@ extern "C" int pp_band_lu_solve(int nfib, int N, int kl, int ku, const void *lub
                                 hipStream_t stream)
 {
   if (nfib <= 0 || N <= 0 || kl < 0 || ku < 0 || !lub || !piv || !x) return PP_EARG;
-  hipLaunchKernelGGL(pv_gbtrs, dim3((nfib + 63) / 64), dim3(64), 0, stream, nfib, N, kl, ku, (const zd *)lub, piv,
-                     (zd *)x);
+  const int ldab = 2 * kl + ku + 1;
+  const size_t lds = sizeof(zd) * ((size_t)N + (size_t)PV_GB_ROWS * ldab) + sizeof(int) * PV_GB_ROWS;
+  if (kl + ku < 64 && lds <= 64 * 1024)
+    hipLaunchKernelGGL(pv_gbtrs_lds, dim3(nfib), dim3(64), lds, stream, N, kl, ku, (const zd *)lub, piv, (zd *)x);
+  else
+    hipLaunchKernelGGL(pv_gbtrs, dim3((nfib + 63) / 64), dim3(64), 0, stream, nfib, N, kl, ku, (const zd *)lub, piv,
+                       (zd *)x);
   return hipGetLastError() == hipSuccess ? PP_OK : PP_EHIP;
 }

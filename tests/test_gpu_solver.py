@@ -237,3 +237,46 @@ def test_cayley_steps_match_oracle(precondition):
         assert rel < 1e-10
     if precondition:
         assert abs(prob.psi.GetNorm() - n0) < 1e-9
+
+
+@pytest.mark.parametrize("N,kl,nfib", [(800, 6, 5), (37, 6, 3), (5000, 6, 2), (130, 2, 4)])
+def test_band_lu_solve_matches_lapack(N, kl, nfib):
+    """pp_band_lu_solve (LDS kernel for N = 37 .. 800, the one-thread-per-fiber
+    kernel for N = 5000, whose vector does not fit its LDS budget) against
+    LAPACK zgbtrs on the same zgbtrf factors of random diagonally weighted
+    band matrices (the diagonal is not dominant, so most rows pivot),
+    relative 1e-13 (measured <= 3.5e-15): the device divides by multiplying
+    with conj(d)/|d|^2 where LAPACK uses Smith's complex division, so the
+    last bits differ, scaled by the condition number (7e3 at N = 800); an
+    indexing or pivoting mistake gives O(1) errors."""
+    import scipy.linalg.lapack as lapack
+    print("band LU solve: N %d, kl %d, fibers %d" % (N, kl, nfib), flush=True)
+    rng = np.random.default_rng(N + kl)
+    ku = kl
+    ldab = 2 * kl + ku + 1
+    lub = np.zeros((nfib, N, ldab), dtype=np.complex128)
+    piv = np.zeros((nfib, N), dtype=np.int32)
+    b = rng.standard_normal((nfib, N)) + 1j * rng.standard_normal((nfib, N))
+    ref = np.zeros_like(b)
+    for a in range(nfib):
+        ab = np.zeros((ldab, N), dtype=np.complex128, order="F")
+        ab[kl:] = rng.standard_normal((kl + ku + 1, N)) + 1j * rng.standard_normal((kl + ku + 1, N))
+        ab[kl + ku] += 0.5
+        lu, pv, info = lapack.zgbtrf(ab, kl, ku)
+        assert info == 0
+        x, info = lapack.zgbtrs(lu, kl, ku, b[a], pv)
+        assert info == 0
+        ref[a] = x
+        lub[a] = lu.T
+        piv[a] = pv
+    t = dev.torch()
+    xd = dev.to_device(b)
+    lubd = dev.to_device(lub)
+    pivd = t.from_numpy(piv).to(xd.device)
+    nat.check(nat.lib().pp_band_lu_solve(nfib, N, kl, ku, dev.ptr(lubd),
+                                         ctypes.cast(ctypes.c_void_p(pivd.data_ptr()), ctypes.POINTER(ctypes.c_int)),
+                                         dev.ptr(xd), dev.stream_ptr()), "pp_band_lu_solve")
+    got = xd.cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print("  rel err %.3e" % err, flush=True)
+    assert err < 1e-13
