@@ -250,7 +250,7 @@ int pp_pamp3_destroy(pp_pamp3 *p);
  * m x np complex128; out must not overlap V. */
 int pp_gram_schmidt(const void *V, long n, int k, void *w, int passes, double *h, double *wnorm, pp_stream stream);
 int pp_basis_combine(const void *V, long n, int m, const double *Q, int np, void *out, pp_stream stream);
-/* Per-fiber banded LU solve of the shift-invert preconditioner (the device
+/* Per-fiber banded LU solve of the shift-invert and Cayley preconditioner (the device
  * side of pyprop's BSplinePreconditioner.Solve,
  * examples/tensor/helium_stabilization_sae/example.py:275-318): x_a := A_a^-1
  * x_a for nfib fibers of length N, A_a factored by LAPACK zgbtrf (lub: nfib
