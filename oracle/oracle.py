@@ -572,16 +572,18 @@ def cayley_step(apply_h, S_dense, M_solve, psi, dt, basis_size, tol=1.4901161193
     """One CayleyPropagator.AdvanceStep (pyprop/propagator/CayleyPropagator.py:73-140):
     rhs = M^-1 (S - i dt/2 H) psi, then psi' solves M^-1 (S + i dt/2 H) psi'
     = rhs by gmres_solve; H = apply_h(x) is the raw (non-balanced)
-    Hamiltonian at t + |dt/2|, S the overlap applied along the B-spline rank
-    (S_dense: N x N), M_solve the left preconditioner (identity when None).
-    psi is (L, N).  Returns (psi', error estimate, operator count)."""
+    Hamiltonian at t + |dt/2|, S the overlap: an N x N matrix applied along
+    the last rank, or a callable (two-electron states: both radial ranks),
+    M_solve the left preconditioner (identity when None).  Returns (psi',
+    error estimate, operator count)."""
     psi = np.asarray(psi, dtype=np.complex128)
     shape = psi.shape
     prec = M_solve if M_solve is not None else (lambda x: x)
+    apply_s = S_dense if callable(S_dense) else (lambda x: x @ S_dense.T)
 
     def sph(x, f):
         x = x.reshape(shape)
-        return (x @ S_dense.T + f * apply_h(x)).reshape(-1)
+        return (apply_s(x) + f * apply_h(x)).reshape(-1)
 
     rhs = prec(sph(psi, -0.5j * dt).reshape(shape)).reshape(-1)
     x, err, nops = gmres_solve(lambda v: prec(sph(v, 0.5j * dt).reshape(shape)).reshape(-1), rhs, basis_size, tol)
