@@ -6,6 +6,8 @@
  *  -1  invalid argument (null pointer, shape / storage mismatch)
  *  -2  combination not implemented on this path
  *  -3  HIP runtime error
+ *  -4  a grid barrier of a persistent launch timed out: the launch's
+ *      results are invalid (reported by the *_diagnostics calls)
  *  >0  numerical failure (band Cholesky: LAPACK info; see each function)
  * Device pointers are complex128 (interleaved re, im; the layout of
  * std::complex<double> / blitz::Array<cplx, R>), C-ordered with the largest
@@ -104,7 +106,11 @@ int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re, double dt
                       const double *scal, pp_stream stream);
 /* Per step of the last launch (host, nsteps x 4): |last Krylov coefficient|
  * (pamp.h:666; -1 if the Pade denominator was singular, pamp.h:544-550),
- * basis size used (< m after happy breakdown), |U psi|, re-orthogonalisation passes. */
+ * basis size used (< m after happy breakdown), |U psi|, re-orthogonalisation passes.
+ * |U psi| = 0 marks a zero wavefunction (the step leaves it zero; the
+ * reference prints "Wavefunction is zero" and divides by the zero norm).
+ * Returns -4 if a grid barrier of the launch timed out.  The Python layer
+ * raises RuntimeError on each of these, as the reference throws. */
 int pp_pamp_diagnostics(pp_pamp *p, int nsteps, double *diag, pp_stream stream);
 /* nsteps consecutive ExpokitPropagator.AdvanceStep(t_n, dt)
  * (pyprop/propagator/KrylovPropagator.py:41-45, core.krylov_ExpokitPropagator_R,
@@ -120,8 +126,10 @@ int pp_expokit_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re, double
 /* Per step of the last pp_expokit_propagate (host, nsteps x 6): matvecs,
  * exponentials, substeps, rejected substeps, basis size of the last substep,
  * zgexpv iflag (0 ok, 1 more than 500 substeps, -9 Pade failure, -8 more than
- * 100 rejections of one substep, a bound zgexpv does not have); the
- * reference throws on iflag != 0 (expokitpropagator.cpp:96-100). */
+ * 100 rejections of one substep, a bound zgexpv does not have, -10 zero
+ * wavefunction: no substep, psi stays zero); the reference throws on
+ * iflag != 0 (expokitpropagator.cpp:96-100).  Returns -4 if a grid barrier
+ * of the launch timed out. */
 int pp_expokit_diagnostics(pp_pamp *p, int nsteps, double *stats, pp_stream stream);
 /* Batched form (no reference counterpart; the reference runs one problem per
  * process): nsteps pAMP steps of each of ninst independent engines (own

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libpyprop_amd.so")
 if os.environ.get("PYPROP_AMD_LIB"):   # an alternative in-tree build (launch-shape tuning)
     LIB_PATH = os.path.abspath(os.environ["PYPROP_AMD_LIB"])
 
-PP_OK, PP_EARG, PP_EUNSUP, PP_EHIP = 0, -1, -2, -3
+PP_OK, PP_EARG, PP_EUNSUP, PP_EHIP, PP_ETIMEOUT = 0, -1, -2, -3, -4
 
 # storage ids (tensorpotentialmultiply_generator.py:1336)
 STORAGE_IDS = {"Ident": 0, "Diag": 1, "Simp": 2, "Herm": 3, "BandNH": 4, "Band": 5}
@@ -44,7 +44,8 @@ class NativeError(RuntimeError):
 def check(rc, what):
     if rc == PP_OK:
         return
-    names = {PP_EARG: "invalid argument", PP_EUNSUP: "unsupported on the HIP path", PP_EHIP: "HIP runtime error"}
+    names = {PP_EARG: "invalid argument", PP_EUNSUP: "unsupported on the HIP path", PP_EHIP: "HIP runtime error",
+             PP_ETIMEOUT: "a grid barrier of the persistent launch timed out (its results are invalid)"}
     raise NativeError("%s failed: %s (status %d)" % (what, names.get(rc, "numerical failure"), rc))
 
 
@@ -55,6 +56,11 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise NativeError("libpyprop_amd.so not built; run pyprop_amd.build.build() (hipcc, gfx950)")
+    # torch ships its own libamdhip64.so.7; importing it first makes the
+    # library bind to that same HIP runtime (one runtime per process).  Loaded
+    # the other way round, /opt/rocm's runtime comes first and the library's
+    # own calls then report "no ROCm-capable device is detected".
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIB_PATH)
     P = ctypes.c_void_p
     I = ctypes.c_int

@@ -12,13 +12,26 @@
 #define PP_EARG (-1)       // bad argument (shape, storage id, null pointer)
 #define PP_EUNSUP (-2)     // combination not implemented on this path
 #define PP_EHIP (-3)       // HIP runtime error
+#define PP_ETIMEOUT (-4)   // a grid barrier of a persistent launch timed out (its results are invalid)
 #define PP_ENUM 1          // numerical failure (Cholesky info, singular Pade denominator)
 
+#ifdef PP_DEBUG_ERRORS   // a diagnostic build: name the failing call
+#include <cstdio>
+#define PP_HIP(call)                                                                            \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) {                                                                     \
+      fprintf(stderr, "pyprop_amd: %s:%d: %s: %s\n", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+      return PP_EHIP;                                                                           \
+    }                                                                                           \
+  } while (0)
+#else
 #define PP_HIP(call)                                   \
   do {                                                 \
     hipError_t e_ = (call);                            \
     if (e_ != hipSuccess) return PP_EHIP;              \
   } while (0)
+#endif
 
 typedef double2 zd;
 

@@ -536,7 +536,12 @@ __device__ __attribute__((always_inline)) inline void pp_pamp_body(const pp_pamp
       const int nk = j + 1, ncomp = 2 * nk;
       grid_partial_sum(A.part, A.pstr, NRM, 1, L, hred, scr);
       beta = sqrt(hred[0]);
-      if (j == 0) beta0 = beta;
+      if (j == 0) {
+        beta0 = beta;
+        // zero (or non-finite) start vector: no Arnoldi step; exp(A) 0 = 0,
+        // and the host reports "Wavefunction is zero" from |psi~| = 0
+        if (!(beta > 0.0 && beta <= 1.79e308)) { broken = true; mEff = 0; }
+      }
       else if (!broken) {
         if (fabs(beta) < tol) { broken = true; mEff = j; }   // happy breakdown (pamp.h:305-312)
         else if (a == 0 && tid == 0) A.Hm[j + (j - 1) * m] = zmk(beta, 0.0);
@@ -868,7 +873,12 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
         }
       grid_partial_sum_wide(A.part, A.pstr, NRM, 1, nwg, hred);
       beta = sqrt(hred[0]);
-      if (j == 0) beta0 = beta;
+      if (j == 0) {
+        beta0 = beta;
+        // zero (or non-finite) start vector: no Arnoldi step; exp(A) 0 = 0,
+        // and the host reports "Wavefunction is zero" from |psi~| = 0
+        if (!(beta > 0.0 && beta <= 1.79e308)) { broken = true; mEff = 0; }
+      }
       else if (!broken) {
         if (fabs(beta) < tol) { broken = true; mEff = j; }   // happy breakdown (pamp.h:305-312)
         else if (lead && tid == 0) A.Hm[j + (j - 1) * m] = zmk(beta, 0.0);
@@ -1205,10 +1215,13 @@ __global__ void __launch_bounds__(PP_THREADS) pp_expokit_kernel(pp_pamp_args A, 
     grid_barrier(A.bar, ep, L, a);
     double beta = sqrt(reduce1(P0NRM));
     int nmult = 0, nexph = 0, nstep = 0, nreject = 0, mbrkdwn = m, k1 = 2, iflag = 0;
+    // zero (or non-finite) psi: no substep (w = 0 stays 0), iflag -10; every
+    // workgroup holds the same beta, so all of them skip alike
+    if (!(beta > 0.0 && beta <= 1.79e308)) iflag = -10;
     double t_now = 0.0, err_loc = 0.0, t_step = 0.0, avnorm = 0.0, xmloc = 1.0 / m;
     const double p2 = tol * pow((m + 1) / 2.72, (double)(m + 1)) * sqrt((m + 1) * 6.28);
     double t_new = pp_roundstep((1.0 / E.anorm) * pow(p2 / (4.0 * beta * E.anorm), 1.0 / m));
-    while (t_now < t_out) {
+    while (!iflag && t_now < t_out) {
       nstep++;
       t_step = fmin(t_out - t_now, t_new);
       const double inv = 1.0 / beta;
@@ -2121,7 +2134,7 @@ extern "C" int pp_expokit_diagnostics(pp_pamp *p, int nsteps, double *stats, hip
   PP_HIP(hipMemcpyAsync(stats, p->diag, sizeof(double) * 6 * (size_t)nsteps, hipMemcpyDeviceToHost, stream));
   PP_HIP(hipMemcpyAsync(&hb, p->bar, sizeof(hb), hipMemcpyDeviceToHost, stream));
   PP_HIP(hipStreamSynchronize(stream));
-  if (hb.err) return PP_EHIP;
+  if (hb.err) return PP_ETIMEOUT;
   return PP_OK;
 }
 
@@ -2135,7 +2148,7 @@ extern "C" int pp_pamp_diagnostics(pp_pamp *p, int nsteps, double *diag, hipStre
   PP_HIP(hipMemcpyAsync(diag, p->diag, sizeof(double) * 4 * (size_t)nsteps, hipMemcpyDeviceToHost, stream));
   PP_HIP(hipMemcpyAsync(&hb, p->bar, sizeof(hb), hipMemcpyDeviceToHost, stream));
   PP_HIP(hipStreamSynchronize(stream));
-  if (hb.err) return PP_EHIP;
+  if (hb.err) return PP_ETIMEOUT;
   return PP_OK;
 }
 

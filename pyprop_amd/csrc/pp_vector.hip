@@ -81,24 +81,28 @@ static unsigned pv_grid(long n)
   return (unsigned)(g < 1024 ? (g > 0 ? g : 1) : 1024);
 }
 
-// Classical Gram-Schmidt of w (n) against V_0 .. V_{k-1}, `passes` times
-// (1 or 2): h = sum over passes of V^H w, w -= V h each pass; then |w|.
-// h (host, 2k doubles re/im) and wnorm (host) are returned; synchronises the
-// stream.  k = 0 computes only the norm.
-extern "C" int pp_gram_schmidt(const void *V, long n, int k, void *w, int passes, double *h, double *wnorm,
-                               hipStream_t stream)
+// Device scratch of one Gram-Schmidt call: the dot kernel's partial sums,
+// 2 max(k, 1) rows of PV_BLOCKS doubles (the norm pass uses rows 0 and 1 even
+// when k = 0), then [h accumulated over passes | h of this pass | |w|^2].
+static size_t pv_gs_partial_doubles(int k) { return 2 * (size_t)(k > 0 ? k : 1) * PV_BLOCKS; }
+static size_t pv_gs_scalar_doubles(int k) { return 4 * (size_t)k + 1; }
+
+static bool pv_gs_args_ok(const void *V, long n, int k, const void *w, int passes, const double *h,
+                          const double *wnorm)
 {
-  if (n <= 0 || k < 0 || k > 4096 || (k > 0 && (!V || !h)) || !w || !wnorm || passes < 1 || passes > 3)
-    return PP_EARG;
+  return !(n <= 0 || k < 0 || k > 4096 || (k > 0 && (!V || !h)) || !w || !wnorm || passes < 1 || passes > 3);
+}
+
+// The launches and the read-back of one Gram-Schmidt call on caller-given
+// device scratch: part (pv_gs_partial_doubles(k)) and hd
+// (pv_gs_scalar_doubles(k)).  Every scratch value is written before it is
+// read, so the scratch needs no clearing.  Synchronises the stream.
+static int pv_gram_schmidt_on(const zd *Vz, long n, int k, zd *wz, int passes, double *h, double *wnorm,
+                              double *part, double *hd, hipStream_t stream)
+{
   const int nblk = PV_BLOCKS;
   const size_t nc = 2 * (size_t)k;
-  // device scratch: partial sums, then [h accumulated over passes | h of this pass | |w|^2]
-  double *part = nullptr, *hd = nullptr;
-  PP_HIP(hipMallocAsync((void **)&part, sizeof(double) * (nc > 0 ? nc : 1) * nblk, stream));
-  PP_HIP(hipMallocAsync((void **)&hd, sizeof(double) * (2 * nc + 1), stream));
   double *hacc = hd, *hcur = hd + nc, *nrm = hd + 2 * nc;
-  const zd *Vz = (const zd *)V;
-  zd *wz = (zd *)w;
   for (int ps = 0; k > 0 && ps < passes; ps++) {
     hipLaunchKernelGGL(pv_dots, dim3(nblk, k), dim3(PV_THREADS), 0, stream, Vz, n, (const zd *)wz, part);
     hipLaunchKernelGGL(pv_reduce, dim3(1), dim3(PV_THREADS), 0, stream, (const double *)part, nblk, (int)nc, hcur,
@@ -109,13 +113,30 @@ extern "C" int pp_gram_schmidt(const void *V, long n, int k, void *w, int passes
   hipLaunchKernelGGL(pv_reduce, dim3(1), dim3(PV_THREADS), 0, stream, (const double *)part, nblk, 1, nrm,
                      (double *)nullptr, 0);
   PP_HIP(hipGetLastError());
-  std::vector<double> out(2 * nc + 1);
-  PP_HIP(hipMemcpyAsync(out.data(), hd, sizeof(double) * (2 * nc + 1), hipMemcpyDeviceToHost, stream));
-  PP_HIP(hipFreeAsync(part, stream));
-  PP_HIP(hipFreeAsync(hd, stream));
+  double nn = 0.0;
+  if (nc > 0) PP_HIP(hipMemcpyAsync(h, hacc, sizeof(double) * nc, hipMemcpyDeviceToHost, stream));
+  PP_HIP(hipMemcpyAsync(&nn, nrm, sizeof(double), hipMemcpyDeviceToHost, stream));
   PP_HIP(hipStreamSynchronize(stream));
-  for (size_t c = 0; c < nc; c++) h[c] = out[c];
-  *wnorm = sqrt(out[2 * nc] > 0.0 ? out[2 * nc] : 0.0);
+  *wnorm = sqrt(nn > 0.0 ? nn : 0.0);
+  return PP_OK;
+}
+
+// Classical Gram-Schmidt of w (n) against V_0 .. V_{k-1}, `passes` times
+// (1 or 2): h = sum over passes of V^H w, w -= V h each pass; then |w|.
+// h (host, 2k doubles re/im) and wnorm (host) are returned; synchronises the
+// stream.  k = 0 computes only the norm.
+extern "C" int pp_gram_schmidt(const void *V, long n, int k, void *w, int passes, double *h, double *wnorm,
+                               hipStream_t stream)
+{
+  if (!pv_gs_args_ok(V, n, k, w, passes, h, wnorm)) return PP_EARG;
+  double *part = nullptr, *hd = nullptr;
+  PP_HIP(hipMallocAsync((void **)&part, sizeof(double) * pv_gs_partial_doubles(k), stream));
+  PP_HIP(hipMallocAsync((void **)&hd, sizeof(double) * pv_gs_scalar_doubles(k), stream));
+  const int rc = pv_gram_schmidt_on((const zd *)V, n, k, (zd *)w, passes, h, wnorm, part, hd, stream);
+  const hipError_t e1 = hipFreeAsync(part, stream), e2 = hipFreeAsync(hd, stream);
+  if (rc != PP_OK) return rc;
+  if (e1 != hipSuccess || e2 != hipSuccess) return PP_EHIP;
+  PP_HIP(hipStreamSynchronize(stream));
   return PP_OK;
 }
 

@@ -105,6 +105,40 @@ def tp_multiply(storage_ids, potential, scaling, src, dst, pairs=None):
     return dst
 
 
+def check_pamp_steps(diag, what="pAMP"):
+    """Raise the way the reference's Krylov step does when a step of the
+    last launch failed.  diag: per step (|last Krylov coefficient| or -1
+    for a singular Pade denominator, basis size used, |U psi|, ...).
+
+    * |U psi| = 0 (or not finite): "Wavefunction is zero".  The reference
+      prints that (expokitpropagator.cpp:48-54) and then divides by the zero
+      norm (pamp.h:619-620, zgexpv.cpp:304,322); here the step raises.
+    * Pade denominator singular: "Error in pAMP" (pamp.h:544-550).
+    * A non-finite Krylov coefficient: the step produced NaN/Inf.
+    """
+    d = np.asarray(diag)
+    for n in range(d.shape[0]):
+        beta = d[n, 2]
+        if not np.isfinite(beta) or beta == 0.0:
+            raise RuntimeError("Wavefunction is zero: initial norm = %r (%s step %d of the launch)" % (beta, what, n))
+        if d[n, 0] == -1.0:
+            raise RuntimeError("Error in %s: singular Pade denominator (step %d of the launch)" % (what, n))
+        if not np.isfinite(d[n, 0]):
+            raise RuntimeError("Error in %s: non-finite Krylov coefficient (step %d of the launch)" % (what, n))
+
+
+def check_expokit_steps(stats):
+    """zgexpv's iflag of every step of the last launch (expokitpropagator.cpp:
+    96-100 throws on iflag != 0); -10 marks a zero wavefunction."""
+    st = np.asarray(stats)
+    bad = np.nonzero(st[:, 5] != 0)[0]
+    if len(bad):
+        flag = int(st[bad[0], 5])
+        if flag == -10:
+            raise RuntimeError("Wavefunction is zero (Expokit step %d of the launch)" % bad[0])
+        raise RuntimeError("Expokit returned nonzero value %d (step %d of the launch)" % (flag, bad[0]))
+
+
 class PampEngine:
     """Persistent-kernel pAMP propagator for rank-2 (angular x B-spline) states.
 
@@ -135,30 +169,38 @@ class PampEngine:
         nat.check(nat.lib().pp_pamp_add_term(self.h, aid, P0, pp, rid, bw, ptr(potential)), "pp_pamp_add_term")
         self.terms.append((potential, keep))
 
-    def propagate(self, psi, nsteps, dt, scalings):
+    def propagate(self, psi, nsteps, dt, scalings, check=True):
+        """nsteps pAMP steps in one launch.  check: read the launch's status
+        (barrier time-out, zero wavefunction, singular Pade denominator)
+        and raise as the reference does; this synchronises the stream."""
         dt = complex(dt)
         sc = np.ascontiguousarray(np.asarray(scalings, dtype=np.float64).reshape(nsteps, len(self.terms)))
         nat.check(nat.lib().pp_pamp_propagate(self.h, ptr(psi), int(nsteps), dt.real, dt.imag, nat.dptr(sc),
                                               stream_ptr()), "pp_pamp_propagate")
         self._last = nsteps
+        if check and nsteps > 0:
+            check_pamp_steps(self.diagnostics(nsteps))
         return psi
 
-    def propagate_expokit(self, psi, nsteps, dt, scalings, tol=0.0, anorm=1e6):
-        """nsteps ExpokitPropagator.AdvanceStep's in one launch (zgexpv)."""
+    def propagate_expokit(self, psi, nsteps, dt, scalings, tol=0.0, anorm=1e6, check=True):
+        """nsteps ExpokitPropagator.AdvanceStep's in one launch (zgexpv);
+        check: raise on a barrier time-out or iflag != 0 (synchronises)."""
         dt = complex(dt)
         sc = np.ascontiguousarray(np.asarray(scalings, dtype=np.float64).reshape(nsteps, len(self.terms)))
         nat.check(nat.lib().pp_expokit_propagate(self.h, ptr(psi), int(nsteps), dt.real, dt.imag, nat.dptr(sc),
                                                  float(tol), float(anorm), stream_ptr()), "pp_expokit_propagate")
         self._last = nsteps
+        if check and nsteps > 0:
+            self.expokit_stats(nsteps)
         return psi
 
-    def expokit_stats(self, nsteps=None):
+    def expokit_stats(self, nsteps=None, check=True):
         """Per step: matvecs, exponentials, substeps, rejections, basis size, iflag."""
         n = self._last if nsteps is None else nsteps
         d = np.zeros((n, 6))
         nat.check(nat.lib().pp_expokit_diagnostics(self.h, n, nat.dptr(d), stream_ptr()), "pp_expokit_diagnostics")
-        if np.any(d[:, 5] != 0):
-            raise RuntimeError("Expokit returned nonzero value %d" % int(d[d[:, 5] != 0][0, 5]))
+        if check:
+            check_expokit_steps(d)
         return d
 
     def diagnostics(self, nsteps=None):
@@ -223,7 +265,7 @@ class BatchGraph:
                   "pp_pamp_batch_graph_create")
         self.h = h
 
-    def launch(self, scalings, psis=None):
+    def launch(self, scalings, psis=None, check=True):
         if psis is not None and [ptr(p).value for p in psis] != self._ptrs:
             raise RuntimeError("a wavefunction buffer changed since the graph was recorded")
         n = len(self.engines)
@@ -233,6 +275,9 @@ class BatchGraph:
         nat.check(nat.lib().pp_pamp_batch_graph_launch(self.h, S, stream_ptr()), "pp_pamp_batch_graph_launch")
         for e in self.engines:
             e._last = self.nsteps
+        if check:
+            for i, e in enumerate(self.engines):
+                check_pamp_steps(e.diagnostics(self.nsteps), "pAMP (batch instance %d)" % i)
 
     def __del__(self):
         if getattr(self, "h", None) and nat._lib is not None:
@@ -240,9 +285,10 @@ class BatchGraph:
             self.h = None
 
 
-def pamp_batch_propagate(engines, psis, nsteps, dts, scalings):
+def pamp_batch_propagate(engines, psis, nsteps, dts, scalings, check=True):
     """nsteps steps of many independent engines in packed cooperative
-    launches (pp_pamp_batch_propagate).  scalings[i]: (nsteps, nterms_i)."""
+    launches (pp_pamp_batch_propagate).  scalings[i]: (nsteps, nterms_i).
+    check: raise if any instance's step failed (synchronises)."""
     n = len(engines)
     if not (len(psis) == len(dts) == len(scalings) == n):
         raise ValueError("engines, psis, dts and scalings must have the same length")
@@ -256,3 +302,6 @@ def pamp_batch_propagate(engines, psis, nsteps, dts, scalings):
               "pp_pamp_batch_propagate")
     for e in engines:
         e._last = nsteps
+    if check:
+        for i, e in enumerate(engines):
+            check_pamp_steps(e.diagnostics(nsteps), "pAMP (batch instance %d)" % i)

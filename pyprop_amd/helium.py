@@ -389,7 +389,9 @@ class Pamp3Engine:
         nat.check(nat.lib().pp_pamp3_apply(self.h, dev.ptr(src), dev.ptr(dst), nat.dptr(sc) if sc is not None else None,
                                            int(bool(balanced)), dev.stream_ptr()), "pp_pamp3_apply")
 
-    def propagate(self, psi, nsteps, dt, scal=None):
+    def propagate(self, psi, nsteps, dt, scal=None, check=True):
+        """nsteps pAMP steps of psi; check: raise on a zero wavefunction or
+        a failed Pade step (reads the diagnostics, synchronises)."""
         self._check(psi)
         dt = complex(dt)
         sc = self._scal(scal, nsteps) if self.nslots else None
@@ -397,6 +399,8 @@ class Pamp3Engine:
                                                nat.dptr(sc) if sc is not None else None, dev.stream_ptr()),
                   "pp_pamp3_propagate")
         self._last_steps = int(nsteps)
+        if check and nsteps > 0:
+            dev.check_pamp_steps(self.diagnostics(), "pAMP (two-electron)")
 
     def set_timing(self, enable=True):
         nat.check(nat.lib().pp_pamp3_set_timing(self.h, int(bool(enable))), "pp_pamp3_set_timing")

@@ -469,3 +469,30 @@ def test_step_kernels_repeat_bitwise():
         for rep in (1, 2):
             assert np.array_equal(out[0], out[rep]), "q=%d run %d differs by %.3e" % (
                 q, rep, np.abs(out[0] - out[rep]).max())
+
+
+@pytest.mark.parametrize("prop", ["PamPropagator", "ExpokitPropagator"])
+def test_zero_wavefunction_raises(prop):
+    """A zero wavefunction through Problem.AdvanceStep: the reference prints
+    "Wavefunction is zero" (expokitpropagator.cpp:48-54) and then divides by
+    the zero norm; here the step raises RuntimeError with that message, and
+    the state stays zero (exp(-iH dt) 0 = 0)."""
+    import pyprop_amd
+    from pyprop_amd.examples import hydrogen as H
+    print("zero wavefunction through", prop, flush=True)
+    conf = H.load("hydrogen_small")
+    conf.SetValue("AngularRepresentation", "maxl", 3)
+    conf.SetValue("RadialRepresentation", "xsize", 30)
+    conf.SetValue("RadialRepresentation", "xmax", 30.0)
+    conf.SetValue("Propagation", "krylov_basis_size", 10)
+    conf = pyprop_amd.Config(conf.cfgObj)
+    conf.SetValue("Propagation", "propagator", getattr(pyprop_amd, prop), ignoreSerializationError=True)
+    prob = pyprop_amd.Problem(conf)
+    prob.SetupStep()
+    psi0 = H.ground_state(prob)
+    prob.SetWavefunction(psi0)
+    prob.AdvanceStep()                        # a normal step first
+    prob.psi.GetData().zero_()
+    with pytest.raises(RuntimeError, match="Wavefunction is zero"):
+        prob.AdvanceStep()
+    assert not prob.psi.GetData().abs().max().item() > 0.0

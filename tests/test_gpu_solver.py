@@ -33,6 +33,23 @@ def _dense_levels(prob):
     return np.sort(np.array(E))
 
 
+def test_gram_schmidt_norm_only_matches_numpy():
+    """k = 0: only |w| (the GMRES residual norm).  Regression test for the
+    norm pass's partial sums (two rows of row-block sums, re and im), which
+    the scratch must hold even when there is no basis vector."""
+    print("pp_gram_schmidt, k = 0, n = 70001", flush=True)
+    rng = np.random.default_rng(11)
+    n = 70001
+    w0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    w = dev.to_device(w0)
+    nrm = ctypes.c_double()
+    h = np.zeros(2)
+    nat.check(nat.lib().pp_gram_schmidt(None, n, 0, dev.ptr(w), 1, nat.dptr(h), ctypes.byref(nrm),
+                                        dev.stream_ptr()), "pp_gram_schmidt")
+    assert abs(nrm.value - np.linalg.norm(w0)) < 1e-13 * np.linalg.norm(w0)
+    assert np.array_equal(w.cpu().numpy(), w0)
+
+
 def test_vector_kernels_match_numpy():
     print("pp_gram_schmidt / pp_basis_combine, n = 5000, k = 7", flush=True)
     t = dev.torch()

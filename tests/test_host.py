@@ -139,3 +139,32 @@ def test_expokit_chunk_fits_lds():
     assert B == 64
     assert lib.pp_pamp_lds_estimate(64, 800, 24, 3, 6, B) + 16 * (26 * 26 + 26) <= 160 * 1024
     assert lib.pp_pamp_lds_estimate(64, 800, 24, 3, 6, 32) + 16 * (26 * 26 + 26) > 160 * 1024
+
+
+def test_krylov_status_mapping_raises_like_the_reference():
+    """Per-step diagnostics -> exceptions (no GPU): the reference throws
+    runtime_error for a singular Pade denominator (pamp.h:544-550) and for
+    zgexpv's iflag != 0 (expokitpropagator.cpp:91-100), and divides by the
+    norm of a zero wavefunction (expokitpropagator.cpp:48-54); here each is a
+    RuntimeError naming the step."""
+    from pyprop_amd import device as D
+    ok = np.array([[1e-12, 20, 1.0, 0], [3e-11, 20, 0.99, 0]])
+    D.check_pamp_steps(ok)
+    cases = [(1, 2, 0.0, "Wavefunction is zero"), (1, 2, np.nan, "Wavefunction is zero"),
+             (0, 0, -1.0, "singular Pade"), (1, 0, np.inf, "non-finite")]
+    for row, col, val, msg in cases:
+        d = ok.copy()
+        d[row, col] = val
+        with pytest.raises(RuntimeError, match=msg):
+            D.check_pamp_steps(d)
+    st = np.zeros((3, 6))
+    D.check_expokit_steps(st)
+    for flag, msg in [(1, "Expokit returned nonzero value 1"), (-9, "nonzero value -9"), (-8, "nonzero value -8"),
+                      (-10, "Wavefunction is zero")]:
+        s = st.copy()
+        s[2, 5] = flag
+        with pytest.raises(RuntimeError, match=msg):
+            D.check_expokit_steps(s)
+    from pyprop_amd import _native as nat
+    with pytest.raises(RuntimeError, match="grid barrier"):
+        nat.check(nat.PP_ETIMEOUT, "pp_pamp_diagnostics")
