@@ -258,6 +258,15 @@ int pp_pamp3_destroy(pp_pamp3 *p);
  * m x np complex128; out must not overlap V. */
 int pp_gram_schmidt(const void *V, long n, int k, void *w, int passes, double *h, double *wnorm, pp_stream stream);
 int pp_basis_combine(const void *V, long n, int m, const double *Q, int np, void *out, pp_stream stream);
+/* pp_gram_schmidt without any allocation.  No reference counterpart: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch is a fixed device array of the library, one per GPU, sized for
+ * k <= 128 basis vectors (PP_EARG beyond); every call on a GPU shares that
+ * GPU's array, so calls that overlap on one GPU race.  Same kernels, same
+ * results as pp_gram_schmidt; synchronises the stream. */
+int pp_gram_schmidt_noalloc(const void *V, long n, int k, void *w, int passes, double *h, double *wnorm,
+                            pp_stream stream);
 /* Per-fiber banded LU solve of the shift-invert and Cayley preconditioner (the device
  * side of pyprop's BSplinePreconditioner.Solve,
  * examples/tensor/helium_stabilization_sae/example.py:275-318): x_a := A_a^-1

@@ -27,7 +27,7 @@ EXPORTS = [
     "pp_band_cholesky_upper", "pp_overlap_create", "pp_overlap_create_chunked", "pp_overlap_factor", "pp_overlap_destroy",
     "pp_overlap_apply", "pp_tp_multiply", "pp_pamp_create", "pp_pamp_add_term", "pp_pamp_add_factored_term",
     "pp_lowrank_factor", "pp_pamp_propagate", "pp_pamp_diagnostics", "pp_pamp_set_mixed", "pp_pamp_set_fiber_split", "pp_pamp_set_pade_scaling", "pp_expokit_propagate", "pp_expokit_diagnostics", "pp_pamp_batch_propagate", "pp_pamp_batch_graph_create", "pp_pamp_batch_graph_launch", "pp_pamp_batch_graph_destroy",
-    "pp_gram_schmidt", "pp_basis_combine", "pp_band_lu_solve", "pp_pamp_lds_estimate", "pp_pamp_set_timing", "pp_pamp_timings", "pp_pamp_destroy",
+    "pp_gram_schmidt", "pp_gram_schmidt_noalloc", "pp_basis_combine", "pp_band_lu_solve", "pp_pamp_lds_estimate", "pp_pamp_set_timing", "pp_pamp_timings", "pp_pamp_destroy",
     "pp_pamp3_create", "pp_pamp3_add_kron_term", "pp_pamp3_add_multipole_term", "pp_pamp3_apply",
     "pp_pamp3_propagate", "pp_pamp3_diagnostics", "pp_pamp3_set_timing", "pp_pamp3_kernel_time",
     "pp_pamp3_set_radial_supports", "pp_pamp3_set_mixed",
@@ -90,6 +90,7 @@ def lib():
     L.pp_pamp_batch_graph_launch.argtypes = [P, ctypes.POINTER(dp), P]
     L.pp_pamp_batch_graph_destroy.argtypes = [P]
     L.pp_gram_schmidt.argtypes = [P, ctypes.c_long, I, P, I, dp, dp, P]
+    L.pp_gram_schmidt_noalloc.argtypes = [P, ctypes.c_long, I, P, I, dp, dp, P]
     L.pp_basis_combine.argtypes = [P, ctypes.c_long, I, dp, I, P, P]
     L.pp_band_lu_solve.argtypes = [I, I, I, I, P, ctypes.POINTER(ctypes.c_int), P, P]
     L.pp_pamp_set_timing.argtypes = [P, I]

@@ -11,6 +11,7 @@
 // Basis layout: V[j * n + i] (vector j contiguous), complex128 interleaved.
 #include "pp_common.h"
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 #define PV_THREADS 256
@@ -138,6 +139,52 @@ extern "C" int pp_gram_schmidt(const void *V, long n, int k, void *w, int passes
   if (e1 != hipSuccess || e2 != hipSuccess) return PP_EHIP;
   PP_HIP(hipStreamSynchronize(stream));
   return PP_OK;
+}
+
+// ---- the same call without allocation ----
+// One file-scope device array per GPU holds the scratch of
+// pp_gram_schmidt_noalloc: the partial sums of PV_GS_NOALLOC_MAXK basis
+// vectors and the scalars after them.  Host code never passes the array's
+// name to a kernel (on the host it is only a stand-in for the symbol): the
+// device address is looked up once per device with hipGetSymbolAddress and
+// kept in pv_gs_scratch_ptr.
+#define PV_GS_NOALLOC_MAXK 128
+#define PV_GS_NOALLOC_DOUBLES (2 * PV_GS_NOALLOC_MAXK * PV_BLOCKS + 4 * PV_GS_NOALLOC_MAXK + 1)
+__device__ __align__(256) double scratch[PV_GS_NOALLOC_DOUBLES];
+
+#define PV_MAX_DEVICES 64
+static double *pv_gs_scratch_ptr[PV_MAX_DEVICES];
+static std::mutex pv_gs_scratch_mutex;
+
+static int pv_gs_scratch(double **out)
+{
+  int dev = 0;
+  PP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= PV_MAX_DEVICES) return PP_EUNSUP;
+  std::lock_guard<std::mutex> lock(pv_gs_scratch_mutex);
+  if (!pv_gs_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return PP_EHIP;
+    pv_gs_scratch_ptr[dev] = (double *)ptr;
+  }
+  *out = pv_gs_scratch_ptr[dev];
+  return PP_OK;
+}
+
+// pp_gram_schmidt with its scratch in the device array above instead of
+// stream-ordered allocations: k <= PV_GS_NOALLOC_MAXK (PP_EARG beyond).
+// Calls on one GPU share that GPU's array, so calls that overlap on one GPU
+// (other streams or threads) race; calls on one stream are ordered.
+extern "C" int pp_gram_schmidt_noalloc(const void *V, long n, int k, void *w, int passes, double *h,
+                                       double *wnorm, hipStream_t stream)
+{
+  if (!pv_gs_args_ok(V, n, k, w, passes, h, wnorm) || k > PV_GS_NOALLOC_MAXK) return PP_EARG;
+  double *base = nullptr;
+  const int rc = pv_gs_scratch(&base);
+  if (rc != PP_OK) return rc;
+  return pv_gram_schmidt_on((const zd *)V, n, k, (zd *)w, passes, h, wnorm, base,
+                            base + pv_gs_partial_doubles(PV_GS_NOALLOC_MAXK), stream);
 }
 
 // out[p] = sum_j V_j Q[j, p] for p < np (Q host, column-major m x np,
