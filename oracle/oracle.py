@@ -18,6 +18,7 @@ operator assembly the reference performs in Python:
   (`gmres_solve`, `cayley_step`; numpy, dense operators on small shapes).
 """
 import ctypes
+import math
 import os
 import subprocess
 
@@ -118,6 +119,44 @@ def exponential_breakpoints(rmin, rmax, n, gamma):
     h = rmax - rmin
     return np.array([rmin + h * (np.exp(gamma * float(i - 1) / float(n - 1)) - 1) / (np.exp(gamma) - 1)
                      for i in range(1, n + 1)])
+
+
+def _py2_round(x):
+    """Python 2 round(): halves away from zero (the reference runs on Python 2)."""
+    return math.floor(x + 0.5) if x >= 0 else -math.floor(-x + 0.5)
+
+
+def exponential_linear_breakpoints(rmin, rpartition, rmax, n, gamma):
+    """breakpointsequences.py:72-90: exponential inner points, then
+    pylab.frange(rpartition + h, rmax, h), which holds
+    round((rmax - rpartition - h)/h) + 1 points (matplotlib.mlab.frange,
+    closed interval)."""
+    xi = [rmin + (rpartition - rmin) * (math.exp(gamma * float(i - 1) / float(n - 1)) - 1) / (math.exp(gamma) - 1)
+          for i in range(1, n + 1)]
+    h = xi[-1] - xi[-2]
+    x0 = rpartition + h
+    npts = int(_py2_round((rmax - x0) / h)) + 1
+    for i in range(npts):
+        xi.append(x0 + i * h)
+    return np.array(xi)
+
+
+def center_exponential_linear_breakpoints(rmin, rpartition, rmax, n, gamma):
+    """breakpointsequences.py:92-121: the exponential-linear sequence from the
+    centre point outwards, then mirrored through zero."""
+    right = list(exponential_linear_breakpoints((rmax + rmin) / 2.0, rpartition, rmax, n, gamma))
+    return np.array([-x for x in reversed(right[1:])] + right)
+
+
+def quadratic_linear_breakpoints(rmin, rmax, n, joinPoint):
+    """breakpointsequences.py:41-70."""
+    rmin, rmax = float(rmin), float(rmax)
+    r0 = (rmax * (joinPoint - 1) + rmin * (n - joinPoint)) / (2.0 * n - joinPoint - 1.0)
+    alpha = (r0 - rmin) / float(joinPoint - 1) ** 2
+    beta = (rmax - r0) / float(n - joinPoint)
+    xi = [rmin + alpha * (i - 1) ** 2 for i in range(1, joinPoint)]
+    xi += [r0 + beta * (i - joinPoint) for i in range(joinPoint, n + 1)]
+    return np.array(xi)
 
 
 class BSpline:

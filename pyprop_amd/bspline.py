@@ -13,35 +13,65 @@ import numpy as np
 
 
 # ---------------------------------------------------------------- breakpoints
-# pyprop/bspline/breakpointsequences.py
+# pyprop/bspline/breakpointsequences.py.  The formulas define the grid (a
+# config names a sequence and its parameters), so they are the reference's;
+# the code evaluates them as numpy arrays and returns lists, as the reference.
+def _exp_points(r0, r1, n, gamma):
+    """n points from r0 to r1 with spacings growing by exp(gamma/(n-1))
+    (breakpointsequences.py:6-28)."""
+    s = np.arange(n, dtype=np.float64) / float(n - 1)
+    return r0 + (r1 - r0) * np.expm1(gamma * s) / np.expm1(gamma)
+
+
+def _closed_frange(start, stop, step):
+    """matplotlib.mlab.frange(start, stop, step) as the reference imports it
+    through pylab: the closed interval, round((stop - start)/step) + 1 points
+    start + i*step (Python 2 round, half away from zero)."""
+    q = (stop - start) / step
+    npts = int(np.sign(q) * np.floor(abs(q) + 0.5)) + 1
+    return start + step * np.arange(max(npts, 0), dtype=np.float64)
+
+
 def LinearBreakpointSequence(rmin, rmax, n):
-    return [rmin + (rmax - rmin) * i / float(n - 1) for i in range(n)]
+    """n equidistant points on [rmin, rmax] (breakpointsequences.py:31-38)."""
+    return list(rmin + (rmax - rmin) * np.arange(n) / float(n - 1))
 
 
 def ExponentialBreakpointSequence(rmin, rmax, n, gamma):
-    h = rmax - rmin
-    return [rmin + h * (np.exp(gamma * float(i - 1) / float(n - 1)) - 1) / (np.exp(gamma) - 1)
-            for i in range(1, n + 1)]
+    """breakpointsequences.py:6-28."""
+    return list(_exp_points(rmin, rmax, n, gamma))
 
 
 def ExponentialLinearBreakpointSequence(rmin, rpartition, rmax, n, gamma):
-    h = rpartition - rmin
-    xi = [rmin + h * (np.exp(gamma * float(i - 1) / float(n - 1)) - 1) / (np.exp(gamma) - 1)
-          for i in range(1, n + 1)]
-    h = xi[-1] - xi[-2]
-    xi += list(np.arange(rpartition + h, rmax, h))
-    return xi
+    """n exponential points on [rmin, rpartition], then points spaced like
+    the last two of them from rpartition + h up to rmax, both ends included
+    (breakpointsequences.py:72-90)."""
+    inner = _exp_points(rmin, rpartition, n, gamma)
+    h = inner[-1] - inner[-2]
+    return list(inner) + list(_closed_frange(rpartition + h, rmax, h))
+
+
+def CenterExponentialLinearBreakpointSequence(rmin, rpartition, rmax, n, gamma):
+    """The exponential-linear sequence from the centre (rmin + rmax)/2
+    outwards, mirrored through x -> -x (breakpointsequences.py:92-121; the
+    mirror is about zero, so the grid is centred on (rmin + rmax)/2 only when
+    that is zero)."""
+    c = (rmax + rmin) / 2.0
+    right = ExponentialLinearBreakpointSequence(c, rpartition, rmax, n, gamma)
+    return [-x for x in reversed(right[1:])] + right
 
 
 def QuadraticLinearBreakpointSequence(rmin, rmax, n, joinPoint):
-    rmin = float(rmin)
-    rmax = float(rmax)
-    r0 = (rmax * (joinPoint - 1) + rmin * (n - joinPoint)) / (2.0 * n - joinPoint - 1.0)
-    alpha = (r0 - rmin) / float(joinPoint - 1) ** 2
-    beta = (rmax - r0) / float(n - joinPoint)
-    xi = [rmin + alpha * (i - 1) ** 2 for i in range(1, joinPoint)]
-    xi += [r0 + beta * (i - joinPoint) for i in range(joinPoint, n + 1)]
-    return xi
+    """joinPoint - 1 points on a parabola from rmin, then n - joinPoint + 1
+    equidistant points to rmax; r0 is chosen so the slope is continuous at
+    the join, 2 alpha (joinPoint - 1) = beta (breakpointsequences.py:41-70)."""
+    rmin, rmax, j = float(rmin), float(rmax), int(joinPoint)
+    r0 = (rmax * (j - 1) + rmin * (n - j)) / (2.0 * n - j - 1.0)
+    alpha = (r0 - rmin) / float(j - 1) ** 2
+    beta = (rmax - r0) / float(n - j)
+    quad = rmin + alpha * np.arange(j - 1, dtype=np.float64) ** 2
+    lin = r0 + beta * np.arange(n - j + 1, dtype=np.float64)
+    return list(quad) + list(lin)
 
 
 def gauss_legendre_rule(n):
@@ -248,6 +278,9 @@ def InitBSpline(conf):
         bps = QuadraticLinearBreakpointSequence(conf.xmin, conf.xmax, conf.xsize, conf.joinpoint)
     elif t == "exponentiallinear":
         bps = ExponentialLinearBreakpointSequence(conf.xmin, conf.xpartition, conf.xmax, conf.xsize, conf.gamma)
+    elif t == "centerexponentiallinear":
+        bps = CenterExponentialLinearBreakpointSequence(conf.xmin, conf.xpartition, conf.xmax, conf.xsize,
+                                                        conf.gamma)
     else:
         raise NameError("Sequence not recognized!")
     return BSplineBasis(bps, conf.order, conf.continuity,

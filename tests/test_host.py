@@ -168,3 +168,47 @@ def test_krylov_status_mapping_raises_like_the_reference():
     from pyprop_amd import _native as nat
     with pytest.raises(RuntimeError, match="grid barrier"):
         nat.check(nat.PP_ETIMEOUT, "pp_pamp_diagnostics")
+
+
+def test_config_import_order_and_python2_rules(tmp_path):
+    """Import priority (the importing file over its imports, an earlier entry
+    of files over a later one, breadth first), Python 2 ConfigParser rules
+    (repeated keys: the last counts; ';' inline comments only; '#' stays in
+    the value and is an expression comment), Python 2 integer division, base
+    chains, and a cycle of bases refused."""
+    (tmp_path / "sub").mkdir()
+    (tmp_path / "sub" / "deep.ini").write_text("[S]\na = 'deep'\nd = 'deep'\n")
+    (tmp_path / "f1.ini").write_text("[Import]\nfiles = ['sub/deep.ini']\n[S]\na = 'f1'\nb = 'f1'\n")
+    (tmp_path / "f2.ini").write_text("[S]\na = 'f2'\nb = 'f2'\nc = 'f2'\n")
+    (tmp_path / "main.ini").write_text(
+        "[Import]\nfiles = ['f1.ini', 'f2.ini']\n"
+        "[S]\na = 'main'\n"
+        "[T]\nx = 5\nx = 7\nh = 7 / 2\nf = 7.0 / 2\nneg = -7 / 2\ns = 'a#b' # trailing comment\n"
+        "t = 3 ; an inline comment\n"
+        "[U]\nbase = 'T'\nh = h * 10\n"
+        "[V]\nbase = 'W'\n[W]\nbase = 'V'\n")
+    files = P.config.config_files(str(tmp_path / "main.ini"))
+    assert [os.path.basename(f) for f in files] == ["deep.ini", "f2.ini", "f1.ini", "main.ini"]
+    parser = P.config._parser()
+    for f in files:
+        P.config._read_file(parser, f)
+    sec = P.config.Section("S", parser)
+    assert (sec.a, sec.b, sec.c, sec.d) == ("main", "f1", "f2", "deep")
+    T = P.config.Section("T", parser)
+    assert T.x == 7 and T.h == 3 and T.f == 3.5 and T.neg == -4 and T.s == "a#b" and T.t == 3
+    U = P.config.Section("U", parser)
+    assert U.h == 30 and U.x == 7
+    with pytest.raises(ValueError, match="cycle"):
+        P.config.Section("V", parser)
+
+
+def test_exponential_linear_grid_of_the_sae_paper_config():
+    """InitBSpline on an exponential-linear section with the SAE paper's
+    grid (xmin 0, xpartition 10, xmax 160, 30 inner points, gamma 1): 310
+    breakpoints, as pylab.frange's closed interval gives."""
+    from pyprop_amd import bspline as B
+    conf = P.LoadString("[R]\nbpstype = 'exponentiallinear'\nxmin = 0.0\nxmax = 160.0\nxsize = 30\n"
+                        "gamma = 1.0\nxpartition = 10\norder = 7\ncontinuity = 'zero'\n").R
+    basis = B.InitBSpline(conf)
+    assert len(basis.Breakpoints) == 310
+    assert basis.Breakpoints[-1] > 159.7

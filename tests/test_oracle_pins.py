@@ -356,3 +356,77 @@ def test_cayley_step_against_dense_crank_nicolson(small):
         n0 = np.vdot(psi.reshape(-1), Sb @ psi.reshape(-1)).real
         n1 = np.vdot(out.reshape(-1), Sb @ out.reshape(-1)).real
         assert abs(n1 - n0) < 1e-9 * n0
+
+
+def test_breakpoint_sequences_closed_forms():
+    """The oracle's breakpoint sequences against what their definitions fix:
+    end points, the constant spacing ratio exp(gamma/(n-1)) of the
+    exponential part, the constant outer spacing and closed end of
+    pylab.frange, second differences 2 alpha and the slope continuity
+    2 alpha (j - 1) = beta of the quadratic-linear join, and the mirror
+    symmetry of the centred sequence."""
+    # exponential
+    x = O.exponential_breakpoints(0.0, 20.0, 24, 2.297)
+    d = np.diff(x)
+    assert x[0] == 0.0 and abs(x[-1] - 20.0) < 1e-13
+    assert np.abs(d[1:] / d[:-1] - np.exp(2.297 / 23)).max() < 1e-12
+    # exponential-linear on the SAE paper grid (config_paper.ini:15-22):
+    # 30 inner points, h = spacing of the last two, 280 outer points
+    x = O.exponential_linear_breakpoints(0.0, 10.0, 160.0, 30, 1.0)
+    inner, outer = x[:30], x[30:]
+    h = inner[-1] - inner[-2]
+    assert len(x) == 310
+    assert abs(h - 10.0 * (np.e - np.exp(28.0 / 29.0)) / (np.e - 1.0)) < 1e-13
+    assert np.abs(np.diff(inner)[1:] / np.diff(inner)[:-1] - np.exp(1.0 / 29.0)).max() < 1e-12
+    assert abs(outer[0] - (10.0 + h)) < 1e-13 and np.abs(np.diff(outer) - h).max() < 1e-12
+    assert abs(outer[-1] - 160.0) <= h / 2 + 1e-12 and abs(outer[-1] + h - 160.0) > h / 2
+    # frange is closed: an exact multiple ends on rmax; fractions round half away from zero
+    xe = O.exponential_linear_breakpoints(0.0, 1.0, 1.0 + 10 * (1.0 / 3.0), 4, 1e-9)   # h = 1/3
+    he = xe[3] - xe[2]
+    assert abs(he - 1.0 / 3.0) < 1e-8
+    assert abs(xe[-1] - (1.0 + 10 * (1.0 / 3.0))) < 1e-7 and len(xe) == 4 + 10
+    for frac, extra in [(0.49, 0), (0.51, 1)]:
+        xr = O.exponential_linear_breakpoints(0.0, 1.0, 1.0 + (6 + frac) * he, 4, 1e-9)
+        assert len(xr) == 4 + 6 + extra
+    # the helium config's grid (helium_stabilization/config.ini:15-23, xpartition = 5: last value wins)
+    xh = O.exponential_linear_breakpoints(0.0, 5.0, 20.0, 24, 2.29709842182)
+    hh = xh[23] - xh[22]
+    assert len(xh) == 24 + int(np.floor((20.0 - 5.0 - hh) / hh + 0.5)) + 1
+    # quadratic-linear
+    n, j, a, b = 40, 12, 0.0, 50.0
+    x = O.quadratic_linear_breakpoints(a, b, n, j)
+    q, l = x[:j - 1], x[j - 1:]
+    alpha = (q[1] - q[0])
+    beta = l[1] - l[0]
+    assert len(x) == n and x[0] == a and abs(x[-1] - b) < 1e-12
+    assert np.abs(np.diff(q, 2) - 2 * alpha).max() < 1e-12
+    assert np.abs(np.diff(l) - beta).max() < 1e-12
+    assert abs(2 * alpha * (j - 1) - beta) < 1e-12          # slope continuous at the join
+    assert abs(l[0] - (a + alpha * (j - 1) ** 2)) < 1e-12   # the parabola reaches r0 at the join
+    # centred exponential-linear: mirror image about zero of the right half
+    x = O.center_exponential_linear_breakpoints(-40.0, 8.0, 40.0, 12, 2.0)
+    assert np.abs(x + x[::-1]).max() < 1e-13
+    right = O.exponential_linear_breakpoints(0.0, 8.0, 40.0, 12, 2.0)
+    assert np.array_equal(x[len(right) - 1:], right)
+
+
+def test_product_breakpoints_match_oracle():
+    """pyprop_amd.bspline's sequences (the product's grids) equal the
+    oracle's, including the SAE paper grid's 310 points."""
+    from pyprop_amd import bspline as B
+    cases = [
+        (B.LinearBreakpointSequence(0.0, 100.0, 100), O.linear_breakpoints(0.0, 100.0, 100)),
+        (B.ExponentialBreakpointSequence(0.0, 20.0, 24, 2.297), O.exponential_breakpoints(0.0, 20.0, 24, 2.297)),
+        (B.ExponentialLinearBreakpointSequence(0.0, 10.0, 160.0, 30, 1.0),
+         O.exponential_linear_breakpoints(0.0, 10.0, 160.0, 30, 1.0)),
+        (B.ExponentialLinearBreakpointSequence(0.0, 5.0, 20.0, 24, 2.29709842182),
+         O.exponential_linear_breakpoints(0.0, 5.0, 20.0, 24, 2.29709842182)),
+        (B.QuadraticLinearBreakpointSequence(0.0, 50.0, 40, 12), O.quadratic_linear_breakpoints(0.0, 50.0, 40, 12)),
+        (B.CenterExponentialLinearBreakpointSequence(-40.0, 8.0, 40.0, 12, 2.0),
+         O.center_exponential_linear_breakpoints(-40.0, 8.0, 40.0, 12, 2.0)),
+    ]
+    for got, ref in cases:
+        got = np.asarray(got)
+        assert got.shape == ref.shape
+        assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+    assert len(B.ExponentialLinearBreakpointSequence(0.0, 10.0, 160.0, 30, 1.0)) == 310
