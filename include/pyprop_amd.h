@@ -258,6 +258,12 @@ int pp_pamp3_destroy(pp_pamp3 *p);
  * m x np complex128; out must not overlap V. */
 int pp_gram_schmidt(const void *V, long n, int k, void *w, int passes, double *h, double *wnorm, pp_stream stream);
 int pp_basis_combine(const void *V, long n, int m, const double *Q, int np, void *out, pp_stream stream);
+/* out (host, 2 doubles re/im) = sum_i conj(a_i) b_i over n complex128
+ * (device): the zdotc of CombinedRepresentation::InnerProduct
+ * (core/representation/combinedrepresentation.cpp:423-474) after the overlap
+ * multiply, and of Wavefunction::GetNorm (core/wavefunction.cpp:194-198);
+ * fixed-order reduction; synchronises the stream. */
+int pp_inner_product(const void *a, const void *b, long n, double *out, pp_stream stream);
 /* pp_gram_schmidt without any allocation.  No reference counterpart: it is
  * for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  Its

@@ -399,10 +399,14 @@ ORC_API int orc_angular_tables(int lmax, int m, double *xq, double *wq, double *
 /* ------------------------------------------------------------------ */
 enum { ST_IDENT = 0, ST_DIAG = 1, ST_SIMP = 2, ST_HERM = 3, ST_BANDNH = 4, ST_BAND = 5 };
 
-typedef struct { int row, col; long pidx; int conj; } orc_entry;
+/* flags of an entry: conjugate V; the innermost Band rank's zhbmv call     */
+/* (:562-575) replaces the conjugation the outer ranks asked for by its own */
+/* (ENT_OWN); zhbmv reads only the real part of the diagonal (ENT_REAL).    */
+enum { ENT_CONJ = 1, ENT_OWN = 2, ENT_REAL = 4 };
+typedef struct { int row, col; long pidx; int flags; } orc_entry;
 
 static int build_entries(int storage, int potExt, int srcExt, const int *pairs, int npairs,
-                         orc_entry **out)
+                         int innermost, orc_entry **out)
 {
     int cap = 16 + 2 * potExt + 2 * npairs + srcExt * 4;
     if (storage == ST_BANDNH || storage == ST_BAND) cap = potExt * 2 + 16;
@@ -421,7 +425,7 @@ static int build_entries(int storage, int potExt, int srcExt, const int *pairs, 
     case ST_HERM:
         for (int i = 0; i < potExt; i++) {
             e[n++] = (orc_entry){ pairs[2 * i], pairs[2 * i + 1], i, 0 };
-            e[n++] = (orc_entry){ pairs[2 * i + 1], pairs[2 * i], i, 1 };
+            e[n++] = (orc_entry){ pairs[2 * i + 1], pairs[2 * i], i, ENT_CONJ };
         }
         break;
     case ST_BANDNH: {
@@ -434,13 +438,23 @@ static int build_entries(int storage, int potExt, int srcExt, const int *pairs, 
         break;
     }
     case ST_BAND: {
-        /* zhbmv 'L' storage (N, k): element (row>=col) at col*k + row-col,  */
-        /* hermitian: the upper part uses the conjugate (generator :467-618). */
+        /* band storage (N, k): slot j*k + d, d = 0..k-1, belongs to the pair */
+        /* (j, j+d).  Innermost rank, zhbmv 'L' (:562-575): A[j+d][j] = V,     */
+        /* A[j][j+d] = conj V, diagonal Re V, the outer conjugation dropped.   */
+        /* Outer rank, the explicit loops (:513-551): A[j][j+d] = V,           */
+        /* A[j+d][j] = conj V, complex diagonal.                               */
         int N = srcExt, k = potExt / N;
-        for (int col = 0; col < N; col++)
-            for (int row = col; row < N && row < col + k; row++) {
-                e[n++] = (orc_entry){ row, col, (long)col * k + row - col, 0 };
-                if (row != col) e[n++] = (orc_entry){ col, row, (long)col * k + row - col, 1 };
+        for (int j = 0; j < N; j++)
+            for (int d = 0; d < k && j + d < N; d++) {
+                long slot = (long)j * k + d;
+                if (innermost) {
+                    if (d == 0) { e[n++] = (orc_entry){ j, j, slot, ENT_OWN | ENT_REAL }; continue; }
+                    e[n++] = (orc_entry){ j + d, j, slot, ENT_OWN };
+                    e[n++] = (orc_entry){ j, j + d, slot, ENT_OWN | ENT_CONJ };
+                } else {
+                    e[n++] = (orc_entry){ j, j + d, slot, 0 };
+                    if (d > 0) e[n++] = (orc_entry){ j + d, j, slot, ENT_CONJ };
+                }
             }
         break;
     }
@@ -463,7 +477,7 @@ ORC_API int orc_tp_multiply(int R, const int *storage, const int *potShape, cons
     int ne[3] = { 1, 1, 1 };
     for (int r = 0; r < R; r++) {
         ne[r] = build_entries(storage[r], potShape[r], shape[r], pairs ? pairs[r] : NULL,
-                              potShape[r], &E[r]);
+                              potShape[r], r == R - 1, &E[r]);
         if (ne[r] < 0) { for (int q = 0; q < r; q++) free(E[q]); return -2; }
     }
     long dstStride[3], potStride[3];
@@ -498,8 +512,11 @@ ORC_API int orc_tp_multiply(int R, const int *storage, const int *potShape, cons
                 long di = e0.row * dstStride[0] + (R > 1 ? e1.row * dstStride[1] : 0) + (R > 2 ? e2.row * dstStride[2] : 0);
                 long si = e0.col * dstStride[0] + (R > 1 ? e1.col * dstStride[1] : 0) + (R > 2 ? e2.col * dstStride[2] : 0);
                 long pi = e0.pidx * potStride[0] + (R > 1 ? e1.pidx * potStride[1] : 0) + (R > 2 ? e2.pidx * potStride[2] : 0);
-                int cj = e0.conj ^ e1.conj ^ e2.conj;
+                const orc_entry *last = R == 1 ? &e0 : (R == 2 ? &e1 : &e2);
+                int cj = (e0.flags ^ e1.flags ^ e2.flags) & ENT_CONJ;
+                if (last->flags & ENT_OWN) cj = last->flags & ENT_CONJ;
                 zc v = cj ? conj(V[pi]) : V[pi];
+                if (last->flags & ENT_REAL) v = creal(v);
                 dst[di] += v * scaling * src[si];
             }
         }

@@ -7,7 +7,9 @@ way the reference turns C++ ``runtime_error`` into Python ``RuntimeError``.
 There is no CPU fallback: if the library is missing, importing the compute
 path raises.
 """
+import atexit
 import ctypes
+import itertools
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -27,7 +29,7 @@ EXPORTS = [
     "pp_band_cholesky_upper", "pp_overlap_create", "pp_overlap_create_chunked", "pp_overlap_factor", "pp_overlap_destroy",
     "pp_overlap_apply", "pp_tp_multiply", "pp_pamp_create", "pp_pamp_add_term", "pp_pamp_add_factored_term",
     "pp_lowrank_factor", "pp_pamp_propagate", "pp_pamp_diagnostics", "pp_pamp_set_mixed", "pp_pamp_set_fiber_split", "pp_pamp_set_pade_scaling", "pp_expokit_propagate", "pp_expokit_diagnostics", "pp_pamp_batch_propagate", "pp_pamp_batch_graph_create", "pp_pamp_batch_graph_launch", "pp_pamp_batch_graph_destroy",
-    "pp_gram_schmidt", "pp_gram_schmidt_noalloc", "pp_basis_combine", "pp_band_lu_solve", "pp_pamp_lds_estimate", "pp_pamp_set_timing", "pp_pamp_timings", "pp_pamp_destroy",
+    "pp_gram_schmidt", "pp_gram_schmidt_noalloc", "pp_inner_product", "pp_basis_combine", "pp_band_lu_solve", "pp_pamp_lds_estimate", "pp_pamp_set_timing", "pp_pamp_timings", "pp_pamp_destroy",
     "pp_pamp3_create", "pp_pamp3_add_kron_term", "pp_pamp3_add_multipole_term", "pp_pamp3_apply",
     "pp_pamp3_propagate", "pp_pamp3_diagnostics", "pp_pamp3_set_timing", "pp_pamp3_kernel_time",
     "pp_pamp3_set_radial_supports", "pp_pamp3_set_mixed",
@@ -91,6 +93,7 @@ def lib():
     L.pp_pamp_batch_graph_destroy.argtypes = [P]
     L.pp_gram_schmidt.argtypes = [P, ctypes.c_long, I, P, I, dp, dp, P]
     L.pp_gram_schmidt_noalloc.argtypes = [P, ctypes.c_long, I, P, I, dp, dp, P]
+    L.pp_inner_product.argtypes = [P, P, ctypes.c_long, dp, P]
     L.pp_basis_combine.argtypes = [P, ctypes.c_long, I, dp, I, P, P]
     L.pp_band_lu_solve.argtypes = [I, I, I, I, P, ctypes.POINTER(ctypes.c_int), P, P]
     L.pp_pamp_set_timing.argtypes = [P, I]
@@ -114,6 +117,44 @@ def lib():
     L.pp_pamp_lds_estimate.restype = ctypes.c_long
     _lib = L
     return L
+
+
+# Native objects (overlaps, engines, graphs) owned by Python wrappers.  They
+# are destroyed when their wrapper is collected, or, for those still alive
+# when the interpreter exits, by an exit hook that runs while the HIP runtime
+# is still up, newest first (engines and graphs before the overlaps they
+# use).  After that hook no wrapper calls into the library: a finalizer that
+# runs during interpreter teardown would otherwise free device memory after
+# the runtime (or a profiler attached to it) has shut down.
+_live = {}
+_serial = itertools.count()
+_closed = False
+
+
+def track(destroy, handle):
+    """Register a native handle with its destroy entry point; returns the
+    token to release it with."""
+    token = next(_serial)
+    _live[token] = (destroy, handle)
+    return token
+
+
+def release(token):
+    """Destroy a registered handle now; a no-op once the exit hook ran."""
+    item = _live.pop(token, None)
+    if item is None or _closed or _lib is None:
+        return
+    getattr(_lib, item[0])(item[1])
+
+
+@atexit.register
+def _release_all():
+    global _closed
+    for token in sorted(_live, reverse=True):
+        destroy, handle = _live.pop(token)
+        if _lib is not None:
+            getattr(_lib, destroy)(handle)
+    _closed = True
 
 
 def dptr(a):

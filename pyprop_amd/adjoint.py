@@ -29,9 +29,10 @@ class PopulationAdjoint:
     """Adjoint gradients for a set-up Problem whose PamPropagator runs a
     static Diag term and one time-dependent Simp dipole term (the hydrogen
     operator).  target: the state phi (L, N) whose population |<phi|S|psi_K>|^2
-    is the objective."""
+    is the objective.  derivative_krylov: Krylov dimension of the doubled
+    (Frechet-derivative) steps, default the propagator's."""
 
-    def __init__(self, problem, target, checkpoint=50):
+    def __init__(self, problem, target, checkpoint=50, derivative_krylov=None):
         prop = problem.Propagator
         if prop.Engine is None:
             raise RuntimeError("call problem.SetupStep() first")
@@ -54,7 +55,8 @@ class PopulationAdjoint:
         V = pots[1].DeviceData()
         pairs = np.asarray(pots[1].GeometryList[0].GetBasisPairs(), dtype=np.int32)
         L = self.L
-        aug = dev.PampEngine(2 * L, self.N, self.m, self.overlap, reorth=prop.DoubleOrthogonalization)
+        self.mder = int(derivative_krylov) if derivative_krylov else self.m
+        aug = dev.PampEngine(2 * L, self.N, self.mder, self.overlap, reorth=prop.DoubleOrthogonalization)
         aug.add_term("Diag", torch.cat([H0, H0]).contiguous())
         aug.add_term("Simp", torch.cat([V, V]).contiguous(), pairs=np.concatenate([pairs, pairs + L]))
         aug.add_term("Simp", V, pairs=np.stack([pairs[:, 0], pairs[:, 1] + L], axis=1))
@@ -67,10 +69,14 @@ class PopulationAdjoint:
 
     # ---------------------------------------------------------------- helpers
     def _sdot(self, a, b):
-        """<a|S|b> over the radial rank of (L, N) device tensors."""
+        """<a|S|b> over the radial rank of (L, N) device tensors (overlap
+        kernel, then the device zdotc)."""
         tmp = b.clone()
         self.overlap.apply(nat.OV_MUL, 1, tmp)
-        return complex((a.conj() * tmp).sum().item())
+        out = np.zeros(2)
+        nat.check(nat.lib().pp_inner_product(dev.ptr(a.contiguous()), dev.ptr(tmp), int(tmp.numel()),
+                                             nat.dptr(out), dev.stream_ptr()), "pp_inner_product")
+        return complex(out[0], out[1])
 
     def _steps(self, psi, s, k0, n, sign=1.0):
         sc = np.stack([np.ones(n), np.asarray(s[k0:k0 + n], dtype=np.float64)], axis=1)
@@ -123,6 +129,34 @@ class PopulationAdjoint:
                 self._steps(lam, s, k, 1, sign=-1.0)
         torch.cuda.synchronize()
         return P, grad
+
+
+def central_differences(adjoint, s, indices, h=1e-6):
+    """(P(s + h e_k) - P(s - h e_k)) / 2h for each k in indices: 2 forward
+    runs per index, all of them in one batched set of packed launches
+    (pp_pamp_batch_propagate) from the adjoint's start state, with the same
+    operator, Pade scaling and step as the adjoint's forward map."""
+    torch = dev.torch()
+    s = np.asarray(s, dtype=np.float64)
+    K = len(s)
+    eng0, pots = adjoint.engine, adjoint.problem.Propagator._pots
+    pairs = pots[1].GeometryList[0].GetBasisPairs()
+    engines, psis, scs = [], [], []
+    for k in indices:
+        for sgn in (1.0, -1.0):
+            e = dev.PampEngine(eng0.L, eng0.N, eng0.m, eng0.overlap)
+            e.add_term("Diag", pots[0].DeviceData())
+            e.add_term("Simp", pots[1].DeviceData(), pairs=pairs)
+            e.set_pade_scaling(1)
+            sv = s.copy()
+            sv[k] += sgn * h
+            engines.append(e)
+            psis.append(adjoint.psi0.clone())
+            scs.append(np.stack([np.ones(K), sv], axis=1))
+    dev.pamp_batch_propagate(engines, psis, K, [adjoint.dt] * len(engines), scs)
+    torch.cuda.synchronize()
+    pops = [abs(adjoint._sdot(adjoint.target, p)) ** 2 for p in psis]
+    return np.array([(pops[2 * i] - pops[2 * i + 1]) / (2 * h) for i in range(len(indices))])
 
 
 def population_function(adjoint):

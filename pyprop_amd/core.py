@@ -174,12 +174,19 @@ class CombinedRepresentation(RepresentationBase):
         self._apply(nat.OV_MUL, psi)
 
     def InnerProduct(self, psi1, psi2):
-        """<psi1|S|psi2> (Algo 4, combinedrepresentation.cpp:423-474)."""
+        """<psi1|S|psi2> (Algo 4, combinedrepresentation.cpp:423-474): S psi2
+        by the overlap kernels along every non-orthogonal rank, then the
+        device zdotc (pp_inner_product)."""
+        a = psi1.GetData()
         tmp = psi2.GetData().clone()
+        if tuple(a.shape) != tuple(tmp.shape):
+            raise ValueError("wavefunctions of different shapes")
         for i in self._ranks():
             self.Representations[i].GetOverlap().apply(nat.OV_MUL, i, tmp)
-        t = dev.torch()
-        return complex(t.sum(t.conj(psi1.GetData()) * tmp).item())
+        out = np.zeros(2)
+        nat.check(nat.lib().pp_inner_product(dev.ptr(a), dev.ptr(tmp), int(tmp.numel()), nat.dptr(out),
+                                             dev.stream_ptr()), "pp_inner_product")
+        return complex(out[0], out[1])
 
 
 def _combined(rank):

@@ -249,4 +249,7 @@ struct pp_rank_csr {
   std::vector<int> rowptr;   // extent+1
   std::vector<int> ent;      // 3 ints per entry: source col, potential index, conjugate flag
 };
-int pp_build_csr(int storage, int potExt, int srcExt, const int *pairs, pp_rank_csr &csr);
+// entry flags: conjugate V; the entry's own conjugation replaces the outer
+// ranks' (innermost Band, zhbmv); only the real part of V (zhbmv's diagonal)
+enum { PP_ENT_CONJ = 1, PP_ENT_OWN = 2, PP_ENT_REAL = 4 };
+int pp_build_csr(int storage, int potExt, int srcExt, const int *pairs, pp_rank_csr &csr, bool innermost = false);

@@ -14,16 +14,21 @@
 #include <cstring>
 
 
-// Expand one rank's storage into destination-row lists.
-// Ident :436-462, Diag :403-433, Simp :302-363, Herm :370-398,
-// BandNH :622-763 (general band, pidx = col*bandCount + s+row-col),
-// Band :467-618 (hermitian lower band storage, conj for the mirrored part).
-int pp_build_csr(int storage, int potExt, int srcExt, const int *pairs, pp_rank_csr &csr)
+// Expand one rank's storage into destination-row lists of (source col,
+// potential index, flags).  Ident :436-462, Diag :403-433, Simp :302-363,
+// Herm :370-398, BandNH :622-763 (general band, pidx = col*bandCount +
+// s+row-col), Band :467-618: slot j*k + d (d = 0..k-1) belongs to the pair
+// (j, j+d); on the innermost rank the generator calls zhbmv 'L' (:562-575),
+// so A[j+d][j] = V, A[j][j+d] = conj V, the diagonal's real part only and
+// no conjugation from an outer Herm mirror (PP_ENT_OWN); on an outer rank
+// its loops (:513-551) apply A[j][j+d] = V and A[j+d][j] = conj V with the
+// complex diagonal.
+int pp_build_csr(int storage, int potExt, int srcExt, const int *pairs, pp_rank_csr &csr, bool innermost)
 {
   std::vector<std::vector<int>> rows((size_t)srcExt);
-  auto add = [&](int row, int col, int pidx, int cj) -> bool {
+  auto add = [&](int row, int col, int pidx, int flags) -> bool {
     if (row < 0 || row >= srcExt || col < 0 || col >= srcExt) return false;
-    rows[row].push_back(col); rows[row].push_back(pidx); rows[row].push_back(cj);
+    rows[row].push_back(col); rows[row].push_back(pidx); rows[row].push_back(flags);
     return true;
   };
   switch (storage) {
@@ -43,7 +48,7 @@ int pp_build_csr(int storage, int potExt, int srcExt, const int *pairs, pp_rank_
       if (!pairs) return PP_EARG;
       for (int i = 0; i < potExt; i++) {
         if (!add(pairs[2 * i], pairs[2 * i + 1], i, 0)) return PP_EARG;
-        if (!add(pairs[2 * i + 1], pairs[2 * i], i, 1)) return PP_EARG;
+        if (!add(pairs[2 * i + 1], pairs[2 * i], i, PP_ENT_CONJ)) return PP_EARG;
       }
       break;
     case PP_BANDNH: {
@@ -55,10 +60,17 @@ int pp_build_csr(int storage, int potExt, int srcExt, const int *pairs, pp_rank_
     }
     case PP_BAND: {
       int N = srcExt, k = potExt / N;
-      for (int col = 0; col < N; col++)
-        for (int row = col; row < N && row < col + k; row++) {
-          add(row, col, col * k + row - col, 0);
-          if (row != col) add(col, row, col * k + row - col, 1);
+      for (int j = 0; j < N; j++)
+        for (int d = 0; d < k && j + d < N; d++) {
+          const int slot = j * k + d;
+          if (innermost) {
+            if (d == 0) { add(j, j, slot, PP_ENT_OWN | PP_ENT_REAL); continue; }
+            add(j + d, j, slot, PP_ENT_OWN);
+            add(j, j + d, slot, PP_ENT_OWN | PP_ENT_CONJ);
+          } else {
+            add(j, j + d, slot, 0);
+            if (d > 0) add(j + d, j, slot, PP_ENT_CONJ);
+          }
         }
       break;
     }
@@ -93,31 +105,31 @@ __global__ void __launch_bounds__(256) pp_tp_kernel(pp_csr_dev C, const zd *__re
   long rem = id;
   for (int r = R - 1; r >= 0; r--) { row[r] = (int)(rem % C.shape[r]); rem /= C.shape[r]; }
   zd acc = zmk(0.0, 0.0);
+  // innermost entry with flags f, outer ranks' conjugation cj
+  auto term = [&](zd v, zd x, int cj, int f) {
+    if (f & PP_ENT_OWN) cj = 0;
+    if (f & PP_ENT_REAL) v.y = 0.0;
+    if ((cj ^ f) & PP_ENT_CONJ) zfmac(acc, v, x); else zfma(acc, v, x);
+  };
   for (int e0 = C.rowptr[0][row[0]]; e0 < C.rowptr[0][row[0] + 1]; e0++) {
     const int *E0 = C.ent[0] + 3 * e0;
     long s0 = (long)E0[0] * C.srcStride[0], p0 = (long)E0[1] * C.potStride[0];
-    int c0 = E0[2];
+    int c0 = E0[2] & PP_ENT_CONJ;
     if (R == 1) {
-      zd v = pot[p0];
-      zd x = src[s0];
-      if (c0) zfmac(acc, v, x); else zfma(acc, v, x);
+      term(pot[p0], src[s0], 0, E0[2]);
       continue;
     }
     for (int e1 = C.rowptr[1][row[1]]; e1 < C.rowptr[1][row[1] + 1]; e1++) {
       const int *E1 = C.ent[1] + 3 * e1;
       long s1 = s0 + (long)E1[0] * C.srcStride[1], p1 = p0 + (long)E1[1] * C.potStride[1];
-      int c1 = c0 ^ E1[2];
       if (R == 2) {
-        zd v = pot[p1];
-        zd x = src[s1];
-        if (c1) zfmac(acc, v, x); else zfma(acc, v, x);
+        term(pot[p1], src[s1], c0, E1[2]);
         continue;
       }
+      const int c1 = c0 ^ (E1[2] & PP_ENT_CONJ);
       for (int e2 = C.rowptr[2][row[2]]; e2 < C.rowptr[2][row[2] + 1]; e2++) {
         const int *E2 = C.ent[2] + 3 * e2;
-        zd v = pot[p1 + (long)E2[1] * C.potStride[2]];
-        zd x = src[s1 + (long)E2[0] * C.srcStride[2]];
-        if (c1 ^ E2[2]) zfmac(acc, v, x); else zfma(acc, v, x);
+        term(pot[p1 + (long)E2[1] * C.potStride[2]], src[s1 + (long)E2[0] * C.srcStride[2]], c1, E2[2]);
       }
     }
   }
@@ -141,7 +153,7 @@ extern "C" int pp_tp_multiply(int R, const int *storage, const int *potShape, co
   memset(&C, 0, sizeof(C));
   long n = 1;
   for (int r = 0; r < R; r++) {
-    int rc = pp_build_csr(storage[r], potShape[r], shape[r], pairs ? pairs[r] : nullptr, csr[r]);
+    int rc = pp_build_csr(storage[r], potShape[r], shape[r], pairs ? pairs[r] : nullptr, csr[r], r == R - 1);
     if (rc) return rc;
     n *= shape[r];
     C.shape[r] = shape[r];

@@ -141,6 +141,28 @@ extern "C" int pp_gram_schmidt(const void *V, long n, int k, void *w, int passes
   return PP_OK;
 }
 
+// out (host, 2 doubles) = sum_i conj(a_i) b_i: the zdotc after the overlap
+// multiply of CombinedRepresentation::InnerProduct (combinedrepresentation.cpp:
+// 423-474), as 240 row-block partial sums added in block order (the same
+// kernels as the Gram-Schmidt norms, so the result is reproducible).
+// Synchronises the stream.
+extern "C" int pp_inner_product(const void *a, const void *b, long n, double *out, hipStream_t stream)
+{
+  if (!a || !b || !out || n <= 0) return PP_EARG;
+  double *part = nullptr;
+  PP_HIP(hipMallocAsync((void **)&part, sizeof(double) * (2 * PV_BLOCKS + 2), stream));
+  hipLaunchKernelGGL(pv_dots, dim3(PV_BLOCKS, 1), dim3(PV_THREADS), 0, stream, (const zd *)a, n, (const zd *)b, part);
+  hipLaunchKernelGGL(pv_reduce, dim3(1), dim3(PV_THREADS), 0, stream, (const double *)part, PV_BLOCKS, 2,
+                     part + 2 * PV_BLOCKS, (double *)nullptr, 0);
+  const hipError_t le = hipGetLastError();
+  const hipError_t ce = le == hipSuccess ? hipMemcpyAsync(out, part + 2 * PV_BLOCKS, 2 * sizeof(double),
+                                                          hipMemcpyDeviceToHost, stream) : le;
+  const hipError_t fe = hipFreeAsync(part, stream);
+  if (ce != hipSuccess || fe != hipSuccess) return PP_EHIP;
+  PP_HIP(hipStreamSynchronize(stream));
+  return PP_OK;
+}
+
 // ---- the same call without allocation ----
 // One file-scope device array per GPU holds the scratch of
 // pp_gram_schmidt_noalloc: the partial sums of PV_GS_NOALLOC_MAXK basis

@@ -9,6 +9,15 @@ import numpy as np
 from . import _native as nat
 
 
+def _release(obj):
+    """Finalizer body of the native wrappers: destroy the handle unless the
+    interpreter is shutting down (then the exit hook of _native did it)."""
+    token = getattr(obj, "_token", None)
+    if token is not None and nat is not None:
+        nat.release(token)
+        obj._token = None
+
+
 def torch():
     import torch as _t
     return _t
@@ -62,6 +71,7 @@ class Overlap:
             raise RuntimeError("overlap matrix is not positive definite (zpbtrf info=%d)" % info.value)
         nat.check(rc, "pp_overlap_create")
         self.h = h
+        self._token = nat.track("pp_overlap_destroy", h)
         self.N, self.kd, self.chunk = N, kd, chunk
 
     def factor(self):
@@ -76,9 +86,7 @@ class Overlap:
         return x
 
     def __del__(self):
-        if getattr(self, "h", None) and nat._lib is not None:
-            nat._lib.pp_overlap_destroy(self.h)
-            self.h = None
+        _release(self)
 
 
 def tp_multiply(storage_ids, potential, scaling, src, dst, pairs=None):
@@ -150,6 +158,7 @@ class PampEngine:
         h = ctypes.c_void_p()
         nat.check(nat.lib().pp_pamp_create(L, N, m, int(bool(reorth)), overlap.h, ctypes.byref(h)), "pp_pamp_create")
         self.h = h
+        self._token = nat.track("pp_pamp_destroy", h)
         self.L, self.N, self.m = L, N, m
         self.overlap = overlap
         self.terms = []
@@ -240,9 +249,7 @@ class PampEngine:
         return np.array(buf[:cnt.value], dtype=np.float64) / 1e8
 
     def __del__(self):
-        if getattr(self, "h", None) and nat._lib is not None:
-            nat._lib.pp_pamp_destroy(self.h)
-            self.h = None
+        _release(self)
 
 
 class BatchGraph:
@@ -264,6 +271,7 @@ class BatchGraph:
         nat.check(nat.lib().pp_pamp_batch_graph_create(n, H, Pp, self.nsteps, nat.dptr(dt), ctypes.byref(h)),
                   "pp_pamp_batch_graph_create")
         self.h = h
+        self._token = nat.track("pp_pamp_batch_graph_destroy", h)
 
     def launch(self, scalings, psis=None, check=True):
         if psis is not None and [ptr(p).value for p in psis] != self._ptrs:
@@ -280,9 +288,7 @@ class BatchGraph:
                 check_pamp_steps(e.diagnostics(self.nsteps), "pAMP (batch instance %d)" % i)
 
     def __del__(self):
-        if getattr(self, "h", None) and nat._lib is not None:
-            nat._lib.pp_pamp_batch_graph_destroy(self.h)
-            self.h = None
+        _release(self)
 
 
 def pamp_batch_propagate(engines, psis, nsteps, dts, scalings, check=True):

@@ -30,8 +30,19 @@ def LaserFunctionVelocity(conf, t):
 WORKLOADS = {
     "hydrogen_small": ("hydrogen_small.ini", 2000),   # order 7, 100 breakpoints, l<=10, m=20, dt=0.01
     "hydrogen_large": ("hydrogen_large.ini", 20000),  # order 7, N=800, l<=63, m=24, dt=0.005
-    "hydrogen_adjoint": ("hydrogen_adjoint.ini", 4000),  # order 7, N=400, l<=31, m=20, dt=0.01
+    "hydrogen_adjoint": ("hydrogen_adjoint.ini", 4000),  # order 7, N=400, r<=800, l<=31, m=24, dt=0.005
 }
+
+
+def adjoint_control_samples(K, dt, seed=20261203, E0=0.05, omega=0.3, sigma=0.005):
+    """Control samples of the adjoint workload: the interaction time factor
+    (velocity-gauge vector potential) at t_k = k dt, a sin^2-enveloped
+    carrier A0 sin^2(pi t/T) (-cos(omega t)), A0 = E0/omega, T = K dt, plus
+    seeded Gaussian noise of sigma on every sample."""
+    rng = np.random.default_rng(seed)
+    t = dt * np.arange(K)
+    T = dt * K
+    return (E0 / omega) * np.sin(np.pi * t / T) ** 2 * (-np.cos(omega * t)) + sigma * rng.standard_normal(K)
 
 
 def register():
@@ -152,24 +163,30 @@ def bound_state(problem, l, index):
     return out, E[index]
 
 
-def populations(problem, psi, emax=0.0):
-    """Bound populations per l: sum_{E<0} |c^H S psi_l|^2 with c the S-normalised
-    eigenvectors of (H0_l, S) (analysis.py:92-140,344-364)."""
+def bound_projectors(problem, emax=0.0):
+    """Per angular channel, the S-normalised eigenvectors of (H0_l, S) with
+    E < emax, and S (analysis.py:92-140): what populations() projects on."""
     import scipy.linalg
     prop = problem.Propagator.BasePropagator
     repr = problem.psi.GetRepresentation()
     L, N = repr.GetFullShape()
-    bs = repr.GetRepresentation(1).GetBSplineObject()
-    S = bs.overlap_dense()
-    per = []
+    S = repr.GetRepresentation(1).GetBSplineObject().overlap_dense()
+    out = []
     for li in range(L):
         H = np.zeros((N, N), dtype=np.complex128)
         for pot in prop.PotentialList:
             if pot.IsTimeDependent or pot.StorageIds() != ["Diag", "BandNH"]:
                 continue
             H += _block(pot, li, N, pot.PotentialData.shape[1] // N)
-        E, V = scipy.linalg.eigh(H, S)
-        V = V[:, E < emax]
-        c = V.conj().T @ (S @ psi[li])
-        per.append(np.abs(c) ** 2)
+        E, V = scipy.linalg.eigh(H, S, subset_by_value=(-np.inf, emax))
+        out.append(V)
+    return out, S
+
+
+def populations(problem, psi, emax=0.0, projectors=None):
+    """Bound populations per l: sum_{E<0} |c^H S psi_l|^2 with c the S-normalised
+    eigenvectors of (H0_l, S) (analysis.py:92-140,344-364).  projectors: the
+    result of bound_projectors(problem, emax), to reuse for several states."""
+    Vs, S = projectors if projectors is not None else bound_projectors(problem, emax)
+    per = [np.abs(V.conj().T @ (S @ psi[li])) ** 2 for li, V in enumerate(Vs)]
     return sum(p.sum() for p in per), per

@@ -324,13 +324,13 @@ class Pamp3Engine:
         nat.check(nat.lib().pp_pamp3_create(self.C, self.N, self.m, int(bool(reorth)), overlap.h,
                                             ctypes.byref(h)), "pp_pamp3_create")
         self.h = h
+        self._token = nat.track("pp_pamp3_destroy", h)
         self.nslots = 0
         self._last_steps = 0
 
     def __del__(self):
-        if getattr(self, "h", None) and nat._lib is not None:
-            nat._lib.pp_pamp3_destroy(self.h)
-            self.h = None
+        if dev is not None:
+            dev._release(self)
 
     def add_kron(self, term, slot=-1):
         coef = term.coef.view(np.float64)

@@ -70,13 +70,15 @@ def _storage_case(st, n, rng):
     (O.ST_IDENT,), (O.ST_DIAG,), (O.ST_SIMP,), (O.ST_HERM,), (O.ST_BANDNH,), (O.ST_BAND,),
     (O.ST_DIAG, O.ST_BANDNH), (O.ST_SIMP, O.ST_BANDNH), (O.ST_HERM, O.ST_BAND), (O.ST_IDENT, O.ST_BAND),
     (O.ST_DIAG, O.ST_DIAG), (O.ST_SIMP, O.ST_SIMP, O.ST_BANDNH), (O.ST_DIAG, O.ST_HERM, O.ST_BAND),
-    (O.ST_IDENT, O.ST_SIMP, O.ST_BANDNH)])
+    (O.ST_IDENT, O.ST_SIMP, O.ST_BANDNH), (O.ST_BAND, O.ST_BANDNH), (O.ST_BAND, O.ST_DIAG), (O.ST_BAND, O.ST_BAND),
+    (O.ST_BAND, O.ST_HERM, O.ST_BAND), (O.ST_HERM, O.ST_BAND, O.ST_BANDNH)])
 def test_tensor_multiply_all_storages_match_oracle(storages):
     """pp_tp_multiply against the oracle's restatement of the generated
     TensorPotentialMultiply_<ids> loops for every storage id of the
     generator (Ident, Diag, Simp, Herm with its conjugate mirror, BandNH,
-    Band with its hermitian upper part) at ranks 1-3, random complex
-    potentials, dst accumulated (+=): relative 1e-13."""
+    Band: zhbmv's reading on the innermost rank, the upper-band loops on an
+    outer rank) at ranks 1-3, random complex potentials, dst accumulated
+    (+=): relative 1e-13."""
     from pyprop_amd import device as D
     rng = np.random.default_rng(sum(storages) * 7 + len(storages))
     shape = (9, 7, 11)[:len(storages)]
@@ -496,3 +498,52 @@ def test_zero_wavefunction_raises(prop):
     with pytest.raises(RuntimeError, match="Wavefunction is zero"):
         prob.AdvanceStep()
     assert not prob.psi.GetData().abs().max().item() > 0.0
+
+
+def test_observables_match_oracle():
+    """Wavefunction.GetNorm, InnerProduct (<phi|S|psi> by the overlap kernel
+    and the device zdotc) and the bound populations (analysis.py:92-140,
+    344-364) of a propagated state, against the oracle's S-norm, its
+    orc_ham_inner and its populations: same psi (after 10 GPU steps of the
+    small hydrogen config with the field on), 1e-13 relative."""
+    import pyprop_amd
+    from pyprop_amd.examples import hydrogen as H
+    print("observables: norm / inner product / populations vs oracle", flush=True)
+    conf = H.load("hydrogen_small")
+    conf.SetValue("AngularRepresentation", "maxl", 4)
+    conf.SetValue("RadialRepresentation", "xsize", 40)
+    conf.SetValue("RadialRepresentation", "xmax", 40.0)
+    conf.SetValue("PulseParameters", "frequency", 0.3)
+    conf = pyprop_amd.Config(conf.cfgObj)
+    prob = pyprop_amd.Problem(conf)
+    prob.SetupStep()
+    psi0 = H.ground_state(prob)
+    psi0[1] = 0.3 * psi0[0]
+    prob.SetWavefunction(psi0)
+    prob.PropagatedTime = 30.0
+    prob.AdvanceSteps(10)
+    psi = prob.psi.GetData().cpu().numpy()
+    h = O.Hydrogen(rmax=40.0, nbp=40, order=7, lmax=4, frequency=0.3, cycles=6, E0=0.05)
+    ham = h.ham()
+    # norm
+    nref = h.norm(psi)
+    assert abs(prob.psi.GetNorm() - nref) <= 1e-13 * nref
+    # <phi|S|psi> against the oracle's inner product (U-factored S)
+    phi_h, _ = H.bound_state(prob, 1, 0)          # 2p
+    phi = prob.psi.Copy()
+    phi.GetData().copy_(dev_tensor(phi_h))
+    ip = phi.InnerProduct(prob.psi)
+    iref = ham.inner(phi_h, psi)
+    assert abs(ip - iref) <= 1e-13 * max(abs(iref), 1e-300) + 1e-15
+    assert abs(prob.psi.InnerProduct(prob.psi) - nref ** 2) <= 1e-13
+    # bound populations of the product's analysis against the oracle's
+    tot, per = H.populations(prob, psi)
+    tref, pref = h.populations(psi)
+    assert abs(tot - tref) <= 1e-12
+    for a, b in zip(per, pref):
+        assert len(a) == len(b) and np.abs(np.asarray(a) - np.asarray(b)).max() <= 1e-12
+
+
+def dev_tensor(a):
+    from pyprop_amd import device as D
+    return D.to_device(a)

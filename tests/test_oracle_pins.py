@@ -430,3 +430,98 @@ def test_product_breakpoints_match_oracle():
         assert got.shape == ref.shape
         assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
     assert len(B.ExponentialLinearBreakpointSequence(0.0, 10.0, 160.0, 30, 1.0)) == 310
+
+
+def _zhbmv(v_band, x):
+    """scipy's zhbmv with the reference's innermost Band call: lower band
+    storage, lda = k (tensorpotentialmultiply_generator.py:562-575);
+    v_band is the (N, k) slot array V[j*k + d]."""
+    import scipy.linalg.blas as blas
+    N, k = v_band.shape
+    return blas.zhbmv(k - 1, 1.0, np.ascontiguousarray(v_band.T), x, lower=1)
+
+
+def _zgbmv(v_band, x):
+    """scipy's zgbmv as the innermost BandNH call (:702-723): (N, bw) slots,
+    column j at j*bw, kl = ku = (bw - 1)/2."""
+    import scipy.linalg.blas as blas
+    N, bw = v_band.shape
+    s = (bw - 1) // 2
+    return blas.zgbmv(N, N, s, s, 1.0, np.ascontiguousarray(v_band.T), x)
+
+
+def test_band_and_identity_storages_vs_blas():
+    """The oracle's Band and Ident storages against scipy BLAS:
+    * Band on the innermost rank is the reference's zhbmv 'L' call: the
+      lower band, the diagonal's real part only, and no conjugation from an
+      outer Herm mirror (the call ignores it);
+    * Band on an outer rank is the generator's explicit loops (:513-551),
+      A[j][j+d] = V, A[j+d][j] = conj V with the complex diagonal; as a
+      matrix that is the transpose of zhbmv's Z plus i Im of the diagonal,
+      so A x = conj(Z conj x) + i Im(v_0) x;
+    * Ident applies potential index 0 on the diagonal of its rank."""
+    rng = np.random.default_rng(17)
+    c = lambda *s: rng.standard_normal(s) + 1j * rng.standard_normal(s)
+    N, k, sc = 13, 4, 0.7
+    # innermost Band, rank 1
+    V = c(N * k)
+    x = c(N)
+    y = np.zeros(N, complex)
+    O.tp_multiply((O.ST_BAND,), V, sc, x, y)
+    ref = sc * _zhbmv(V.reshape(N, k), x)
+    assert np.abs(y - ref).max() < 1e-13 * np.abs(ref).max()
+    # outer Band over an inner Diag: each inner index is its own outer band
+    M = 5
+    V = c(N * k, M)
+    x = c(N, M)
+    y = np.zeros((N, M), complex)
+    O.tp_multiply((O.ST_BAND, O.ST_DIAG), V, sc, x, y)
+    ref = np.zeros_like(y)
+    for i in range(M):
+        vb = V[:, i].reshape(N, k)
+        ref[:, i] = sc * (np.conj(_zhbmv(vb, np.conj(x[:, i]))) + 1j * vb[:, 0].imag * x[:, i])
+    assert np.abs(y - ref).max() < 1e-13 * np.abs(ref).max()
+    # outer Band over an inner BandNH (no conjugation passes into zgbmv on
+    # the upper part; the mirrored part conjugates the whole inner band)
+    N2, bw = 9, 5
+    V = c(N * k, N2 * bw)
+    x = c(N, N2)
+    y = np.zeros((N, N2), complex)
+    O.tp_multiply((O.ST_BAND, O.ST_BANDNH), V, sc, x, y)
+    ref = np.zeros_like(y)
+    for j in range(N):
+        for d in range(k):
+            if j + d >= N:
+                continue
+            vb = V[j * k + d].reshape(N2, bw)
+            ref[j] += sc * _zgbmv(vb, x[j + d])
+            if d > 0:
+                ref[j + d] += sc * np.conj(_zgbmv(vb, np.conj(x[j])))
+    assert np.abs(y - ref).max() < 1e-13 * np.abs(ref).max()
+    # Herm outer, Band inner: both halves of every pair through the same zhbmv
+    P = 3
+    pairs = np.array([[0, 1], [1, 3], [2, 2]], dtype=np.int32)
+    V = c(P, N * k)
+    x = c(4, N)
+    y = np.zeros((4, N), complex)
+    O.tp_multiply((O.ST_HERM, O.ST_BAND), V, sc, x, y, [pairs, None])
+    ref = np.zeros_like(y)
+    for p, (r, cc) in enumerate(pairs):
+        ref[r] += sc * _zhbmv(V[p].reshape(N, k), x[cc])
+        ref[cc] += sc * _zhbmv(V[p].reshape(N, k), x[r])
+    assert np.abs(y - ref).max() < 1e-13 * np.abs(ref).max()
+    # Ident: potential index 0 on every diagonal element of its rank
+    V = c(1, N2 * bw)
+    x = c(4, N2)
+    y = np.zeros((4, N2), complex)
+    O.tp_multiply((O.ST_IDENT, O.ST_BANDNH), V, sc, x, y)
+    ref = np.array([sc * _zgbmv(V[0].reshape(N2, bw), x[i]) for i in range(4)])
+    assert np.abs(y - ref).max() < 1e-13 * np.abs(ref).max()
+    V = c(4, 1)
+    y = np.zeros((4, N2), complex)
+    O.tp_multiply((O.ST_DIAG, O.ST_IDENT), V, sc, x, y)
+    assert np.abs(y - sc * V * x).max() < 1e-13 * np.abs(x).max()
+    V = c(1)
+    y = np.zeros(N2, complex)
+    O.tp_multiply((O.ST_IDENT,), V, sc, x[0].copy(), y)
+    assert np.abs(y - sc * V[0] * x[0]).max() < 1e-13 * np.abs(x).max()
