@@ -30,14 +30,51 @@ _LIB = None
 ST_IDENT, ST_DIAG, ST_SIMP, ST_HERM, ST_BANDNH, ST_BAND = range(6)
 
 
+_BUILD = "liborc.so"
+# builds of pyprop_oracle.c (oracle/Makefile): the checker the tests pin
+# (-O2, x86-64-v2), and two -O3 builds for the CPU baseline timing
+_BUILDS = {"checker": "liborc.so", "v3": "liborc_v3.so", "v4": "liborc_v4.so"}
+_ISA = {"v3": {"avx2", "fma", "bmi1", "bmi2", "f16c", "movbe", "abm"},
+        "v4": {"avx2", "fma", "bmi1", "bmi2", "f16c", "movbe", "abm",
+               "avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}}
+
+
 def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def fastest_build():
+    """The -O3 build whose instruction set this host has (x86-64-v4:
+    AVX-512; v3: AVX2/FMA), else the checker build."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            flags = set()
+            for line in fh:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    break
+    except OSError:
+        return "checker"
+    for name in ("v4", "v3"):
+        if _ISA[name] <= flags and os.path.exists(os.path.join(_HERE, _BUILDS[name])):
+            return name
+    return "checker"
+
+
+def use_build(name):
+    """Select the build later lib() calls load ("checker", "v3", "v4");
+    only before the first lib() call of the process."""
+    global _BUILD
+    if _LIB is not None and _BUILDS[name] != _BUILD:
+        raise RuntimeError("the oracle library is already loaded")
+    _BUILD = _BUILDS[name]
+    return name
 
 
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liborc.so")
+        path = os.path.join(_HERE, _BUILD)
         if not os.path.exists(path):
             build()
         L = ctypes.CDLL(path)

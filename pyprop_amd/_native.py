@@ -140,11 +140,13 @@ def track(destroy, handle):
 
 
 def release(token):
-    """Destroy a registered handle now; a no-op once the exit hook ran."""
-    item = _live.pop(token, None)
-    if item is None or _closed or _lib is None:
+    """Destroy a registered handle now; a no-op once the exit hook ran (or
+    while the interpreter tears this module down)."""
+    if _closed is not False or _live is None or _lib is None:
         return
-    getattr(_lib, item[0])(item[1])
+    item = _live.pop(token, None)
+    if item is not None:
+        getattr(_lib, item[0])(item[1])
 
 
 @atexit.register

@@ -85,7 +85,7 @@ class Overlap:
                   "pp_overlap_apply")
         return x
 
-    def __del__(self):
+    def __del__(self, _release=_release):   # bound now: module globals are gone at teardown
         _release(self)
 
 
@@ -248,7 +248,7 @@ class PampEngine:
         nat.check(nat.lib().pp_pamp_timings(self.h, buf, 512, ctypes.byref(cnt)), "pp_pamp_timings")
         return np.array(buf[:cnt.value], dtype=np.float64) / 1e8
 
-    def __del__(self):
+    def __del__(self, _release=_release):   # bound now: module globals are gone at teardown
         _release(self)
 
 
@@ -287,7 +287,7 @@ class BatchGraph:
             for i, e in enumerate(self.engines):
                 check_pamp_steps(e.diagnostics(self.nsteps), "pAMP (batch instance %d)" % i)
 
-    def __del__(self):
+    def __del__(self, _release=_release):   # bound now: module globals are gone at teardown
         _release(self)
 
 

@@ -328,9 +328,8 @@ class Pamp3Engine:
         self.nslots = 0
         self._last_steps = 0
 
-    def __del__(self):
-        if dev is not None:
-            dev._release(self)
+    def __del__(self, _release=dev._release):   # bound now: module globals are gone at teardown
+        _release(self)
 
     def add_kron(self, term, slot=-1):
         coef = term.coef.view(np.float64)

@@ -83,9 +83,9 @@ class PamPropagator(PropagatorBase):
         if hasattr(section, "krylov_double_orthogonalization"):
             self.DoubleOrthogonalization = bool(section.krylov_double_orthogonalization)
         # extension key (no reference counterpart): "fp64" (default, the
-        # drop-in) or "mixed" (one-electron states: time-dependent potentials
-        # applied in complex FP32; two-electron states: the electron-electron
-        # multipole terms in FP32; the rest of the Hamiltonian, Gram-Schmidt,
+        # drop-in) or "mixed" (one-electron states: every potential applied
+        # in complex FP32; two-electron states: the electron-electron
+        # multipole terms, the bulk of the apply, in FP32; Gram-Schmidt,
         # overlap solves and Pade stay FP64)
         if hasattr(section, "matvec_precision"):
             prec = str(section.matvec_precision).lower()
@@ -124,11 +124,10 @@ class PamPropagator(PropagatorBase):
             pairs = pot.GeometryList[0].GetBasisPairs() if ids[0] in ("Simp", "Herm") else None
             eng.add_term(ids[0], pot.DeviceData(), pairs=pairs)
         if self.MatvecPrecision == "mixed":
-            # the static part carries the stiff spectrum (centrifugal and
-            # kinetic terms reach ~1e4 at large l near the origin), so FP32
-            # rounding there would dominate the error; the time-dependent
-            # couplings are small and take the FP32 path
-            eng.set_mixed([i for i, p in enumerate(pots) if p.IsTimeDependent])
+            # every term's radial bands in FP32 with FP32 band sums (the
+            # complex64 operator apply); the exchange, Gram-Schmidt, the
+            # overlap solves and Pade stay FP64
+            eng.set_mixed(list(range(len(pots))))
         self._pots = pots
         return eng
 

@@ -131,29 +131,37 @@ def test_pamp_persistent_matches_oracle(lmax, nbp, m, nsteps, dt, reorth):
 
 
 @pytest.mark.parametrize("lmax,nbp,m,nsteps,dt", [(10, 100, 20, 20, 0.01), (63, 120, 24, 6, 0.005)])
-def test_pamp_mixed_close_to_oracle(lmax, nbp, m, nsteps, dt):
-    """Mixed form (the dipole term applied in complex FP32, FP64 elsewhere)
-    against the FP64 oracle: the difference must sit at FP32 rounding of that
-    term (between 1e-13 and 1e-6 of max|psi|, so both a silent FP64 path and a
-    broken FP32 path fail), and the norm must still be conserved to 1e-8."""
+@pytest.mark.parametrize("mask", [(1,), (0, 1)])
+def test_pamp_mixed_close_to_oracle(lmax, nbp, m, nsteps, dt, mask):
+    """Mixed form against the FP64 oracle, with the dipole term alone or
+    every term (the matvec_precision = "mixed" form) applied in complex
+    FP32: the difference must exceed 1e-13 of max|psi| (no silent FP64
+    path) and stay within FP32 rounding amplified by the step (1e-6 for the
+    dipole alone, 1e-4 for every term: the kinetic and centrifugal bands
+    are large and nearly cancel) or 100x the oracle's own sensitivity to a
+    1e-16 perturbation; the norm is conserved to 1e-8."""
     from pyprop_amd import device as D
     h = _hydrogen(lmax=lmax, nbp=nbp, rmax=float(nbp))
     psi0 = h.ground_state()
     psi0[1] = 0.1 * psi0[0]
     ts = O.step_times(nsteps, dt, t0=10.0)
     sc = np.array([[1.0, h.scaling(t)] for t in ts])
-    ref = h.ham().propagate_pamp(psi0, nsteps, dt, sc, m, False)
+    ham = h.ham()
+    ref = ham.propagate_pamp(psi0, nsteps, dt, sc, m, False)
+    rng = np.random.default_rng(11)
+    ref2 = ham.propagate_pamp(psi0 * (1 + 1e-16 * rng.standard_normal(psi0.shape)), nsteps, dt, sc, m, False)
+    sens = np.abs(ref2 - ref).max() / np.abs(ref).max()
     ov = D.Overlap(h.S, h.N, h.kd)
     eng = D.PampEngine(h.L, h.N, m, ov)
     eng.add_term("Diag", D.to_device(h.H0))
     eng.add_term("Simp", D.to_device(h.Hint), pairs=h.pairs)
-    eng.set_mixed([1])
+    eng.set_mixed(list(mask))
     psi = D.to_device(psi0)
     eng.propagate(psi, nsteps, dt, sc)
     got = psi.cpu().numpy()
     err = np.abs(got - ref).max() / np.abs(ref).max()
-    print("mixed", lmax, nbp, m, nsteps, dt, "rel err", err)
-    assert 1e-13 < err < 1e-6
+    print("mixed", mask, lmax, nbp, m, nsteps, dt, "rel err %.3e, oracle sensitivity %.3e" % (err, sens), flush=True)
+    assert 1e-13 < err < max(1e-6 if mask == (1,) else 1e-4, 100 * sens)
     assert abs(h.norm(got) - h.norm(ref)) < 1e-8
 
 
