@@ -115,6 +115,7 @@ struct pp_pamp3 {
   // buffers
   zd *Y, *X, *Z, *P, *V, *Ym;
   double *part, *scal, *diag, *scal_dt;
+  double *urd;             // reciprocals of U's diagonal (the fiber solves)
   zd *Hm, *cvec, *hcoef;
   double *beta;
   int *state;              // [0] broken, [1] mEff
@@ -553,21 +554,31 @@ __global__ void __launch_bounds__(256) p3_near(const T *const *Rn, const int *bl
   ZT acc[4];
 #pragma unroll
   for (int t = 0; t < 4; t++) acc[t] = p3_czero<ZT>();
-  if (any) {
+  // this wavefront's copy of R[d1][.][.] for the current (block, d1): the
+  // coefficients are shared by the 16 channel lanes, so one coalesced load
+  // per lane fills it and the taps read LDS instead of issuing 52 loads each
+  __shared__ T rs[P3_NG4][P3_NW * BW];
+  T *rw = rs[grp];
+  {
     bool pv[WIN];
 #pragma unroll
-    for (int p = 0; p < WIN; p++) pv[p] = p0 + p >= 0 && p0 + p < N;
+    for (int p = 0; p < WIN; p++) pv[p] = any && p0 + p >= 0 && p0 + p < N;
     for (int k = grp; k < nb; k += P3_NG4) {
-      const T *R = sR[k] + (size_t)w0 * BW;
+      const T *R = sR[k];
       const ZT *Yb = Ym + sY[k] + cl + (long)p0 * P3_CHUNK;
       for (int d1 = 0; d1 < BW; d1++) {
         const int j1 = i1 + d1 - S;
         if (j1 < 0 || j1 >= N) continue;
+        const T *Rd = R + (size_t)d1 * P3_NW * BW;
+        for (int e = lane; e < P3_NW * BW; e += 64) rw[e] = Rd[e];
         const ZT *Yr = Yb + (long)j1 * N * P3_CHUNK;
         ZT y[WIN];
 #pragma unroll
         for (int p = 0; p < WIN; p++) y[p] = pv[p] ? Yr[(long)p * P3_CHUNK] : p3_czero<ZT>();
-        const T *Rr = R + (size_t)d1 * P3_NW * BW;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const T *Rr = rw + (size_t)w0 * BW;
 #pragma unroll
         for (int t = 0; t < 4; t++) {
           ZT a = p3_czero<ZT>();
@@ -575,6 +586,7 @@ __global__ void __launch_bounds__(256) p3_near(const T *const *Rn, const int *bl
           for (int d2 = 0; d2 < BW; d2++) p3_cfmar(a, Rr[t * BW + d2], y[t + d2]);
           acc[t] = p3_cadd(acc[t], a);
         }
+        __builtin_amdgcn_wave_barrier();
       }
     }
   }
@@ -605,8 +617,9 @@ __global__ void __launch_bounds__(256) p3_near(const T *const *Rn, const int *bl
 // 'U','C'), OP 2: x := U x (ztbmv).  U(i, j) at Ub[j*(KD+1) + KD + i - j]
 // (overlapmatrix.cpp:22-32).
 #define P3_FT 32
+#define P3_SB 8            // rows per group of the substitution's interior
 template <int KD, int OP>
-__global__ void __launch_bounds__(256) p3_ftile(const double *Ub, zd *x, int C, int N, int rank)
+__global__ void __launch_bounds__(256) p3_ftile(const double *Ub, const double *rdg, zd *x, int C, int N, int rank)
 {
   extern __shared__ double p3_lds[];
   double *uL = p3_lds;                                   // N*(KD+1) band
@@ -616,7 +629,7 @@ __global__ void __launch_bounds__(256) p3_ftile(const double *Ub, zd *x, int C, 
   const int ld = N + 1;
   const long NN = (long)N * N;
   for (int e = threadIdx.x; e < N * (KD + 1); e += 256) uL[e] = Ub[e];
-  for (int i = threadIdx.x; i < N; i += 256) rd[i] = 1.0 / Ub[(size_t)i * (KD + 1) + KD];
+  for (int i = threadIdx.x; i < N; i += 256) rd[i] = rdg[i];
   int nf;
   zd *g;
   long gstr, fstr;   // global element i of fiber f at g[f*fstr + i*gstr]
@@ -658,6 +671,25 @@ __global__ void __launch_bounds__(256) p3_ftile(const double *Ub, zd *x, int C, 
         for (int d = KD - 1; d >= 1; d--) win[d] = win[d - 1];
         win[0] = xi;
       }
+      // interior rows in groups of P3_SB: the group's right-hand sides are
+      // loaded together, so the recurrence waits on LDS once per group
+      for (; i >= P3_SB - 1; i -= P3_SB) {
+        zd bv[P3_SB];
+#pragma unroll
+        for (int r = 0; r < P3_SB; r++) bv[r] = b[i - r];
+#pragma unroll
+        for (int r = 0; r < P3_SB; r++) {
+          const int ii = i - r;
+          zd s = bv[r];
+#pragma unroll
+          for (int d = KD; d >= 1; d--) { const double u = U(ii, ii + d); s.x -= u * win[d - 1].x; s.y -= u * win[d - 1].y; }
+          const zd xi = zscale(s, rd[ii]);
+          b[ii] = xi;
+#pragma unroll
+          for (int d = KD - 1; d >= 1; d--) win[d] = win[d - 1];
+          win[0] = xi;
+        }
+      }
       for (; i >= 0; i--) {
         zd s = b[i];
 #pragma unroll
@@ -671,11 +703,39 @@ __global__ void __launch_bounds__(256) p3_ftile(const double *Ub, zd *x, int C, 
     } else if (OP == 1) {
 #pragma unroll
       for (int d = 0; d < KD; d++) win[d] = zmk(0.0, 0.0);
-      for (int i = 0; i < N; i++) {
+      int i = 0;
+      for (; i < N && i < KD; i++) {   // the first KD rows see the start of the band
         zd s = b[i];
 #pragma unroll
         for (int d = KD; d >= 1; d--)
           if (i - d >= 0) { const double u = U(i - d, i); s.x -= u * win[d - 1].x; s.y -= u * win[d - 1].y; }
+        const zd xi = zscale(s, rd[i]);
+        b[i] = xi;
+#pragma unroll
+        for (int d = KD - 1; d >= 1; d--) win[d] = win[d - 1];
+        win[0] = xi;
+      }
+      for (; i + P3_SB <= N; i += P3_SB) {   // interior rows in groups, as above
+        zd bv[P3_SB];
+#pragma unroll
+        for (int r = 0; r < P3_SB; r++) bv[r] = b[i + r];
+#pragma unroll
+        for (int r = 0; r < P3_SB; r++) {
+          const int ii = i + r;
+          zd s = bv[r];
+#pragma unroll
+          for (int d = KD; d >= 1; d--) { const double u = U(ii - d, ii); s.x -= u * win[d - 1].x; s.y -= u * win[d - 1].y; }
+          const zd xi = zscale(s, rd[ii]);
+          b[ii] = xi;
+#pragma unroll
+          for (int d = KD - 1; d >= 1; d--) win[d] = win[d - 1];
+          win[0] = xi;
+        }
+      }
+      for (; i < N; i++) {
+        zd s = b[i];
+#pragma unroll
+        for (int d = KD; d >= 1; d--) { const double u = U(i - d, i); s.x -= u * win[d - 1].x; s.y -= u * win[d - 1].y; }
         const zd xi = zscale(s, rd[i]);
         b[i] = xi;
 #pragma unroll
@@ -874,6 +934,15 @@ static void p3_free_plan(pp_pamp3 *p)
 
 static void p3_collect(pp_pamp3 *p);
 
+// rd[i] = 1 / U(i, i) of the overlap factor (the fiber solves' diagonal)
+__global__ void p3_recip_diag(const double *Ub, double *rd, int N, int kd)
+{
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+    rd[i] = 1.0 / Ub[(size_t)i * (kd + 1) + kd];
+}
+
+extern "C" int pp_pamp3_destroy(pp_pamp3 *p);
+
 extern "C" int pp_pamp3_create(int C, int N, int m, int reorth, const pp_overlap *ov, pp_pamp3 **out)
 {
   if (!out || !ov || C <= 0 || N <= 0 || m < 1 || m > 48) return PP_EARG;
@@ -896,8 +965,11 @@ extern "C" int pp_pamp3_create(int C, int N, int m, int reorth, const pp_overlap
             hipMalloc(&p->part, sizeof(double) * P3_RED_BLOCKS * 128) == hipSuccess &&
             hipMalloc(&p->Hm, sizeof(zd) * m * m) == hipSuccess && hipMalloc(&p->cvec, sizeof(zd) * m) == hipSuccess &&
             hipMalloc(&p->hcoef, sizeof(zd) * 64) == hipSuccess && hipMalloc(&p->beta, sizeof(double) * 64) == hipSuccess &&
-            hipMalloc(&p->state, sizeof(int) * 4) == hipSuccess;
-  if (!ok) { delete p; return PP_EHIP; }
+            hipMalloc(&p->state, sizeof(int) * 4) == hipSuccess &&
+            hipMalloc(&p->urd, sizeof(double) * (size_t)N) == hipSuccess;
+  if (!ok) { pp_pamp3_destroy(p); return PP_EHIP; }
+  hipLaunchKernelGGL(p3_recip_diag, dim3(1), dim3(256), 0, 0, ov->Uband, p->urd, N, ov->kd);
+  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) { pp_pamp3_destroy(p); return PP_EHIP; }
   *out = p;
   return PP_OK;
 }
@@ -907,7 +979,8 @@ extern "C" int pp_pamp3_destroy(pp_pamp3 *p)
   if (!p) return PP_OK;
   p3_collect(p);
   p3_free_plan(p);
-  void *bufs[] = {p->Y, p->Z, p->P, p->V, p->part, p->Hm, p->cvec, p->hcoef, p->beta, p->state, p->scal, p->diag};
+  void *bufs[] = {p->Y, p->Z, p->P, p->V, p->part, p->Hm, p->cvec, p->hcoef, p->beta, p->state, p->scal, p->diag,
+                  p->urd};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &t : p->mterms)
@@ -1246,9 +1319,9 @@ static int p3_solve2(pp_pamp3 *p, int op, zd *x, hipStream_t st)
   if (hipFuncSetAttribute(fn[op], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return PP_EHIP;
   for (int rank = 2; rank >= 1; rank--) {
     const dim3 grid = rank == 2 ? g2 : g1;
-    if (op == 0) hipLaunchKernelGGL((p3_ftile<6, 0>), grid, dim3(256), lds, st, p->ov->Uband, x, C, N, rank);
-    else if (op == 1) hipLaunchKernelGGL((p3_ftile<6, 1>), grid, dim3(256), lds, st, p->ov->Uband, x, C, N, rank);
-    else hipLaunchKernelGGL((p3_ftile<6, 2>), grid, dim3(256), lds, st, p->ov->Uband, x, C, N, rank);
+    if (op == 0) hipLaunchKernelGGL((p3_ftile<6, 0>), grid, dim3(256), lds, st, p->ov->Uband, p->urd, x, C, N, rank);
+    else if (op == 1) hipLaunchKernelGGL((p3_ftile<6, 1>), grid, dim3(256), lds, st, p->ov->Uband, p->urd, x, C, N, rank);
+    else hipLaunchKernelGGL((p3_ftile<6, 2>), grid, dim3(256), lds, st, p->ov->Uband, p->urd, x, C, N, rank);
   }
   return hipGetLastError() == hipSuccess ? PP_OK : PP_EHIP;
 }
