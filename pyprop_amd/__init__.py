@@ -17,5 +17,7 @@ from .propagator import PropagatorBase, PamPropagator, ExpokitPropagator, Cayley
 from .tensorpotential import BasisPropagator, TensorPotential  # noqa: F401
 from .coupled import CoupledIndex, DefaultCoupledIndexIterator  # noqa: F401
 from .solver import PiramSolver, GMRESShiftInvertSolver, BSplinePreconditioner, GmresSolver  # noqa: F401
+from .serialization import (SaveWavefunctionHDF, LoadWavefunctionHDF, LoadConfigFromFile,  # noqa: F401
+                            CreateWavefunctionFromFile)
 
 del abs, max, min, round, sum  # keep the builtins for code that star-imports this package

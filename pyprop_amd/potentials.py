@@ -14,11 +14,13 @@ import numpy as np
 
 
 class PotentialType:
-    Static = 0
-    Dynamic = 1
-    RankOne = 2
-    FiniteDifference = 3
-    Matrix = 4
+    """pyprop/Enum.py:2-9 (the values config files may give as numbers)."""
+    Static = 1
+    Dynamic = 3
+    FiniteDifference = 4
+    CrankNicholson = 5
+    Matrix = 6
+    RankOne = 7
 
 
 class PotentialBase:

@@ -13,7 +13,10 @@ class InitialConditionType:
 
 
 class WavefunctionFileFormat:
-    HDF = 1
+    """pyprop/Enum.py:21-24 (the values config files may give as numbers)."""
+    Ascii = 1
+    Binary = 2
+    HDF = 3
 
 
 def CreateRepresentation(config):
@@ -180,6 +183,8 @@ class Problem:
         type_ = ic.type
         if type_ == InitialConditionType.Function:
             self.SetupWavefunctionFunction(self.Config)
+        elif type_ == InitialConditionType.File:
+            self.SetupWavefunctionFile(self.Config)
         elif type_ == InitialConditionType.Class:
             ic.classname(self.Config, self.psi)
         elif type_ == InitialConditionType.Custom:
@@ -201,3 +206,32 @@ class Problem:
 
     def SetWavefunction(self, data):
         self.psi.GetData().copy_(dev.to_device(np.ascontiguousarray(data, dtype=np.complex128)))
+
+    def SetupWavefunctionFile(self, config):
+        """InitialCondition.type = InitialConditionType.File: load the
+        wavefunction from InitialCondition.filename (format HDF: the dataset
+        InitialCondition.dataset of a wavefunction archive; the grid must be
+        the one it was saved on) (Problem.py:393-421)."""
+        ic = config.InitialCondition
+        fmt = getattr(ic, "format", WavefunctionFileFormat.HDF)
+        if fmt != WavefunctionFileFormat.HDF:
+            raise NotImplementedError("wavefunction file format %r (only HDF archives are read here)" % (fmt,))
+        self.LoadWavefunctionHDF(ic.filename, str(getattr(ic, "dataset", "/wavefunction")))
+
+    # ---------------------------------------------------------- (de)serialisation
+    def LoadWavefunctionData(self, newdata):
+        """Problem.py:425-429: copy an array of the wavefunction's shape in."""
+        newdata = np.asarray(newdata)
+        if tuple(newdata.shape) != tuple(self.psi.GetData().shape):
+            raise Exception("Invalid shape on loaded wavefunction, got %s expected %s"
+                            % (newdata.shape, tuple(self.psi.GetData().shape)))
+        self.SetWavefunction(newdata)
+
+    def LoadWavefunctionHDF(self, filename, datasetPath):
+        from . import serialization
+        serialization.LoadWavefunctionHDF(filename, datasetPath, self.psi)
+
+    def SaveWavefunctionHDF(self, filename, datasetPath):
+        """Store psi with this problem's config (Problem.py:441-442)."""
+        from . import serialization
+        serialization.SaveWavefunctionHDF(filename, datasetPath, self.psi, conf=self.Config)

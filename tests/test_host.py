@@ -212,3 +212,61 @@ def test_exponential_linear_grid_of_the_sae_paper_config():
     basis = B.InitBSpline(conf)
     assert len(basis.Breakpoints) == 310
     assert basis.Breakpoints[-1] > 159.7
+
+
+def test_wavefunction_archive_round_trip(tmp_path):
+    """SaveWavefunctionHDF / LoadWavefunctionHDF / LoadConfigFromFile on
+    host tensors: bitwise round trip, several datasets per file addressed
+    by path, replacing a dataset, the stored config object evaluating to
+    the same values, shape checks, and a refusal of real HDF5 files."""
+    import torch
+    from pyprop_amd import serialization as SZ
+
+    class _Repr:
+        def __init__(self, shape):
+            self.shape = shape
+
+        def GetFullShape(self):
+            return self.shape
+
+    class _Psi:
+        def __init__(self, data):
+            self.data = data
+
+        def GetData(self):
+            return self.data
+
+        def GetRepresentation(self):
+            return _Repr(tuple(self.data.shape))
+
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal((5, 7)) + 1j * rng.standard_normal((5, 7))
+    b = rng.standard_normal((3, 4, 4)) + 1j * rng.standard_normal((3, 4, 4))
+    conf = P.LoadString("[Propagation]\ntimestep = 0.01\nduration = 2 * pi / 4\nname2 = 'x#y'\n")
+    f = str(tmp_path / "wf.h5")
+    SZ.SaveWavefunctionHDF(f, "/wavefunction", _Psi(torch.from_numpy(a.copy())), conf=conf)
+    SZ.SaveWavefunctionHDF(f, "/run1/psi", _Psi(torch.from_numpy(b.copy())))
+    assert SZ.ListDatasets(f) == ["/run1/psi", "/wavefunction"]
+    out = _Psi(torch.zeros((5, 7), dtype=torch.complex128))
+    SZ.LoadWavefunctionHDF(f, "/wavefunction", out)
+    assert np.array_equal(out.data.numpy(), a)
+    out3 = _Psi(torch.zeros((3, 4, 4), dtype=torch.complex128))
+    SZ.LoadWavefunctionHDF(f, "run1/psi", out3)
+    assert np.array_equal(out3.data.numpy(), b)
+    # replace a dataset; the other one stays
+    SZ.SaveWavefunctionHDF(f, "/wavefunction", _Psi(torch.from_numpy(2 * a)), conf=conf)
+    SZ.LoadWavefunctionHDF(f, "/wavefunction", out)
+    assert np.array_equal(out.data.numpy(), 2 * a)
+    SZ.LoadWavefunctionHDF(f, "/run1/psi", out3)
+    assert np.array_equal(out3.data.numpy(), b)
+    c = SZ.LoadConfigFromFile(f, "/wavefunction")
+    assert c.Propagation.timestep == 0.01 and c.Propagation.duration == 2 * np.pi / 4
+    assert c.Propagation.name2 == "x#y"
+    with pytest.raises(ValueError, match="Invalid shape"):
+        SZ.LoadWavefunctionHDF(f, "/wavefunction", out3)
+    with pytest.raises(KeyError):
+        SZ.LoadWavefunctionHDF(f, "/missing", out)
+    h5 = tmp_path / "ref.h5"
+    h5.write_bytes(b"\x89HDF\r\n\x1a\n" + b"\0" * 64)
+    with pytest.raises(IOError, match="HDF5"):
+        SZ.LoadWavefunctionHDF(str(h5), "/wavefunction", out)
