@@ -280,6 +280,25 @@ int pp_gram_schmidt_noalloc(const void *V, long n, int k, void *w, int passes, d
  * x N x (2kl+ku+1) complex128 column bands, piv: nfib x N 0-based). */
 int pp_band_lu_solve(int nfib, int N, int kl, int ku, const void *lub, const int *piv, void *x, pp_stream stream);
 
+/* Two-electron radial preconditioner (the device side of pyprop's
+ * RadialTwoElectronPreconditioner Setup / Solve,
+ * examples/tensor/helium_stabilization/preconditioner.py:77-244): on a
+ * (C, N, N) state, x_c := M_c^-1 x_c with M_c = b S (x) S + a (H1_c (x) S +
+ * S (x) H2_c), by fast diagonalisation.  create copies, for nmat distinct
+ * real symmetric radial matrices H, the S-orthonormal generalised
+ * eigenvectors V (nmat x N x N row-major, column n the n-th eigenvector,
+ * V^T S V = I) and eigenvalues lam (nmat x N); idx1[c], idx2[c] select
+ * channel c's matrix on radial rank 1 and 2 (host arrays).  solve runs
+ * V1 [(V1^T X V2) ./ (b + a (lam1_i + lam2_k))] V2^T per channel,
+ * asynchronously on `stream`; solves that overlap on one handle race (one
+ * intermediate buffer per handle). */
+typedef struct pp_radial_prec pp_radial_prec;
+int pp_radial_prec_create(int C, int N, int nmat, const int *idx1, const int *idx2, const double *V,
+                          const double *lam, pp_radial_prec **out);
+int pp_radial_prec_solve(pp_radial_prec *p, void *psi, double a_re, double a_im, double b_re, double b_im,
+                         pp_stream stream);
+int pp_radial_prec_destroy(pp_radial_prec *p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,8 @@ from .propagator import PropagatorBase, PamPropagator, ExpokitPropagator, Cayley
 from .tensorpotential import BasisPropagator, TensorPotential  # noqa: F401
 from .coupled import CoupledIndex, DefaultCoupledIndexIterator  # noqa: F401
 from .solver import PiramSolver, GMRESShiftInvertSolver, BSplinePreconditioner, GmresSolver  # noqa: F401
+from .helium import (RadialTwoElectronPreconditioner, RadialTwoElectronPreconditionerSuperLU,  # noqa: F401
+                     RadialTwoElectronPreconditionerIfpack)
 from .serialization import (SaveWavefunctionHDF, LoadWavefunctionHDF, LoadConfigFromFile,  # noqa: F401
                             CreateWavefunctionFromFile)
 

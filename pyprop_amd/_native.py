@@ -34,6 +34,7 @@ EXPORTS = [
     "pp_pamp3_propagate", "pp_pamp3_diagnostics", "pp_pamp3_set_timing", "pp_pamp3_kernel_time",
     "pp_pamp3_set_radial_supports", "pp_pamp3_set_mixed",
     "pp_pamp3_multipole_flops", "pp_pamp3_destroy",
+    "pp_radial_prec_create", "pp_radial_prec_solve", "pp_radial_prec_destroy",
 ]
 
 _lib = None
@@ -112,6 +113,9 @@ def lib():
     L.pp_pamp3_set_mixed.argtypes = [P, I]
     L.pp_pamp3_kernel_time.argtypes = [P, dp, ctypes.POINTER(ctypes.c_long)]
     L.pp_pamp3_multipole_flops.argtypes = [P, dp]
+    L.pp_radial_prec_create.argtypes = [I, I, I, ip, ip, dp, dp, ctypes.POINTER(P)]
+    L.pp_radial_prec_solve.argtypes = [P, P, D, D, D, D, P]
+    L.pp_radial_prec_destroy.argtypes = [P]
     for name in EXPORTS:
         getattr(L, name).restype = ctypes.c_int
     L.pp_pamp_lds_estimate.restype = ctypes.c_long

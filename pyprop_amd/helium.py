@@ -426,3 +426,187 @@ class Pamp3Engine:
         out = np.zeros(4 * max(n, 1))
         nat.check(nat.lib().pp_pamp3_diagnostics(self.h, n, nat.dptr(out), dev.stream_ptr()), "pp_pamp3_diagnostics")
         return out[:4 * n].reshape(n, 4)
+
+
+# ---------------------------------------------------------------- preconditioner
+def band_dense(band):
+    """Dense N x N matrix of a row-major band (N, 2k-1), j = i + d - (k-1)."""
+    N, bw = band.shape
+    s = (bw - 1) // 2
+    out = np.zeros((N, N))
+    for d in range(bw):
+        i = np.arange(max(0, s - d), min(N, N + s - d))
+        out[i, i + d - s] = band[i, d]
+    return out
+
+
+class RadialTwoElectronPreconditioner:
+    """Left preconditioner of the two-electron Cayley and shift-invert
+    solves: x = M^-1 y with M = scalingS S + scalingH (P1 + P2 + ...) per
+    coupled channel (pyprop's interface,
+    examples/tensor/helium_stabilization/preconditioner.py:77-140: the
+    section's potential_evaluation list and the config's OverlapPotential
+    section, SetHamiltonianScaling / SetOverlapScaling, Setup(prop),
+    Solve(psi); the potentials must be diagonal in the angular rank).
+
+    The reference factorises each channel's N^2 x N^2 radial block (SuperLU,
+    or an incomplete LU with Ifpack).  Here the block's separable part, b S
+    (x) S + a (H1_c (x) S + S (x) H2_c) with H1_c, H2_c the listed terms that
+    carry S on the other radial rank (radial and angular kinetic energy,
+    nuclear attraction), is inverted exactly by fast diagonalisation on the
+    device (pp_radial_prec_*): the S-orthonormal eigenvectors of each
+    distinct H_c come from a host eigensolver once, and a solve is four dense
+    N x N products per channel.  Listed terms that are not separable (the
+    channel-diagonal electron-electron multipoles) are left out of M and
+    named in OmittedTerms; M stays a preconditioner, GMRES solves the full
+    system."""
+
+    def __init__(self, psi):
+        self.Rank = psi.GetRank()
+        self.psi = psi
+        self.HamiltonianScaling = 1.0
+        self.OverlapScaling = 1.0
+        self.OverlapSection = None
+        self.PotentialSections = []
+        self.OmittedTerms = []
+        self._h = None
+
+    def __del__(self, _release=dev._release):
+        _release(self)
+
+    def ApplyConfigSection(self, conf):
+        cfg = conf.Config
+        self.OverlapSection = getattr(cfg, "OverlapPotential", None)
+        self.PotentialSections = [cfg.GetSection(s) for s in conf.potential_evaluation]
+
+    def SetHamiltonianScaling(self, scalingH):
+        self.HamiltonianScaling = scalingH
+
+    def SetOverlapScaling(self, scalingS):
+        self.OverlapScaling = scalingS
+
+    def GetHamiltonianScaling(self):
+        return self.HamiltonianScaling
+
+    def GetOverlapScaling(self):
+        return self.OverlapScaling
+
+    def _radial_terms(self, prop, S):
+        """(H1, H2): (C, N, N) real matrices of the listed potentials' terms
+        A (x) S and S (x) B, summed per channel."""
+        repr = self.psi.GetRepresentation()
+        C, N, _ = repr.GetFullShape()
+        H1 = np.zeros((C, N, N))
+        H2 = np.zeros((C, N, N))
+        self.OmittedTerms = []
+        for conf in self.PotentialSections:
+            if conf.classname == "CustomPotentialEvaluationR12" and str(conf.geometry0).strip().lower() == "diagonal":
+                # only multipole terms, none separable: not generated at all
+                self.OmittedTerms.append("%s (electron-electron multipoles)" % conf.name)
+                continue
+            pot = prop.BasePropagator.GeneratePotential(conf)
+            if not isinstance(pot, CoupledTensorPotential):
+                raise NotImplementedError("RadialTwoElectronPreconditioner needs two-electron potentials")
+            for row, col in pot.Pairs:
+                if row != col:
+                    raise ValueError("%i != %i, angular rank must be diagonal for potential %s" % (row, col, pot.Name))
+            for t in pot.Terms:
+                if not isinstance(t, KronTerm):
+                    self.OmittedTerms.append("%s (multipole l = %d)" % (pot.Name, t.l))
+                    continue
+                if np.any(t.coef.imag != 0.0):
+                    raise NotImplementedError("potential %s has complex coefficients; the preconditioner's "
+                                              "eigenvectors need real symmetric radial matrices" % pot.Name)
+                ch = t.pairs[:, 0]
+                if np.array_equal(t.B, S):
+                    np.add.at(H1, ch, t.coef.real[:, None, None] * band_dense(t.A)[None])
+                elif np.array_equal(t.A, S):
+                    np.add.at(H2, ch, t.coef.real[:, None, None] * band_dense(t.B)[None])
+                else:
+                    self.OmittedTerms.append("%s (A (x) B term)" % pot.Name)
+        return H1, H2
+
+    def Decompose(self, prop):
+        """(idx1, idx2, V, lam): the S-orthonormal generalised eigenvectors
+        H V = S V diag(lam), V^T S V = I, of each distinct radial matrix of
+        the listed potentials (host), and the matrix index of each channel
+        on radial rank 1 and 2."""
+        import scipy.linalg
+        repr = self.psi.GetRepresentation()
+        C, N, N2 = repr.GetFullShape()
+        if self.Rank != 3 or N != N2:
+            raise NotImplementedError("RadialTwoElectronPreconditioner needs (channels, N, N) two-electron states")
+        S = repr.RadialMatrices()["S"]
+        if self.OverlapSection is not None:
+            ov = prop.BasePropagator.GeneratePotential(self.OverlapSection)
+            ok = (isinstance(ov, CoupledTensorPotential) and len(ov.Terms) == 1 and isinstance(ov.Terms[0], KronTerm)
+                  and np.array_equal(ov.Terms[0].A, S) and np.array_equal(ov.Terms[0].B, S)
+                  and np.array_equal(ov.Terms[0].pairs[:, 0], np.arange(C))
+                  and np.array_equal(ov.Terms[0].pairs[:, 1], np.arange(C)) and np.all(ov.Terms[0].coef == 1.0))
+            if not ok:
+                raise NotImplementedError("the OverlapPotential section must be the overlap S (x) S on every channel")
+        H1, H2 = self._radial_terms(prop, S)
+        Sd = band_dense(S)
+        mats, idx, uniq = {}, [], []
+        for H in list(H1) + list(H2):
+            key = H.tobytes()
+            if key not in mats:
+                mats[key] = len(uniq)
+                uniq.append(H)
+            idx.append(mats[key])
+        lam = np.zeros((len(uniq), N))
+        V = np.zeros((len(uniq), N, N))
+        for m, H in enumerate(uniq):
+            if np.abs(H - H.T).max() > 1e-12 * max(np.abs(H).max(), 1.0):
+                raise NotImplementedError("radial matrix %d of the preconditioner is not symmetric" % m)
+            lam[m], V[m] = scipy.linalg.eigh(0.5 * (H + H.T), Sd)
+        return np.array(idx[:C], dtype=np.int32), np.array(idx[C:], dtype=np.int32), V, lam
+
+    def Setup(self, prop):
+        """Decompose the listed potentials and set up the device solver for
+        the scalings set now (the reference factorises here too,
+        preconditioner.py:111-140)."""
+        idx1, idx2, V, lam = self.Decompose(prop)
+        C, N = len(idx1), V.shape[1]
+        a = complex(self.HamiltonianScaling)
+        b = complex(self.OverlapScaling)
+        absD = np.abs(b + a * (lam[idx1][:, :, None] + lam[idx2][:, None, :]))
+        if not np.all(np.isfinite(absD)) or absD.min() <= 1e-14 * absD.max():
+            c = int(np.argmin(absD.reshape(C, -1).min(axis=1)))
+            raise RuntimeError("preconditioner matrix of channel %d is singular" % c)
+        dev._release(self)
+        h = ctypes.c_void_p()
+        V = np.ascontiguousarray(V)
+        lam = np.ascontiguousarray(lam)
+        nat.check(nat.lib().pp_radial_prec_create(C, N, len(V), nat.iptr(idx1), nat.iptr(idx2), nat.dptr(V),
+                                                  nat.dptr(lam), ctypes.byref(h)), "pp_radial_prec_create")
+        self.h = h
+        self._token = nat.track("pp_radial_prec_destroy", h)
+        self._a, self._b = a, b
+        self._shape = (C, N, N)
+
+    def Solve(self, psi):
+        data = psi.GetData()
+        if getattr(self, "_token", None) is None:
+            raise RuntimeError("call Setup(prop) first")
+        t = dev.torch()
+        if tuple(data.shape) != self._shape or data.dtype != t.complex128 or not data.is_contiguous():
+            raise ValueError("expected a contiguous complex128 %s device tensor" % (self._shape,))
+        nat.check(nat.lib().pp_radial_prec_solve(self.h, dev.ptr(data), self._a.real, self._a.imag,
+                                                 self._b.real, self._b.imag, dev.stream_ptr()),
+                  "pp_radial_prec_solve")
+
+
+class RadialTwoElectronPreconditionerSuperLU(RadialTwoElectronPreconditioner):
+    """Config name of the exact variant (preconditioner.py:154-198); the
+    device solve is the same."""
+
+
+class RadialTwoElectronPreconditionerIfpack(RadialTwoElectronPreconditioner):
+    """Config name of the incomplete-LU variant (preconditioner.py:200-244);
+    its drop `cutoff` is read and kept, the device solve is exact for the
+    separable part, so there is nothing to drop."""
+
+    def ApplyConfigSection(self, conf):
+        RadialTwoElectronPreconditioner.ApplyConfigSection(self, conf)
+        self.Cutoff = conf.cutoff

@@ -8,6 +8,8 @@ the matvec callback and the Pade exponential all run inside one persistent HIP
 launch; the time functions are sampled on the host at t_n for each step
 (H held at the step start, Problem.py:166, TensorPotential.py:98-100).
 """
+import warnings
+
 import numpy as np
 
 from . import _native as nat
@@ -266,8 +268,9 @@ class CayleyPropagator(PropagatorBase):
     and krylov_double_orthogonalization keep their meaning.  psi(t) is the
     GMRES right-hand side's source and psi(t+dt) its solution (zero start).
     Rank-2 states apply H through the base propagator's tensor-potential
-    kernels; rank-3 (two-electron) states through the rank-3 engine's raw
-    apply (no preconditioner: BSplinePreconditioner covers rank-2 states)."""
+    kernels (preconditioner: BSplinePreconditioner); rank-3 (two-electron)
+    states through the rank-3 engine's raw apply (preconditioner:
+    RadialTwoElectronPreconditioner and its SuperLU / Ifpack names)."""
 
     def __init__(self, psi):
         PropagatorBase.__init__(self, psi)
@@ -301,8 +304,6 @@ class CayleyPropagator(PropagatorBase):
         self.IsOrthogonalBasis = all(repr.IsOrthogonalBasis(i) for i in range(self.Rank))
         self._engine3 = None
         if self.Rank == 3:
-            if self.Preconditioner is not None:
-                raise NotImplementedError("the Cayley preconditioner on the HIP path covers rank-2 states")
             # raw H of the two-electron engine (its Krylov buffers are sized
             # by the GMRES basis but unused here)
             helper = PamPropagator(self.psi)
@@ -352,6 +353,11 @@ class CayleyPropagator(PropagatorBase):
         self.LastErrorEstimate = self.Solver.GetErrorEstimate()
         if self.LastErrorEstimate > 1e-4:
             print("Error = %s" % self.LastErrorEstimate)
+        if self.LastErrorEstimate > self.Solver.Tolerance:
+            warnings.warn("CayleyPropagator: GMRES did not converge in %d vectors (residual estimate %.3e > "
+                          "tolerance %.3e); the step is inaccurate" % (self.Solver.BasisSize, self.LastErrorEstimate,
+                                                                       self.Solver.Tolerance),
+                          RuntimeWarning, stacklevel=2)
 
     def AdvanceSteps(self, times, dt):
         for t in times:

@@ -270,3 +270,50 @@ def test_wavefunction_archive_round_trip(tmp_path):
     h5.write_bytes(b"\x89HDF\r\n\x1a\n" + b"\0" * 64)
     with pytest.raises(IOError, match="HDF5"):
         SZ.LoadWavefunctionHDF(str(h5), "/wavefunction", out)
+
+
+def test_radial_preconditioner_decomposition_matches_oracle():
+    """RadialTwoElectronPreconditioner.Decompose on the small helium config:
+    the eigenvectors are S-orthonormal, one decomposition per distinct l, and
+    S V diag(lam) V^T S rebuilds each channel's H1 (x) S + S (x) H2, whose
+    action equals the oracle's four channel-diagonal one-electron potentials
+    (radial kinetic on both ranks, angular kinetic, nuclear attraction,
+    projected on the product grid) to rounding; the channel-diagonal e-e
+    multipoles are the omitted, non-separable terms."""
+    from pyprop_amd.examples import helium as HE
+    conf = HE.load("helium_small", field_free=True)
+    repr = CreateRepresentation(conf)
+    psi = Wavefunction(repr)
+    bp = P.BasisPropagator(psi)
+    conf.Apply(bp)
+    prec = conf.RadialPreconditioner.type(psi)
+    conf.RadialPreconditioner.Apply(prec)
+    assert isinstance(prec, P.RadialTwoElectronPreconditionerIfpack) and prec.Cutoff == 0
+
+    class _Prop:
+        BasePropagator = bp
+    idx1, idx2, V, lam = prec.Decompose(_Prop())
+    h = O.Helium(lmax=1, Ls=(0, 1), nbp=6, rmax=8.0, laser=False)
+    C, N = h.C, h.N
+    assert (len(idx1), V.shape[1]) == (C, N)
+    assert V.shape[0] == 2                                   # l = 0, 1
+    ch = repr.GetRepresentation(0).Channels
+    assert all(idx1[c] == idx1[d] for c in range(C) for d in range(C) if ch[c].l1 == ch[d].l1)
+    assert all(idx1[c] == idx2[d] for c in range(C) for d in range(C) if ch[c].l1 == ch[d].l2)
+    from pyprop_amd.helium import band_dense
+    S = band_dense(repr.RadialMatrices()["S"])
+    for m in range(V.shape[0]):
+        assert np.abs(V[m].T @ S @ V[m] - np.eye(N)).max() < 1e-11
+    Hm = [S @ V[m] @ np.diag(lam[m]) @ V[m].T @ S for m in range(V.shape[0])]
+    statics = []
+    for kind, pk, pot in h.terms[:4]:
+        assert pk == "diag"
+        statics.append((O.ST_SIMP, O.ST_SIMP, O.ST_SIMP, pot,
+                        [h.pairs[pk].astype(np.int32), h.packed.astype(np.int32), h.packed.astype(np.int32)]))
+    ham = O.Ham(h.shape, statics, {1: (h.U, h.kd), 2: (h.U, h.kd)})
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(h.shape) + 1j * rng.standard_normal(h.shape)
+    ref = ham.potentials(x)
+    got = np.stack([Hm[idx1[c]] @ x[c] @ S + S @ x[c] @ Hm[idx2[c]] for c in range(C)])
+    assert np.abs(got - ref).max() < 1e-11 * np.abs(ref).max()
+    assert prec.OmittedTerms and all("ElectronicCouplingPotentialMonopoleTerm" in t for t in prec.OmittedTerms)
