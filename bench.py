@@ -16,10 +16,11 @@ The JSON line carries:
   from the same start (S-norm of the difference, bound populations, norm),
   next to the oracle's own change under a 1e-16 perturbation of psi0;
 * mixed: the matvec_precision = "mixed" form (every term's radial bands in
-  FP32 with FP32 band sums; Gram-Schmidt, overlap solves and Pade FP64), its
-  speed and its error against the FP64 run, next to the FP64 run's own change
-  under a 1e-16 perturbation of psi0 (the floor no precision change can go
-  below);
+  FP32 with FP32 band sums; Gram-Schmidt, overlap solves and Pade FP64): its
+  speed over the timed steps, and its error against FP64 over the workload's
+  whole window (20000 steps from t = 0, checkpoints every 2000) next to the
+  FP64 run's own change under a 1e-16 perturbation of psi0 (the floor no
+  precision change can go below);
 * cpu_baseline: the C oracle (-O3 build for the host's instruction set) on
   the GPU box's host, 10 warm-up steps, median of 3 timed samples, on one
   thread and on all;
@@ -111,6 +112,44 @@ def timed_advance(prob, nsteps):
     return time.perf_counter() - t0, ev0.elapsed_time(ev1) / 1e3
 
 
+def window_study(set_form, start, t0, total, chunk, measure):
+    """The mixed form's error over a whole window: FP64, FP64 from a start
+    perturbed by 1e-16 relative noise (the FP64 run's own floor) and mixed,
+    each `total` steps from t0 in launches of `chunk` steps;
+    measure(a, b) -> {name: value} compares two states at each checkpoint.
+    set_form(mixed) switches the engine and returns the problem."""
+    import numpy as np
+    rng = np.random.default_rng(20261204)
+    forms = [("fp64", False, start), ("fp64_perturbed", False, start * (1 + 1e-16 * rng.standard_normal(start.shape))),
+             ("mixed", True, start)]
+    snaps = {}
+    for name, mixed, psi0 in forms:
+        prob = set_form(mixed)
+        prob.SetWavefunction(psi0)
+        prob.PropagatedTime = t0
+        out = []
+        done = 0
+        while done < total:
+            n = min(chunk, total - done)
+            prob.AdvanceSteps(n)
+            done += n
+            out.append(prob.psi.GetData().cpu().numpy())
+        snaps[name] = out
+        log("  %s: %d steps from t = %g" % (name, total, t0))
+    set_form(False)
+    rows = []
+    for k in range(len(snaps["fp64"])):
+        mx = measure(snaps["mixed"][k], snaps["fp64"][k])
+        fl = measure(snaps["fp64_perturbed"][k], snaps["fp64"][k])
+        rows.append({"step": min((k + 1) * chunk, total), "mixed": mx, "fp64_floor": fl})
+    keys = list(rows[-1]["mixed"].keys())
+    return {"steps": total, "start_time": t0, "checkpoint_every": chunk,
+            "final": rows[-1], "max_over_checkpoints": {
+                "mixed": {k: max(r["mixed"][k] for r in rows) for k in keys},
+                "fp64_floor": {k: max(r["fp64_floor"][k] for r in rows) for k in keys}},
+            "checkpoints": rows}
+
+
 def run_hydrogen(args):
     """The default line: hydrogen_large FP64 (timed), parity against the
     oracle, the mixed form against the FP64 floor, the CPU baseline."""
@@ -160,28 +199,33 @@ def run_hydrogen(args):
         pb = H.populations(prob, b, projectors=proj)[1]
         return max(float(np.abs(x - y).max()) for x, y in zip(pa, pb) if len(x))
 
-    # the FP64 run's own floor: the same steps from psi0 (1 + 1e-16 eta)
-    rng = np.random.default_rng(20261204)
-    psi_pert, _, _ = run([], psi_g * (1 + 1e-16 * rng.standard_normal(psi_g.shape)), False)
-    floor = {"population_error_max": pop_err(psi_pert, psi64), "psi_diff_snorm": snorm(psi_pert - psi64)}
-
     mixed = None
     if not args.no_mixed:
-        psi32, wall32, kern32 = run(list(range(nterms)), psi_g, True)
+        _, wall32, kern32 = run(list(range(nterms)), psi_g, True)
+        eng.set_mixed([])
+        full = int(round(conf.Propagation.duration / dt.real))
+        log("mixed form: %.1f steps/s; error over the workload's %d steps from t = 0" % (K / wall32, full))
+
+        def set_form(m):
+            eng.set_mixed(list(range(nterms)) if m else [])
+            return prob
+        window = window_study(set_form, psi_g, 0.0, full, 2000,
+                              lambda a, b: {"population_error_max": pop_err(a, b), "psi_diff_snorm": snorm(a - b)})
+        fin = window["final"]
         mixed = {"value": K / wall32, "unit": "steps/s", "ms_per_step": 1e3 * wall32 / K,
                  "speedup_vs_fp64": (K / wall32) / steps_per_s,
-                 "population_error_max": pop_err(psi32, psi64), "psi_diff_snorm": snorm(psi32 - psi64),
-                 "fp64_floor": floor, "steps_compared": W + K, "start_time": T_START,
-                 "norm_mixed": float(np.sqrt(np.real(np.sum(np.conj(psi32) * (psi32 @ S.T))))),
+                 "population_error_max": fin["mixed"]["population_error_max"],
+                 "psi_diff_snorm": fin["mixed"]["psi_diff_snorm"], "fp64_floor": fin["fp64_floor"],
+                 "window": window,
                  "dtype": "c64 operator apply (every term: FP32 radial bands, FP32 band sums), "
                           "c128 exchange / Gram-Schmidt / overlap solves / Pade"}
-        eng.set_mixed([])
-        log("mixed: %.1f steps/s, population error %.2e (FP64 floor %.2e)"
-            % (mixed["value"], mixed["population_error_max"], floor["population_error_max"]))
+        log("mixed: %.1f steps/s, population error %.2e after %d steps (FP64 floor %.2e)"
+            % (mixed["value"], mixed["population_error_max"], full, fin["fp64_floor"]["population_error_max"]))
 
     O = None
     parity = None
     cpu = None
+    rng = np.random.default_rng(20261204)
     if not args.no_cpu:
         O = oracle_for_baseline()
         h = O.Hydrogen(rmax=conf.RadialRepresentation.xmax, nbp=conf.RadialRepresentation.xsize, order=7,
@@ -351,6 +395,19 @@ def run_helium(args, steps, warmup, name="helium"):
         log("helium mixed: %.2f steps/s, population error %.2e (FP64 floor %.2e)"
             % (mixed["value"], mixed["population_error"], floor["population_error"]))
         del mx
+        if getattr(args, "mixed_study", False):
+            full = int(round(prob.Duration / dt.real))
+            log("helium mixed form over the workload's %d steps from t = 0" % full)
+
+            def set_form(m):
+                eng.set_mixed(m)
+                return prob
+            mixed["window"] = window_study(set_form, ground.GetData().cpu().numpy(), 0.0, full, 500,
+                                           lambda a, b: {"population_error": abs(pop_of(a) - pop_of(b)),
+                                                         "psi_diff_snorm": snorm_diff(a, b)})
+            fin = mixed["window"]["final"]
+            log("helium mixed after %d steps: population error %.2e (FP64 floor %.2e)"
+                % (full, fin["mixed"]["population_error"], fin["fp64_floor"]["population_error"]))
 
     cpu = None
     if not args.no_cpu:
@@ -700,6 +757,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision runs")
     ap.add_argument("--no-helium", action="store_true", help="default line without the helium object")
+    ap.add_argument("--mixed-study", action="store_true",
+                    help="helium: also compare the mixed form with FP64 and the FP64 floor over the workload's "
+                         "whole duration (5000 steps, several minutes)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB before the workload, so that a "
                          "rocprofv3 --pmc run can scale FETCH_SIZE/WRITE_SIZE by a copy of known size")
