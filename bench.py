@@ -601,24 +601,37 @@ def run_batch(args):
 
 
 def run_cayley(args):
-    """CayleyPropagator on the hydrogen_large operator (L = 64, N = 800):
-    Crank-Nicolson steps solved by the single-cycle GMRES (basis 20,
-    tolerance eps^(2/3)) with the RadialPreconditioner block solve, from the
-    1s state at t = 20 (inside the pulse)."""
+    """CayleyPropagator: Crank-Nicolson steps solved by the single-cycle GMRES
+    (tolerance eps^(2/3)) with the config's left preconditioner.
+    hydrogen_cayley: the hydrogen_large operator (L = 64, N = 800), GMRES
+    basis 20, BSplinePreconditioner, from the 1s state at t = 20.
+    helium_cayley: the helium workload (66 channels x 120 x 120), GMRES basis
+    20, RadialTwoElectronPreconditioner (fast diagonalisation; its solve
+    timed with HIP events on the launch stream and priced against the FP64
+    peak: 4 passes of 4 N^3 flops per channel), from the ground state at
+    t = 10."""
     import torch
     import pyprop_amd
-    from pyprop_amd.examples import hydrogen as H
-
-    conf = H.load("hydrogen_large")
+    helium = args.workload == "helium_cayley"
+    if helium:
+        from pyprop_amd.examples import helium as HE
+        psi_g, E0 = HE.ground_state("helium", steps=args.imag_steps, dt=0.05)
+        conf = HE.load("helium")
+        t_start = 10.0
+    else:
+        from pyprop_amd.examples import hydrogen as H
+        conf = H.load("hydrogen_large")
+        t_start = 20.0
     conf.Propagation.propagator = pyprop_amd.CayleyPropagator
     conf.Propagation.preconditioner = "RadialPreconditioner"
     conf.Propagation.krylov_basis_size = 20
     prob = pyprop_amd.Problem(conf)
     prob.SetupStep()
-    prob.SetWavefunction(H.ground_state(prob))
-    prob.PropagatedTime = 20.0
+    prob.SetWavefunction(psi_g if helium else H.ground_state(prob))
+    prob.psi.Normalize()
+    prob.PropagatedTime = t_start
     warm = min(args.warmup, 5)
-    print("cayley: %d warmup steps" % warm, file=sys.stderr, flush=True)
+    log("cayley: %d warmup steps" % warm)
     prob.AdvanceSteps(warm)
     solver = prob.Propagator.Solver
     ops0 = solver.GetOperatorCount()
@@ -628,20 +641,41 @@ def run_cayley(args):
     prob.AdvanceSteps(K)
     torch.cuda.synchronize()
     dt_s = time.perf_counter() - t0
+    shape = tuple(prob.psi.GetData().shape)
     line = {
         "metric": METRIC,
         "value": K / dt_s, "unit": "steps/s",
         "n_gpus": 1, "steps": K, "warmup": warm, "ms_per_step": 1e3 * dt_s / K,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
-        "data": "synthetic (deterministic hydrogen operator, 1s start state by inverse iteration)",
-        "config": {"workload": "hydrogen_cayley", "n": 64 * 800, "L": 64, "N": 800, "gmres_basis": 20,
-                   "dt": prob.TimeStep.real, "preconditioner": "BSplinePreconditioner (rank 1)"},
+        "data": ("synthetic (deterministic helium operator, ground state by imaginary-time pAMP)" if helium else
+                 "synthetic (deterministic hydrogen operator, 1s start state by inverse iteration)"),
+        "config": {"workload": args.workload, "shape": shape, "gmres_basis": 20, "dt": prob.TimeStep.real,
+                   "start_time": t_start,
+                   "preconditioner": type(prob.Propagator.Preconditioner).__name__},
         "operator_calls_per_step": (solver.GetOperatorCount() - ops0) / K,
         "gmres_error_estimate_last": solver.GetErrorEstimate(),
         "norm_after": prob.psi.GetNorm(),
     }
+    if helium:
+        C, N, _ = shape
+        prec = prob.Propagator.Preconditioner
+        x = prob.psi.Copy()
+        reps = 50
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            prec.Solve(x)
+        e1.record()
+        torch.cuda.synchronize()
+        sec = e0.elapsed_time(e1) / 1e3 / reps
+        flops = 4 * 4.0 * C * N ** 3
+        line["roofline"] = {"bound": "fp64", "achieved": flops / sec / 1e12, "peak": FP64_PEAK_TFLOPS,
+                            "unit": "TFLOP/s", "frac": flops / sec / 1e12 / FP64_PEAK_TFLOPS, "traffic": None,
+                            "kernel": "pr_mm x 4 (RadialTwoElectronPreconditioner.Solve)",
+                            "algorithmic_flops_per_solve": flops, "solve_us": sec * 1e6,
+                            "algorithmic_bytes_per_solve": 4 * 2 * 16 * C * N * N}
+        line["checks"] = {"ground_energy": E0}
     print(json.dumps(line))
-
 
 
 def run_adjoint(args):
@@ -752,7 +786,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="hydrogen_large",
                     help="hydrogen_large (default; with the helium object), hydrogen_small, helium, "
-                         "batch512, hydrogen_adjoint, hydrogen_cayley")
+                         "batch512, hydrogen_adjoint, hydrogen_cayley, helium_cayley")
     ap.add_argument("--imag-steps", type=int, default=150, help="imaginary-time steps to the helium ground state")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision runs")
@@ -765,7 +799,8 @@ def main():
                          "rocprofv3 --pmc run can scale FETCH_SIZE/WRITE_SIZE by a copy of known size")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = {"hydrogen_adjoint": 4000, "helium": 300, "helium_small": 50, "hydrogen_cayley": 400}.get(
+        args.steps = {"hydrogen_adjoint": 4000, "helium": 300, "helium_small": 50, "hydrogen_cayley": 400,
+                      "helium_cayley": 300}.get(
             args.workload, 20 if args.workload.startswith("batch") else 20000)
     if args.gpus != 1:
         print("bench.py: this build runs one GPU holding the whole wavefunction; "
@@ -785,7 +820,7 @@ def main():
         return run_batch(args)
     if args.workload == "hydrogen_adjoint":
         return run_adjoint(args)
-    if args.workload == "hydrogen_cayley":
+    if args.workload in ("hydrogen_cayley", "helium_cayley"):
         return run_cayley(args)
     if args.workload.startswith("helium"):
         return run_helium_line(args)

@@ -164,6 +164,12 @@ long pp_pamp_lds_estimate(int L, int N, int m, int nslots, int kd, int B);
  * keeps the Pade(6) error at rounding level and makes a step a smooth
  * function of its inputs (used by the adjoint gradients). */
 int pp_pamp_set_pade_scaling(pp_pamp *p, int mode);
+/* Exponential of the Krylov matrix in later launches: the reference's
+ * krylov_exponentiation_method (core/krylov/pamp/pampwrapper.cpp:56-61):
+ * 0 scaling + Pade(6) (pamp.h:430-564, the default), 1 the eigenvector
+ * exponential R exp(Lambda) R^H e1 (pAMP::EigenvectorExponential,
+ * pamp.h:567-607; Jacobi on the Hermitian part of the Krylov matrix). */
+int pp_pamp_set_exponentiation(pp_pamp *p, int method);
 /* Mixed-precision form (no reference counterpart; the reference is FP64
  * throughout): later launches apply the terms whose bit is set in term_mask
  * (bit t = t-th added term) in complex FP32 (FP32 radial bands, FP32 band
@@ -236,6 +242,9 @@ int pp_pamp3_diagnostics(pp_pamp3 *p, int nsteps, double *diag, pp_stream stream
  * supports and P_l, Q_l of every multipole term): later calls return
  * PP_EUNSUP otherwise. */
 int pp_pamp3_set_mixed(pp_pamp3 *p, int mixed);
+/* krylov_exponentiation_method for the two-electron engine, as
+ * pp_pamp_set_exponentiation (pampwrapper.cpp:56-61). */
+int pp_pamp3_set_exponentiation(pp_pamp3 *p, int method);
 /* Measurement hooks (no reference counterpart): with timing enabled, every
  * multipole contraction launch of later calls is bracketed by HIP events on
  * its stream; pp_pamp3_kernel_time returns their summed duration (ms) and

@@ -118,6 +118,8 @@ def lib():
         L.orc_pamp_error_estimate.argtypes = [P]
         L.orc_pamp_error_estimate.restype = ctypes.c_double
         L.orc_propagate_pamp.argtypes = [P, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int, ctypes.c_int]
+        L.orc_propagate_pamp_ex.argtypes = [P, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int, ctypes.c_int,
+                                            _EXPFN]
         L.orc_zgpadm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, dp, dp, ip]
         L.orc_zgexpv.argtypes = [P, ctypes.c_int, ctypes.c_double, dp, dp, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_int, ip]
@@ -571,11 +573,17 @@ class Ham:
             raise RuntimeError("Expokit returned nonzero value %d" % rc)
         return psi, st
 
-    def propagate_pamp(self, psi, nsteps, dt, scalings, m, reorth=False):
+    def propagate_pamp(self, psi, nsteps, dt, scalings, m, reorth=False, exponentiation=0):
+        """exponentiation: 0 scaling + Pade (pamp.h:430-564), 1 the
+        eigenvector exponential (pamp.h:567-607, `eigenvector_exponential`)."""
         psi = np.array(psi, dtype=np.complex128, order="C")
         sc = np.ascontiguousarray(np.asarray(scalings, dtype=np.float64).reshape(nsteps, self.nterms))
-        rc = lib().orc_propagate_pamp(self.h, _dp(psi.view(np.float64)), nsteps, float(dt), _dp(sc),
-                                      int(m), int(bool(reorth)))
+        if exponentiation:
+            rc = lib().orc_propagate_pamp_ex(self.h, _dp(psi.view(np.float64)), nsteps, float(dt), _dp(sc),
+                                             int(m), int(bool(reorth)), _EIG_EXPFN)
+        else:
+            rc = lib().orc_propagate_pamp(self.h, _dp(psi.view(np.float64)), nsteps, float(dt), _dp(sc),
+                                          int(m), int(bool(reorth)))
         if rc:
             raise RuntimeError("oracle pAMP failed (%d)" % rc)
         return psi
@@ -686,6 +694,35 @@ def pade_exp(A, order=6):
     info = lib().orc_pade_exp(m, _dp(A.reshape(-1, order="F").view(np.float64)),
                               _dp(out.reshape(-1, order="F").view(np.float64)), order)
     return out, info
+
+
+def eigenvector_exponential(A):
+    """pAMP::EigenvectorExponential (pamp.h:567-607): R diag(exp(lambda)) R^H
+    with lambda, R the eigenvalues and unit right eigenvectors of A from
+    LAPACK's zgeev (numpy.linalg.eig calls the same routine).  pAMP stores its
+    Hessenberg matrix so that the column-major view zgeev and zgemm see is H
+    itself (pamp.h:289,315,345 write H(j, j+1) = beta into a row-major
+    array), so this is the matrix of A, not of its transpose."""
+    A = np.asarray(A, dtype=np.complex128)
+    lam, R = np.linalg.eig(A)
+    return (R * np.exp(lam)[None, :]) @ R.conj().T
+
+
+EIG_EXPFN_CALLS = [0]
+
+
+def _eig_expfn(m, a_ptr, out_ptr):
+    a = np.ctypeslib.as_array(a_ptr, shape=(2 * m * m,)).view(np.complex128).reshape((m, m), order="F")
+    E = eigenvector_exponential(a)
+    out = np.ctypeslib.as_array(out_ptr, shape=(2 * m * m,)).view(np.complex128)
+    out[:] = E.reshape(-1, order="F")
+    EIG_EXPFN_CALLS[0] += 1
+    return 0
+
+
+_EXPFN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                          ctypes.POINTER(ctypes.c_double))
+_EIG_EXPFN = _EXPFN(_eig_expfn)
 
 
 def scaling_pade_exp(A):

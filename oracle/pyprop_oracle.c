@@ -914,6 +914,12 @@ ORC_API int orc_scaling_pade_exp(int m, const double *Ad, double *outd)
 /* pAMP::PropagateVector (pamp.h:613-669), Euclidean inner products,  */
 /* CGS (pamp.h:293-350), optional re-orthogonalisation (:353-400).    */
 /* ------------------------------------------------------------------ */
+/* exponential of the m x m Krylov matrix A = -i dt H (column-major): out = */
+/* exp(A) by pAMP's scaling + Pade, or by a caller-supplied routine (the    */
+/* eigenvector exponential, pamp.h:567-607, needs LAPACK's zgeev, which     */
+/* this file does not restate: oracle.py passes numpy's).                   */
+typedef int (*orc_expfn)(int m, const double *A, double *out);
+
 typedef struct {
     int m;
     int reorth;
@@ -924,6 +930,7 @@ typedef struct {
     double err_estimate;
     int happy;
     int basis_used;
+    orc_expfn expfn;   /* NULL: orc_scaling_pade_exp */
 } orc_pamp;
 
 static zc zdotc(long n, const zc *a, const zc *b)
@@ -1037,7 +1044,8 @@ ORC_API int orc_pamp_step(orc_pamp *p, orc_ham *h, double *psid, double dt_re, d
     p->basis_used = j + 1;
     int mm = m;   /* pAMP exponentiates the full (zero-padded) m x m matrix */
     for (int i = 0; i < mm * mm; i++) H[i] *= -dt * I;
-    int info = orc_scaling_pade_exp(mm, (const double *)H, (double *)p->Hexp);
+    int info = p->expfn ? p->expfn(mm, (const double *)H, (double *)p->Hexp)
+                        : orc_scaling_pade_exp(mm, (const double *)H, (double *)p->Hexp);
     if (info) return info;
     /* psi = norm * V * exp(H) e1 (:657-662) */
     zc *v = p->Hexp;   /* first column */
@@ -1057,10 +1065,11 @@ ORC_API int orc_pamp_step(orc_pamp *p, orc_ham *h, double *psid, double dt_re, d
 /* psi~ = U psi ; pAMP step on U^-H H(t_n) U^-1 ; psi = U^-1 psi~.     */
 /* scalings[step*nterms + t] = time factor of term t at t_n.           */
 /* ------------------------------------------------------------------ */
-ORC_API int orc_propagate_pamp(orc_ham *h, double *psid, int nsteps, double dt,
-                               const double *scalings, int m, int reorth)
+ORC_API int orc_propagate_pamp_ex(orc_ham *h, double *psid, int nsteps, double dt,
+                                  const double *scalings, int m, int reorth, orc_expfn expfn)
 {
     orc_pamp *p = orc_pamp_create(h->n, m, reorth);
+    p->expfn = expfn;
     int rc = 0;
     for (int s = 0; s < nsteps && rc == 0; s++) {
         orc_ham_set_scaling(h, scalings + (size_t)s * h->nterms);
@@ -1072,6 +1081,12 @@ ORC_API int orc_propagate_pamp(orc_ham *h, double *psid, int nsteps, double dt,
     }
     orc_pamp_destroy(p);
     return rc;
+}
+
+ORC_API int orc_propagate_pamp(orc_ham *h, double *psid, int nsteps, double dt,
+                               const double *scalings, int m, int reorth)
+{
+    return orc_propagate_pamp_ex(h, psid, nsteps, dt, scalings, m, reorth, NULL);
 }
 
 /* ------------------------------------------------------------------ */

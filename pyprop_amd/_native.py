@@ -35,6 +35,7 @@ EXPORTS = [
     "pp_pamp3_set_radial_supports", "pp_pamp3_set_mixed",
     "pp_pamp3_multipole_flops", "pp_pamp3_destroy",
     "pp_radial_prec_create", "pp_radial_prec_solve", "pp_radial_prec_destroy",
+    "pp_pamp_set_exponentiation", "pp_pamp3_set_exponentiation",
 ]
 
 _lib = None
@@ -116,6 +117,8 @@ def lib():
     L.pp_radial_prec_create.argtypes = [I, I, I, ip, ip, dp, dp, ctypes.POINTER(P)]
     L.pp_radial_prec_solve.argtypes = [P, P, D, D, D, D, P]
     L.pp_radial_prec_destroy.argtypes = [P]
+    L.pp_pamp_set_exponentiation.argtypes = [P, I]
+    L.pp_pamp3_set_exponentiation.argtypes = [P, I]
     for name in EXPORTS:
         getattr(L, name).restype = ctypes.c_int
     L.pp_pamp_lds_estimate.restype = ctypes.c_long

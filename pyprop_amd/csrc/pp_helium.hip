@@ -98,6 +98,7 @@ struct pp_pamp3 {
   const double **dRn, **dPt, **dQt;
   const float **fRnt, **fPt, **fQt;    // per-term FP32 tables of the mixed form
   int mixed;                           // multipole stage in FP32 (complex64 intermediates)
+  int expmethod;                       // 0 Pade, 1 eigenvector exponential
   zd *Tq, *Tp;
   std::vector<mterm> mterms;
   int nslots;
@@ -955,7 +956,7 @@ extern "C" int pp_pamp3_create(int C, int N, int m, int reorth, const pp_overlap
   p->dmats = nullptr; p->rrow = p->lrow = p->mrow = nullptr; p->rent = p->lent = p->ment = nullptr;
   p->cb_ptr = p->cb_blk = p->blk_term = nullptr; p->dR = nullptr; p->X = nullptr; p->Ym = nullptr;
   p->nr = nullptr; p->dRn = p->dPt = p->dQt = nullptr; p->Tq = p->Tp = nullptr; p->split = false;
-  p->fRnt = p->fPt = p->fQt = nullptr; p->mixed = 0;
+  p->fRnt = p->fPt = p->fQt = nullptr; p->mixed = 0; p->expmethod = 0;
   p->K = 0; p->nblk = 0; p->nchunk = (C + P3_CHUNK - 1) / P3_CHUNK;
   p->scal = p->diag = nullptr; p->scal_cap = p->diag_cap = 0;
   p->timing = 0; p->ee_ms = 0.0; p->ee_count = 0;
@@ -1408,7 +1409,7 @@ extern "C" int pp_pamp3_propagate(pp_pamp3 *p, void *psi, int nsteps, double dt_
                          p->beta, j + 1, p->state);
     }
     hipLaunchKernelGGL(p3_pade, dim3(1), dim3(PP_THREADS), lds, stream, m, p->Hm, dt_re, dt_im, p->cvec,
-                       p->diag + (size_t)step * 4, p->state, p->beta, 0);
+                       p->diag + (size_t)step * 4, p->state, p->beta, p->expmethod ? 2 : 0);
     hipLaunchKernelGGL(p3_combine, dim3(gridv), dim3(256), 0, stream, p->V, p->cvec, p->beta, p->state, ps, n);
     if ((rc = p3_solve2(p, 0, ps, stream))) return rc;
     if (hipGetLastError() != hipSuccess) return PP_EHIP;
@@ -1430,6 +1431,15 @@ static void p3_collect(pp_pamp3 *p)
     (void)hipEventDestroy(p->ev[i + 1]);
   }
   p->ev.clear();
+}
+
+// Exponential of the Krylov matrix (krylov_exponentiation_method): 0 Pade,
+// 1 the eigenvector exponential, as pp_pamp_set_exponentiation.
+extern "C" int pp_pamp3_set_exponentiation(pp_pamp3 *p, int method)
+{
+  if (!p || method < 0 || method > 1) return PP_EARG;
+  p->expmethod = method;
+  return PP_OK;
 }
 
 extern "C" int pp_pamp3_set_mixed(pp_pamp3 *p, int mixed)

@@ -105,6 +105,7 @@ struct pp_pamp {
   zd *V, *y, *Hm, *cvec;
   unsigned mixed;                          // bit t: term t applied in complex FP32
   int normscale;                           // Pade scaling by the 1-norm
+  int expmethod;                           // 0 Pade, 1 eigenvector exponential
   double *part, *diag, *scal;
   pp_barrier *bar;
   int diag_cap, scal_cap, pstr;
@@ -1363,6 +1364,7 @@ extern "C" int pp_pamp_create(int L, int N, int m, int reorth, const pp_overlap 
   p->V = p->y = p->Hm = p->cvec = nullptr;
   p->mixed = 0;
   p->normscale = 0;
+  p->expmethod = 0;
   p->gen = 0;
   p->part = p->diag = p->scal = nullptr;
   p->bar = nullptr; p->tlog = nullptr; p->timing = 0;
@@ -1684,7 +1686,7 @@ static int pamp_args(pp_pamp *p, void *psi, int nsteps, double dt_re, double dt_
   }
   memset(&A, 0, sizeof(A));
   A.L = p->L; A.N = p->N; A.m = p->m; A.s = p->s; A.bw = p->bw; A.nterms = p->nterms;
-  A.nsteps = nsteps; A.reorth = p->reorth; A.nslots = p->nslots; A.normscale = p->normscale;
+  A.nsteps = nsteps; A.reorth = p->reorth; A.nslots = p->nslots; A.normscale = p->expmethod ? 2 : p->normscale;
   A.ncombo = 0;
   for (int t = 0; t < p->nterms; t++) A.ncombo += p->term[t].rank;
   if (A.ncombo > PP_MAXCOMBO) return PP_EUNSUP;
@@ -2159,6 +2161,17 @@ extern "C" int pp_pamp_set_pade_scaling(pp_pamp *p, int mode)
 {
   if (!p || mode < 0 || mode > 1) return PP_EARG;
   p->normscale = mode;
+  p->gen++;
+  return PP_OK;
+}
+
+// Exponential of the Krylov matrix in later launches (the reference's
+// krylov_exponentiation_method, pampwrapper.cpp:56-61): 0 scaling + Pade(6)
+// (pamp.h:430-564), 1 the eigenvector exponential (pamp.h:567-607).
+extern "C" int pp_pamp_set_exponentiation(pp_pamp *p, int method)
+{
+  if (!p || method < 0 || method > 1) return PP_EARG;
+  p->expmethod = method;
   p->gen++;
   return PP_OK;
 }

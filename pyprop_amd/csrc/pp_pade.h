@@ -27,10 +27,18 @@ __device__ inline zd zld(const zd *p) { return zmk(xld(&p->x), xld(&p->y)); }
 // 1-norm of A / 2^s is at most 1/2, which keeps the Pade(6) error near
 // rounding and makes the step a smooth function of its inputs (the largest
 // element can be several times smaller than the norm, and a change of s
-// then moves the result by the Pade error).
+// then moves the result by the Pade error); 2 selects the eigenvector
+// exponential (eig_block) instead of Pade.
+static __device__ __attribute__((noinline)) void eig_block(int m, const zd *Hg, double dt_re, double dt_im, int off,
+                                                    zd *cvec_out, double *diag);
+
 static __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg, double dt_re, double dt_im, int off,
                                                      zd *cvec_out, double *diag, int normscale)
 {
+  if (normscale == 2) {
+    eig_block(m, Hg, dt_re, dt_im, off, cvec_out, diag);
+    return;
+  }
   extern __shared__ double pp_lds[];
   zd *lds = (zd *)(pp_lds + off);
   const int tid = threadIdx.x;
@@ -201,3 +209,125 @@ static __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg,
   __syncthreads();
 }
 
+
+// ------------------------------------------------------------------ eigenvector exponential
+// pAMP::EigenvectorExponential (pamp.h:567-607, krylov_exponentiation_method
+// = 1): exp(A) e1 = R exp(Lambda) R^H e1 with A = -i dt H and R its unit
+// eigenvectors.  pAMP's H is the Arnoldi matrix of a Hermitian operator,
+// Hermitian up to the rounding of the Arnoldi inner products, so A = -i dt H
+// is normal and R is unitary: the eigenvectors and eigenvalues come from the
+// Hermitian part (H + H^H)/2 by the cyclic Jacobi method (the order-m matrix
+// padded to even order n2; per sweep n2-1 rounds of n2/2 disjoint rotations
+// in round-robin pairing, applied as one column pass over A and R and one row
+// pass over A; sweeps until the off-diagonal norm is below n2 eps of the
+// Frobenius norm -- 6 to 8 sweeps at m = 12..40 -- at most 20), then c_k = sum_n R_kn exp(-i dt lambda_n)
+// conj(R_0n).  Column-major n2 x n2 matrices in the dynamic LDS at double
+// offset `off` (2 n2^2 complex entries and the rotation table: less than
+// pade_block's region).
+static __device__ double eig_block_sum(double v, double *red)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < PP_THREADS / 64; w++) t += red[w];
+  return t;
+}
+
+static __device__ __attribute__((noinline)) void eig_block(int m, const zd *Hg, double dt_re, double dt_im, int off,
+                                                    zd *cvec_out, double *diag)
+{
+  extern __shared__ double pp_lds[];
+  const int tid = threadIdx.x;
+  const int n2 = m + (m & 1);
+  const int np = n2 / 2;
+  zd *A = (zd *)(pp_lds + off);
+  zd *R = A + n2 * n2;
+  zd *ph = R + n2 * n2;                       // e^{i phi} of each rotation
+  double *rc = (double *)(ph + np), *rs = rc + np;
+  int *rp = (int *)(rs + np), *rq = rp + np;
+  double *red = (double *)(rq + np);          // rp, rq: 2 np ints = np doubles, so still 8-byte aligned
+  for (int e = tid; e < n2 * n2; e += PP_THREADS) {
+    const int i = e % n2, j = e / n2;
+    zd v = zmk(0.0, 0.0);
+    if (i < m && j < m) {   // (H + H^H)/2
+      const zd a = Hg[i + j * m], b = Hg[j + i * m];
+      v = zmk(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+    }
+    A[e] = v;
+    R[e] = zmk(i == j ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  double fro = 0.0;
+  for (int e = tid; e < n2 * n2; e += PP_THREADS) fro += zabs2(A[e]);
+  fro = eig_block_sum(fro, red);
+  const double tol = n2 * 2.220446049250313e-16;
+  const double stop = fro * tol * tol;         // off-diagonal norm <= n2 eps of the Frobenius norm
+  for (int sweep = 0; sweep < 20; sweep++) {
+    double offd = 0.0;
+    for (int e = tid; e < n2 * n2; e += PP_THREADS)
+      if (e % n2 != e / n2) offd += zabs2(A[e]);
+    offd = eig_block_sum(offd, red);
+    if (!(offd > stop)) break;
+    for (int r = 0; r < n2 - 1; r++) {
+      if (tid < np) {   // pair tid of round r: the circle method with index 0 fixed
+        const int a = tid == 0 ? 0 : 1 + (tid - 1 + r) % (n2 - 1);
+        const int b = 1 + (n2 - 2 - tid + r) % (n2 - 1);
+        const int p = a < b ? a : b, q = a < b ? b : a;
+        const zd apq = A[p + q * n2];
+        const double g = sqrt(zabs2(apq));
+        double c = 1.0, sn = 0.0;
+        zd e = zmk(1.0, 0.0);
+        if (q < m && g > 1e-300) {
+          // J = diag(1, e^{-i phi}) [[c, s], [-s, c]] zeroes A_pq of the
+          // Hermitian 2 x 2 block [[a_pp, |a_pq| e^{i phi}], [c.c., a_qq]]
+          e = zmk(apq.x / g, apq.y / g);
+          const double tau = (A[q + q * n2].x - A[p + p * n2].x) / (2.0 * g);
+          const double t = fabs(tau) > 1e150 ? 0.5 / tau
+                                             : (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+          c = 1.0 / sqrt(1.0 + t * t);
+          sn = t * c;
+        }
+        rp[tid] = p; rq[tid] = q; rc[tid] = c; rs[tid] = sn; ph[tid] = e;
+      }
+      __syncthreads();
+      for (int e = tid; e < np * n2; e += PP_THREADS) {   // columns: A J, R J
+        const int k = e / n2, i = e % n2;
+        const int p = rp[k], q = rq[k];
+        const double c = rc[k], sn = rs[k];
+        const zd em = zconj(ph[k]);
+        zd x = A[i + p * n2], y = zmul(A[i + q * n2], em);
+        A[i + p * n2] = zsub(zscale(x, c), zscale(y, sn));
+        A[i + q * n2] = zadd(zscale(x, sn), zscale(y, c));
+        x = R[i + p * n2]; y = zmul(R[i + q * n2], em);
+        R[i + p * n2] = zsub(zscale(x, c), zscale(y, sn));
+        R[i + q * n2] = zadd(zscale(x, sn), zscale(y, c));
+      }
+      __syncthreads();
+      for (int e = tid; e < np * n2; e += PP_THREADS) {   // rows: J^H (A J)
+        const int k = e / n2, j = e % n2;
+        const int p = rp[k], q = rq[k];
+        const double c = rc[k], sn = rs[k];
+        const zd x = A[p + j * n2], y = zmul(A[q + j * n2], ph[k]);
+        A[p + j * n2] = zsub(zscale(x, c), zscale(y, sn));
+        A[q + j * n2] = zadd(zscale(x, sn), zscale(y, c));
+      }
+      __syncthreads();
+    }
+  }
+  // c_k = sum_n R_kn exp(-i dt lambda_n) conj(R_0n)
+  for (int k = tid; k < m; k += PP_THREADS) {
+    zd acc = zmk(0.0, 0.0);
+    for (int n = 0; n < m; n++) {
+      const double lam = A[n + n * n2].x;
+      const double mag = exp(dt_im * lam), ang = -dt_re * lam;
+      const zd ex = zmk(mag * cos(ang), mag * sin(ang));
+      zfma(acc, R[k + n * n2], zmul(ex, zconj(R[n * n2])));
+    }
+    zst(cvec_out + k, acc);
+    if (k == m - 1) diag[0] = sqrt(zabs2(acc));
+  }
+  __syncthreads();
+}

@@ -232,6 +232,11 @@ class PampEngine:
         """0: largest element (pAMP); 1: 1-norm, accurate and smooth."""
         nat.check(nat.lib().pp_pamp_set_pade_scaling(self.h, int(mode)), "pp_pamp_set_pade_scaling")
 
+    def set_exponentiation(self, method):
+        """Exponential of the Krylov matrix: 0 scaling + Pade(6), 1 the
+        eigenvector exponential (krylov_exponentiation_method)."""
+        nat.check(nat.lib().pp_pamp_set_exponentiation(self.h, int(method)), "pp_pamp_set_exponentiation")
+
     def set_fiber_split(self, q=0):
         """Workgroups per angular fiber in propagate: 0 automatic (fill the
         CUs), 1 one per fiber, q >= 2 fixed."""

@@ -404,6 +404,11 @@ class Pamp3Engine:
     def set_timing(self, enable=True):
         nat.check(nat.lib().pp_pamp3_set_timing(self.h, int(bool(enable))), "pp_pamp3_set_timing")
 
+    def set_exponentiation(self, method):
+        """Exponential of the Krylov matrix: 0 scaling + Pade(6), 1 the
+        eigenvector exponential (krylov_exponentiation_method)."""
+        nat.check(nat.lib().pp_pamp3_set_exponentiation(self.h, int(method)), "pp_pamp3_set_exponentiation")
+
     def set_mixed(self, enable=True):
         """Mixed form: the multipole terms in FP32 with complex64
         intermediates (everything else FP64); needs the near/far split."""

@@ -97,12 +97,15 @@ class PamPropagator(PropagatorBase):
 
     def SetupStep(self, dt):
         self.BasePropagator.SetupStep(dt)
-        if self.ExponentiationMethod != 0:
-            raise NotImplementedError("krylov_exponentiation_method=1 (eigenvector) is not on the HIP path")
+        if self.ExponentiationMethod not in (0, 1):
+            raise ValueError("krylov_exponentiation_method must be 0 (Pade) or 1 (eigenvector), got %r"
+                             % (self.ExponentiationMethod,))
         repr = self.psi.GetRepresentation()
         self.IsOrthogonalBasis = all(repr.IsOrthogonalBasis(i) for i in range(self.Rank))
         self.UseBalancedOverlap = hasattr(self.BasePropagator, "MultiplyHamiltonianBalancedOverlap")
         self.Engine = self._build_engine()
+        # pampwrapper.cpp:56-61: 0 scaling + Pade(6), 1 the eigenvector exponential
+        self.Engine.set_exponentiation(self.ExponentiationMethod)
 
     def _build_engine(self):
         """Map the consolidated tensor potentials onto the persistent engine:

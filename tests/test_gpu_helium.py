@@ -75,6 +75,29 @@ def test_helium_pamp_matches_oracle():
     assert d.shape == (n, 4) and np.all(d[:, 1] == 20)
 
 
+def test_helium_eigenvector_exponential_matches_oracle():
+    """The two-electron engine with krylov_exponentiation_method = 1 (read
+    from the config section, as pampwrapper.cpp:56-61 does): 12 laser steps
+    against the oracle's eigenvector exponential (numpy's zgeev), 1e-11."""
+    print("helium pAMP, eigenvector exponential, small, 12 steps from t = 5", flush=True)
+    prob = _problem(overrides={("Propagation", "krylov_exponentiation_method"): 1})
+    assert prob.Propagator.ExponentiationMethod == 1
+    h = O.Helium(**SMALL)
+    psi0 = HE.start_state(prob) + 0.05 * _rand(prob.psi.GetData().shape, 5)
+    prob.SetWavefunction(psi0)
+    prob.psi.Normalize()
+    psi0 = prob.psi.GetData().cpu().numpy()
+    t0, n, dt = 5.0, 12, 0.01
+    prob.PropagatedTime = t0
+    prob.AdvanceSteps(n)
+    got = prob.psi.GetData().cpu().numpy()
+    sc = np.array([_oracle_scalings(h, h.scaling(tt)) for tt in O.step_times(n, dt, t0)])
+    ref = h.ham().propagate_pamp(psi0, n, dt, sc, 20, exponentiation=1)
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print("  rel err %.3e" % err, flush=True)
+    assert err <= 1e-11
+
+
 def test_helium_imaginary_time_ground_state():
     print("helium imaginary time, small", flush=True)
     h = O.Helium(**SMALL, laser=False)

@@ -130,6 +130,48 @@ def test_pamp_persistent_matches_oracle(lmax, nbp, m, nsteps, dt, reorth):
     assert err <= tol
 
 
+@pytest.mark.parametrize("lmax,nbp,m,nsteps,dt,split", [
+    (6, 40, 12, 5, 0.05, 0), (6, 40, 12, 5, 0.05, 1), (10, 100, 20, 20, 0.01, 0), (10, 100, 21, 10, 0.01, 1)])
+def test_pamp_eigenvector_exponential_matches_oracle(lmax, nbp, m, nsteps, dt, split):
+    """krylov_exponentiation_method = 1 (pamp.h:567-607): the step kernels'
+    Jacobi eigen-decomposition against the oracle's, which calls LAPACK's
+    zgeev through numpy as the reference does; fiber-split and
+    one-workgroup-per-fiber kernels, even and odd m.  Same bar as the Pade
+    test (10x the oracle's 1e-16 sensitivity, floor 1e-12); and the
+    eigenvector and Pade forms of the GPU step agree to 1e-11 (both are
+    exp(-i dt H_m) e1 for the Hermitian Krylov matrix)."""
+    from pyprop_amd import device as D
+    h = _hydrogen(lmax=lmax, nbp=nbp, rmax=float(nbp))
+    psi0 = h.ground_state()
+    psi0[1] = 0.1 * psi0[0]
+    ts = O.step_times(nsteps, dt, t0=10.0)
+    sc = np.array([[1.0, h.scaling(t)] for t in ts])
+    ham = h.ham()
+    calls = O.EIG_EXPFN_CALLS[0]
+    ref = ham.propagate_pamp(psi0, nsteps, dt, sc, m, False, exponentiation=1)
+    assert O.EIG_EXPFN_CALLS[0] == calls + nsteps
+    rng = np.random.default_rng(12)
+    ref2 = ham.propagate_pamp(psi0 * (1 + 1e-16 * rng.standard_normal(psi0.shape)), nsteps, dt, sc, m, False,
+                              exponentiation=1)
+    tol = max(1e-12, 10 * np.abs(ref2 - ref).max())
+    ov = D.Overlap(h.S, h.N, h.kd)
+    out = {}
+    for method in (1, 0):
+        eng = D.PampEngine(h.L, h.N, m, ov)
+        eng.add_term("Diag", D.to_device(h.H0))
+        eng.add_term("Simp", D.to_device(h.Hint), pairs=h.pairs)
+        eng.set_fiber_split(split)
+        eng.set_exponentiation(method)
+        psi = D.to_device(psi0)
+        eng.propagate(psi, nsteps, dt, sc)
+        out[method] = psi.cpu().numpy()
+    err = np.abs(out[1] - ref).max()
+    print("eigenvector exponential", lmax, nbp, m, nsteps, dt, split, "err", err, "tol", tol,
+          "vs GPU Pade", np.abs(out[1] - out[0]).max(), flush=True)
+    assert err <= tol
+    assert np.abs(out[1] - out[0]).max() <= 1e-11 * np.abs(out[0]).max()
+
+
 @pytest.mark.parametrize("lmax,nbp,m,nsteps,dt", [(10, 100, 20, 20, 0.01), (63, 120, 24, 6, 0.005)])
 @pytest.mark.parametrize("mask", [(1,), (0, 1)])
 def test_pamp_mixed_close_to_oracle(lmax, nbp, m, nsteps, dt, mask):

@@ -525,3 +525,34 @@ def test_band_and_identity_storages_vs_blas():
     y = np.zeros(N2, complex)
     O.tp_multiply((O.ST_IDENT,), V, sc, x[0].copy(), y)
     assert np.abs(y - sc * V[0] * x[0]).max() < 1e-13 * np.abs(x).max()
+
+
+def test_eigenvector_exponential_restatement():
+    """oracle.eigenvector_exponential (pAMP's krylov_exponentiation_method =
+    1, zgeev through numpy): for A = -i dt H with H Hermitian (normal A, as
+    pAMP's Krylov matrix is) it equals scipy's expm; on a complex symmetric
+    non-Hermitian H it does not (R R^H != I), which pins that the formula is
+    R exp(Lambda) R^H as in the reference and not R exp(Lambda) R^-1; and the
+    pAMP propagation through the callback agrees with the Pade path to the
+    Pade error on a resolved step, calling the callback once per step."""
+    import scipy.linalg
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((9, 9)) + 1j * rng.standard_normal((9, 9))
+    for dt in (0.3, 0.05 - 0.02j):
+        A = -1j * dt * (X + X.conj().T)
+        assert np.abs(O.eigenvector_exponential(A) - scipy.linalg.expm(A)).max() < 1e-12 * np.abs(
+            scipy.linalg.expm(A)).max()
+    B = -0.3j * (X + X.T)
+    lam, R = np.linalg.eig(B)
+    assert np.abs(O.eigenvector_exponential(B) - scipy.linalg.expm(B)).max() > 1e-3
+    assert np.abs(O.eigenvector_exponential(B) - (R * np.exp(lam)) @ R.conj().T).max() < 1e-12
+    h = O.Hydrogen(rmax=30.0, nbp=30, order=7, lmax=3, frequency=0.3, cycles=6, E0=0.05)
+    ham = h.ham()
+    psi0 = np.zeros((4, h.N), complex)
+    psi0[0] = rng.standard_normal(h.N)
+    sc = [[1.0, h.scaling(t)] for t in O.step_times(5, 0.01, t0=5.0)]
+    calls = O.EIG_EXPFN_CALLS[0]
+    a = ham.propagate_pamp(psi0, 5, 0.01, sc, 12)
+    b = ham.propagate_pamp(psi0, 5, 0.01, sc, 12, exponentiation=1)
+    assert O.EIG_EXPFN_CALLS[0] == calls + 5
+    assert np.abs(a - b).max() < 1e-11 * np.abs(a).max()
