@@ -518,30 +518,35 @@ __global__ void __launch_bounds__(256) p3_farL(const T *const *Pt, const T *cons
 
 // Z[c][i1][i2] += sum_blocks sum_{d1,d2} Rn_t[i1][d1][w][d2] Ym[b][i1+d1-S][i2+d2-S][cl]
 // for the near band i2 = nlo[i1] + w <= nhi[i1]; workgroup: (chunk, i1),
-// 256 threads = P3_NG4 groups x 4 w-quads x 16 channel lanes.  A lane
+// P3_NG wavefronts = P3_NG groups x (4 w-quads x 16 channel lanes).  A lane
 // computes the 4 outputs w = 4 wq .. 4 wq + 3 of its channel, so each input
 // row is loaded once per lane as 16 consecutive positions shared by the 4
 // outputs' 13-tap windows (a quarter of the loads of one output per lane).
-// Group g takes the blocks g, g + P3_NG4, ... and the groups' partial sums
-// are added in LDS in group order, so the result is deterministic.  Lanes
-// whose outputs fall past the band accumulate values that are never stored.
-#define P3_NG4 4
+// The (block, tap row d1) pairs of the workgroup are dealt round-robin to the
+// groups (pair index g, g + P3_NG, ...: equal shares), which keeps P3_NG
+// short dependent load chains in flight per workgroup; the groups' partial
+// sums are added in LDS in group order, so the result is deterministic.
+// Lanes whose outputs fall past the band accumulate values that are never
+// stored.
+#ifndef P3_NG
+#define P3_NG 16
+#endif
 template <int S, class T>
-__global__ void __launch_bounds__(256) p3_near(const T *const *Rn, const int *blk_term, const int *cb_ptr,
-                                               const int *cb_blk, const int2 *nr,
-                                               const typename p3_cx<T>::type *Ym, zd *Z, int C, int N, int nchunk,
-                                               const int *state)
+__global__ void __launch_bounds__(64 * P3_NG) p3_near(const T *const *Rn, const int *blk_term, const int *cb_ptr,
+                                                      const int *cb_blk, const int2 *nr,
+                                                      const typename p3_cx<T>::type *Ym, zd *Z, int C, int N,
+                                                      int nchunk, const int *state)
 {
   typedef typename p3_cx<T>::type ZT;
   if (state[0]) return;
   constexpr int BW = 2 * S + 1, WIN = 4 + 2 * S;
   __shared__ const T *sR[P3_MAXB];
   __shared__ long sY[P3_MAXB];
-  __shared__ ZT part[P3_NG4][64 * 4];
+  __shared__ ZT part[P3_NG][64 * 4];
   const int chunk = blockIdx.x % nchunk, i1 = blockIdx.x / nchunk;
   const long NN = (long)N * N;
   const int b0 = cb_ptr[chunk], nb = cb_ptr[chunk + 1] - b0;
-  for (int k = threadIdx.x; k < nb; k += 256) {
+  for (int k = threadIdx.x; k < nb; k += 64 * P3_NG) {
     const int b = cb_blk[b0 + k];
     sR[k] = Rn[blk_term[b]] + (size_t)i1 * BW * P3_NW * BW;
     sY[k] = (long)b * NN * P3_CHUNK;
@@ -552,43 +557,42 @@ __global__ void __launch_bounds__(256) p3_near(const T *const *Rn, const int *bl
   const int2 r = nr[i1];
   const int w0 = 4 * wq, p0 = r.x + w0 - S;     // first input position of this lane's window
   const bool any = r.x + w0 <= r.y;
+  // tap rows with j1 = i1 + d1 - S inside the grid
+  const int d1lo = S - i1 > 0 ? S - i1 : 0, d1hi = N - 1 - i1 + S < BW - 1 ? N - 1 - i1 + S : BW - 1;
+  const int nd = d1hi - d1lo + 1;
   ZT acc[4];
 #pragma unroll
   for (int t = 0; t < 4; t++) acc[t] = p3_czero<ZT>();
   // this wavefront's copy of R[d1][.][.] for the current (block, d1): the
   // coefficients are shared by the 16 channel lanes, so one coalesced load
   // per lane fills it and the taps read LDS instead of issuing 52 loads each
-  __shared__ T rs[P3_NG4][P3_NW * BW];
+  __shared__ T rs[P3_NG][P3_NW * BW];
   T *rw = rs[grp];
   {
     bool pv[WIN];
 #pragma unroll
     for (int p = 0; p < WIN; p++) pv[p] = any && p0 + p >= 0 && p0 + p < N;
-    for (int k = grp; k < nb; k += P3_NG4) {
-      const T *R = sR[k];
-      const ZT *Yb = Ym + sY[k] + cl + (long)p0 * P3_CHUNK;
-      for (int d1 = 0; d1 < BW; d1++) {
-        const int j1 = i1 + d1 - S;
-        if (j1 < 0 || j1 >= N) continue;
-        const T *Rd = R + (size_t)d1 * P3_NW * BW;
-        for (int e = lane; e < P3_NW * BW; e += 64) rw[e] = Rd[e];
-        const ZT *Yr = Yb + (long)j1 * N * P3_CHUNK;
-        ZT y[WIN];
+    for (int it = grp; it < nb * nd; it += P3_NG) {
+      const int k = it / nd, d1 = d1lo + it % nd;
+      const int j1 = i1 + d1 - S;
+      const T *Rd = sR[k] + (size_t)d1 * P3_NW * BW;
+      for (int e = lane; e < P3_NW * BW; e += 64) rw[e] = Rd[e];
+      const ZT *Yr = Ym + sY[k] + cl + (long)p0 * P3_CHUNK + (long)j1 * N * P3_CHUNK;
+      ZT y[WIN];
 #pragma unroll
-        for (int p = 0; p < WIN; p++) y[p] = pv[p] ? Yr[(long)p * P3_CHUNK] : p3_czero<ZT>();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const T *Rr = rw + (size_t)w0 * BW;
+      for (int p = 0; p < WIN; p++) y[p] = pv[p] ? Yr[(long)p * P3_CHUNK] : p3_czero<ZT>();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const T *Rr = rw + (size_t)w0 * BW;
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-          ZT a = p3_czero<ZT>();
+      for (int t = 0; t < 4; t++) {
+        ZT a = p3_czero<ZT>();
 #pragma unroll
-          for (int d2 = 0; d2 < BW; d2++) p3_cfmar(a, Rr[t * BW + d2], y[t + d2]);
-          acc[t] = p3_cadd(acc[t], a);
-        }
-        __builtin_amdgcn_wave_barrier();
+        for (int d2 = 0; d2 < BW; d2++) p3_cfmar(a, Rr[t * BW + d2], y[t + d2]);
+        acc[t] = p3_cadd(acc[t], a);
       }
+      __builtin_amdgcn_wave_barrier();
     }
   }
 #pragma unroll
@@ -602,7 +606,7 @@ __global__ void __launch_bounds__(256) p3_near(const T *const *Rn, const int *bl
     if (c >= C || i2 > r.y) continue;
     ZT v = part[0][lane * 4 + t];
 #pragma unroll
-    for (int g = 1; g < P3_NG4; g++) v = p3_cadd(v, part[g][lane * 4 + t]);
+    for (int g = 1; g < P3_NG; g++) v = p3_cadd(v, part[g][lane * 4 + t]);
     const long o = (long)c * NN + (long)i1 * N + i2;
     Z[o] = zadd(Z[o], p3_ctod(v));
   }
@@ -1283,7 +1287,7 @@ static int p3_apply_raw(pp_pamp3 *p, const zd *y, zd *z, const double *scal, hip
       const size_t lds = sizeof(float2) * (size_t)(N + 12) * P3_CHUNK;
       hipLaunchKernelGGL((p3_mixT<6, float>), dim3(p->nblk * N), dim3(256), lds, st, p->mrow, p->ment, y, Ym, p->fPt,
                          p->fQt, p->blk_term, p->nr, Tq, Tp, N, scal, p->state);
-      hipLaunchKernelGGL((p3_near<6, float>), dim3(p->nchunk * N), dim3(256), 0, st, p->fRnt, p->blk_term,
+      hipLaunchKernelGGL((p3_near<6, float>), dim3(p->nchunk * N), dim3(64 * P3_NG), 0, st, p->fRnt, p->blk_term,
                          p->cb_ptr, p->cb_blk, p->nr, Ym, z, C, N, p->nchunk, p->state);
       hipLaunchKernelGGL((p3_farL<6, float>), dim3(p->nchunk * ntile * ngrp), dim3(256), 0, st, p->fPt, p->fQt,
                          p->blk_term, p->cb_ptr, p->cb_blk, p->nr, Tq, Tp, z, C, N, p->nchunk, p->state);
@@ -1291,7 +1295,7 @@ static int p3_apply_raw(pp_pamp3 *p, const zd *y, zd *z, const double *scal, hip
       const size_t lds = sizeof(zd) * (size_t)(N + 12) * P3_CHUNK;
       hipLaunchKernelGGL((p3_mixT<6, double>), dim3(p->nblk * N), dim3(256), lds, st, p->mrow, p->ment, y, p->Ym,
                          p->dPt, p->dQt, p->blk_term, p->nr, p->Tq, p->Tp, N, scal, p->state);
-      hipLaunchKernelGGL((p3_near<6, double>), dim3(p->nchunk * N), dim3(256), 0, st, p->dRn, p->blk_term,
+      hipLaunchKernelGGL((p3_near<6, double>), dim3(p->nchunk * N), dim3(64 * P3_NG), 0, st, p->dRn, p->blk_term,
                          p->cb_ptr, p->cb_blk, p->nr, p->Ym, z, C, N, p->nchunk, p->state);
       hipLaunchKernelGGL((p3_farL<6, double>), dim3(p->nchunk * ntile * ngrp), dim3(256), 0, st, p->dPt, p->dQt,
                          p->blk_term, p->cb_ptr, p->cb_blk, p->nr, p->Tq, p->Tp, z, C, N, p->nchunk, p->state);
