@@ -48,7 +48,8 @@ int pp_band_cholesky_upper(int N, int kd, double *ab);
 /* OverlapMatrix::Setup (overlapmatrix.cpp:9-104): S in upper band storage on
  * the host, factorised and uploaded with both solve plans.  *info = zpbtrf
  * info (the reference leaves it unchecked; here info>0 returns 1).
- * Supported: kd in {4,5,6,8} (B-spline orders 5,6,7,9), N <= 1024. */
+ * Supported: 2 <= kd <= 8 (B-spline orders 3 to 9); N up to 64 chunks of the
+ * solver (N <= 16384 with the default chunk choice). */
 int pp_overlap_create(int N, int kd, const double *S_upper, pp_overlap **out, int *info);
 /* Same with an explicit solver chunk length B (power of two >= max(kd, 8),
  * at most 64 chunks; 0 = default).  No reference counterpart: the reference's

@@ -232,12 +232,19 @@ inline int pp_plan_chunk(int N, int kd)
   return 0;
 }
 
-// Dispatch helper over the supported super-diagonal counts (B-spline order-1).
+// Dispatch helper over the supported super-diagonal counts kd = order - 1:
+// B-spline orders 3 to 9 (every order the reference's example configs use,
+// and cubic splines).
+#define PP_KD_MIN 2
+#define PP_KD_MAX 8
 #define PP_KD_SWITCH(kd, ...) \
   switch (kd) {                      \
+    case 2: { constexpr int KD_ = 2; __VA_ARGS__; } break; \
+    case 3: { constexpr int KD_ = 3; __VA_ARGS__; } break; \
     case 4: { constexpr int KD_ = 4; __VA_ARGS__; } break; \
     case 5: { constexpr int KD_ = 5; __VA_ARGS__; } break; \
     case 6: { constexpr int KD_ = 6; __VA_ARGS__; } break; \
+    case 7: { constexpr int KD_ = 7; __VA_ARGS__; } break; \
     case 8: { constexpr int KD_ = 8; __VA_ARGS__; } break; \
     default: break; }
 

@@ -165,7 +165,7 @@ extern "C" int pp_overlap_create(int N, int kd, const double *S_upper, pp_overla
 extern "C" int pp_overlap_create_chunked(int N, int kd, const double *S_upper, int B, pp_overlap **out, int *info)
 {
   if (!out || !S_upper || N < 1 || kd < 1 || B < 0) return PP_EARG;
-  if (!(kd == 4 || kd == 5 || kd == 6 || kd == 8)) return PP_EUNSUP;
+  if (kd < PP_KD_MIN || kd > PP_KD_MAX) return PP_EUNSUP;
   if (B == 0) B = pp_plan_chunk(N, kd);
   if (B == 0 || B < kd || B < 8 || (B & (B - 1)) || (N + B - 1) / B > 64) return PP_EUNSUP;
   std::vector<double> ab(S_upper, S_upper + (size_t)2 * N * (kd + 1));

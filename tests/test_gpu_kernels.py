@@ -12,14 +12,17 @@ except ImportError:  # pragma: no cover
 pytestmark = pytest.mark.gpu
 
 
-def _hydrogen(lmax=6, nbp=40, rmax=40.0):
-    return O.Hydrogen(rmax=rmax, nbp=nbp, order=7, lmax=lmax, frequency=0.057, cycles=6, E0=0.05)
+def _hydrogen(lmax=6, nbp=40, rmax=40.0, order=7):
+    return O.Hydrogen(rmax=rmax, nbp=nbp, order=order, lmax=lmax, frequency=0.057, cycles=6, E0=0.05)
 
 
-def test_overlap_solves_match_oracle():
+@pytest.mark.parametrize("order", [3, 4, 5, 6, 7, 8, 9])
+def test_overlap_solves_match_oracle(order):
+    """U^-1, U^-H and U x on the B-spline rank for every supported order
+    (kd = order - 1 = 2..8) against the oracle's ztbsv / ztbmv restatement."""
     from pyprop_amd import device as D
     torch = D.torch()
-    h = _hydrogen()
+    h = _hydrogen(order=order)
     ov = D.Overlap(h.S, h.N, h.kd)
     rng = np.random.default_rng(1)
     x = rng.standard_normal(h.shape) + 1j * rng.standard_normal(h.shape)
@@ -128,6 +131,46 @@ def test_pamp_persistent_matches_oracle(lmax, nbp, m, nsteps, dt, reorth):
     err = np.abs(got - ref).max()
     print("pamp", lmax, nbp, m, nsteps, dt, reorth, "err", err, "tol", tol, eng.diagnostics()[-1])
     assert err <= tol
+
+
+@pytest.mark.parametrize("order", [3, 4, 8])
+def test_pamp_other_spline_orders_match_oracle(order):
+    """pAMP steps (fiber-split and one-workgroup-per-fiber kernels) and an
+    Expokit step on B-splines of order 3, 4 and 8 (kd = 2, 3, 7: the band
+    width of the operator and of the overlap factor change with the order)
+    against the oracle; same sensitivity bar as the order-7 tests."""
+    from pyprop_amd import device as D
+    h = _hydrogen(lmax=6, nbp=40, rmax=40.0, order=order)
+    psi0 = h.ground_state()
+    psi0[1] = 0.1 * psi0[0]
+    nsteps, dt, m = 5, 0.05, 12
+    ts = O.step_times(nsteps, dt, t0=10.0)
+    sc = np.array([[1.0, h.scaling(t)] for t in ts])
+    ham = h.ham()
+    ref = ham.propagate_pamp(psi0, nsteps, dt, sc, m)
+    rng = np.random.default_rng(13)
+    ref2 = ham.propagate_pamp(psi0 * (1 + 1e-16 * rng.standard_normal(psi0.shape)), nsteps, dt, sc, m)
+    tol = max(1e-12, 10 * np.abs(ref2 - ref).max())
+    ov = D.Overlap(h.S, h.N, h.kd)
+    for split in (0, 1):
+        eng = D.PampEngine(h.L, h.N, m, ov)
+        eng.add_term("Diag", D.to_device(h.H0))
+        eng.add_term("Simp", D.to_device(h.Hint), pairs=h.pairs)
+        eng.set_fiber_split(split)
+        psi = D.to_device(psi0)
+        eng.propagate(psi, nsteps, dt, sc)
+        err = np.abs(psi.cpu().numpy() - ref).max()
+        print("order", order, "split", split, "pAMP err", err, "tol", tol, flush=True)
+        assert err <= tol
+    eref, _ = ham.propagate_expokit(psi0, 1, dt, sc[:1], m)
+    eng = D.PampEngine(h.L, h.N, m, ov)
+    eng.add_term("Diag", D.to_device(h.H0))
+    eng.add_term("Simp", D.to_device(h.Hint), pairs=h.pairs)
+    psi = D.to_device(psi0)
+    eng.propagate_expokit(psi, 1, dt, sc[:1])
+    err = np.abs(psi.cpu().numpy() - eref).max() / np.abs(eref).max()
+    print("order", order, "Expokit rel err", err, flush=True)
+    assert err <= 1e-12
 
 
 @pytest.mark.parametrize("lmax,nbp,m,nsteps,dt,split", [
