@@ -302,6 +302,19 @@ int pp_band_lu_solve(int nfib, int N, int kl, int ku, const void *lub, const int
  * V1 [(V1^T X V2) ./ (b + a (lam1_i + lam2_k))] V2^T per channel,
  * asynchronously on `stream`; solves that overlap on one handle race (one
  * intermediate buffer per handle). */
+/* The B-spline rank's projection step of tensor-potential generation
+ * (TensorPotentialGenerator.GeneratePotential, pyprop/tensorpotential/
+ * TensorPotentialGenerator.py:72-256, with basis_bspline.h's
+ * <B_i| f d^d |B_j>): out[r][e] = sum_q MT[e][q] f[r][q] for rows x G
+ * weighted quadrature samples f (device complex128), MT the sparse map to
+ * the band pairs as host CSR (ptr nout+1, idx/val nnz, copied at create).
+ * Selected by the `generate_on_device = True` key of a potential section
+ * (no reference counterpart: the reference projects on the host). */
+typedef struct pp_projector pp_projector;
+int pp_projector_create(int nout, int G, const int *ptr, const int *idx, const double *val, pp_projector **out);
+int pp_projector_apply(const pp_projector *p, long rows, const void *f, void *out, pp_stream stream);
+int pp_projector_destroy(pp_projector *p);
+
 typedef struct pp_radial_prec pp_radial_prec;
 int pp_radial_prec_create(int C, int N, int nmat, const int *idx1, const int *idx2, const double *V,
                           const double *lam, pp_radial_prec **out);

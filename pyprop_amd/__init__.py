@@ -20,6 +20,6 @@ from .solver import PiramSolver, GMRESShiftInvertSolver, BSplinePreconditioner, 
 from .helium import (RadialTwoElectronPreconditioner, RadialTwoElectronPreconditionerSuperLU,  # noqa: F401
                      RadialTwoElectronPreconditionerIfpack)
 from .serialization import (SaveWavefunctionHDF, LoadWavefunctionHDF, LoadConfigFromFile,  # noqa: F401
-                            CreateWavefunctionFromFile)
+                            CreateWavefunctionFromFile, SaveTensorPotential, LoadTensorPotential)
 
 del abs, max, min, round, sum  # keep the builtins for code that star-imports this package

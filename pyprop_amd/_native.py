@@ -36,6 +36,7 @@ EXPORTS = [
     "pp_pamp3_multipole_flops", "pp_pamp3_destroy",
     "pp_radial_prec_create", "pp_radial_prec_solve", "pp_radial_prec_destroy",
     "pp_pamp_set_exponentiation", "pp_pamp3_set_exponentiation",
+    "pp_projector_create", "pp_projector_apply", "pp_projector_destroy",
 ]
 
 _lib = None
@@ -119,6 +120,9 @@ def lib():
     L.pp_radial_prec_destroy.argtypes = [P]
     L.pp_pamp_set_exponentiation.argtypes = [P, I]
     L.pp_pamp3_set_exponentiation.argtypes = [P, I]
+    L.pp_projector_create.argtypes = [I, I, ip, ip, dp, ctypes.POINTER(P)]
+    L.pp_projector_apply.argtypes = [P, ctypes.c_long, P, P, P]
+    L.pp_projector_destroy.argtypes = [P]
     for name in EXPORTS:
         getattr(L, name).restype = ctypes.c_int
     L.pp_pamp_lds_estimate.restype = ctypes.c_long

@@ -226,18 +226,27 @@ class BSplineBasis:
         cache[d] = M
         return M
 
-    def project_bandnh(self, f, d=0):
+    def project_bandnh(self, f, d=0, device=False):
         """<B_i| f d^d |B_j> in BandNH storage, shape (..., N*(2k-1)).
 
         f: samples on the global quadrature grid, shape (..., G); element
         (i,j) is stored at j*(2k-1) + (k-1)+i-j (GeometryInfo.py:433-454).
         For i > j the reference integrates the transposed pair and multiplies
-        by (-1)^d (bspline.cpp:380-392)."""
+        by (-1)^d (bspline.cpp:380-392).  device: the sparse projection runs
+        on the GPU (pp_projector_apply, the same sums) and the result comes
+        back as a numpy array."""
         f = np.asarray(f)
         lead = f.shape[:-1]
         fw = (f * self.QuadratureWeightsGlobal).reshape(-1, f.shape[-1])
         M = self._projector(d)
-        out = np.asarray((M.T @ fw.T).T)
+        if device:
+            from . import device as dev
+            cache = self.__dict__.setdefault("_dev_proj", {})
+            if d not in cache:
+                cache[d] = dev.Projector(M.T.tocsr())
+            out = cache[d].apply(np.ascontiguousarray(fw, dtype=np.complex128))
+        else:
+            out = np.asarray((M.T @ fw.T).T)
         return out.reshape(lead + (M.shape[1],))
 
     def overlap_upper(self):

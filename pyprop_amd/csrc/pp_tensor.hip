@@ -186,3 +186,86 @@ extern "C" int pp_tp_multiply(int R, const int *storage, const int *potShape, co
   PP_HIP(hipFreeAsync(dev, stream));
   return PP_OK;
 }
+
+// ---------------------------------------------------------------- potential projection
+// The B-spline rank's step of TensorPotentialGenerator.GeneratePotential
+// (TensorPotentialGenerator.py:72-256 with basis_bspline.h's projection):
+// out[r][e] = sum_q M^T[e][q] f[r][q] for every row r of the grid-evaluated
+// potential, with M^T the sparse map from quadrature samples to the band
+// pairs (<B_i| f d^d |B_j> = sum_q w_q B_i(q) f(q) d^d B_j(q) over the
+// quadrature points the two supports share).  CSR rows of M^T in ascending
+// q, so a pair sums its points in the same order on every run.
+struct pp_projector {
+  int nout, G;
+  int *ptr, *idx;    // device CSR of M^T (nout + 1, nnz)
+  double *val;       // device, nnz
+};
+
+// one thread per (row, pair); consecutive threads take consecutive pairs of a
+// row, whose point ranges overlap, so the row's samples come from L1 / L2
+__global__ void __launch_bounds__(256) pp_project_kernel(pp_projector pj, long rows, const zd *__restrict__ f,
+                                                         zd *__restrict__ out)
+{
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= rows * pj.nout) return;
+  const long r = t / pj.nout;
+  const int e = (int)(t % pj.nout);
+  const zd *fr = f + r * pj.G;
+  zd acc = zmk(0.0, 0.0);
+  for (int k = pj.ptr[e]; k < pj.ptr[e + 1]; k++) zfmar(acc, pj.val[k], fr[pj.idx[k]]);
+  out[t] = acc;
+}
+
+// CSR of M^T (host arrays: ptr nout+1, idx/val nnz; idx in [0, G)); copied.
+extern "C" int pp_projector_create(int nout, int G, const int *ptr, const int *idx, const double *val,
+                                   pp_projector **out)
+{
+  if (!out) return PP_EARG;
+  *out = nullptr;
+  if (nout < 1 || G < 1 || !ptr || !idx || !val || ptr[0] != 0) return PP_EARG;
+  const int nnz = ptr[nout];
+  for (int e = 0; e < nout; e++)
+    if (ptr[e + 1] < ptr[e]) return PP_EARG;
+  for (int k = 0; k < nnz; k++)
+    if (idx[k] < 0 || idx[k] >= G) return PP_EARG;
+  pp_projector *p = new pp_projector();
+  p->nout = nout; p->G = G;
+  p->ptr = p->idx = nullptr;
+  p->val = nullptr;
+  hipError_t e = hipMalloc((void **)&p->ptr, sizeof(int) * (nout + 1));
+  if (e == hipSuccess) e = hipMalloc((void **)&p->idx, sizeof(int) * (nnz > 0 ? nnz : 1));
+  if (e == hipSuccess) e = hipMalloc((void **)&p->val, sizeof(double) * (nnz > 0 ? nnz : 1));
+  if (e == hipSuccess) e = hipMemcpy(p->ptr, ptr, sizeof(int) * (nout + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess && nnz > 0) e = hipMemcpy(p->idx, idx, sizeof(int) * nnz, hipMemcpyHostToDevice);
+  if (e == hipSuccess && nnz > 0) e = hipMemcpy(p->val, val, sizeof(double) * nnz, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(p->ptr); (void)hipFree(p->idx); (void)hipFree(p->val);
+    delete p;
+    return PP_EHIP;
+  }
+  *out = p;
+  return PP_OK;
+}
+
+// out (rows x nout complex128, device) = f (rows x G complex128, device,
+// the samples already multiplied by the quadrature weights) projected.
+extern "C" int pp_projector_apply(const pp_projector *p, long rows, const void *f, void *out, hipStream_t stream)
+{
+  if (!p || rows < 0 || (!f && rows > 0) || (!out && rows > 0)) return PP_EARG;
+  const long n = rows * p->nout;
+  if (n == 0) return PP_OK;
+  if ((n + 255) / 256 > 0x7fffffffL) return PP_EARG;
+  hipLaunchKernelGGL(pp_project_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *p, rows,
+                     (const zd *)f, (zd *)out);
+  return hipGetLastError() == hipSuccess ? PP_OK : PP_EHIP;
+}
+
+extern "C" int pp_projector_destroy(pp_projector *p)
+{
+  if (!p) return PP_OK;
+  (void)hipFree(p->ptr);
+  (void)hipFree(p->idx);
+  (void)hipFree(p->val);
+  delete p;
+  return PP_OK;
+}

@@ -316,3 +316,37 @@ def pamp_batch_propagate(engines, psis, nsteps, dts, scalings, check=True):
     if check:
         for i, e in enumerate(engines):
             check_pamp_steps(e.diagnostics(nsteps), "pAMP (batch instance %d)" % i)
+
+
+class Projector:
+    """pp_projector_*: the sparse map from weighted quadrature samples to the
+    band pairs of one B-spline rank, applied on the device."""
+
+    def __init__(self, MT):
+        MT = MT.tocsr()
+        MT.sort_indices()
+        ptr = np.ascontiguousarray(MT.indptr, dtype=np.int32)
+        idx = np.ascontiguousarray(MT.indices, dtype=np.int32)
+        val = np.ascontiguousarray(MT.data, dtype=np.float64)
+        if np.iscomplexobj(MT.data):
+            raise ValueError("the projection map is real")
+        self.nout, self.G = MT.shape
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().pp_projector_create(self.nout, self.G, nat.iptr(ptr), nat.iptr(idx), nat.dptr(val),
+                                                ctypes.byref(h)), "pp_projector_create")
+        self.h = h
+        self._token = nat.track("pp_projector_destroy", h)
+
+    def __del__(self, _release=_release):
+        _release(self)
+
+    def apply(self, fw):
+        """fw (rows x G complex128, host): the weighted samples; returns
+        (rows x nout) complex128 on the host."""
+        t = torch()
+        fw = np.ascontiguousarray(fw, dtype=np.complex128).reshape(-1, self.G)
+        src = to_device(fw)
+        out = t.empty((fw.shape[0], self.nout), dtype=t.complex128, device=device())
+        nat.check(nat.lib().pp_projector_apply(self.h, int(fw.shape[0]), ptr(src), ptr(out), stream_ptr()),
+                  "pp_projector_apply")
+        return out.cpu().numpy()

@@ -160,7 +160,13 @@ class CoupledTensorPotential:
                 raise NotImplementedError("radial geometry %s is not on the rank-3 engine" % g)
         self.Pairs = np.array(cp.basis_pairs(self.Channels, g0), dtype=np.int32).reshape(-1, 2)
         self.GeometryNames = [g0, conf.geometry1, conf.geometry2]
-        self.Terms = _evaluate(conf, self.Channels, self.Pairs, self.Basis, repr.RadialMatrices())
+        if hasattr(conf, "filename"):
+            # TensorPotential.py:56-71: the terms come from the file, checked
+            # against the angular pairs of the current representation
+            from .serialization import LoadTensorPotential
+            LoadTensorPotential(conf.filename, conf.dataset, self)
+        else:
+            self.Terms = _evaluate(conf, self.Channels, self.Pairs, self.Basis, repr.RadialMatrices())
 
     def GetTimeScaling(self, t):
         return float(np.real(self.TimeFunction(t))) if self.IsTimeDependent else 1.0

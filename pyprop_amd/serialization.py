@@ -131,3 +131,122 @@ def CreateWavefunctionFromFile(filename, datasetPath="/wavefunction"):
     psi = CreateWavefunction(conf)
     LoadWavefunctionHDF(filename, datasetPath, psi)
     return psi
+
+
+# ---------------------------------------------------------------- tensor potentials
+_REGENERATE = ("Organization of basis pairs has changed from when this potential was generated. "
+               "Please regenerate potential")
+
+
+def _group(groupPath):
+    if str(groupPath).strip() == "/":
+        raise Exception("Storing tensor potentials on '/' is a bad idea (trust us, it is)!")
+    return _key(groupPath)
+
+
+def _drop(members, key):
+    for name in [n for n in members if n == key or n.startswith(key + "/")]:
+        del members[name]
+
+
+def _factored(potential):
+    return hasattr(potential, "Terms") and not hasattr(potential, "GeometryList")
+
+
+def SaveTensorPotential(filename, groupPath, potential, distributedModel=None, conf=None):
+    """Store a tensor potential under groupPath (TensorPotentialHDF.py:10-80):
+    `<group>/potential` the full data array (complex128, the reference
+    layout: one axis per rank over that rank's basis pairs), and per rank
+    `<group>/basisPairs<i>` with its `storageId<i>`, plus the config object.
+    A two-electron potential held as factored terms is stored as its terms
+    (`<group>/term<k>/...`: angular pairs, coefficients and the radial band
+    matrices or multipole tensor) with the angular pairs as basisPairs0.
+    The whole array is stored: one GPU holds the whole wavefunction, so the
+    distributed model's slabs are the full extents."""
+    key = _group(groupPath)
+    members = _read_members(filename)
+    _drop(members, key)
+    members["__format__"] = np.array(FORMAT)
+    if _factored(potential):
+        members[key + "/kind"] = np.array("factored")
+        members[key + "/basisPairs0"] = np.ascontiguousarray(potential.Pairs, dtype=np.int64)
+        members[key + "/name"] = np.array(potential.Name or "")
+        for k, t in enumerate(potential.Terms):
+            tk = "%s/term%d" % (key, k)
+            members[tk + "/pairs"] = np.ascontiguousarray(t.pairs, dtype=np.int32)
+            members[tk + "/coef"] = np.ascontiguousarray(t.coef, dtype=np.complex128)
+            if hasattr(t, "R"):
+                members[tk + "/type"] = np.array("multipole")
+                members[tk + "/l"] = np.array(int(t.l))
+                members[tk + "/R"] = np.ascontiguousarray(t.R, dtype=np.float64)
+                if t.P is not None:
+                    members[tk + "/P"] = np.ascontiguousarray(t.P, dtype=np.float64)
+                    members[tk + "/Q"] = np.ascontiguousarray(t.Q, dtype=np.float64)
+            else:
+                members[tk + "/type"] = np.array("kron")
+                members[tk + "/A"] = np.ascontiguousarray(t.A, dtype=np.float64)
+                members[tk + "/B"] = np.ascontiguousarray(t.B, dtype=np.float64)
+    else:
+        data = np.asarray(potential.PotentialData)
+        shape = tuple(len(g.GetBasisPairs()) for g in potential.GeometryList)
+        if tuple(data.shape) != shape:
+            raise ValueError("potential data %s does not match its basis pairs %s" % (data.shape, shape))
+        members[key + "/kind"] = np.array("tensor")
+        members[key + "/potential"] = np.ascontiguousarray(data, dtype=np.complex128)
+        for r, g in enumerate(potential.GeometryList):
+            members["%s/basisPairs%d" % (key, r)] = np.ascontiguousarray(g.GetBasisPairs(), dtype=np.int64)
+            members["%s/storageId%d" % (key, r)] = np.array(g.GetStorageId())
+    members[key + "/configObject"] = np.array(_config_text(conf))
+    _write_members(filename, members)
+
+
+def _check_pairs(members, key, rank, current):
+    """CheckLocalSlab (TensorPotentialHDF.py:131-148): the stored basis pairs
+    of every rank equal the ones the current representation generates."""
+    for r in range(rank):
+        stored = members.get("%s/basisPairs%d" % (key, r))
+        now = np.asarray(current[r])
+        if stored is None or stored.shape != now.shape or not np.array_equal(stored, now):
+            raise Exception(_REGENERATE)
+
+
+def LoadTensorPotential(filename, groupPath, potential, distributedModel=None):
+    """Fill a set-up potential from groupPath (TensorPotentialHDF.py:83-128):
+    the stored basis pairs must equal the current ones (else the reference's
+    "regenerate potential" error); the data (or the factored terms) replace
+    the potential's own."""
+    key = _group(groupPath)
+    members = _read_members(_filename(filename))
+    if key + "/kind" not in members:
+        raise KeyError("no tensor potential %s in %s" % (groupPath, _filename(filename)))
+    kind = str(members[key + "/kind"])
+    if _factored(potential):
+        if kind != "factored":
+            raise ValueError("%s holds a tensor-layout potential, not factored terms" % groupPath)
+        _check_pairs(members, key, 1, [potential.Pairs])
+        from .helium import KronTerm, MultipoleTerm
+        terms, k = [], 0
+        while "%s/term%d/type" % (key, k) in members:
+            tk = "%s/term%d" % (key, k)
+            if str(members[tk + "/type"]) == "kron":
+                terms.append(KronTerm(members[tk + "/pairs"], members[tk + "/coef"], members[tk + "/A"],
+                                      members[tk + "/B"]))
+            else:
+                terms.append(MultipoleTerm(members[tk + "/pairs"], members[tk + "/coef"], int(members[tk + "/l"]),
+                                           members[tk + "/R"], members.get(tk + "/P"), members.get(tk + "/Q")))
+            k += 1
+        potential.Terms = terms
+        return
+    if kind != "tensor":
+        raise ValueError("%s holds factored terms, not a tensor-layout potential" % groupPath)
+    geoms = potential.GeometryList
+    _check_pairs(members, key, len(geoms), [g.GetBasisPairs() for g in geoms])
+    for r, g in enumerate(geoms):
+        sid = members.get("%s/storageId%d" % (key, r))
+        if sid is not None and str(sid) != g.GetStorageId():
+            raise Exception(_REGENERATE)
+    data = members[key + "/potential"]
+    if potential.PotentialData is not None and tuple(np.shape(potential.PotentialData)) != tuple(data.shape):
+        raise ValueError("Invalid shape on loaded potential, got %s expected %s"
+                         % (data.shape, np.shape(potential.PotentialData)))
+    potential.PotentialData = np.ascontiguousarray(data, dtype=np.complex128)

@@ -211,15 +211,16 @@ class BasisfunctionBSpline:
             return GeometryInfoCommonDense(N, True)
         raise UnsupportedGeometryException("Geometry '%s' not supported by BasisfunctionBSpline" % geometryName)
 
-    def RepresentPotentialInBasis(self, source, rank, geometryInfo, differentiation):
+    def RepresentPotentialInBasis(self, source, rank, geometryInfo, differentiation, device=False):
         """source: potential with the grid on axis `rank`; returns the array with
-        that axis replaced by the basis pairs (basis_bspline.h:11-47)."""
+        that axis replaced by the basis pairs (basis_bspline.h:11-47);
+        device: the projection runs on the GPU."""
         sid = geometryInfo.GetStorageId()
         src = np.moveaxis(source, rank, -1)
         if sid == "Ident":
             out = src[..., :1]
             return np.moveaxis(out, -1, rank)
-        band = self.BSplineObject.project_bandnh(src, differentiation)
+        band = self.BSplineObject.project_bandnh(src, differentiation, device=device)
         if sid == "BandNH":
             out = band
         else:
@@ -309,11 +310,18 @@ class TensorPotentialGenerator:
         data = np.broadcast_to(data, tuple(len(g) for g in grids)).copy()
         if getattr(configSection, "debug_potential", False):
             self.OriginalPotential = data.copy()
+        # extension key (no reference counterpart): generate_on_device = True
+        # projects the B-spline ranks on the GPU (pp_projector_apply)
+        device = bool(getattr(configSection, "generate_on_device", False))
         for rank in reversed(range(self.Rank)):
             g = geometryList[rank]
             if g.UseGridRepresentation():
                 d = getattr(configSection, "differentiation%i" % rank, 0)
-                data = self.BasisList[rank].RepresentPotentialInBasis(data, rank, g, d)
+                basis = self.BasisList[rank]
+                if device and isinstance(basis, BasisfunctionBSpline):
+                    data = basis.RepresentPotentialInBasis(data, rank, g, d, device=True)
+                else:
+                    data = basis.RepresentPotentialInBasis(data, rank, g, d)
         return np.ascontiguousarray(data, dtype=np.complex128)
 
 
@@ -340,9 +348,14 @@ class TensorPotential:
         self.GeometryList = generator.GetGeometryList(configSection)
         self.Name = configSection.name
         if hasattr(configSection, "filename"):
-            raise NotImplementedError("loading tensor potentials from HDF5 is not on this path")
-        self.PotentialData = generator.GeneratePotential(configSection)
-        self.OriginalPotential = getattr(generator, "OriginalPotential", None)
+            # TensorPotential.py:56-71: load the data instead of generating it
+            from .serialization import LoadTensorPotential
+            shape = tuple(len(g.GetBasisPairs()) for g in self.GeometryList)
+            self.PotentialData = np.zeros(shape, dtype=np.complex128)
+            LoadTensorPotential(configSection.filename, configSection.dataset, self)
+        else:
+            self.PotentialData = generator.GeneratePotential(configSection)
+            self.OriginalPotential = getattr(generator, "OriginalPotential", None)
         self._device = None
 
     def StorageIds(self):

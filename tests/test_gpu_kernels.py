@@ -679,3 +679,31 @@ def test_checkpoint_resume_is_bitwise(tmp_path):
     assert np.array_equal(prob2.psi.GetData().cpu().numpy(), ref)
     psi3 = pyprop_amd.CreateWavefunctionFromFile(f, "/wavefunction")
     assert tuple(psi3.GetData().shape) == ref.shape
+
+
+def test_potential_generation_on_device_matches_host():
+    """generate_on_device = True (the B-spline ranks projected by
+    pp_projector_apply) against the host projection for every potential of
+    the hydrogen_small config (kinetic d = 2, SAE, the velocity-gauge d = 1
+    and 1/r parts): relative 1e-14, and the consolidated list the same."""
+    import pyprop_amd as P
+    from pyprop_amd.core import Wavefunction
+    from pyprop_amd.examples import hydrogen as H
+    from pyprop_amd.problem import CreateRepresentation
+    conf = H.load("hydrogen_small")
+    bp = P.BasisPropagator(Wavefunction(CreateRepresentation(conf)))
+    conf.Apply(bp)
+    confd = H.load("hydrogen_small")
+    for n in confd.Propagation.grid_potential_list:
+        confd.SetValue(n, "generate_on_device", True)
+    bpd = P.BasisPropagator(Wavefunction(CreateRepresentation(confd)))
+    for n in confd.Propagation.grid_potential_list:
+        a = bp.GeneratePotential(conf.GetSection(n)).PotentialData
+        b = bpd.GeneratePotential(confd.GetSection(n)).PotentialData
+        err = np.abs(a - b).max() / max(np.abs(a).max(), 1e-300)
+        print("potential", n, "rel err %.2e" % err, flush=True)
+        assert err <= 1e-14
+    confd.Apply(bpd)
+    for a, b in zip(bp.PotentialList, bpd.PotentialList):
+        assert a.Name == b.Name
+        assert np.abs(a.PotentialData - b.PotentialData).max() <= 1e-14 * np.abs(a.PotentialData).max()

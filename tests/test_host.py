@@ -317,3 +317,63 @@ def test_radial_preconditioner_decomposition_matches_oracle():
     got = np.stack([Hm[idx1[c]] @ x[c] @ S + S @ x[c] @ Hm[idx2[c]] for c in range(C)])
     assert np.abs(got - ref).max() < 1e-11 * np.abs(ref).max()
     assert prec.OmittedTerms and all("ElectronicCouplingPotentialMonopoleTerm" in t for t in prec.OmittedTerms)
+
+
+def test_tensor_potential_files(tmp_path):
+    """SaveTensorPotential / LoadTensorPotential and a potential section's
+    `filename` / `dataset` keys (TensorPotentialHDF.py:10-171,
+    TensorPotential.py:56-71): every hydrogen potential saved and loaded back
+    through the config equals the generated one bitwise, also after
+    consolidation; a potential stored for another radial grid is refused with
+    the reference's "regenerate" error; '/' is refused as a group; the
+    factored two-electron terms of the helium config round-trip bitwise
+    (their reference layout included)."""
+    from pyprop_amd import serialization as SZ
+    f = str(tmp_path / "pots.h5")
+    conf, repr, bp = _assemble("hydrogen_small")
+    names = list(conf.Propagation.grid_potential_list)
+    gen = {}
+    for n in names:
+        pot = bp.GeneratePotential(conf.GetSection(n))
+        gen[n] = pot.PotentialData.copy()
+        SZ.SaveTensorPotential(f, "/potentials/" + n, pot, None, conf)
+    with pytest.raises(Exception, match="bad idea"):
+        SZ.SaveTensorPotential(f, "/", pot, None)
+    conf2 = H.load("hydrogen_small")
+    for n in names:
+        conf2.SetValue(n, "filename", f)
+        conf2.SetValue(n, "dataset", "/potentials/" + n)
+    repr2 = CreateRepresentation(conf2)
+    bp2 = P.BasisPropagator(Wavefunction(repr2))
+    for n in names:
+        pot = bp2.GeneratePotential(conf2.GetSection(n))
+        assert np.array_equal(pot.PotentialData, gen[n])
+    conf2.Apply(bp2)
+    assert [p.Name for p in bp2.PotentialList] == [p.Name for p in bp.PotentialList]
+    for a, b in zip(bp.PotentialList, bp2.PotentialList):
+        assert np.array_equal(a.PotentialData, b.PotentialData)
+    # another grid: the stored radial pairs no longer match
+    conf3 = H.load("hydrogen_small")
+    conf3.SetValue("RadialRepresentation", "xsize", 90)
+    conf3.SetValue("RadialKineticEnergy", "filename", f)
+    conf3.SetValue("RadialKineticEnergy", "dataset", "/potentials/RadialKineticEnergy")
+    bp3 = P.BasisPropagator(Wavefunction(CreateRepresentation(conf3)))
+    with pytest.raises(Exception, match="regenerate potential"):
+        bp3.GeneratePotential(conf3.GetSection("RadialKineticEnergy"))
+    # two-electron potentials: factored terms
+    from pyprop_amd.examples import helium as HE
+    hconf = HE.load("helium_small")
+    hrepr = CreateRepresentation(hconf)
+    hbp = P.BasisPropagator(Wavefunction(hrepr))
+    sec = hconf.GetSection("ElectronicCouplingPotential")
+    ee = hbp.GeneratePotential(sec)
+    SZ.SaveTensorPotential(f, "/helium/ee", ee, None, hconf)
+    hconf.SetValue("ElectronicCouplingPotential", "filename", f)
+    hconf.SetValue("ElectronicCouplingPotential", "dataset", "/helium/ee")
+    ee2 = hbp.GeneratePotential(hconf.GetSection("ElectronicCouplingPotential"))
+    assert len(ee2.Terms) == len(ee.Terms) > 0
+    for a, b in zip(ee.Terms, ee2.Terms):
+        assert type(a) is type(b) and np.array_equal(a.pairs, b.pairs) and np.array_equal(a.coef, b.coef)
+    assert np.array_equal(ee.ReferenceLayout(), ee2.ReferenceLayout())
+    with pytest.raises(ValueError, match="factored"):
+        SZ.LoadTensorPotential(f, "/potentials/RadialKineticEnergy", ee2)
