@@ -16,13 +16,19 @@ __device__ inline zd zld(const zd *p) { return zmk(xld(&p->x), xld(&p->y)); }
 
 // ------------------------------------------------------------------ Pade on one workgroup
 // pAMP::ScalingPadeExponential + PadeExponential (pamp.h:430-564): A = -i dt H
-// scaled by 2^-s with s = max(0, ceil(log2 max|A_ij|)), Pade(6) numerator
-// and denominator by Horner in A^2, X = (Q - P)^-1 P, E = +-(I + 2X), then s
-// squarings; the first column of E goes to cvec_out (exchanged).  Column-major
-// m x m complex matrices in the dynamic LDS at double offset `off`.  The
-// linear solve is Gauss-Jordan with the same partial pivoting as zgesv
-// (largest |re|+|im|, first index on ties) on ping-pong buffers, one barrier
-// per column.
+// scaled by 2^-s with s = max(0, ceil(log2 max|A_ij|)), the Pade(6)
+// numerator and denominator, X = (Q - P)^-1 P, E = I + 2X, then s
+// squarings; the first column of E goes to cvec_out (exchanged).  Column-
+// major m x m complex matrices in the dynamic LDS at double offset `off`.
+// The even and odd parts are formed from the powers A^2, A^4, A^6 (Q = c0 +
+// c2 A^2 + c4 A^4 + c6 A^6, P = A (c1 + c3 A^2 + c5 A^4): four matrix
+// products instead of Horner's seven, the same polynomials).  The linear
+// solve is Gauss-Jordan with zgesv's partial pivoting (largest |re|+|im|,
+// first index on ties) on ping-pong buffers, one barrier per column; the
+// elimination pass also leaves the next column's magnitudes in LDS, so every
+// lane finds the pivot from one array instead of a cross-lane reduction.
+// The last squaring forms only the first column.  Matrix products run on
+// v_mfma_f64_16x16x4f64.
 // normscale: 0 scales by the largest |A_ij| as pAMP does; 1 scales until the
 // 1-norm of A / 2^s is at most 1/2, which keeps the Pade(6) error near
 // rounding and makes the step a smooth function of its inputs (the largest
@@ -31,6 +37,43 @@ __device__ inline zd zld(const zd *p) { return zmk(xld(&p->x), xld(&p->y)); }
 // exponential (eig_block) instead of Pade.
 static __device__ __attribute__((noinline)) void eig_block(int m, const zd *Hg, double dt_re, double dt_im, int off,
                                                     zd *cvec_out, double *diag);
+
+typedef double pp_d4 __attribute__((ext_vector_type(4)));
+
+// Z = X Y (+ c on the diagonal when diag), m x m complex, column-major, LDS:
+// 16 x 16 output tiles, one per wavefront at a time, with the real and
+// imaginary parts accumulated by v_mfma_f64_16x16x4f64 (FP64 products and
+// sums).  Operand lanes: A[i][k] from lane i + 16k, B[k][j] from lane j + 16k;
+// D[4r + l/16][l%16] comes back in lane l, register r.  Ends with a barrier.
+__device__ __attribute__((always_inline)) inline void pade_mm(int m, const zd *X, const zd *Y, zd *Z, double c, bool diag)
+{
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int nt = (m + 15) >> 4;
+  for (int t = wave; t < nt * nt; t += PP_THREADS / 64) {
+    const int ti = t % nt, tj = t / nt;
+    const int i = ti * 16 + li, j = tj * 16 + li;
+    pp_d4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < m; k0 += 4) {
+      const int k = k0 + lk;
+      const zd x = (i < m && k < m) ? X[i + k * m] : zmk(0.0, 0.0);
+      const zd y = (k < m && j < m) ? Y[k + j * m] : zmk(0.0, 0.0);
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.y, ci, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.y, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.x, ci, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = ti * 16 + 4 * r + lk, col = tj * 16 + li;
+      if (row < m && col < m) {
+        zd v = zmk(cr[r], ci[r]);
+        if (diag && row == col) v.x += c;
+        Z[row + col * m] = v;
+      }
+    }
+  }
+  __syncthreads();
+}
 
 static __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg, double dt_re, double dt_im, int off,
                                                      zd *cvec_out, double *diag, int normscale)
@@ -43,11 +86,9 @@ static __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg,
   zd *lds = (zd *)(pp_lds + off);
   const int tid = threadIdx.x;
   const int mm = m * m;
-  zd *A = lds, *A2 = lds + mm, *P = lds + 2 * mm, *Q = lds + 3 * mm, *T = lds + 4 * mm;
+  zd *A = lds, *A2 = lds + mm, *A4 = lds + 2 * mm, *A6 = lds + 3 * mm, *Wt = lds + 4 * mm;
   zd *G0 = lds + 5 * mm, *G1 = G0 + 2 * mm;          // Gauss-Jordan buffers (m x 2m)
   double *red = (double *)(G1 + 2 * mm);
-  // element e = tid + r*PP_THREADS of an m x m matrix: row i = e % m, column j = e / m
-  const int i0 = tid % m, j0 = tid / m, di = PP_THREADS % m, dj = PP_THREADS / m;
   const zd fac = zmk(dt_im, -dt_re);  // -i * dt
   for (int e = tid; e < mm; e += PP_THREADS) A[e] = zmul(fac, Hg[e]);
   __syncthreads();
@@ -70,84 +111,57 @@ static __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg,
   const double inv = 1.0 / pow(2.0, (double)sc);
   for (int e = tid; e < mm; e += PP_THREADS) A[e] = zscale(A[e], inv);
   __syncthreads();
-  auto matmul = [&](const zd *X, const zd *Y, zd *Z) {   // Z = X Y
-    int i = i0, j = j0;
-    for (int e = tid; e < mm; e += PP_THREADS) {
-      const zd *xr = X + i, *yc = Y + j * m;
-      zd a0 = zmk(0.0, 0.0), a1 = zmk(0.0, 0.0), a2 = zmk(0.0, 0.0), a3 = zmk(0.0, 0.0);
-      int k = 0;
-      for (; k + 3 < m; k += 4) {   // four independent chains keep 8 LDS reads in flight
-        const zd x0 = xr[k * m], x1 = xr[(k + 1) * m], x2 = xr[(k + 2) * m], x3 = xr[(k + 3) * m];
-        const zd y0 = yc[k], y1 = yc[k + 1], y2 = yc[k + 2], y3 = yc[k + 3];
-        zfma(a0, x0, y0);
-        zfma(a1, x1, y1);
-        zfma(a2, x2, y2);
-        zfma(a3, x3, y3);
-      }
-      for (; k < m; k++) zfma(a0, xr[k * m], yc[k]);
-      Z[e] = zadd(zadd(a0, a1), zadd(a2, a3));
-      i += di; j += dj;
-      if (i >= m) { i -= m; j++; }
-    }
-    __syncthreads();
-  };
-  matmul(A, A, A2);
   const int order = 6;
   double c[order + 1];
   c[0] = 1.0;
   for (int i = 0; i < order; i++) c[i + 1] = c[i] * (double)(order - i) / (double)((i + 1) * (2 * order - i));
-  {
-    int i = i0, j = j0;
-    for (int e = tid; e < mm; e += PP_THREADS) {
-      Q[e] = zmk(i == j ? c[order] : 0.0, 0.0);
-      P[e] = zmk(i == j ? c[order - 1] : 0.0, 0.0);
-      i += di; j += dj;
-      if (i >= m) { i -= m; j++; }
-    }
+  pade_mm(m, A, A, A2, 0.0, false);
+  pade_mm(m, A2, A2, A4, 0.0, false);
+  pade_mm(m, A4, A2, A6, 0.0, false);
+  // even part Q (over A4), the odd part's cofactor c1 + c3 A^2 + c5 A^4 (Wt)
+  for (int e = tid; e < mm; e += PP_THREADS) {
+    const int i = e % m, j = e / m;
+    const zd a2 = A2[e], a4 = A4[e], a6 = A6[e];
+    zd w = zadd(zscale(a4, c[5]), zscale(a2, c[3]));
+    zd q = zadd(zadd(zscale(a6, c[6]), zscale(a4, c[4])), zscale(a2, c[2]));
+    if (i == j) { w.x += c[1]; q.x += c[0]; }
+    Wt[e] = w;
+    A4[e] = q;
   }
   __syncthreads();
-  bool odd = true;
-  for (int q = order - 2; q >= 0; q--) {
-    zd *X = odd ? Q : P;
-    matmul(X, A2, T);
-    int i = i0, j = j0;
-    for (int e = tid; e < mm; e += PP_THREADS) {
-      zd v = T[e];
-      if (i == j) v.x += c[q];
-      X[e] = v;
-      i += di; j += dj;
-      if (i >= m) { i -= m; j++; }
-    }
-    __syncthreads();
-    odd = !odd;
-  }
-  zd *Pm = odd ? Q : P;   // the odd polynomial gets the extra factor A
-  matmul(Pm, A, T);
-  // Gauss-Jordan on [Q - P | P] (m x 2m, column-major)
+  pade_mm(m, A, Wt, A6, 0.0, false);   // P = A (c1 + c3 A^2 + c5 A^4)
+  // Gauss-Jordan on [Q - P | P] (m x 2m, column-major).  mag: |re| + |im| of
+  // the next pivot column, two buffers of MS doubles over the (now unused)
+  // scaled A
+  double *mag = (double *)A;
+  const int MS = (m + 3) & ~1;
   for (int e = tid; e < mm; e += PP_THREADS) {
-    const zd qv = odd ? T[e] : Q[e];       // Q after its final update
-    const zd pv = odd ? P[e] : T[e];       // P after its final update
-    G0[e] = zsub(qv, pv);
+    const zd pv = A6[e];
+    const zd g = zsub(A4[e], pv);
+    G0[e] = g;
     G0[mm + e] = pv;
+    if (e < m) mag[e] = fabs(g.x) + fabs(g.y);
   }
   __syncthreads();
   zd *gc = G0, *gn = G1;
   const int w2 = 2 * m * m;
+  const int ti = tid % m, tj = tid / m, di = PP_THREADS % m, dj = PP_THREADS / m;
   for (int k = 0; k < m; k++) {
-    // every wavefront finds the pivot itself: rows k..m-1 of column k, one
-    // per lane, reduced by shuffles (first index on ties, like izamax)
-    int p = m;
+    // pivot: the first largest magnitude of rows k .. m-1 (izamax), every
+    // lane comparing the values in row order
+    const double *mk = mag + (k & 1) * MS;
+    int p = k;
     double best = -1.0;
-    const int lane = tid & 63;
-    for (int i = k + lane; i < m; i += 64) {
-      const double v = fabs(gc[i + k * m].x) + fabs(gc[i + k * m].y);
-      if (v > best) { best = v; p = i; }
-    }
+    for (int i0 = k & ~1; i0 < m; i0 += 8) {
+      double2 v[4];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ob = __shfl_xor(best, o, 64);
-      const int op = __shfl_xor(p, o, 64);
-      if (ob > best || (ob == best && op < p)) { best = ob; p = op; }
+      for (int u = 0; u < 4; u++) v[u] = i0 + 2 * u < m ? *(const double2 *)(mk + i0 + 2 * u) : make_double2(-1.0, -1.0);
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int r = i0 + 2 * u;
+        if (r >= k && r < m && v[u].x > best) { best = v[u].x; p = r; }
+        if (r + 1 >= k && r + 1 < m && v[u].y > best) { best = v[u].y; p = r + 1; }
+      }
     }
     if (best == 0.0) {   // singular denominator (pamp.h:544-550)
       if (tid == 0) diag[0] = -1.0;
@@ -156,56 +170,68 @@ static __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg,
     const zd piv = gc[p + k * m];
     const double dn = zabs2(piv);
     const zd pinv = zmk(piv.x / dn, -piv.y / dn);
-    int i = i0, j = j0;
-    for (int e = tid; e < w2; e += PP_THREADS) {
-      if (j >= k) {   // eliminated columns are not read again
-        const zd rk = zmul(gc[p + j * m], pinv);   // new pivot row
-        zd v;
-        if (i == k) v = rk;
-        else {
+    // columns >= k (eliminated columns are not read again), the operands of
+    // four elements loaded before any is written
+    int i = ti, j = k + tj;
+    for (int e0 = k * m + tid; e0 < w2; e0 += 4 * PP_THREADS) {
+      zd ga[4], gb[4], gk[4];
+      int iu[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        iu[u] = i;
+        if (e0 + u * PP_THREADS < w2) {
           const int si = (i == p) ? k : i;       // row swap k <-> p
-          v = zsub(gc[si + j * m], zmul(gc[si + k * m], rk));
+          ga[u] = gc[p + j * m];
+          gb[u] = gc[si + j * m];
+          gk[u] = gc[si + k * m];
         }
-        gn[e] = v;
+        i += di; j += dj;
+        if (i >= m) { i -= m; j++; }
       }
-      i += di; j += dj;
-      if (i >= m) { i -= m; j++; }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int e = e0 + u * PP_THREADS;
+        if (e < w2) {
+          const zd rk = zmul(ga[u], pinv);     // new pivot row
+          const zd v = iu[u] == k ? rk : zsub(gb[u], zmul(gk[u], rk));
+          gn[e] = v;
+          if (e - iu[u] == (k + 1) * m) mag[((k + 1) & 1) * MS + iu[u]] = fabs(v.x) + fabs(v.y);
+        }
+      }
     }
     __syncthreads();
     zd *t = gc; gc = gn; gn = t;
   }
-  // X = right half of gc;  E = +-(I + 2X)
-  const bool neg = odd;
+  // X = right half of gc;  E = I + 2X
   if (sc == 0) {   // only the first column of E is needed
     for (int i = tid; i < m; i += PP_THREADS) {
       zd v = zscale(gc[mm + i], 2.0);
       if (i == 0) v.x += 1.0;
-      if (neg) v = zmk(-v.x, -v.y);
       zst(cvec_out + i, v);
       if (i == m - 1) diag[0] = sqrt(zabs2(v));
     }
     __syncthreads();
     return;
   }
-  {
-    int i = i0, j = j0;
-    for (int e = tid; e < mm; e += PP_THREADS) {
-      zd v = zscale(gc[mm + e], 2.0);
-      if (i == j) v.x += 1.0;
-      if (neg) v = zmk(-v.x, -v.y);
-      A[e] = v;
-      i += di; j += dj;
-      if (i >= m) { i -= m; j++; }
-    }
+  zd *E = A, *F = A2;
+  for (int e = tid; e < mm; e += PP_THREADS) {
+    const int i = e % m, j = e / m;
+    zd v = zscale(gc[mm + e], 2.0);
+    if (i == j) v.x += 1.0;
+    E[e] = v;
   }
   __syncthreads();
-  for (int s = 0; s < sc; s++) {
-    matmul(A, A, T);
-    for (int e = tid; e < mm; e += PP_THREADS) A[e] = T[e];
-    __syncthreads();
+  for (int s = 0; s + 1 < sc; s++) {   // all squarings but the last, ping-pong
+    pade_mm(m, E, E, F, 0.0, false);
+    zd *t = E; E = F; F = t;
   }
-  for (int i = tid; i < m; i += PP_THREADS) zst(cvec_out + i, A[i]);
-  if (tid == 0) diag[0] = sqrt(zabs2(A[m - 1]));
+  // the last squaring: only its first column, E (E e1)
+  for (int i = tid; i < m; i += PP_THREADS) {
+    zd acc = zmk(0.0, 0.0);
+    for (int k = 0; k < m; k++) zfma(acc, E[i + k * m], E[k]);
+    zst(cvec_out + i, acc);
+    if (i == m - 1) diag[0] = sqrt(zabs2(acc));
+  }
   __syncthreads();
 }
 

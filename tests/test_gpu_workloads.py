@@ -170,7 +170,8 @@ def test_batch_scan_of_512_instances():
     full basis; 4 instances spanning N and lmax match the oracle within its
     sensitivity.  Mixed form (the graph recorded again): bound-population
     error against the FP64 batch on 8 instances, within 1e-8 or 10x the
-    FP64 form's own floor (its change under a 1e-16 perturbation of psi0:
+    FP64 form's own floor (its largest change under three independent 1e-16
+    perturbations of psi0:
     at m = 16, dt = 0.01 and l up to 60, |H^| dt reaches ~6000, and the
     fixed-m steps amplify rounding to ~1e-4 in 20 steps)."""
     import pyprop_amd
@@ -218,13 +219,16 @@ def test_batch_scan_of_512_instances():
         print("  instance %d (N %d, lmax %d): max |psi - psi_ref| %.3e (tol %.3e)" % (i, d["N"], d["lmax"], err, tol),
               flush=True)
         assert err <= tol
-    # the FP64 form's own floor: the same batch from psi0 (1 + 1e-16 eta)
-    rng = np.random.default_rng(13)
-    for p, x in zip(probs, psi0s):
-        p.SetWavefunction(x * (1 + 1e-16 * rng.standard_normal(x.shape)))
-        p.PropagatedTime = p.StartTime
-    batch.AdvanceSteps(K)
-    pert = [p.psi.GetData().cpu().numpy() for p in probs]
+    # the FP64 form's own floor: the same batch from psi0 (1 + 1e-16 eta), three
+    # independent perturbations (one draw is a noisy estimate of the sensitivity)
+    perts = []
+    for seed in (13, 14, 15):
+        rng = np.random.default_rng(seed)
+        for p, x in zip(probs, psi0s):
+            p.SetWavefunction(x * (1 + 1e-16 * rng.standard_normal(x.shape)))
+            p.PropagatedTime = p.StartTime
+        batch.AdvanceSteps(K)
+        perts.append([p.psi.GetData().cpu().numpy() for p in probs])
     # mixed form: same instances, the graph recorded again
     for p, x in zip(probs, psi0s):
         p.SetWavefunction(x)
@@ -236,9 +240,11 @@ def test_batch_scan_of_512_instances():
         proj = H.bound_projectors(probs[i])
         a = H.populations(probs[i], got[i], projectors=proj)[1]
         b = H.populations(probs[i], probs[i].psi.GetData().cpu().numpy(), projectors=proj)[1]
-        c = H.populations(probs[i], pert[i], projectors=proj)[1]
         mixed = max(float(np.abs(x - y).max()) for x, y in zip(a, b) if len(x))
-        floor = max(float(np.abs(x - y).max()) for x, y in zip(a, c) if len(x))
+        floor = 0.0
+        for pert in perts:
+            c = H.populations(probs[i], pert[i], projectors=proj)[1]
+            floor = max(floor, max(float(np.abs(x - y).max()) for x, y in zip(a, c) if len(x)))
         print("  instance %d (N %d, lmax %d): mixed vs FP64 bound-population error %.2e, FP64 floor %.2e"
               % (i, draws[i]["N"], draws[i]["lmax"], mixed, floor), flush=True)
         assert mixed <= max(1e-8, 10 * floor)
