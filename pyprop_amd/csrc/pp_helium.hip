@@ -427,6 +427,9 @@ __global__ void __launch_bounds__(256) p3_mixT(const int *row, const p3_ent *ent
 // Z[c][i1][i2] += sum_blocks [ i2 > nhi[i1]:  sum_d1 P[i1][d1] Tq[b][i1+d1-S][i2][cl]
 //                              i2 < nlo[i1]:  sum_d1 Q[i1][d1] Tp[b][i1+d1-S][i2][cl] ]
 // workgroup: (chunk, P3_RT rows i1, 16 columns i2); thread: (i2, channel lane).
+#ifndef P3_FRT
+#define P3_FRT 12          // output rows i1 per far-field workgroup (its 24-row window: 2x re-read, 8 rows: 2.5x)
+#endif
 template <int S, class T>
 __global__ void __launch_bounds__(256) p3_farL(const T *const *Pt, const T *const *Qt, const int *blk_term,
                                                const int *cb_ptr, const int *cb_blk, const int2 *nr,
@@ -435,18 +438,18 @@ __global__ void __launch_bounds__(256) p3_farL(const T *const *Pt, const T *cons
 {
   typedef typename p3_cx<T>::type ZT;
   if (state[0]) return;
-  constexpr int BW = 2 * S + 1, WIN = P3_RT + 2 * S;
-  const int ntile = (N + P3_RT - 1) / P3_RT, ngrp = (N + 15) / 16;
+  constexpr int BW = 2 * S + 1, WIN = P3_FRT + 2 * S;
+  const int ntile = (N + P3_FRT - 1) / P3_FRT, ngrp = (N + 15) / 16;
   int g = blockIdx.x;
   const int chunk = g % nchunk; g /= nchunk;
-  const int i1a = (g % ntile) * P3_RT; g /= ntile;
+  const int i1a = (g % ntile) * P3_FRT; g /= ntile;
   const int i2 = g * 16 + (threadIdx.x >> 4), cl = threadIdx.x & (P3_CHUNK - 1);
   (void)ngrp;
   const long NN = (long)N * N;
   const bool col_ok = i2 < N;
-  bool below[P3_RT], above[P3_RT];
+  bool below[P3_FRT], above[P3_FRT];
 #pragma unroll
-  for (int rr = 0; rr < P3_RT; rr++) {
+  for (int rr = 0; rr < P3_FRT; rr++) {
     const int i1 = i1a + rr;
     const int2 r = i1 < N ? nr[i1] : make_int2(N, -1);
     below[rr] = col_ok && i1 < N && i2 > r.y;
@@ -454,7 +457,7 @@ __global__ void __launch_bounds__(256) p3_farL(const T *const *Pt, const T *cons
   }
   bool anyb = false, anya = false;
 #pragma unroll
-  for (int rr = 0; rr < P3_RT; rr++) { anyb = anyb || below[rr]; anya = anya || above[rr]; }
+  for (int rr = 0; rr < P3_FRT; rr++) { anyb = anyb || below[rr]; anya = anya || above[rr]; }
   __shared__ const T *sP[P3_MAXB];
   __shared__ const T *sQ[P3_MAXB];
   __shared__ long sT[P3_MAXB];
@@ -466,9 +469,9 @@ __global__ void __launch_bounds__(256) p3_farL(const T *const *Pt, const T *cons
     sT[k] = (long)b * NN * P3_CHUNK;
   }
   __syncthreads();
-  ZT acc[P3_RT];
+  ZT acc[P3_FRT];
 #pragma unroll
-  for (int rr = 0; rr < P3_RT; rr++) acc[rr] = p3_czero<ZT>();
+  for (int rr = 0; rr < P3_FRT; rr++) acc[rr] = p3_czero<ZT>();
   for (int k = 0; k < nb; k++) {
     const T *P = sP[k], *Q = sQ[k];
     const long tb = sT[k];
@@ -480,7 +483,7 @@ __global__ void __launch_bounds__(256) p3_farL(const T *const *Pt, const T *cons
         win[w] = (j1 >= 0 && j1 < N && col_ok) ? Tq[tb + ((long)j1 * N + i2) * P3_CHUNK + cl] : p3_czero<ZT>();
       }
 #pragma unroll
-      for (int rr = 0; rr < P3_RT; rr++) {
+      for (int rr = 0; rr < P3_FRT; rr++) {
         if (!below[rr]) continue;
         const T *pr = P + (size_t)(i1a + rr) * BW;
         ZT t = p3_czero<ZT>();
@@ -496,7 +499,7 @@ __global__ void __launch_bounds__(256) p3_farL(const T *const *Pt, const T *cons
         win[w] = (j1 >= 0 && j1 < N && col_ok) ? Tp[tb + ((long)j1 * N + i2) * P3_CHUNK + cl] : p3_czero<ZT>();
       }
 #pragma unroll
-      for (int rr = 0; rr < P3_RT; rr++) {
+      for (int rr = 0; rr < P3_FRT; rr++) {
         if (!above[rr]) continue;
         const T *qr = Q + (size_t)(i1a + rr) * BW;
         ZT t = p3_czero<ZT>();
@@ -509,7 +512,7 @@ __global__ void __launch_bounds__(256) p3_farL(const T *const *Pt, const T *cons
   const int c = chunk * P3_CHUNK + cl;
   if (c >= C || !col_ok) return;
 #pragma unroll
-  for (int rr = 0; rr < P3_RT; rr++) {
+  for (int rr = 0; rr < P3_FRT; rr++) {
     if (!(below[rr] || above[rr])) continue;
     const long o = (long)c * NN + (long)(i1a + rr) * N + i2;
     Z[o] = zadd(Z[o], p3_ctod(acc[rr]));
@@ -621,10 +624,15 @@ __global__ void __launch_bounds__(64 * P3_NG) p3_near(const T *const *Rn, const 
 // ztbsv 'U','N'), OP 1: U^T x = b (forward; U is real so this is U^H, ztbsv
 // 'U','C'), OP 2: x := U x (ztbmv).  U(i, j) at Ub[j*(KD+1) + KD + i - j]
 // (overlapmatrix.cpp:22-32).
-#define P3_FT 32
+#ifndef P3_FT
+#define P3_FT 32           // fibers per workgroup
+#endif
+#ifndef P3_FTT
+#define P3_FTT 256         // threads per workgroup
+#endif
 #define P3_SB 8            // rows per group of the substitution's interior
 template <int KD, int OP>
-__global__ void __launch_bounds__(256) p3_ftile(const double *Ub, const double *rdg, zd *x, int C, int N, int rank)
+__global__ void __launch_bounds__(P3_FTT) p3_ftile(const double *Ub, const double *rdg, zd *x, int C, int N, int rank)
 {
   extern __shared__ double p3_lds[];
   double *uL = p3_lds;                                   // N*(KD+1) band
@@ -633,8 +641,8 @@ __global__ void __launch_bounds__(256) p3_ftile(const double *Ub, const double *
   zd *tile = (zd *)(p3_lds + toff);                      // fiber f, element i at f*(N+1) + i
   const int ld = N + 1;
   const long NN = (long)N * N;
-  for (int e = threadIdx.x; e < N * (KD + 1); e += 256) uL[e] = Ub[e];
-  for (int i = threadIdx.x; i < N; i += 256) rd[i] = rdg[i];
+  for (int e = threadIdx.x; e < N * (KD + 1); e += P3_FTT) uL[e] = Ub[e];
+  for (int i = threadIdx.x; i < N; i += P3_FTT) rd[i] = rdg[i];
   int nf;
   zd *g;
   long gstr, fstr;   // global element i of fiber f at g[f*fstr + i*gstr]
@@ -642,13 +650,13 @@ __global__ void __launch_bounds__(256) p3_ftile(const double *Ub, const double *
     const long r0 = (long)blockIdx.x * P3_FT, nrow = (long)C * N;
     nf = nrow - r0 < P3_FT ? (int)(nrow - r0) : P3_FT;
     g = x + r0 * N; fstr = N; gstr = 1;
-    for (int e = threadIdx.x; e < nf * N; e += 256) tile[(e / N) * ld + e % N] = g[e];
+    for (int e = threadIdx.x; e < nf * N; e += P3_FTT) tile[(e / N) * ld + e % N] = g[e];
   } else {
     const int ncol = (N + P3_FT - 1) / P3_FT;
     const int c = blockIdx.x / ncol, i2a = (blockIdx.x % ncol) * P3_FT;
     nf = N - i2a < P3_FT ? N - i2a : P3_FT;
     g = x + (long)c * NN + i2a; fstr = 1; gstr = N;
-    for (int e = threadIdx.x; e < nf * N; e += 256) {
+    for (int e = threadIdx.x; e < nf * N; e += P3_FTT) {
       const int i = e / nf, f = e % nf;                   // consecutive threads: consecutive columns
       tile[f * ld + i] = g[(long)i * N + f];
     }
@@ -759,9 +767,9 @@ __global__ void __launch_bounds__(256) p3_ftile(const double *Ub, const double *
   }
   __syncthreads();
   if (rank == 2) {
-    for (int e = threadIdx.x; e < nf * N; e += 256) g[e] = tile[(e / N) * ld + e % N];
+    for (int e = threadIdx.x; e < nf * N; e += P3_FTT) g[e] = tile[(e / N) * ld + e % N];
   } else {
-    for (int e = threadIdx.x; e < nf * N; e += 256) {
+    for (int e = threadIdx.x; e < nf * N; e += P3_FTT) {
       const int i = e / nf, f = e % nf;
       g[(long)i * N + f] = tile[f * ld + i];
     }
@@ -790,18 +798,38 @@ __global__ void __launch_bounds__(256) p3_dots(const zd *V, const zd *w, long n,
     if (tid == 0) part[(long)blockIdx.x * pstr] = (red[0] + red[1]) + (red[2] + red[3]);
     return;
   }
-  for (int k = wv; k < nk; k += 4) {
-    const zd *vk = V + (long)k * n;
-    double re = 0.0, im = 0.0;
-#pragma unroll 4
-    for (long i = lo + lane; i < hi; i += 64) {
-      const zd v = vk[i], x = w[i];
-      re = fma(v.x, x.x, re); re = fma(v.y, x.y, re);
-      im = fma(v.x, x.y, im); im = fma(-v.y, x.x, im);
+  // groups of 8 vectors: each thread reads w[i] once per group and carries
+  // 8 partial inner products; one block reduction per group, summed over the
+  // waves in a fixed order
+  __shared__ double wred[4][16];
+  for (int k0 = 0; k0 < nk; k0 += 8) {
+    const int kn = nk - k0 < 8 ? nk - k0 : 8;
+    double re[8], im[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) { re[u] = 0.0; im[u] = 0.0; }
+    const zd *vk = V + (long)k0 * n;
+    for (long i = lo + tid; i < hi; i += 256) {
+      const zd x = w[i];
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (u < kn) {
+          const zd v = vk[(long)u * n + i];
+          re[u] = fma(v.x, x.x, re[u]); re[u] = fma(v.y, x.y, re[u]);
+          im[u] = fma(v.x, x.y, im[u]); im[u] = fma(-v.y, x.x, im[u]);
+        }
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) { re += __shfl_xor(re, o, 64); im += __shfl_xor(im, o, 64); }
-    if (lane == 0) { part[(long)blockIdx.x * pstr + 2 * k] = re; part[(long)blockIdx.x * pstr + 2 * k + 1] = im; }
+    for (int u = 0; u < 8; u++) {
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) { re[u] += __shfl_xor(re[u], o, 64); im[u] += __shfl_xor(im[u], o, 64); }
+    }
+    if (lane == 0)
+#pragma unroll
+      for (int u = 0; u < 8; u++) { wred[wv][2 * u] = re[u]; wred[wv][2 * u + 1] = im[u]; }
+    __syncthreads();
+    if (tid < 2 * kn)
+      part[(long)blockIdx.x * pstr + 2 * k0 + tid] = (wred[0][tid] + wred[1][tid]) + (wred[2][tid] + wred[3][tid]);
+    __syncthreads();
   }
 }
 
@@ -1280,7 +1308,7 @@ static int p3_apply_raw(pp_pamp3 *p, const zd *y, zd *z, const double *scal, hip
       if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return PP_EHIP;
       (void)hipEventRecord(e0, st);
     }
-    const int ntile = (N + P3_RT - 1) / P3_RT, ngrp = (N + 15) / 16;
+    const int ntile = (N + P3_FRT - 1) / P3_FRT, ngrp = (N + 15) / 16;
     if (p->split && p->mixed) {
       // FP32 stage: the complex64 intermediates use the first half of the FP64 buffers
       float2 *Ym = (float2 *)p->Ym, *Tq = (float2 *)p->Tq, *Tp = (float2 *)p->Tp;
@@ -1324,9 +1352,9 @@ static int p3_solve2(pp_pamp3 *p, int op, zd *x, hipStream_t st)
   if (hipFuncSetAttribute(fn[op], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return PP_EHIP;
   for (int rank = 2; rank >= 1; rank--) {
     const dim3 grid = rank == 2 ? g2 : g1;
-    if (op == 0) hipLaunchKernelGGL((p3_ftile<6, 0>), grid, dim3(256), lds, st, p->ov->Uband, p->urd, x, C, N, rank);
-    else if (op == 1) hipLaunchKernelGGL((p3_ftile<6, 1>), grid, dim3(256), lds, st, p->ov->Uband, p->urd, x, C, N, rank);
-    else hipLaunchKernelGGL((p3_ftile<6, 2>), grid, dim3(256), lds, st, p->ov->Uband, p->urd, x, C, N, rank);
+    if (op == 0) hipLaunchKernelGGL((p3_ftile<6, 0>), grid, dim3(P3_FTT), lds, st, p->ov->Uband, p->urd, x, C, N, rank);
+    else if (op == 1) hipLaunchKernelGGL((p3_ftile<6, 1>), grid, dim3(P3_FTT), lds, st, p->ov->Uband, p->urd, x, C, N, rank);
+    else hipLaunchKernelGGL((p3_ftile<6, 2>), grid, dim3(P3_FTT), lds, st, p->ov->Uband, p->urd, x, C, N, rank);
   }
   return hipGetLastError() == hipSuccess ? PP_OK : PP_EHIP;
 }
