@@ -1412,7 +1412,7 @@ extern "C" int pp_pamp3_propagate(pp_pamp3 *p, void *psi, int nsteps, double dt_
   const size_t vb = sizeof(zd) * (size_t)n;
   zd *ps = (zd *)psi;
   const int gridv = 1024;
-  const size_t lds = sizeof(zd) * (size_t)(9 * m * m) + 64 * sizeof(double);
+  const size_t lds = sizeof(double) * (size_t)pp_pade_lds_doubles(m) + 64 * sizeof(double);
   if (hipFuncSetAttribute((const void *)p3_pade, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
     return PP_EHIP;
   for (int step = 0; step < nsteps; step++) {

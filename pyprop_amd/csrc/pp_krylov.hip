@@ -1620,7 +1620,7 @@ static size_t pamp_lds_bytes(int L, int N, int m, int nslots, int kd, int P, int
   size_t fib = sizeof(double) * nle + sizeof(zd) * (size_t)(2 * padN + P * kd) +
                sizeof(double) * (size_t)pamp_scr_doubles(L, N, m, nslots, padN) +
                sizeof(double) * (size_t)(16 + (2 * m + 4) + PP_THREADS + 2 * m) + sizeof(zd) * (size_t)m + 64;
-  size_t pade = sizeof(double) * nle + sizeof(zd) * (size_t)(9 * m * m) + sizeof(double) * (PP_THREADS / 64) + 64;
+  size_t pade = sizeof(double) * (nle + (size_t)pp_pade_lds_doubles(m)) + 64;
   return fib > pade ? fib : pade;
 }
 
@@ -1761,7 +1761,7 @@ static size_t pamp_split_lds_bytes(const pp_pamp *p, int Q)
   const size_t nle = (size_t)((pl.nlev + 1) & ~1) + PP_CMB_DOUBLES;
   const int maxch = (pl.P + Q - 1) / Q;
   const size_t fib = sizeof(double) * (nle + (size_t)pp_split_doubles(p->nslots, p->m, p->ov->kd, pl.B, maxch)) + 64;
-  const size_t pade = sizeof(double) * nle + sizeof(zd) * (size_t)(9 * p->m * p->m) + sizeof(double) * (PP_THREADS / 64) + 64;
+  const size_t pade = sizeof(double) * (nle + (size_t)pp_pade_lds_doubles(p->m)) + 64;
   return fib > pade ? fib : pade;
 }
 

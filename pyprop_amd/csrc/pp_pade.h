@@ -40,6 +40,17 @@ static __device__ __attribute__((noinline)) void eig_block(int m, const zd *Hg, 
 
 typedef double pp_d4 __attribute__((ext_vector_type(4)));
 
+// Doubles of dynamic LDS the exponential of an m x m Krylov matrix needs at
+// its offset: five m x m complex buffers (Pade) or the eigenvector form's two
+// padded matrices and rotation table, then the pivot magnitudes and the
+// wavefront maxima.
+__host__ __device__ inline int pp_pade_lds_doubles(int m)
+{
+  const int n2 = m + (m & 1), np = n2 / 2;
+  const int pade = 2 * 5 * m * m, eig = 2 * 2 * n2 * n2 + 2 * np + 2 * np + np + 2;
+  return (pade > eig ? pade : eig) + 2 * ((m + 3) & ~1) + PP_THREADS / 64 + 2;
+}
+
 // Z = X Y (+ c on the diagonal when diag), m x m complex, column-major, LDS:
 // 16 x 16 output tiles, one per wavefront at a time, with the real and
 // imaginary parts accumulated by v_mfma_f64_16x16x4f64 (FP64 products and
@@ -86,9 +97,14 @@ static __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg,
   zd *lds = (zd *)(pp_lds + off);
   const int tid = threadIdx.x;
   const int mm = m * m;
+  // five m x m buffers; Gauss-Jordan ping-pongs between [A | A2] and
+  // [A4 | A6] (m x 2m each) once Q and P are formed, and the squarings use Wt
+  // and A, so the exponential needs 5 m^2 complex values in all
   zd *A = lds, *A2 = lds + mm, *A4 = lds + 2 * mm, *A6 = lds + 3 * mm, *Wt = lds + 4 * mm;
-  zd *G0 = lds + 5 * mm, *G1 = G0 + 2 * mm;          // Gauss-Jordan buffers (m x 2m)
-  double *red = (double *)(G1 + 2 * mm);
+  zd *G0 = A, *G1 = A4;                               // Gauss-Jordan buffers (m x 2m)
+  double *mag = (double *)(lds + 5 * mm);              // pivot magnitudes, two buffers
+  const int MS = (m + 3) & ~1;
+  double *red = mag + 2 * MS;
   const zd fac = zmk(dt_im, -dt_re);  // -i * dt
   for (int e = tid; e < mm; e += PP_THREADS) A[e] = zmul(fac, Hg[e]);
   __syncthreads();
@@ -130,11 +146,8 @@ static __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg,
   }
   __syncthreads();
   pade_mm(m, A, Wt, A6, 0.0, false);   // P = A (c1 + c3 A^2 + c5 A^4)
-  // Gauss-Jordan on [Q - P | P] (m x 2m, column-major).  mag: |re| + |im| of
-  // the next pivot column, two buffers of MS doubles over the (now unused)
-  // scaled A
-  double *mag = (double *)A;
-  const int MS = (m + 3) & ~1;
+  // Gauss-Jordan on [Q - P | P] (m x 2m, column-major) in [A | A2] (A and A2
+  // are no longer needed); mag: |re| + |im| of the next pivot column
   for (int e = tid; e < mm; e += PP_THREADS) {
     const zd pv = A6[e];
     const zd g = zsub(A4[e], pv);
@@ -213,7 +226,7 @@ static __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg,
     __syncthreads();
     return;
   }
-  zd *E = A, *F = A2;
+  zd *E = Wt, *F = gc == G0 ? G1 : G0;   // free: Wt and the left half of the other Gauss-Jordan buffer
   for (int e = tid; e < mm; e += PP_THREADS) {
     const int i = e % m, j = e / m;
     zd v = zscale(gc[mm + e], 2.0);

@@ -126,19 +126,22 @@ def test_lowrank_factor_finds_exact_rank_of_batch_operators():
 
 def test_expokit_chunk_fits_lds():
     """The Expokit engine's solver chunk: the default plan where zgexpv's
-    Hessenberg workspace still fits 160 KB of LDS (small shapes), a longer
-    chunk at the hydrogen_large shape (L = 64, N = 800, m = 24, kd = 6),
-    where the default B = 32 plan needs 157,856 B before the 11,232 B of
-    zgexpv workspace."""
+    Hessenberg workspace still fits 160 KB of LDS (small shapes, and the
+    hydrogen_large shape L = 64, N = 800, kd = 6 at m = 24), a longer chunk
+    where it does not (the same shape at m = 30: the default B = 32 plan and
+    the Pade's 5 m^2 buffers leave less than the 16,896 B of zgexpv
+    workspace)."""
     from pyprop_amd import _native as nat
     from pyprop_amd.propagator import expokit_chunk
     lib = nat.lib()
     assert lib.pp_pamp_lds_estimate(64, 800, 24, 3, 6, 0) == lib.pp_pamp_lds_estimate(64, 800, 24, 3, 6, 32)
     assert expokit_chunk(11, 103, 20, 6, 0) is None
-    B = expokit_chunk(64, 800, 24, 6, 0)
+    assert expokit_chunk(64, 800, 24, 6, 0) is None
+    assert lib.pp_pamp_lds_estimate(64, 800, 24, 3, 6, 32) + 16 * (26 * 26 + 26) <= 160 * 1024
+    B = expokit_chunk(64, 800, 30, 6, 0)
     assert B == 64
-    assert lib.pp_pamp_lds_estimate(64, 800, 24, 3, 6, B) + 16 * (26 * 26 + 26) <= 160 * 1024
-    assert lib.pp_pamp_lds_estimate(64, 800, 24, 3, 6, 32) + 16 * (26 * 26 + 26) > 160 * 1024
+    assert lib.pp_pamp_lds_estimate(64, 800, 30, 3, 6, B) + 16 * (32 * 32 + 32) <= 160 * 1024
+    assert lib.pp_pamp_lds_estimate(64, 800, 30, 3, 6, 32) + 16 * (32 * 32 + 32) > 160 * 1024
 
 
 def test_krylov_status_mapping_raises_like_the_reference():
