@@ -117,6 +117,8 @@ struct pp_pamp3 {
   zd *Y, *X, *Z, *P, *V, *Ym;
   double *part, *scal, *diag, *scal_dt;
   double *urd;             // reciprocals of U's diagonal (the fiber solves)
+  double *tinv;            // U^-1 and U^-T as MFMA blocks (p3_tgemm; null: substitution tiles)
+  int tnp;                 // their padded size (0: substitution tiles)
   zd *Hm, *cvec, *hcoef;
   double *beta;
   int *state;              // [0] broken, [1] mEff
@@ -650,15 +652,32 @@ __global__ void __launch_bounds__(P3_FTT) p3_ftile(const double *Ub, const doubl
     const long r0 = (long)blockIdx.x * P3_FT, nrow = (long)C * N;
     nf = nrow - r0 < P3_FT ? (int)(nrow - r0) : P3_FT;
     g = x + r0 * N; fstr = N; gstr = 1;
-    for (int e = threadIdx.x; e < nf * N; e += P3_FTT) tile[(e / N) * ld + e % N] = g[e];
+    const int tot = nf * N;
+    for (int e0 = 0; e0 < tot; e0 += 8 * P3_FTT) {   // 8 loads in flight per thread (clamped: no branches)
+      zd v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) { const int e = e0 + threadIdx.x + u * P3_FTT; v[u] = g[e < tot ? e : tot - 1]; }
+#pragma unroll
+      for (int u = 0; u < 8; u++) { const int e = e0 + threadIdx.x + u * P3_FTT; if (e < tot) tile[(e / N) * ld + e % N] = v[u]; }
+    }
   } else {
     const int ncol = (N + P3_FT - 1) / P3_FT;
     const int c = blockIdx.x / ncol, i2a = (blockIdx.x % ncol) * P3_FT;
     nf = N - i2a < P3_FT ? N - i2a : P3_FT;
     g = x + (long)c * NN + i2a; fstr = 1; gstr = N;
-    for (int e = threadIdx.x; e < nf * N; e += P3_FTT) {
-      const int i = e / nf, f = e % nf;                   // consecutive threads: consecutive columns
-      tile[f * ld + i] = g[(long)i * N + f];
+    const int tot = nf * N;
+    for (int e0 = 0; e0 < tot; e0 += 8 * P3_FTT) {   // consecutive threads: consecutive columns
+      zd v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int e = e0 + threadIdx.x + u * P3_FTT, ec = e < tot ? e : tot - 1;
+        v[u] = g[(long)(ec / nf) * N + ec % nf];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int e = e0 + threadIdx.x + u * P3_FTT, i = e / nf, f = e % nf;
+        if (e < tot) tile[f * ld + i] = v[u];
+      }
     }
   }
   __syncthreads();
@@ -777,7 +796,147 @@ __global__ void __launch_bounds__(P3_FTT) p3_ftile(const double *Ub, const doubl
   (void)fstr; (void)gstr;
 }
 
+// Dense triangular products for the overlap solves at N <= 128: y = T x
+// per fiber with T = U^-1 (OP 0, upper triangular) or U^-T (OP 1, lower;
+// U is real, so this is U^-H), padded with zeros to NP = 16 * NB rows and
+// columns, on v_mfma_f64_16x16x4f64.  The substitution over a fiber is a
+// chain of N dependent steps on one lane; the product has no chain.  T is
+// kept as its nonzero 16 x 4 blocks (row block rb, column step ks), each 64
+// doubles in MFMA operand order (lane i + 16k: T[16rb + i][4ks + k]); it
+// stays in L2 and every wavefront streams the blocks it needs.  One
+// wavefront per workgroup owns 16 real columns (fiber f, part ri at column
+// 2f + ri) and computes all NP rows of them: rank 1 (fibers along i1,
+// 8 consecutive i2 of one channel) D = T X with T as the A operand; rank 2
+// (fibers along i2, 8 consecutive rows c*N + i1) D = X^T T^T with the same
+// T blocks as the B operand.  The wave loads all of its columns first (so
+// the product is in place), T one column step ahead of the MFMAs; terms are
+// added in k order, four per MFMA, over the blocks inside the triangle.
+#ifndef P3_TPROD
+#define P3_TPROD 1         // overlap solves as products with the dense inverses where N <= 128
+#endif
+__host__ __device__ inline int p3_tblk_first(int OP, int rb) { return OP == 0 ? 4 * rb : 0; }
+__host__ __device__ inline int p3_tblk_end(int OP, int rb, int KS) { return OP == 0 ? KS : 4 * rb + 4; }
+__host__ __device__ inline int p3_tblk_base(int OP, int rb, int KS)   // blocks of row blocks < rb
+{
+  return OP == 0 ? rb * KS - 2 * rb * (rb - 1) : 2 * rb * (rb + 1);
+}
+
+template <int NB, int OP, int RANK>
+__global__ void __launch_bounds__(64) p3_tgemm(const double *__restrict__ Tb, double *x, int C, int N)
+{
+  constexpr int NP = 16 * NB, KS = NP / 4;
+  const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
+  double *g;
+  int ncols;        // valid real columns of this wave (<= 16)
+  long kstr;        // doubles from element k to k + 1 of a column
+  if (RANK == 1) {
+    const int nb = (2 * N + 15) / 16, c = blockIdx.x / nb, j = blockIdx.x - c * nb;
+    g = x + (long)c * 2 * N * N + 16 * j;
+    ncols = 2 * N - 16 * j < 16 ? 2 * N - 16 * j : 16;
+    kstr = 2L * N;
+  } else {
+    const long f0 = (long)blockIdx.x * 8, nrow = (long)C * N;
+    g = x + f0 * 2 * N;
+    ncols = nrow - f0 < 8 ? 2 * (int)(nrow - f0) : 16;
+    kstr = 2;
+  }
+  // the lane's operand column li (B[k][j] for rank 1, A[c][k] for rank 2; k = lk)
+  const bool colok = li < ncols;
+  const double *src = RANK == 1 ? g + li : g + (long)(li >> 1) * 2 * N + (li & 1);
+  double xa[KS];
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++) {
+    const int k = 4 * ks + lk;
+    xa[ks] = (colok && k < N) ? src[(long)k * kstr] : 0.0;
+  }
+  auto tblk = [&](int rb, int ks) { return Tb[(p3_tblk_base(OP, rb, KS) + ks - p3_tblk_first(OP, rb)) * 64 + lane]; };
+  auto inside = [](int rb, int ks) { return ks >= p3_tblk_first(OP, rb) && ks < p3_tblk_end(OP, rb, KS); };
+  double tn[NB];
+#pragma unroll
+  for (int rb = 0; rb < NB; rb++) tn[rb] = inside(rb, 0) ? tblk(rb, 0) : 0.0;
+  pp_d4 acc[NB];
+#pragma unroll
+  for (int rb = 0; rb < NB; rb++) acc[rb] = pp_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++) {
+    double tc[NB];
+#pragma unroll
+    for (int rb = 0; rb < NB; rb++) {
+      tc[rb] = tn[rb];
+      if (ks + 1 < KS && inside(rb, ks + 1)) tn[rb] = tblk(rb, ks + 1);
+    }
+#pragma unroll
+    for (int rb = 0; rb < NB; rb++) {
+      if (!inside(rb, ks)) continue;
+      if (RANK == 1) acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(tc[rb], xa[ks], acc[rb], 0, 0, 0);
+      else acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[ks], tc[rb], acc[rb], 0, 0, 0);
+    }
+  }
+  // D[4r + lane/16][lane%16] in register r of each block
+#pragma unroll
+  for (int rb = 0; rb < NB; rb++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      if (RANK == 1) {   // row i = 16rb + 4r + lk, column li
+        const int i = 16 * rb + 4 * r + lk;
+        if (i < N && colok) g[(long)i * 2 * N + li] = acc[rb][r];
+      } else {           // column 4r + lk, row i = 16rb + li
+        const int col = 4 * r + lk, i = 16 * rb + li;
+        if (i < N && col < ncols) g[(long)(col >> 1) * 2 * N + 2 * i + (col & 1)] = acc[rb][r];
+      }
+    }
+}
+
+// Columns of U^-1 by back substitution (one thread per column j: U t = e_j,
+// t_i = 0 for i > j), into T0 (row-major U^-1) and T1 (row-major U^-T);
+// both NP x NP and zeroed beforehand.
+__global__ void p3_tinv_build(const double *Ub, double *T0, double *T1, int N, int NP, int kd)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  auto U = [&](int i, int k) { return Ub[(size_t)k * (kd + 1) + kd + i - k]; };
+  for (int i = j; i >= 0; i--) {
+    double s = i == j ? 1.0 : 0.0;
+    for (int d = 1; d <= kd && i + d <= j; d++) s -= U(i, i + d) * T0[(size_t)(i + d) * NP + j];
+    const double t = s / U(i, i);
+    T0[(size_t)i * NP + j] = t;
+    T1[(size_t)j * NP + i] = t;
+  }
+}
+
 // ------------------------------------------------------------------ Gram-Schmidt
+// The thread's partial <V_u, w> (u < KN, V_u at vk + u*n) over i = lo+tid,
+// lo+tid+256, ... < hi, accumulated in that order; the loads of P3_DUR
+// consecutive elements are issued before their multiply-adds.
+#ifndef P3_DUR
+#define P3_DUR 4
+#endif
+template <int KN>
+__device__ __attribute__((always_inline)) inline void p3_dots_group(const zd *vk, const zd *w, long n, long lo, long hi,
+                                                                    double *re, double *im)
+{
+  for (long i = lo + threadIdx.x; i < hi; i += 256 * P3_DUR) {
+    long ii[P3_DUR];
+    zd x[P3_DUR], v[P3_DUR][KN];
+#pragma unroll
+    for (int r = 0; r < P3_DUR; r++) ii[r] = i + 256 * r < hi ? i + 256 * r : i;   // clamped: no branches on the loads
+#pragma unroll
+    for (int r = 0; r < P3_DUR; r++) {
+      x[r] = w[ii[r]];
+#pragma unroll
+      for (int u = 0; u < KN; u++) v[r][u] = vk[(long)u * n + ii[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < P3_DUR; r++)
+      if (r == 0 || i + 256 * r < hi)
+#pragma unroll
+        for (int u = 0; u < KN; u++) {
+          re[u] = fma(v[r][u].x, x[r].x, re[u]); re[u] = fma(v[r][u].y, x[r].y, re[u]);
+          im[u] = fma(v[r][u].x, x[r].y, im[u]); im[u] = fma(-v[r][u].y, x[r].x, im[u]);
+        }
+  }
+}
+
 // part[blk*pstr + 2k..] = partial <V_k, w> over this block's range, k < nk;
 // nk == 0: part[blk*pstr] = partial |w|^2.
 __global__ void __launch_bounds__(256) p3_dots(const zd *V, const zd *w, long n, int nk, double *part, int pstr,
@@ -808,15 +967,15 @@ __global__ void __launch_bounds__(256) p3_dots(const zd *V, const zd *w, long n,
 #pragma unroll
     for (int u = 0; u < 8; u++) { re[u] = 0.0; im[u] = 0.0; }
     const zd *vk = V + (long)k0 * n;
-    for (long i = lo + tid; i < hi; i += 256) {
-      const zd x = w[i];
-#pragma unroll
-      for (int u = 0; u < 8; u++)
-        if (u < kn) {
-          const zd v = vk[(long)u * n + i];
-          re[u] = fma(v.x, x.x, re[u]); re[u] = fma(v.y, x.y, re[u]);
-          im[u] = fma(v.x, x.y, im[u]); im[u] = fma(-v.y, x.x, im[u]);
-        }
+    switch (kn) {
+      case 1: p3_dots_group<1>(vk, w, n, lo, hi, re, im); break;
+      case 2: p3_dots_group<2>(vk, w, n, lo, hi, re, im); break;
+      case 3: p3_dots_group<3>(vk, w, n, lo, hi, re, im); break;
+      case 4: p3_dots_group<4>(vk, w, n, lo, hi, re, im); break;
+      case 5: p3_dots_group<5>(vk, w, n, lo, hi, re, im); break;
+      case 6: p3_dots_group<6>(vk, w, n, lo, hi, re, im); break;
+      case 7: p3_dots_group<7>(vk, w, n, lo, hi, re, im); break;
+      default: p3_dots_group<8>(vk, w, n, lo, hi, re, im); break;
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
@@ -974,6 +1133,33 @@ __global__ void p3_recip_diag(const double *Ub, double *rd, int N, int kd)
     rd[i] = 1.0 / Ub[(size_t)i * (kd + 1) + kd];
 }
 
+// Dense U^-1 and U^-T on the device, then their nonzero 16 x 4 blocks in
+// MFMA operand order (p3_tgemm), both triangles one after the other.
+static int p3_tprod_build(pp_pamp3 *p)
+{
+  const int N = p->N, NP = p->tnp, NB = NP / 16, KS = NP / 4, nblk = 2 * NB * (NB + 1);
+  const size_t dn = (size_t)NP * NP;
+  double *dT = nullptr;
+  if (hipMalloc(&dT, sizeof(double) * 2 * dn) != hipSuccess) return PP_EHIP;
+  std::vector<double> T(2 * dn), B(2 * (size_t)nblk * 64, 0.0);
+  bool ok = hipMemset(dT, 0, sizeof(double) * 2 * dn) == hipSuccess;
+  if (ok) {
+    hipLaunchKernelGGL(p3_tinv_build, dim3((N + 63) / 64), dim3(64), 0, 0, p->ov->Uband, dT, dT + dn, N, NP, p->ov->kd);
+    ok = hipGetLastError() == hipSuccess &&
+         hipMemcpy(T.data(), dT, sizeof(double) * 2 * dn, hipMemcpyDeviceToHost) == hipSuccess;
+  }
+  (void)hipFree(dT);
+  if (!ok) return PP_EHIP;
+  for (int op = 0; op < 2; op++)
+    for (int rb = 0; rb < NB; rb++)
+      for (int ks = p3_tblk_first(op, rb); ks < p3_tblk_end(op, rb, KS); ks++) {
+        double *blk = B.data() + ((size_t)op * nblk + p3_tblk_base(op, rb, KS) + ks - p3_tblk_first(op, rb)) * 64;
+        for (int l = 0; l < 64; l++) blk[l] = T[(size_t)op * dn + (size_t)(16 * rb + (l & 15)) * NP + 4 * ks + (l >> 4)];
+      }
+  if (hipMalloc(&p->tinv, sizeof(double) * B.size()) != hipSuccess) return PP_EHIP;
+  return hipMemcpy(p->tinv, B.data(), sizeof(double) * B.size(), hipMemcpyHostToDevice) == hipSuccess ? PP_OK : PP_EHIP;
+}
+
 extern "C" int pp_pamp3_destroy(pp_pamp3 *p);
 
 extern "C" int pp_pamp3_create(int C, int N, int m, int reorth, const pp_overlap *ov, pp_pamp3 **out)
@@ -992,6 +1178,7 @@ extern "C" int pp_pamp3_create(int C, int N, int m, int reorth, const pp_overlap
   p->K = 0; p->nblk = 0; p->nchunk = (C + P3_CHUNK - 1) / P3_CHUNK;
   p->scal = p->diag = nullptr; p->scal_cap = p->diag_cap = 0;
   p->timing = 0; p->ee_ms = 0.0; p->ee_count = 0;
+  p->tinv = nullptr; p->tnp = 0;
   const size_t vb = sizeof(zd) * (size_t)p->n;
   bool ok = hipMalloc(&p->Y, vb) == hipSuccess && hipMalloc(&p->Z, vb) == hipSuccess &&
             hipMalloc(&p->P, vb) == hipSuccess && hipMalloc(&p->V, vb * (size_t)(m + 1)) == hipSuccess &&
@@ -1003,6 +1190,9 @@ extern "C" int pp_pamp3_create(int C, int N, int m, int reorth, const pp_overlap
   if (!ok) { pp_pamp3_destroy(p); return PP_EHIP; }
   hipLaunchKernelGGL(p3_recip_diag, dim3(1), dim3(256), 0, 0, ov->Uband, p->urd, N, ov->kd);
   if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) { pp_pamp3_destroy(p); return PP_EHIP; }
+  // U^-1 and U^-T as MFMA blocks for the product form of the overlap solves
+  p->tnp = P3_TPROD ? (N <= 64 ? 64 : N <= 128 ? 128 : 0) : 0;
+  if (p->tnp && p3_tprod_build(p) != PP_OK) { pp_pamp3_destroy(p); return PP_EHIP; }
   *out = p;
   return PP_OK;
 }
@@ -1013,7 +1203,7 @@ extern "C" int pp_pamp3_destroy(pp_pamp3 *p)
   p3_collect(p);
   p3_free_plan(p);
   void *bufs[] = {p->Y, p->Z, p->P, p->V, p->part, p->Hm, p->cvec, p->hcoef, p->beta, p->state, p->scal, p->diag,
-                  p->urd};
+                  p->urd, p->tinv};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &t : p->mterms)
@@ -1343,8 +1533,27 @@ static int p3_apply_raw(pp_pamp3 *p, const zd *y, zd *z, const double *scal, hip
 }
 
 // op 0: (U x U)^-1, 1: (U x U)^-H, 2: (U x U) on both radial ranks.
+template <int NB>
+static int p3_tsolve2(pp_pamp3 *p, int op, zd *x, hipStream_t st)
+{
+  const int C = p->C, N = p->N, nblk = 2 * NB * (NB + 1);
+  const double *T = p->tinv + (op == 0 ? 0 : (size_t)nblk * 64);
+  const dim3 g2((unsigned)(((long)C * N + 7) / 8)), g1((unsigned)(C * ((2 * N + 15) / 16)));
+  double *xd = (double *)x;
+  if (op == 0) {
+    hipLaunchKernelGGL((p3_tgemm<NB, 0, 2>), g2, dim3(64), 0, st, T, xd, C, N);
+    hipLaunchKernelGGL((p3_tgemm<NB, 0, 1>), g1, dim3(64), 0, st, T, xd, C, N);
+  } else {
+    hipLaunchKernelGGL((p3_tgemm<NB, 1, 2>), g2, dim3(64), 0, st, T, xd, C, N);
+    hipLaunchKernelGGL((p3_tgemm<NB, 1, 1>), g1, dim3(64), 0, st, T, xd, C, N);
+  }
+  return hipGetLastError() == hipSuccess ? PP_OK : PP_EHIP;
+}
+
 static int p3_solve2(pp_pamp3 *p, int op, zd *x, hipStream_t st)
 {
+  if (op != 2 && p->tnp == 64) return p3_tsolve2<4>(p, op, x, st);
+  if (op != 2 && p->tnp == 128) return p3_tsolve2<8>(p, op, x, st);
   const int C = p->C, N = p->N;
   const size_t lds = sizeof(double) * ((size_t)N * 7 + N + ((N * 7 + N) & 1)) + sizeof(zd) * (size_t)P3_FT * (N + 1);
   const dim3 g2((unsigned)(((long)C * N + P3_FT - 1) / P3_FT)), g1((unsigned)(C * ((N + P3_FT - 1) / P3_FT)));

@@ -186,6 +186,35 @@ def test_helium_apply_multichunk_shapes():
     assert np.abs(out.cpu().numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("lmax,xsize", [(1, 5), (3, 67), (1, 127)])
+def test_helium_balanced_apply_matches_dense_solves(lmax, xsize):
+    """U^-H H U^-1 of the engine (overlap solves on both radial ranks as
+    products with the dense triangular inverses, padded to 64, 128 and 256
+    rows at N = 8, 70, 130) against the factored terms between dense
+    triangular solves with numpy's Cholesky factor of S."""
+    import scipy.linalg
+    from pyprop_amd.helium import band_dense
+    over = {("AngularRepresentation", "index_iterator"):
+            P.DefaultCoupledIndexIterator(lmax=lmax, L=list(range(lmax + 1)), M=[0]),
+            ("RadialRepresentation", "xsize"): xsize, ("RadialRepresentation", "xmax"): 30.0}
+    prob = _problem(overrides=over)
+    C, N, _ = prob.psi.GetData().shape
+    print("helium balanced apply, %d channels, N=%d" % (C, N), flush=True)
+    x = _rand((C, N, N), 31)
+    s = 0.43
+    t = dev.torch()
+    xd = dev.to_device(x)
+    out = t.zeros_like(xd)
+    prob.Propagator.Engine.apply(xd, out, scal=[s], balanced=True)
+    S = band_dense(prob.psi.GetRepresentation().RadialMatrices()["S"])
+    U = np.linalg.cholesky(S).T                       # S = U^T U, U upper (real)
+    Ui = scipy.linalg.solve_triangular(U, np.eye(N))  # U^-1
+    y = Ui @ x @ Ui.T                                 # U^-1 on both radial ranks
+    z = _factored_apply(prob.Propagator.BasePropagator.PotentialList, y, [s])
+    ref = Ui.T @ z @ Ui                               # U^-H on both radial ranks
+    assert np.abs(out.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
 def test_helium_workload_size_properties():
     """The helium workload itself (66 channels, N = 120, 950,400 amplitudes):
     the imaginary-time ground state is stationary under field-free steps and
