@@ -52,6 +52,9 @@ struct p3_ent {            // one coupling of a banded pass or the mixing
   int src, col, mat, slot; // source buffer, source channel, radial matrix, time slot (-1: 1.0)
   double cre, cim;
 };
+struct p3_grp {            // entries of one i2-pass row sharing a band matrix and time slot
+  int m0, m1, mat, slot;   // their channel mix: p3_ent m0 .. m1-1 (col, cre, cim)
+};
 
 // Working precision of the split multipole stage: the FP64 form keeps the
 // mixed input, the far-field intermediates and the radial tables in double;
@@ -106,7 +109,8 @@ struct pp_pamp3 {
   // device plan
   double *dmats;
   int K;                   // intermediates of the banded passes
-  int *rrow, *lrow, *mrow; // CSR row pointers (right: K*C, left: C, mix: nblk*16)
+  int *rrow, *lrow, *mrow; // CSR row pointers (right: K*C rows of groups, left: C, mix: nblk*16)
+  p3_grp *rgrp;            // right pass: (matrix, slot) groups, each a channel mix in rent
   p3_ent *rent, *lent, *ment;
   int nblk;                // multipole (term, chunk) blocks
   int nchunk;
@@ -131,63 +135,114 @@ struct pp_pamp3 {
 };
 
 // ------------------------------------------------------------------ banded passes
+// Logical block of dispatched block b of nb: blocks go round-robin over the
+// 8 XCDs (b % 8), so logical blocks L = 0, 1, 2, ... are given to one XCD in
+// order and neighbouring logical blocks share that XCD's L2 (a bijection for
+// any nb).
+__device__ inline int p3_xcd_order(int b, int nb)
+{
+  const int x = b & 7, slot = b >> 3, q = nb >> 3, rem = nb & 7;
+  return x * q + (x < rem ? x : rem) + slot;
+}
+
 #define P3_RT 8            // rows per workgroup of the banded passes
-// X[k][c][i1][i2] = sum_e coef_e * s_e * sum_d B_e[i2][d] * y[c'_e][i1][i2+d-s]
-// workgroup: (row r = k*C + c, P3_RT rows i1); thread: i2.  The matrix row
-// B_e[i2][.] is loaded once per entry and used for all P3_RT rows.
+// X[k][c][i1][i2] = sum_g sum_d B_g[i2][d] * m_g[i1][i2+d-s],
+// m_g = sum_{e in g} coef_e * s_e * y[c'_e]: the row's couplings grouped by
+// band matrix B_g and time slot, each group's channel mix formed first
+// (rows i1a .. i1a+P3_RT-1 and the i2 window of the chunk with its halo, in
+// LDS) and its 13-tap stencil applied once.  workgroup: (row r = k*C + c,
+// P3_RT rows i1, P3_RC columns i2), thread: one i2.  The mixing loads of
+// four window elements are issued together.  Logical blocks run tile-major
+// (every row r of one i1 tile, then the next tile), so the workgroups that
+// read the same rows of y share an XCD.
+#define P3_RC 128          // i2 columns per workgroup of the i2 pass (= its threads)
 template <int S>
-__global__ void __launch_bounds__(128) p3_right(const double *mats, const int *row, const p3_ent *ent, const zd *y,
-                                                 zd *X, int C, int N, const double *scal, const int *state)
+__global__ void __launch_bounds__(P3_RC) p3_right(const double *mats, const int *grow, const p3_grp *grp,
+                                                   const p3_ent *mix, const zd *y, zd *X, int C, int N,
+                                                   const double *scal, const int *state)
 {
   if (state[0]) return;
-  constexpr int s = S, bw = 2 * S + 1;
-  const int ntile = (N + P3_RT - 1) / P3_RT;
-  const int r = blockIdx.x / ntile, i1a = (blockIdx.x % ntile) * P3_RT;   // r = k*C + c
+  constexpr int s = S, bw = 2 * S + 1, WM = P3_RC + 2 * S, NE = P3_RT * WM;
+  __shared__ zd m[P3_RT * WM];
+  const int ntile = (N + P3_RT - 1) / P3_RT, nch = (N + P3_RC - 1) / P3_RC, nr = gridDim.x / (ntile * nch);
+  const int L = p3_xcd_order(blockIdx.x, gridDim.x);
+  const int r = L % nr, tc = L / nr, i1a = (tc / nch) * P3_RT, i2c = (tc % nch) * P3_RC;   // r = k*C + c
   const long NN = (long)N * N;
-  for (int i2 = threadIdx.x; i2 < N; i2 += blockDim.x) {
-    zd acc[P3_RT];
+  const int tid = threadIdx.x, i2 = i2c + tid;
+  zd acc[P3_RT];
 #pragma unroll
-    for (int rr = 0; rr < P3_RT; rr++) acc[rr] = zmk(0.0, 0.0);
-    const int d0 = s - i2 > 0 ? s - i2 : 0, d1 = N - 1 - i2 + s < bw - 1 ? N - 1 - i2 + s : bw - 1;
-    for (int e = row[r]; e < row[r + 1]; e++) {
-      const p3_ent E = ent[e];
-      const double sc = E.slot < 0 ? 1.0 : scal[E.slot];
-      const zd cf = zmk(E.cre * sc, E.cim * sc);
-      const double *Bm = mats + ((size_t)E.mat * N + i2) * bw;
+  for (int rr = 0; rr < P3_RT; rr++) acc[rr] = zmk(0.0, 0.0);
+  for (int gi = grow[r]; gi < grow[r + 1]; gi++) {
+    const p3_grp G = grp[gi];
+    const double sc = G.slot < 0 ? 1.0 : scal[G.slot];
+    for (int base = 0; base < NE; base += 4 * P3_RC) {
+      long off[4];
+      bool ok[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int idx = base + tid + u * P3_RC, rr = idx / WM, j = i2c - s + (idx - rr * WM), i1 = i1a + rr;
+        ok[u] = idx < NE && i1 < N && j >= 0 && j < N;
+        off[u] = ok[u] ? (long)i1 * N + j : 0;   // clamped: the loads need no branches
+      }
+      zd v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) v[u] = zmk(0.0, 0.0);
+      for (int e = G.m0; e < G.m1; e++) {
+        const p3_ent E = mix[e];
+        const zd cf = zmk(E.cre * sc, E.cim * sc);
+        const zd *src = y + E.col * NN;
+        zd t[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) t[u] = src[off[u]];
+#pragma unroll
+        for (int u = 0; u < 4; u++) zfma(v[u], cf, t[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int idx = base + tid + u * P3_RC;
+        if (idx < NE) m[idx] = ok[u] ? v[u] : zmk(0.0, 0.0);
+      }
+    }
+    __syncthreads();
+    if (i2 < N) {
+      const double *Bm = mats + ((size_t)G.mat * N + i2) * bw;
       double bv[bw];
 #pragma unroll
       for (int d = 0; d < bw; d++) bv[d] = Bm[d];
-      const zd *src = y + E.col * NN + i2 - s;
+      const int d0 = s - i2 > 0 ? s - i2 : 0, d1 = N - 1 - i2 + s < bw - 1 ? N - 1 - i2 + s : bw - 1;
 #pragma unroll
       for (int rr = 0; rr < P3_RT; rr++) {
-        const int i1 = i1a + rr;
-        if (i1 >= N) break;
-        const zd *sr = src + (long)i1 * N;
         zd t = zmk(0.0, 0.0);
 #pragma unroll
         for (int d = 0; d < bw; d++)
-          if (d >= d0 && d <= d1) zfmar(t, bv[d], sr[d]);
-        zfma(acc[rr], cf, t);
+          if (d >= d0 && d <= d1) zfmar(t, bv[d], m[rr * WM + tid + d]);
+        acc[rr] = zadd(acc[rr], t);
       }
     }
+    __syncthreads();
+  }
+  if (i2 < N)
 #pragma unroll
     for (int rr = 0; rr < P3_RT; rr++)
       if (i1a + rr < N) X[(long)r * NN + (long)(i1a + rr) * N + i2] = acc[rr];
-  }
 }
 
 // Z[c][i1][i2] = sum_e coef_e * s_e * sum_d A_e[i1][d] * X[k_e][c'_e][i1+d-s][i2]
-// workgroup: (channel c, P3_RT rows i1); thread: i2.  Per entry the column
-// window X[.][i1a-s .. i1a+P3_RT-1+s][i2] is loaded once into registers.
+// workgroup: (channel c, P3_LT rows i1); thread: i2.  Per entry the column
+// window X[.][i1a-s .. i1a+P3_LT-1+s][i2] is loaded once into registers.
+// Logical blocks run tile-major, so neighbouring tiles (overlapping windows)
+// and the channels that read the same sources share an XCD.
+#ifndef P3_LT
 #define P3_LT 4            // rows per workgroup of the i1 pass
+#endif
 template <int S>
 __global__ void __launch_bounds__(128) p3_left(const double *mats, const int *row, const p3_ent *ent, const zd *X,
                                                 zd *Z, int C, int N, const double *scal, const int *state)
 {
   if (state[0]) return;
   constexpr int s = S, bw = 2 * S + 1, WIN = P3_LT + 2 * S;
-  const int ntile = (N + P3_LT - 1) / P3_LT;
-  const int c = blockIdx.x / ntile, i1a = (blockIdx.x % ntile) * P3_LT;
+  const int L = p3_xcd_order(blockIdx.x, gridDim.x);
+  const int c = L % C, i1a = (L / C) * P3_LT;
   const long NN = (long)N * N;
   for (int i2 = threadIdx.x; i2 < N; i2 += blockDim.x) {
     zd acc[P3_LT];
@@ -1106,7 +1161,7 @@ __global__ void __launch_bounds__(256) p3_combine(const zd *V, const zd *cvec, c
 // ------------------------------------------------------------------ host side
 static void p3_free_plan(pp_pamp3 *p)
 {
-  void *bufs[] = {p->dmats, p->rrow, p->lrow, p->mrow, p->rent, p->lent, p->ment, p->cb_ptr, p->cb_blk,
+  void *bufs[] = {p->dmats, p->rrow, p->rgrp, p->lrow, p->mrow, p->rent, p->lent, p->ment, p->cb_ptr, p->cb_blk,
                   p->blk_term, (void *)p->dR, p->X, p->Ym, p->nr, (void *)p->dRn, (void *)p->dPt, (void *)p->dQt,
                   (void *)p->fRnt, (void *)p->fPt, (void *)p->fQt, p->Tq, p->Tp};
   for (void *b : bufs)
@@ -1120,7 +1175,7 @@ static void p3_free_plan(pp_pamp3 *p)
   }
   p->nr = nullptr; p->dRn = p->dPt = p->dQt = nullptr; p->Tq = p->Tp = nullptr; p->split = false;
   p->fRnt = p->fPt = p->fQt = nullptr;
-  p->dmats = nullptr; p->rrow = p->lrow = p->mrow = nullptr; p->rent = p->lent = p->ment = nullptr;
+  p->dmats = nullptr; p->rrow = p->lrow = p->mrow = nullptr; p->rgrp = nullptr; p->rent = p->lent = p->ment = nullptr;
   p->cb_ptr = p->cb_blk = p->blk_term = nullptr; p->dR = nullptr; p->X = nullptr; p->Ym = nullptr;
 }
 
@@ -1171,7 +1226,7 @@ extern "C" int pp_pamp3_create(int C, int N, int m, int reorth, const pp_overlap
   p->C = C; p->N = N; p->m = m; p->reorth = 0; p->s = ov->kd; p->bw = 2 * ov->kd + 1;
   p->n = (long)C * N * N; p->ov = ov;
   p->nslots = 0; p->dirty = true;
-  p->dmats = nullptr; p->rrow = p->lrow = p->mrow = nullptr; p->rent = p->lent = p->ment = nullptr;
+  p->dmats = nullptr; p->rrow = p->lrow = p->mrow = nullptr; p->rgrp = nullptr; p->rent = p->lent = p->ment = nullptr;
   p->cb_ptr = p->cb_blk = p->blk_term = nullptr; p->dR = nullptr; p->X = nullptr; p->Ym = nullptr;
   p->nr = nullptr; p->dRn = p->dPt = p->dQt = nullptr; p->Tq = p->Tp = nullptr; p->split = false;
   p->fRnt = p->fPt = p->fQt = nullptr; p->mixed = 0; p->expmethod = 0;
@@ -1364,15 +1419,36 @@ static int p3_build(pp_pamp3 *p)
     }
     r.swap(out);
   };
-  for (auto &r : rrows) merge_row(r);
   for (auto &r : lrows) merge_row(r);
+  // i2-pass rows: entries grouped by (band matrix, time slot); within a
+  // group the coefficients of one source channel are summed
+  std::vector<int> rgp(1, 0);
+  std::vector<p3_grp> rg;
+  std::vector<p3_ent> rm;
+  for (auto &r : rrows) {
+    std::vector<char> used(r.size(), 0);
+    for (size_t a = 0; a < r.size(); a++) {
+      if (used[a]) continue;
+      p3_grp G{(int)rm.size(), 0, r[a].mat, r[a].slot};
+      for (size_t b = a; b < r.size(); b++) {
+        if (used[b] || r[b].mat != r[a].mat || r[b].slot != r[a].slot) continue;
+        used[b] = 1;
+        bool found = false;
+        for (int q = G.m0; q < (int)rm.size() && !found; q++)
+          if (rm[q].col == r[b].col) { rm[q].cre += r[b].cre; rm[q].cim += r[b].cim; found = true; }
+        if (!found) rm.push_back({0, r[b].col, r[b].mat, r[b].slot, r[b].cre, r[b].cim});
+      }
+      G.m1 = (int)rm.size();
+      rg.push_back(G);
+    }
+    rgp.push_back((int)rg.size());
+  }
   auto flatten = [](const std::vector<std::vector<p3_ent>> &rows, std::vector<int> &ptr, std::vector<p3_ent> &ent) {
     ptr.assign(1, 0);
     for (auto &r : rows) { ent.insert(ent.end(), r.begin(), r.end()); ptr.push_back((int)ent.size()); }
   };
-  std::vector<int> rp, lp, mp;
-  std::vector<p3_ent> re, le, me;
-  flatten(rrows, rp, re);
+  std::vector<int> lp, mp;
+  std::vector<p3_ent> le, me;
   flatten(lrows, lp, le);
   // multipole blocks: (term, chunk) with any row in the chunk
   std::vector<std::vector<p3_ent>> mrows;
@@ -1417,7 +1493,8 @@ static int p3_build(pp_pamp3 *p)
   std::vector<const double *> rptr;
   for (auto &mt : p->mterms) rptr.push_back(mt.R);
   int rc;
-  if ((rc = p3_upload(&p->dmats, mats)) || (rc = p3_upload(&p->rrow, rp)) || (rc = p3_upload(&p->rent, re)) ||
+  if ((rc = p3_upload(&p->dmats, mats)) || (rc = p3_upload(&p->rrow, rgp)) || (rc = p3_upload(&p->rgrp, rg)) ||
+      (rc = p3_upload(&p->rent, rm)) ||
       (rc = p3_upload(&p->lrow, lp)) || (rc = p3_upload(&p->lent, le)) || (rc = p3_upload(&p->mrow, mp)) ||
       (rc = p3_upload(&p->ment, me)) || (rc = p3_upload(&p->cb_ptr, cbp)) || (rc = p3_upload(&p->cb_blk, cbb)) ||
       (rc = p3_upload(&p->blk_term, blk_term)) || (rc = p3_upload(&p->dR, rptr)))
@@ -1480,9 +1557,9 @@ static int p3_apply_raw(pp_pamp3 *p, const zd *y, zd *z, const double *scal, hip
 {
   const int C = p->C, N = p->N;
   if (p->K > 0) {
-    const int ntile = (N + P3_RT - 1) / P3_RT;
-    hipLaunchKernelGGL(p3_right<6>, dim3(p->K * C * ntile), dim3(128), 0, st, p->dmats, p->rrow, p->rent, y, p->X, C,
-                       N, scal, p->state);
+    const int ntile = (N + P3_RT - 1) / P3_RT, nch = (N + P3_RC - 1) / P3_RC;
+    hipLaunchKernelGGL(p3_right<6>, dim3(p->K * C * ntile * nch), dim3(P3_RC), 0, st, p->dmats, p->rrow, p->rgrp,
+                       p->rent, y, p->X, C, N, scal, p->state);
     const int ltile = (N + P3_LT - 1) / P3_LT;
     hipLaunchKernelGGL(p3_left<6>, dim3(C * ltile), dim3(128), 0, st, p->dmats, p->lrow, p->lent, p->X, z, C, N, scal,
                        p->state);
