@@ -814,11 +814,9 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
   const double tol = 1.4901161193847656e-08;           // sqrt(DBL_EPSILON), pamp.h:75
   zd *Va = A.V + (size_t)a * (m + 1) * N;
   const int nwg = L * Q, wid = a * Q + q;
-  double *mypart = A.part + (size_t)wid * A.pstr;
   unsigned *fw = A.subw + (size_t)a * Q;
   zd *sbase = A.sb + (size_t)a * 2 * P * KD;
   const bool lead = a == 0 && q == 0;
-  const int NRM = 2 * m;
   unsigned ep = 0, sep = 0;   // grid and per-fiber barriers passed
   int nsolve = 0;             // split solves done (selects the boundary buffer)
   auto sbuf = [&]() { return sbase + (size_t)((nsolve++) & 1) * P * KD; };
@@ -828,9 +826,63 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
   };
   pp_load_persistent(A, M, a);
 
+  // U^-1 u of the coupled fibers reaches this workgroup without a grid
+  // barrier: every workgroup counts its productions of U^-1 u (yep) and
+  // posts the count in its word of yflag after the values; a consumer waits
+  // until the producers of the rows it stages (at most nslots * Q of them,
+  // one per lane of wavefront 0) have posted the same count.  A producer
+  // overwrites its rows only after the grid barrier of the iteration in which
+  // every consumer staged them.
+  unsigned *yflag = A.subw + (size_t)nwg;
+  unsigned yep = 0;
+  int prod = -1;
+  {
+    const int ylo = r0 - KD > 0 ? r0 - KD : 0, yhi = r0 + nloc + KD < N ? r0 + nloc + KD : N;
+    int cnt = 0;
+    for (int sl = 0; sl < A.nslots; sl++) {
+      const int col = A.slotcols[a * A.nslots + sl];
+      if (col < 0) continue;
+      for (int q2 = 0; q2 < Q; q2++) {
+        const int rlo = (q2 * P / Q) << lb, rhi0 = ((q2 + 1) * P / Q) << lb, rhi = rhi0 < N ? rhi0 : N;
+        if (rlo < yhi && rhi > ylo) {
+          if (cnt == tid) prod = col * Q + q2;
+          cnt++;
+        }
+      }
+    }
+  }
+  auto ypublish = [&]() {
+    yep++;
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's stores of U^-1 u are acknowledged before the count
+    __syncthreads();
+    if (tid == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __hip_atomic_store(yflag + wid, yep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  };
+  auto ywait = [&]() {
+    if (tid < 64) {
+      long spins = 0;
+      while (true) {
+        const bool ok =
+            prod < 0 || (int)(__hip_atomic_load(yflag + prod, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - yep) >= 0;
+        if (__all(ok)) break;
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > (1l << 24)) {
+          if (tid == 0) __hip_atomic_store(&A.bar->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    __syncthreads();
+  };
+  const int HB = 2 * m + 3;   // partial sums per iteration; two buffers alternate by iteration parity
+
   for (int step = 0; step < A.nsteps; step++) {
-    // ---- psi~ = U psi on the own rows (psi rows up to r0+nloc+KD-1, written
-    // by the neighbouring part at the end of the previous step) ; y = U^-1 psi~
+    // ---- u_0 = psi~ = U psi on the own rows (psi rows up to r0+nloc+KD-1,
+    // written by the neighbouring part at the end of the previous step) ;
+    // y_0 = U^-1 u_0
     if (step > 0) group_barrier(fw, &A.bar->err, sep, Q, q);
     const int phi = r0 + nloc + KD < N ? r0 + nloc + KD : N;
     for (int j = r0 + tid; j < phi; j += PP_THREADS) Ys[j - r0] = zld(A.psi + (size_t)a * N + j);
@@ -847,94 +899,104 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
       X[i] = acc;
       nrm += zabs2(acc);
     }
-    nrm = block_sum(nrm, wsum);
-    if (tid == 0) xst(mypart + NRM, nrm);
+    double unrm = block_sum(nrm, wsum);   // this workgroup's part of |u_j|^2
     pp_split_solve<KD, true>(A, M.nl, X, Z, tb, c0, nch, sbuf(), fw, sep, q);
     for (int i = tid; i < nown; i += PP_THREADS) zst(A.y + (size_t)a * N + r0 + i, Z[i]);
-    stamp(step);
-    grid_barrier(A.bar, ep, nwg, wid);
+    ypublish();
     stamp(step);
 
-    double beta0 = 0.0, beta = 0.0;
+    double beta0 = 0.0;
     int mEff = m;
-    bool broken = false;
     for (int j = 0; j < m; j++) {
-      // ---------------- phase A: beta_j, v_j = w/beta, w' = U^-H H U^-1 v_j, <v_k, w'>
-      const int nk = j + 1, ncomp = 2 * nk;
-      // coupled fibers' U^-1 w, rows r0-KD .. r0+nloc+KD-1 inside [0, N): they
-      // do not depend on beta (1/beta goes into the coefficients), so their
-      // loads are issued before the norm reduction and the two round trips
-      // overlap (unused if this iteration breaks down)
+      // ---- Arnoldi iteration j with the normalisation deferred by one
+      // reduction (pamp.h:293-350: v_j = w / |w|, w = H^ v_j, h = V^H w,
+      // w -= V h).  W holds u_j, the unnormalised residual of iteration j-1
+      // (u_0 = psi~), and the coupled fibers' U^-1 u_j have been posted.
+      // z = H^ u_j = U^-H H U^-1 u_j; one grid reduction of <v_k, z>
+      // (k < j), <u_j, z> and |u_j|^2 gives beta_j = |u_j|, v_j = u_j /
+      // beta_j, h_kj = <v_k, z> / beta_j, h_jj = <u_j, z> / beta_j^2 and
+      // u_{j+1} = z / beta_j - sum_k h_kj v_k: the same quantities with H^
+      // applied to u_j instead of v_j (moves results by rounding only), and
+      // one grid barrier per iteration.
+      const int ncomp = 2 * j + 3;
+      double *mypart = A.part + (size_t)wid * A.pstr + (j & 1) * HB;
+      ywait();
       const int ybase = r0 - KD, ylo = ybase > 0 ? ybase : 0;
-      if (!broken)
-        for (int sl = 0; sl < A.nslots; sl++) {
-          const int col = A.slotcols[a * A.nslots + sl];
-          if (col >= 0)
-            for (int jj = ylo + tid; jj < phi; jj += PP_THREADS) Ys[sl * ystr + jj - ybase] = zld(A.y + (size_t)col * N + jj);
-        }
-      grid_partial_sum_wide(A.part, A.pstr, NRM, 1, nwg, hred);
-      beta = sqrt(hred[0]);
+      for (int sl = 0; sl < A.nslots; sl++) {
+        const int col = A.slotcols[a * A.nslots + sl];
+        if (col >= 0)
+          for (int jj = ylo + tid; jj < phi; jj += PP_THREADS) Ys[sl * ystr + jj - ybase] = zld(A.y + (size_t)col * N + jj);
+      }
+      pp_step_coeffs(A, M, step, zmk(1.0, 0.0));
+      __syncthreads();
+      pp_band_rows<KD, MIXED>(A, M, Ys, ystr, ybase, r0, nloc, X);
+      __syncthreads();
+      stamp(step);
+      pp_split_solve<KD, false>(A, M.nl, X, Z, tb, c0, nch, sbuf(), fw, sep, q);   // z in Z (over Ys)
+      stamp(step);
+      if (j > 0) split_dots(Vl, R, KC, Va + r0, N, Z, j, nown, hred);
+      double ure = 0.0, uim = 0.0;
+      for (int i = tid; i < nown; i += PP_THREADS) {
+        const zd u = W[i], zz = Z[i];
+        ure = fma(u.x, zz.x, ure); ure = fma(u.y, zz.y, ure);
+        uim = fma(u.x, zz.y, uim); uim = fma(-u.y, zz.x, uim);
+      }
+      ure = block_sum(ure, wsum);
+      uim = block_sum(uim, wsum);
+      if (tid == 0) { hred[2 * j] = ure; hred[2 * j + 1] = uim; hred[2 * j + 2] = unrm; }
+      __syncthreads();
+      for (int c = tid; c < ncomp; c += PP_THREADS) xst(mypart + c, hred[c]);
+      stamp(step);
+      grid_barrier(A.bar, ep, nwg, wid);
+      stamp(step);
+      grid_partial_sum_wide(A.part + (j & 1) * HB, A.pstr, 0, ncomp, nwg, hred);
+      const double bj = sqrt(hred[2 * j + 2]);
       if (j == 0) {
-        beta0 = beta;
+        beta0 = bj;
         // zero (or non-finite) start vector: no Arnoldi step; exp(A) 0 = 0,
         // and the host reports "Wavefunction is zero" from |psi~| = 0
-        if (!(beta > 0.0 && beta <= 1.79e308)) { broken = true; mEff = 0; }
+        if (!(bj > 0.0 && bj <= 1.79e308)) { mEff = 0; break; }
+      } else {
+        if (fabs(bj) < tol) { mEff = j; break; }   // happy breakdown (pamp.h:305-312)
+        if (lead && tid == 0) A.Hm[j + (j - 1) * m] = zmk(bj, 0.0);
       }
-      else if (!broken) {
-        if (fabs(beta) < tol) { broken = true; mEff = j; }   // happy breakdown (pamp.h:305-312)
-        else if (lead && tid == 0) A.Hm[j + (j - 1) * m] = zmk(beta, 0.0);
+      const double inv = 1.0 / bj, inv2 = inv * inv;
+      if (lead && tid <= j) {
+        const double f = tid < j ? inv : inv2;
+        A.Hm[tid + j * m] = zmk(hred[2 * tid] * f, hred[2 * tid + 1] * f);
       }
-      if (!broken) {
-        const double inv = 1.0 / beta;
-        for (int i = tid; i < nown; i += PP_THREADS) {
-          const zd v = zscale(W[i], inv);
-          Va[(size_t)j * N + r0 + i] = v;
-          if (j < KC) Vl[(size_t)j * R + i] = v;
-        }
-        pp_step_coeffs(A, M, step, zmk(inv, 0.0));
-        __syncthreads();
-        pp_band_rows<KD, MIXED>(A, M, Ys, ystr, ybase, r0, nloc, X);
-        __syncthreads();
-        stamp(step);
-        pp_split_solve<KD, false>(A, M.nl, X, W, tb, c0, nch, sbuf(), fw, sep, q);
-        stamp(step);
-        split_dots(Vl, R, KC, Va + r0, N, W, nk, nown, hred);
-        for (int c = tid; c < ncomp; c += PP_THREADS) xst(mypart + c, hred[c]);
-      }
-      stamp(step);
-      grid_barrier(A.bar, ep, nwg, wid);
-      stamp(step);
-
-      // ---------------- phase B: h = sum of partials ; w' -= V h ; |w'|^2 ; y = U^-1 w'
-      if (!broken) {
-        grid_partial_sum_wide(A.part, A.pstr, 0, ncomp, nwg, hred);
-        if (lead && tid <= j) A.Hm[tid + j * m] = zmk(hred[2 * tid], hred[2 * tid + 1]);
+      for (int i = tid; i < nown; i += PP_THREADS) {
+        const zd v = zscale(W[i], inv);
+        Va[(size_t)j * N + r0 + i] = v;
+        if (j < KC) Vl[(size_t)j * R + i] = v;
       }
       if (j == m - 1) break;   // the last residual is not used (pamp.h:419-426)
-      if (!broken) {
-        double v = 0.0;
-        for (int i = tid; i < nloc; i += PP_THREADS) {
-          zd w = zmk(0.0, 0.0);
-          if (i < nown) {
-            w = W[i];
-            const int kl = j + 1 < KC ? j + 1 : KC;   // LDS copies first, same k order
+      double v2 = 0.0;
+      for (int i = tid; i < nloc; i += PP_THREADS) {
+        zd w = zmk(0.0, 0.0);
+        if (i < nown) {
+          w = zscale(Z[i], inv);
+          const int kl = j + 1 < KC ? j + 1 : KC;   // LDS copies first, same k order
 #pragma unroll PP_VUNROLL
-            for (int k = 0; k < kl; k++) w = zsub(w, zmul(Vl[(size_t)k * R + i], zmk(hred[2 * k], hred[2 * k + 1])));
-#pragma unroll PP_VUNROLL
-            for (int k = kl; k <= j; k++) w = zsub(w, zmul(Va[(size_t)k * N + r0 + i], zmk(hred[2 * k], hred[2 * k + 1])));
+          for (int k = 0; k < kl; k++) {
+            const double f = k < j ? inv : inv2;
+            w = zsub(w, zmul(Vl[(size_t)k * R + i], zmk(hred[2 * k] * f, hred[2 * k + 1] * f)));
           }
-          W[i] = w;
-          X[i] = w;
-          v += zabs2(w);
+#pragma unroll PP_VUNROLL
+          for (int k = kl; k <= j; k++) {
+            const double f = k < j ? inv : inv2;
+            w = zsub(w, zmul(Va[(size_t)k * N + r0 + i], zmk(hred[2 * k] * f, hred[2 * k + 1] * f)));
+          }
         }
-        v = block_sum(v, wsum);
-        if (tid == 0) xst(mypart + NRM, v);
-        stamp(step);
-        pp_split_solve<KD, true>(A, M.nl, X, Z, tb, c0, nch, sbuf(), fw, sep, q);
-        for (int i = tid; i < nown; i += PP_THREADS) zst(A.y + (size_t)a * N + r0 + i, Z[i]);
+        W[i] = w;
+        X[i] = w;
+        v2 += zabs2(w);
       }
+      unrm = block_sum(v2, wsum);
       stamp(step);
-      grid_barrier(A.bar, ep, nwg, wid);
+      pp_split_solve<KD, true>(A, M.nl, X, Z, tb, c0, nch, sbuf(), fw, sep, q);
+      for (int i = tid; i < nown; i += PP_THREADS) zst(A.y + (size_t)a * N + r0 + i, Z[i]);
+      ypublish();
       stamp(step);
     }
 
@@ -1738,8 +1800,11 @@ static int pamp_split_factor(const pp_pamp *p, int &Q)
   const int P = p->ov->plan.P;
   Q = 1;
   if (p->reorth || p->fsplit == 1) return PP_OK;
+  // the split kernel waits on the producers of its coupled rows with one
+  // lane each: nslots * Q of them at most
+  const int qmax = 64 / (p->nslots > 0 ? p->nslots : 1);
   if (p->fsplit >= 2) {
-    if (p->fsplit > P) return PP_EUNSUP;
+    if (p->fsplit > P || p->fsplit > qmax) return PP_EUNSUP;
     Q = p->fsplit;
     return PP_OK;
   }
@@ -1749,6 +1814,7 @@ static int pamp_split_factor(const pp_pamp *p, int &Q)
   int q = cus / p->L;
   if (q > 8) q = 8;
   if (q > P / 2) q = P / 2;
+  if (q > qmax) q = qmax;
   Q = q >= 2 ? q : 1;
   return PP_OK;
 }
@@ -1795,12 +1861,12 @@ static int pamp_split_buffers(pp_pamp *p, int Q)
     PP_HIP(hipMalloc(&p->part, sizeof(double) * nw * p->pstr));
     p->part_cap = nw * p->pstr;
   }
-  if (2 * nw > p->bar_words) {
+  if (3 * nw > p->bar_words) {   // grid arrival words, per-fiber arrival words, U^-1 u production counts
     p->gen++;
     (void)hipFree(p->bar);
     p->bar = nullptr;
-    PP_HIP(hipMalloc(&p->bar, sizeof(pp_barrier) + sizeof(unsigned) * 2 * nw));
-    p->bar_words = 2 * nw;
+    PP_HIP(hipMalloc(&p->bar, sizeof(pp_barrier) + sizeof(unsigned) * 3 * nw));
+    p->bar_words = 3 * nw;
   }
   const size_t nsb = (size_t)p->L * 2 * p->ov->plan.P * p->ov->kd;
   if (nsb > p->sb_cap) {

@@ -570,6 +570,38 @@ def test_step_kernels_repeat_bitwise():
                 q, rep, np.abs(out[0] - out[rep]).max())
 
 
+def test_step_kernels_launch_split_bitwise():
+    """Steps carry nothing from one to the next but psi: 6 steps in one
+    launch, in two launches of 3 and in six launches of 1 (the same time
+    factors) give bitwise the same state, for the fiber-split step kernel
+    (one grid barrier per Arnoldi iteration, U^-1 u handed to the coupled
+    fibers through per-workgroup production counts) and the
+    one-workgroup-per-fiber kernel."""
+    import pyprop_amd
+    from pyprop_amd.examples import hydrogen as H
+    print("hydrogen_large: 6 steps as 1, 2 and 6 launches, split and unsplit", flush=True)
+    prob = pyprop_amd.Problem(H.load("hydrogen_large"))
+    prob.SetupStep()
+    psi0 = H.ground_state(prob)
+    prop = prob.Propagator
+    eng = prop.Engine
+    dt = prob.TimeStep
+    times = 50.0 + dt.real * np.arange(6)
+    for q in (0, 1):
+        eng.set_fiber_split(q)
+        out = []
+        for parts in ((6,), (3, 3), (1,) * 6):
+            prob.SetWavefunction(psi0)
+            k = 0
+            for n in parts:
+                prop.AdvanceSteps(times[k:k + n], dt)
+                k += n
+            out.append(prob.psi.GetData().cpu().numpy())
+        for i in (1, 2):
+            assert np.array_equal(out[0], out[i]), "q=%d: launch split %d differs by %.3e" % (
+                q, i, np.abs(out[0] - out[i]).max())
+
+
 @pytest.mark.parametrize("prop", ["PamPropagator", "ExpokitPropagator"])
 def test_zero_wavefunction_raises(prop):
     """A zero wavefunction through Problem.AdvanceStep: the reference prints
