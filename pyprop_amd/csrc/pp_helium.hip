@@ -46,7 +46,9 @@
 #include "pp_pade.h"
 
 #define P3_CHUNK 16        // channels per multipole chunk
+#ifndef P3_RED_BLOCKS
 #define P3_RED_BLOCKS 480  // blocks of the Gram-Schmidt reductions
+#endif
 
 struct p3_ent {            // one coupling of a banded pass or the mixing
   int src, col, mat, slot; // source buffer, source channel, radial matrix, time slot (-1: 1.0)
