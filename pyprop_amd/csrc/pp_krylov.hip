@@ -90,6 +90,8 @@ struct pp_pamp_args {
   zd *sb;                // split kernel: L x 2 x P*KD chunk boundary values (exchanged)
   unsigned *subw;        // split kernel: L x Q arrival words of the per-fiber barriers
   int vcache;            // split kernel: Krylov vectors whose own rows are also kept in LDS
+  const double *srows;   // split kernel: per part q (stride sstride doubles) the chunk-boundary rows
+  int sstride;           //   of the band solves (pamp_split_rows), copied to LDS in place of the scan matrices
 };
 
 struct pp_pamp {
@@ -114,6 +116,8 @@ struct pp_pamp {
   int fsplit;                              // workgroups per fiber: 0 automatic, 1 none, >= 2 fixed
   size_t bar_words, part_cap, sb_cap;      // allocated arrival words, partial-sum doubles, boundary values
   zd *sb;
+  double *srows;                           // split kernel: chunk-boundary rows per part (pamp_split_rows)
+  int srows_q, sstride;                    // the split factor they were built for, doubles per part
   unsigned gen;                            // bumped whenever a buffer or setting a recorded launch uses changes
 };
 
@@ -316,10 +320,13 @@ __device__ __attribute__((always_inline)) inline pp_lds_map pp_map_lds(const pp_
 // Scan matrices to LDS and the combo table of row a: one entry per
 // (term, rank) with the angular coefficients of its couplings (time factors
 // applied per matvec).
-__device__ __attribute__((always_inline)) inline void pp_load_persistent(const pp_pamp_args &A, const pp_lds_map &M, int a)
+__device__ __attribute__((always_inline)) inline void pp_load_persistent(const pp_pamp_args &A, const pp_lds_map &M, int a,
+                                                                         const double *nsrc = nullptr, int nlen = -1)
 {
   const int tid = threadIdx.x;
-  for (int i = tid; i < A.plan.nlev; i += PP_THREADS) M.nl[i] = A.plan.nlevs[i];
+  // the solver's persistent matrices: the scan matrices, or those given (nsrc, nlen)
+  if (!nsrc) { nsrc = A.plan.nlevs; nlen = A.plan.nlev; }
+  for (int i = tid; i < nlen; i += PP_THREADS) M.nl[i] = nsrc[i];
   if (tid == 0) {
     int k = 0;
     for (int t = 0; t < A.nterms; t++) {
@@ -747,16 +754,27 @@ __device__ inline void grid_partial_sum_wide(const double *part, int pstr, int o
 }
 
 // x = U^-1 b (UPPER) or U^-T b over the own chunks c0 .. c0+nch-1 of one
-// fiber (in/out: those chunks' rows, LDS; tb: nch*KD complex).  All Q
-// workgroups of the fiber call it in the same sequence.  Ends with a
-// __syncthreads.
+// fiber (in/out: those chunks' rows, LDS; tb: nch*KD complex; bb: P*KD
+// complex).  All Q workgroups of the fiber call it in the same sequence.
+// Stage 1 on the own chunks; their boundary values to the fiber's exchange
+// slot and a barrier among the fiber's workgroups; then, instead of the
+// log-depth scan, the boundary term every own chunk needs from its
+// neighbour (x_{c+1} for U^-1, x_{c-1} for U^-T) as one dot product with a
+// precomputed row over the other chunks' boundary values: the scan's
+// recurrence x_c = b_c + N_c x_{c+1} (x_c = b_c + N'_c x_{c-1}) has the
+// solution x_c = sum_{c' >= c} N_c N_{c+1} .. N_{c'-1} b_{c'}, and the
+// workgroup keeps the rows of those products for its own chunks (srows,
+// pamp_split_rows).  Each row's terms are added by SEG lanes, lane s
+// taking columns s, s+SEG, ..., then a fixed butterfly over the SEG lanes.
+// Stage 3 on the own chunks.  Ends with a __syncthreads.
 template <int KD, bool UPPER>
-__device__ __attribute__((always_inline)) inline void pp_split_solve(const pp_pamp_args &A, const double *nl, const zd *in,
-                                                                     zd *out, zd *tb, int c0, int nch, zd *sbuf,
-                                                                     unsigned *fw, unsigned &sep, int q)
+__device__ __attribute__((always_inline)) inline void pp_split_solve(const pp_pamp_args &A, const double *rows,
+                                                                     const zd *in, zd *out, zd *tb, zd *bb, int c0,
+                                                                     int nch, zd *sbuf, unsigned *fw, unsigned &sep,
+                                                                     int q)
 {
   const pp_band_plan &pl = A.plan;
-  const int tid = threadIdx.x, lb = pl.lb, off = UPPER ? 0 : pl.B - KD;
+  const int tid = threadIdx.x, lb = pl.lb, off = UPPER ? 0 : pl.B - KD, P = pl.P;
   if (tid < PP_SOLVE_THREADS) pp_solve_blocks<UPPER, PP_SOLVE_THREADS>(pl, in, out, c0, nch);
   __syncthreads();
   for (int e = tid; e < nch * KD; e += PP_THREADS) {
@@ -764,21 +782,55 @@ __device__ __attribute__((always_inline)) inline void pp_split_solve(const pp_pa
     zst(sbuf + (size_t)(c0 + cl) * KD + k, out[(cl << lb) + off + k]);
   }
   group_barrier(fw, &A.bar->err, sep, A.Q, q);
-  if (tid < 64)
-    pp_solve_scan_src<KD, UPPER>(pl, nl, [&](int c, int k) { return zld(sbuf + (size_t)c * KD + k); }, tb, c0, c0 + nch);
+  for (int e = tid; e < P * KD; e += PP_THREADS) {
+    const int c = e / KD, k = e - c * KD;
+    bb[e] = (c >= c0 && c < c0 + nch) ? out[((c - c0) << lb) + off + k] : zld(sbuf + e);
+  }
+  __syncthreads();
+  const int nr = nch * KD;
+  int SEG = 64;
+  while (SEG > 1 && SEG * nr > PP_THREADS) SEG >>= 1;
+  const int r = tid / SEG, sg = tid - r * SEG;
+  zd acc = zmk(0.0, 0.0);
+  if (r < nr) {
+    const int cl = r / KD, i = r - cl * KD, c = c0 + cl;
+    // this part's rows: the U^-1 block (row length (P-c-1)*KD, columns from
+    // chunk c+1 on), then the U^-T block (row length c*KD, columns from chunk 0)
+    const int kk = KD * KD;
+    const int ubase = kk * (cl * (P - c0 - 1) - cl * (cl - 1) / 2);
+    const int lbase = kk * (nch * (P - c0 - 1) - nch * (nch - 1) / 2) + kk * (cl * c0 + cl * (cl - 1) / 2);
+    const int len = UPPER ? (P - c - 1) * KD : c * KD;
+    const double *row = rows + (UPPER ? ubase : lbase) + i * len;
+    const zd *b = UPPER ? bb + (c + 1) * KD : bb;
+#pragma unroll 8
+    for (int j = sg; j < len; j += SEG) zfmar(acc, row[j], b[j]);
+  }
+  for (int o = SEG >> 1; o > 0; o >>= 1) {
+    acc.x += __shfl_xor(acc.x, o, 64);
+    acc.y += __shfl_xor(acc.y, o, 64);
+  }
+  if (r < nr && sg == 0) tb[r] = acc;
   __syncthreads();
   if (tid < PP_SOLVE_THREADS) pp_solve_fixup<KD, UPPER, PP_SOLVE_THREADS>(pl, tb, out, c0, nch);
   __syncthreads();
 }
 
 // LDS doubles of the split kernel's per-workgroup buffers above the
-// persistent scan matrices and combo table (maxch = chunks per workgroup,
-// rounded up): W, X (maxch*B each), tb, the staged coupled rows, the
-// reduction scratch and exp(-i dt H) e1.
-__host__ __device__ inline int pp_split_doubles(int nslots, int m, int kd, int B, int maxch)
+// persistent chunk-boundary rows and combo table (maxch = chunks per
+// workgroup, rounded up; P chunks per fiber): W, X (maxch*B each), tb, all
+// chunks' boundary values (P*kd), the staged coupled rows, the reduction
+// scratch and exp(-i dt H) e1.
+__host__ __device__ inline int pp_split_doubles(int nslots, int m, int kd, int B, int maxch, int P)
 {
   const int R = maxch * B;
-  return 2 * (2 * R + maxch * kd + nslots * (R + 2 * kd)) + 16 + (2 * m + 4) + 2 * m;
+  return 2 * (2 * R + maxch * kd + P * kd + nslots * (R + 2 * kd)) + 16 + (2 * m + 4) + 2 * m;
+}
+
+// Doubles of one part's chunk-boundary rows (kd^2 (P-1) per chunk), even.
+__host__ __device__ inline int pp_split_rows_doubles(int kd, int P, int maxch)
+{
+  const int n = kd * kd * (P - 1) * maxch;
+  return n + (n & 1);
 }
 
 template <int KD, bool MIXED>
@@ -798,7 +850,8 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
   M.W = (zd *)(lds_d + A.nlevEven + PP_CMB_DOUBLES);
   M.X = M.W + R;
   M.tb = M.X + R;
-  M.Ys = M.tb + maxch * KD;
+  zd *bb = M.tb + maxch * KD;   // all chunks' boundary values of a split solve
+  M.Ys = bb + P * KD;
   M.Z = M.Ys;
   M.wsum = (double *)(M.Ys + A.nslots * ystr);
   M.hred = M.wsum + 16;
@@ -824,7 +877,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
   auto stamp = [&](int step) {
     if (A.tlog && step == 0 && lead && tid == 0 && tcount < PP_TLOG_MAX) A.tlog[tcount++] = (long long)wall_clock64();
   };
-  pp_load_persistent(A, M, a);
+  pp_load_persistent(A, M, a, A.srows + (size_t)q * A.sstride, A.sstride);
 
   // U^-1 u of the coupled fibers reaches this workgroup without a grid
   // barrier: every workgroup counts its productions of U^-1 u (yep) and
@@ -900,7 +953,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
       nrm += zabs2(acc);
     }
     double unrm = block_sum(nrm, wsum);   // this workgroup's part of |u_j|^2
-    pp_split_solve<KD, true>(A, M.nl, X, Z, tb, c0, nch, sbuf(), fw, sep, q);
+    pp_split_solve<KD, true>(A, M.nl, X, Z, tb, bb, c0, nch, sbuf(), fw, sep, q);
     for (int i = tid; i < nown; i += PP_THREADS) zst(A.y + (size_t)a * N + r0 + i, Z[i]);
     ypublish();
     stamp(step);
@@ -932,7 +985,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
       pp_band_rows<KD, MIXED>(A, M, Ys, ystr, ybase, r0, nloc, X);
       __syncthreads();
       stamp(step);
-      pp_split_solve<KD, false>(A, M.nl, X, Z, tb, c0, nch, sbuf(), fw, sep, q);   // z in Z (over Ys)
+      pp_split_solve<KD, false>(A, M.nl, X, Z, tb, bb, c0, nch, sbuf(), fw, sep, q);   // z in Z (over Ys)
       stamp(step);
       if (j > 0) split_dots(Vl, R, KC, Va + r0, N, Z, j, nown, hred);
       double ure = 0.0, uim = 0.0;
@@ -994,7 +1047,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
       }
       unrm = block_sum(v2, wsum);
       stamp(step);
-      pp_split_solve<KD, true>(A, M.nl, X, Z, tb, c0, nch, sbuf(), fw, sep, q);
+      pp_split_solve<KD, true>(A, M.nl, X, Z, tb, bb, c0, nch, sbuf(), fw, sep, q);
       for (int i = tid; i < nown; i += PP_THREADS) zst(A.y + (size_t)a * N + r0 + i, Z[i]);
       ypublish();
       stamp(step);
@@ -1025,7 +1078,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
       X[i] = zscale(acc, beta0);
     }
     __syncthreads();
-    pp_split_solve<KD, true>(A, M.nl, X, Z, tb, c0, nch, sbuf(), fw, sep, q);
+    pp_split_solve<KD, true>(A, M.nl, X, Z, tb, bb, c0, nch, sbuf(), fw, sep, q);
     for (int i = tid; i < nown; i += PP_THREADS) zst(A.psi + (size_t)a * N + r0 + i, Z[i]);
     __syncthreads();
     stamp(step);
@@ -1430,7 +1483,7 @@ extern "C" int pp_pamp_create(int L, int N, int m, int reorth, const pp_overlap 
   p->gen = 0;
   p->part = p->diag = p->scal = nullptr;
   p->bar = nullptr; p->tlog = nullptr; p->timing = 0;
-  p->fsplit = 0; p->sb = nullptr; p->sb_cap = 0;
+  p->fsplit = 0; p->sb = nullptr; p->sb_cap = 0; p->srows = nullptr; p->srows_q = 0; p->sstride = 0;
   p->bar_words = (size_t)L; p->part_cap = (size_t)L * p->pstr;
   p->diag_cap = p->scal_cap = 0;
   for (int t = 0; t < PP_MAXTERMS; t++) p->termbuf[t] = nullptr;
@@ -1824,9 +1877,9 @@ static int pamp_split_factor(const pp_pamp *p, int &Q)
 static size_t pamp_split_lds_bytes(const pp_pamp *p, int Q)
 {
   const pp_band_plan &pl = p->ov->plan;
-  const size_t nle = (size_t)((pl.nlev + 1) & ~1) + PP_CMB_DOUBLES;
   const int maxch = (pl.P + Q - 1) / Q;
-  const size_t fib = sizeof(double) * (nle + (size_t)pp_split_doubles(p->nslots, p->m, p->ov->kd, pl.B, maxch)) + 64;
+  const size_t nle = (size_t)pp_split_rows_doubles(p->ov->kd, pl.P, maxch) + PP_CMB_DOUBLES;
+  const size_t fib = sizeof(double) * (nle + (size_t)pp_split_doubles(p->nslots, p->m, p->ov->kd, pl.B, maxch, pl.P)) + 64;
   const size_t pade = sizeof(double) * (nle + (size_t)pp_pade_lds_doubles(p->m)) + 64;
   return fib > pade ? fib : pade;
 }
@@ -1837,16 +1890,83 @@ static size_t pamp_split_lds_bytes(const pp_pamp *p, int Q)
 static int pamp_split_vcache(const pp_pamp *p, int Q, size_t &lds)
 {
   const pp_band_plan &pl = p->ov->plan;
-  const size_t nle = (size_t)((pl.nlev + 1) & ~1) + PP_CMB_DOUBLES;
   const int maxch = (pl.P + Q - 1) / Q;
+  const size_t nle = (size_t)pp_split_rows_doubles(p->ov->kd, pl.P, maxch) + PP_CMB_DOUBLES;
   const size_t R = (size_t)maxch << pl.lb;
-  const size_t fib = sizeof(double) * (nle + (size_t)pp_split_doubles(p->nslots, p->m, p->ov->kd, pl.B, maxch)) + 64;
+  const size_t fib = sizeof(double) * (nle + (size_t)pp_split_doubles(p->nslots, p->m, p->ov->kd, pl.B, maxch, pl.P)) + 64;
   lds = pamp_split_lds_bytes(p, Q);
   const size_t cap = 160 * 1024, per = sizeof(zd) * R;
   long kc = fib < cap ? (long)((cap - fib) / per) : 0;
   if (kc > p->m) kc = p->m;
   if (fib + per * (size_t)kc > lds) lds = fib + per * (size_t)kc;
   return (int)kc;
+}
+
+// Chunk-boundary rows of the split band solves for split factor Q: the
+// scan of the band solver solves x_c = b_c + N_c x_{c+1} (U^-1) and x_c =
+// b_c + N'_c x_{c-1} (U^-T) over the chunks' boundary values, with N_c, N'_c
+// its level-0 matrices (the first kd*kd*P doubles of each half of the plan's
+// scan matrices, [(i*kd + k)*P + c]).  Part q (chunks c0 .. c0+nch-1) keeps,
+// per own chunk c and component i, the row of x_{c+1} = sum_{c' > c}
+// (N_{c+1} .. N_{c'-1}) b_{c'} over (c', k) for c' = c+1 .. P-1, then the
+// rows of x_{c-1} = sum_{c' < c} (N'_{c-1} .. N'_{c'+1}) b_{c'} over
+// c' = 0 .. c-1 (pp_split_solve reads them in this order); FP64 products on
+// the host, once per split factor.
+static int pamp_split_rows(pp_pamp *p, int Q)
+{
+  if (p->srows && p->srows_q == Q) return PP_OK;
+  const pp_band_plan &pl = p->ov->plan;
+  const int P = pl.P, kd = p->ov->kd, kk = kd * kd, maxch = (P + Q - 1) / Q;
+  const size_t nL = (size_t)pl.levels * kk * P;
+  std::vector<double> n0((size_t)kk * P), n0t((size_t)kk * P);
+  PP_HIP(hipMemcpy(n0.data(), pl.nlevs, sizeof(double) * n0.size(), hipMemcpyDeviceToHost));
+  PP_HIP(hipMemcpy(n0t.data(), pl.nlevs + nL, sizeof(double) * n0t.size(), hipMemcpyDeviceToHost));
+  const int stride = pp_split_rows_doubles(kd, P, maxch);
+  std::vector<double> rows((size_t)stride * Q, 0.0), T(kk), T2(kk);
+  auto times = [&](const std::vector<double> &Nm, int c) {   // T := T N_c
+    for (int i = 0; i < kd; i++)
+      for (int k = 0; k < kd; k++) {
+        double a = 0.0;
+        for (int u = 0; u < kd; u++) a += T[(size_t)i * kd + u] * Nm[((size_t)u * kd + k) * P + c];
+        T2[(size_t)i * kd + k] = a;
+      }
+    T.swap(T2);
+  };
+  auto identity = [&]() { for (int e = 0; e < kk; e++) T[e] = (e % (kd + 1) == 0) ? 1.0 : 0.0; };
+  for (int q = 0; q < Q; q++) {
+    const int c0 = q * P / Q, nch = (q + 1) * P / Q - c0;
+    double *out = rows.data() + (size_t)q * stride;
+    size_t o = 0;
+    for (int cl = 0; cl < nch; cl++) {   // x_{c+1}: columns c' = c+1 .. P-1
+      const int c = c0 + cl, len = (P - c - 1) * kd;
+      identity();
+      for (int cp = c + 1; cp < P; cp++) {
+        for (int i = 0; i < kd; i++)
+          for (int k = 0; k < kd; k++) out[o + (size_t)i * len + (size_t)(cp - c - 1) * kd + k] = T[(size_t)i * kd + k];
+        if (cp + 1 < P) times(n0, cp);
+      }
+      o += (size_t)kd * len;
+    }
+    for (int cl = 0; cl < nch; cl++) {   // x_{c-1}: columns c' = 0 .. c-1
+      const int c = c0 + cl, len = c * kd;
+      identity();
+      for (int cp = c - 1; cp >= 0; cp--) {
+        for (int i = 0; i < kd; i++)
+          for (int k = 0; k < kd; k++) out[o + (size_t)i * len + (size_t)cp * kd + k] = T[(size_t)i * kd + k];
+        if (cp > 0) times(n0t, cp);
+      }
+      o += (size_t)kd * len;
+    }
+    if (o > (size_t)stride) return PP_EUNSUP;   // cannot happen: kd^2 (P-1) per chunk
+  }
+  if (p->srows) (void)hipFree(p->srows);
+  p->srows = nullptr;
+  p->gen++;
+  PP_HIP(hipMalloc(&p->srows, sizeof(double) * rows.size()));
+  PP_HIP(hipMemcpy(p->srows, rows.data(), sizeof(double) * rows.size(), hipMemcpyHostToDevice));
+  p->srows_q = Q;
+  p->sstride = stride;
+  return PP_OK;
 }
 
 // Buffers of the split kernel: partial sums of L*Q workgroups, grid and
@@ -1889,7 +2009,7 @@ extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re
   int Q = 1;
   int rc = pamp_split_factor(p, Q);
   if (rc) return rc;
-  if (Q > 1 && (rc = pamp_split_buffers(p, Q))) return rc;
+  if (Q > 1 && ((rc = pamp_split_buffers(p, Q)) || (rc = pamp_split_rows(p, Q)))) return rc;
   pp_pamp_args A;
   rc = pamp_prepare(p, psi, nsteps, dt_re, dt_im, scal, stream, A);
   if (rc) return rc;
@@ -1898,6 +2018,9 @@ extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re
   if (Q > 1) {
     A.Q = Q;
     A.sb = p->sb;
+    A.srows = p->srows;
+    A.sstride = p->sstride;
+    A.nlevEven = p->sstride;   // the chunk-boundary rows take the scan matrices' place in LDS
     A.subw = (unsigned *)(p->bar + 1) + (size_t)p->L * Q;
     size_t lds = 0;
     A.vcache = pamp_split_vcache(p, Q, lds);
@@ -2295,6 +2418,7 @@ extern "C" int pp_pamp_destroy(pp_pamp *p)
   if (p->scal) (void)hipFree(p->scal);
   if (p->bar) (void)hipFree(p->bar);
   if (p->sb) (void)hipFree(p->sb);
+  if (p->srows) (void)hipFree(p->srows);
   delete p;
   return PP_OK;
 }
