@@ -833,6 +833,76 @@ __host__ __device__ inline int pp_split_rows_doubles(int kd, int P, int maxch)
   return n + (n & 1);
 }
 
+// Hand-over of U^-1 v between the split kernels' workgroups without a grid
+// barrier: every workgroup counts its productions of U^-1 v (ep) and posts
+// the count in its word of `flag` after the values; a consumer waits until
+// the producers of the rows it stages (at most nslots * Q of them, one per
+// lane of wavefront 0) have posted the same count.  A producer overwrites
+// its rows only after a grid barrier that every consumer reaches after
+// staging them.
+struct pp_yx {
+  unsigned *flag;
+  unsigned ep;
+  int prod, wid;
+};
+
+// rows r0 .. r0+nloc-1 of fiber a staged with a halo of KD rows
+__device__ inline pp_yx pp_yx_init(const pp_pamp_args &A, int a, int wid, int nwg, int r0, int nloc, int KD)
+{
+  const int N = A.N, Q = A.Q, P = A.plan.P, lb = A.plan.lb, tid = threadIdx.x;
+  pp_yx y;
+  y.flag = A.subw + (size_t)nwg;
+  y.ep = 0;
+  y.prod = -1;
+  y.wid = wid;
+  const int ylo = r0 - KD > 0 ? r0 - KD : 0, yhi = r0 + nloc + KD < N ? r0 + nloc + KD : N;
+  int cnt = 0;
+  for (int sl = 0; sl < A.nslots; sl++) {
+    const int col = A.slotcols[a * A.nslots + sl];
+    if (col < 0) continue;
+    for (int q2 = 0; q2 < Q; q2++) {
+      const int rlo = (q2 * P / Q) << lb, rhi0 = ((q2 + 1) * P / Q) << lb, rhi = rhi0 < N ? rhi0 : N;
+      if (rlo < yhi && rhi > ylo) {
+        if (cnt == tid) y.prod = col * Q + q2;
+        cnt++;
+      }
+    }
+  }
+  return y;
+}
+
+// after this workgroup's stores of U^-1 v (all threads call it)
+__device__ inline void pp_yx_publish(pp_yx &y)
+{
+  y.ep++;
+  __builtin_amdgcn_s_waitcnt(0);   // this wave's stores are acknowledged before the count
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __hip_atomic_store(y.flag + y.wid, y.ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// until the producers have posted this workgroup's own count (all threads call it)
+__device__ inline void pp_yx_wait(const pp_yx &y, unsigned *err)
+{
+  if (threadIdx.x < 64) {
+    long spins = 0;
+    while (true) {
+      const bool ok =
+          y.prod < 0 || (int)(__hip_atomic_load(y.flag + y.prod, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - y.ep) >= 0;
+      if (__all(ok)) break;
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > (1l << 24)) {
+        if (threadIdx.x == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+  __syncthreads();
+}
+
 template <int KD, bool MIXED>
 __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args A)
 {
@@ -879,57 +949,8 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
   };
   pp_load_persistent(A, M, a, A.srows + (size_t)q * A.sstride, A.sstride);
 
-  // U^-1 u of the coupled fibers reaches this workgroup without a grid
-  // barrier: every workgroup counts its productions of U^-1 u (yep) and
-  // posts the count in its word of yflag after the values; a consumer waits
-  // until the producers of the rows it stages (at most nslots * Q of them,
-  // one per lane of wavefront 0) have posted the same count.  A producer
-  // overwrites its rows only after the grid barrier of the iteration in which
-  // every consumer staged them.
-  unsigned *yflag = A.subw + (size_t)nwg;
-  unsigned yep = 0;
-  int prod = -1;
-  {
-    const int ylo = r0 - KD > 0 ? r0 - KD : 0, yhi = r0 + nloc + KD < N ? r0 + nloc + KD : N;
-    int cnt = 0;
-    for (int sl = 0; sl < A.nslots; sl++) {
-      const int col = A.slotcols[a * A.nslots + sl];
-      if (col < 0) continue;
-      for (int q2 = 0; q2 < Q; q2++) {
-        const int rlo = (q2 * P / Q) << lb, rhi0 = ((q2 + 1) * P / Q) << lb, rhi = rhi0 < N ? rhi0 : N;
-        if (rlo < yhi && rhi > ylo) {
-          if (cnt == tid) prod = col * Q + q2;
-          cnt++;
-        }
-      }
-    }
-  }
-  auto ypublish = [&]() {
-    yep++;
-    __builtin_amdgcn_s_waitcnt(0);   // this wave's stores of U^-1 u are acknowledged before the count
-    __syncthreads();
-    if (tid == 0) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __hip_atomic_store(yflag + wid, yep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  };
-  auto ywait = [&]() {
-    if (tid < 64) {
-      long spins = 0;
-      while (true) {
-        const bool ok =
-            prod < 0 || (int)(__hip_atomic_load(yflag + prod, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - yep) >= 0;
-        if (__all(ok)) break;
-        __builtin_amdgcn_s_sleep(1);
-        if (++spins > (1l << 24)) {
-          if (tid == 0) __hip_atomic_store(&A.bar->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-      }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    }
-    __syncthreads();
-  };
+  // U^-1 u of the coupled fibers arrives through production counts (pp_yx)
+  pp_yx yx = pp_yx_init(A, a, wid, nwg, r0, nloc, KD);
   const int HB = 2 * m + 3;   // partial sums per iteration; two buffers alternate by iteration parity
 
   for (int step = 0; step < A.nsteps; step++) {
@@ -955,7 +976,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
     double unrm = block_sum(nrm, wsum);   // this workgroup's part of |u_j|^2
     pp_split_solve<KD, true>(A, M.nl, X, Z, tb, bb, c0, nch, sbuf(), fw, sep, q);
     for (int i = tid; i < nown; i += PP_THREADS) zst(A.y + (size_t)a * N + r0 + i, Z[i]);
-    ypublish();
+    pp_yx_publish(yx);
     stamp(step);
 
     double beta0 = 0.0;
@@ -973,7 +994,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
       // one grid barrier per iteration.
       const int ncomp = 2 * j + 3;
       double *mypart = A.part + (size_t)wid * A.pstr + (j & 1) * HB;
-      ywait();
+      pp_yx_wait(yx, &A.bar->err);
       const int ybase = r0 - KD, ylo = ybase > 0 ? ybase : 0;
       for (int sl = 0; sl < A.nslots; sl++) {
         const int col = A.slotcols[a * A.nslots + sl];
@@ -1049,7 +1070,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
       stamp(step);
       pp_split_solve<KD, true>(A, M.nl, X, Z, tb, bb, c0, nch, sbuf(), fw, sep, q);
       for (int i = tid; i < nown; i += PP_THREADS) zst(A.y + (size_t)a * N + r0 + i, Z[i]);
-      ypublish();
+      pp_yx_publish(yx);
       stamp(step);
     }
 
@@ -1457,6 +1478,245 @@ __global__ void __launch_bounds__(PP_THREADS) pp_expokit_kernel(pp_pamp_args A, 
     pp_band_solve<KD, true, PP_SOLVE_THREADS>(pl, nl, X, Z, tb);
     for (int i = tid; i < N; i += PP_THREADS) A.psi[(size_t)a * N + i] = Z[i];
     if (a == 0 && tid == 0) {
+      double *st = E.stats + (size_t)step * 6;
+      st[0] = nmult; st[1] = nexph; st[2] = nstep; st[3] = nreject; st[4] = mbrkdwn; st[5] = iflag;
+    }
+    __syncthreads();
+  }
+}
+
+// Fiber-split form of pp_expokit_kernel for problems with fewer fibers than
+// CUs (the layout of pp_pamp_split_kernel: Q workgroups per fiber, own chunk
+// ranges, split band solves, coupled rows staged with a halo, U^-1 v handed
+// over through production counts).  The same zgexpv: classical
+// Gram-Schmidt applied twice, with the inner products and norms partial over
+// each workgroup's own rows and summed over all L*Q workgroups in one fixed
+// order; every workgroup holds the Hessenberg matrix and runs the
+// exponential on identical data, so the step control takes the same
+// branches everywhere.
+template <int KD, bool MIXED>
+__global__ void __launch_bounds__(PP_THREADS) pp_expokit_split_kernel(pp_pamp_args A, pp_expo_args E)
+{
+  extern __shared__ double lds_d[];
+  const int N = A.N, L = A.L, m = A.m, Q = A.Q, tid = threadIdx.x, mh = m + 2;
+  const pp_band_plan &pl = A.plan;
+  const int P = pl.P, lb = pl.lb;
+  int a, q;
+  pp_split_map(blockIdx.x, L, Q, a, q);
+  const int c0 = q * P / Q, nch = (q + 1) * P / Q - c0, maxch = (P + Q - 1) / Q;
+  const int r0 = c0 << lb, nloc = nch << lb, R = maxch << lb;
+  const int nown = (r0 + nloc < N ? r0 + nloc : N) - r0;
+  const int ystr = R + 2 * KD;
+  pp_lds_map M = pp_map_lds<KD>(A, lds_d);
+  M.W = (zd *)(lds_d + A.nlevEven + PP_CMB_DOUBLES);
+  M.X = M.W + R;
+  M.tb = M.X + R;
+  zd *bb = M.tb + maxch * KD;
+  M.Ys = bb + P * KD;
+  M.Z = M.Ys;
+  M.wsum = (double *)(M.Ys + A.nslots * ystr);
+  M.hred = M.wsum + 16;
+  double *hacc = M.hred + (2 * m + 4);             // 2m doubles (the cv slot of pp_split_doubles)
+  zd *Hm = (zd *)(hacc + 2 * m);                   // mh x mh, column-major
+  zd *ecol = Hm + mh * mh;                         // first column of exp(t H)
+  const int pade_off = (int)((double *)(ecol + mh) - lds_d);
+  zd *W = M.W, *X = M.X, *tb = M.tb, *Ys = M.Ys, *Z = M.Z;
+  double *wsum = M.wsum, *hred = M.hred;
+  zd *Va = A.V + (size_t)a * (m + 1) * N;
+  const int nwg = L * Q, wid = a * Q + q;
+  double *mypart = A.part + (size_t)wid * A.pstr;
+  unsigned *fw = A.subw + (size_t)a * Q;
+  zd *sbase = A.sb + (size_t)a * 2 * P * KD;
+  const bool lead = a == 0 && q == 0;
+  const int D1 = 0, D2 = 2 * m, NRM = 4 * m + 2, ANRM = 4 * m + 3, P0NRM = 4 * m + 4;
+  unsigned ep = 0, sep = 0;
+  int nsolve = 0;
+  auto sbuf = [&]() { return sbase + (size_t)((nsolve++) & 1) * P * KD; };
+  pp_load_persistent(A, M, a, A.srows + (size_t)q * A.sstride, A.sstride);
+  pp_yx yx = pp_yx_init(A, a, wid, nwg, r0, nloc, KD);
+  const zd fac = E.imag ? zmk(-1.0, 0.0) : zmk(0.0, -1.0);
+  const double eps = 2.220446049250313e-16;
+  const double tol = E.tol <= eps ? sqrt(eps) : E.tol;
+  const double rndoff = eps * E.anorm, break_tol = 1e-7, t_out = E.t_out;
+  auto reduce = [&](int slot, int n) { grid_partial_sum_wide(A.part, A.pstr, slot, n, nwg, hred); };
+  // y = U^-1 X (own rows) to the coupled fibers
+  auto publish_y = [&]() {
+    pp_split_solve<KD, true>(A, M.nl, X, Z, tb, bb, c0, nch, sbuf(), fw, sep, q);
+    for (int i = tid; i < nown; i += PP_THREADS) zst(A.y + (size_t)a * N + r0 + i, Z[i]);
+    pp_yx_publish(yx);
+  };
+  // W = fac U^-H H U^-1 v on the own rows, from the coupled fibers' posted U^-1 v
+  auto matvec = [&](int step) {
+    pp_yx_wait(yx, &A.bar->err);
+    const int ybase = r0 - KD, ylo = ybase > 0 ? ybase : 0;
+    const int phi = r0 + nloc + KD < N ? r0 + nloc + KD : N;
+    for (int sl = 0; sl < A.nslots; sl++) {
+      const int col = A.slotcols[a * A.nslots + sl];
+      if (col >= 0)
+        for (int jj = ylo + tid; jj < phi; jj += PP_THREADS) Ys[sl * ystr + jj - ybase] = zld(A.y + (size_t)col * N + jj);
+    }
+    pp_step_coeffs(A, M, step, fac);
+    __syncthreads();
+    pp_band_rows<KD, MIXED>(A, M, Ys, ystr, ybase, r0, nloc, X);
+    __syncthreads();
+    pp_split_solve<KD, false>(A, M.nl, X, W, tb, bb, c0, nch, sbuf(), fw, sep, q);
+  };
+  for (int step = 0; step < A.nsteps; step++) {
+    // ---- psi~ = U psi on the own rows (psi rows up to r0+nloc+KD-1 were
+    // written by the neighbouring part at the end of the previous step)
+    if (step > 0) group_barrier(fw, &A.bar->err, sep, Q, q);
+    const int phi = r0 + nloc + KD < N ? r0 + nloc + KD : N;
+    for (int j = r0 + tid; j < phi; j += PP_THREADS) Ys[j - r0] = zld(A.psi + (size_t)a * N + j);
+    __syncthreads();
+    double nrm = 0.0;
+    for (int i = tid; i < nloc; i += PP_THREADS) {
+      const int gi = r0 + i;
+      zd acc = zmk(0.0, 0.0);
+      if (gi < N) {
+        const int j1 = gi + KD < N - 1 ? gi + KD : N - 1;
+        for (int j = gi; j <= j1; j++) zfmar(acc, A.Uband[(size_t)j * (KD + 1) + KD + gi - j], Ys[j - r0]);
+      }
+      W[i] = acc;
+      nrm += zabs2(acc);
+    }
+    nrm = block_sum(nrm, wsum);
+    if (tid == 0) xst(mypart + P0NRM, nrm);
+    grid_barrier(A.bar, ep, nwg, wid);
+    reduce(P0NRM, 1);
+    double beta = sqrt(hred[0]);
+    int nmult = 0, nexph = 0, nstep = 0, nreject = 0, mbrkdwn = m, k1 = 2, iflag = 0;
+    if (!(beta > 0.0 && beta <= 1.79e308)) iflag = -10;
+    double t_now = 0.0, err_loc = 0.0, t_step = 0.0, avnorm = 0.0, xmloc = 1.0 / m;
+    const double p2 = tol * pow((m + 1) / 2.72, (double)(m + 1)) * sqrt((m + 1) * 6.28);
+    double t_new = pp_roundstep((1.0 / E.anorm) * pow(p2 / (4.0 * beta * E.anorm), 1.0 / m));
+    while (!iflag && t_now < t_out) {
+      nstep++;
+      t_step = fmin(t_out - t_now, t_new);
+      const double inv = 1.0 / beta;
+      for (int i = tid; i < nloc; i += PP_THREADS) {
+        const zd v = i < nown ? zscale(W[i], inv) : zmk(0.0, 0.0);
+        if (i < nown) Va[r0 + i] = v;
+        X[i] = v;
+      }
+      for (int e = tid; e < mh * mh; e += PP_THREADS) Hm[e] = zmk(0.0, 0.0);
+      __syncthreads();
+      publish_y();
+      k1 = 2;
+      mbrkdwn = m;
+      for (int j = 1; j <= m; j++) {
+        nmult++;
+        matvec(step);
+        // first pass: h1 = V^H z
+        split_dots(nullptr, R, 0, Va + r0, N, W, j, nown, hred);
+        for (int c = tid; c < 2 * j; c += PP_THREADS) xst(mypart + D1 + c, hred[c]);
+        grid_barrier(A.bar, ep, nwg, wid);
+        reduce(D1, 2 * j);
+        for (int c = tid; c < 2 * j; c += PP_THREADS) hacc[c] = hred[c];
+        for (int i = tid; i < nown; i += PP_THREADS) {
+          zd w = W[i];
+          for (int k = 0; k < j; k++) w = zsub(w, zmul(Va[(size_t)k * N + r0 + i], zmk(hred[2 * k], hred[2 * k + 1])));
+          W[i] = w;
+        }
+        __syncthreads();
+        // second pass: h2 = V^H z', |z'|^2
+        split_dots(nullptr, R, 0, Va + r0, N, W, j, nown, hred);
+        double v2 = 0.0;
+        for (int i = tid; i < nown; i += PP_THREADS) v2 += zabs2(W[i]);
+        v2 = block_sum(v2, wsum);
+        if (tid == 0) hred[2 * j] = v2;
+        __syncthreads();
+        for (int c = tid; c <= 2 * j; c += PP_THREADS) xst(mypart + D2 + c, hred[c]);
+        grid_barrier(A.bar, ep, nwg, wid);
+        reduce(D2, 2 * j + 1);
+        double h2n = 0.0;
+        for (int k = 0; k < j; k++) h2n += hred[2 * k] * hred[2 * k] + hred[2 * k + 1] * hred[2 * k + 1];
+        const double hj1j = sqrt(fmax(hred[2 * j] - h2n, 0.0));
+        for (int i = tid; i < nown; i += PP_THREADS) {
+          zd w = W[i];
+          for (int k = 0; k < j; k++) w = zsub(w, zmul(Va[(size_t)k * N + r0 + i], zmk(hred[2 * k], hred[2 * k + 1])));
+          W[i] = w;
+        }
+        if (tid < j) Hm[tid + (j - 1) * mh] = zmk(hacc[2 * tid] + hred[2 * tid], hacc[2 * tid + 1] + hred[2 * tid + 1]);
+        __syncthreads();
+        if (hj1j <= break_tol) {   // happy breakdown (zgexpv.cpp:366-372)
+          k1 = 0;
+          mbrkdwn = j;
+          t_step = t_out - t_now;
+          break;
+        }
+        if (tid == 0) Hm[j + (j - 1) * mh] = zmk(hj1j, 0.0);
+        const double hinv = 1.0 / hj1j;
+        for (int i = tid; i < nloc; i += PP_THREADS) {
+          const zd v = i < nown ? zscale(W[i], hinv) : zmk(0.0, 0.0);
+          if (i < nown) Va[(size_t)j * N + r0 + i] = v;
+          X[i] = v;
+        }
+        __syncthreads();
+        publish_y();
+      }
+      if (k1 != 0) {   // the extra matvec of the 2-corrected error estimate
+        nmult++;
+        matvec(step);
+        double v = 0.0;
+        for (int i = tid; i < nown; i += PP_THREADS) v += zabs2(W[i]);
+        v = block_sum(v, wsum);
+        if (tid == 0) xst(mypart + ANRM, v);
+        grid_barrier(A.bar, ep, nwg, wid);
+        reduce(ANRM, 1);
+        avnorm = sqrt(hred[0]);
+      }
+      if (tid == 0) Hm[(m + 1) + m * mh] = zmk(1.0, 0.0);
+      __syncthreads();
+      // exponential with step rejection (zgexpv.cpp:397-470)
+      int ireject = 0;
+      for (;;) {
+        nexph++;
+        const int mx = mbrkdwn + k1;
+        const int ns = pp_zgpadm_block(mx, mh, Hm, t_step, pade_off, ecol);
+        if (ns < 0) { iflag = -9; break; }
+        if (k1 == 0) err_loc = tol;
+        else {
+          const double q1 = sqrt(zabs2(ecol[m])) * beta, q2 = sqrt(zabs2(ecol[m + 1])) * beta * avnorm;
+          if (q1 > 10.0 * q2) { err_loc = q2; xmloc = 1.0 / m; }
+          else if (q1 > q2) { err_loc = q1 * q2 / (q1 - q2); xmloc = 1.0 / m; }
+          else { err_loc = q1; xmloc = 1.0 / (m - 1); }
+        }
+        if (k1 != 0 && err_loc > 1.2 * t_step * tol) {
+          t_step = pp_roundstep(0.9 * t_step * pow(t_step * tol / err_loc, xmloc));
+          nreject++;
+          if (++ireject > 100) { iflag = -8; break; }   // bound (not in zgexpv): no accepted step
+          continue;
+        }
+        break;
+      }
+      if (iflag) break;
+      // w = beta V exp(t H) e1, |w|
+      const int mx = mbrkdwn + (k1 - 1 > 0 ? k1 - 1 : 0);
+      double nw = 0.0;
+      for (int i = tid; i < nloc; i += PP_THREADS) {
+        zd acc = zmk(0.0, 0.0);
+        if (i < nown)
+          for (int k = 0; k < mx; k++) zfma(acc, Va[(size_t)k * N + r0 + i], ecol[k]);
+        acc = zscale(acc, beta);
+        W[i] = acc;
+        nw += zabs2(acc);
+      }
+      nw = block_sum(nw, wsum);
+      if (tid == 0) xst(mypart + NRM, nw);
+      grid_barrier(A.bar, ep, nwg, wid);
+      reduce(NRM, 1);
+      beta = sqrt(hred[0]);
+      t_new = pp_roundstep(0.9 * t_step * pow(t_step * tol / err_loc, xmloc));
+      err_loc = fmax(err_loc, rndoff);
+      t_now += t_step;
+      if (nstep >= 500 && t_now < t_out) { iflag = 1; break; }
+    }
+    // ---- psi = U^-1 w (SolveSqrtOverlap) on the own rows
+    for (int i = tid; i < nloc; i += PP_THREADS) X[i] = i < nown ? W[i] : zmk(0.0, 0.0);
+    __syncthreads();
+    pp_split_solve<KD, true>(A, M.nl, X, Z, tb, bb, c0, nch, sbuf(), fw, sep, q);
+    for (int i = tid; i < nown; i += PP_THREADS) zst(A.psi + (size_t)a * N + r0 + i, Z[i]);
+    if (lead && tid == 0) {
       double *st = E.stats + (size_t)step * 6;
       st[0] = nmult; st[1] = nexph; st[2] = nstep; st[3] = nreject; st[4] = mbrkdwn; st[5] = iflag;
     }
@@ -2006,8 +2266,10 @@ extern "C" int pp_pamp_propagate(pp_pamp *p, void *psi, int nsteps, double dt_re
   if (!p || !psi || nsteps < 0 || (!scal && p->nterms > 0)) return PP_EARG;
   if (nsteps == 0) return PP_OK;
   if (p->nterms == 0) return PP_EARG;
+  int rc = pamp_upload_slots(p);   // nslots bounds the split factor
+  if (rc) return rc;
   int Q = 1;
-  int rc = pamp_split_factor(p, Q);
+  rc = pamp_split_factor(p, Q);
   if (rc) return rc;
   if (Q > 1 && ((rc = pamp_split_buffers(p, Q)) || (rc = pamp_split_rows(p, Q)))) return rc;
   pp_pamp_args A;
@@ -2289,8 +2551,27 @@ extern "C" int pp_expokit_propagate(pp_pamp *p, void *psi, int nsteps, double dt
   if (p->nterms == 0 || p->m < 2) return PP_EARG;
   if (dt_re != 0.0 && dt_im != 0.0) return PP_EUNSUP;   // complex time steps
   if (p->reorth) return PP_EUNSUP;                      // pAMP-only option
+  int rc = pamp_upload_slots(p);   // nslots sizes the split form's LDS
+  if (rc) return rc;
+  const int mh = p->m + 2;
+  const size_t hm = sizeof(double) * 2 * (size_t)(mh * mh + mh);
+  const size_t zgp = sizeof(zd) * (size_t)(5 * mh * mh + 5 * mh) + sizeof(double) * (PP_THREADS / 64) + 64;
+  // the fiber-split form where it applies and its LDS (stage-2 rows, combo
+  // table, split buffers, Hessenberg matrix, zgpadm scratch) fits
+  int Q = 1;
+  rc = pamp_split_factor(p, Q);
+  if (rc) return rc;
+  size_t slds = 0;
+  if (Q > 1) {
+    const pp_band_plan &pl = p->ov->plan;
+    const int maxch = (pl.P + Q - 1) / Q;
+    slds = sizeof(double) * ((size_t)pp_split_rows_doubles(p->ov->kd, pl.P, maxch) + PP_CMB_DOUBLES +
+                             (size_t)pp_split_doubles(p->nslots, p->m, p->ov->kd, pl.B, maxch, pl.P)) + hm + zgp;
+    if (slds > 160 * 1024) Q = 1;
+  }
+  if (Q > 1 && ((rc = pamp_split_buffers(p, Q)) || (rc = pamp_split_rows(p, Q)))) return rc;
   pp_pamp_args A;
-  int rc = pamp_prepare(p, psi, nsteps, dt_re, dt_im, scal, stream, A);
+  rc = pamp_prepare(p, psi, nsteps, dt_re, dt_im, scal, stream, A);
   if (rc) return rc;
   pp_expo_args E;
   E.tol = tol;
@@ -2298,14 +2579,25 @@ extern "C" int pp_expokit_propagate(pp_pamp *p, void *psi, int nsteps, double dt
   E.imag = dt_re == 0.0 ? 1 : 0;
   E.t_out = E.imag ? fabs(dt_im) : fabs(dt_re);
   E.stats = p->diag;
-  const int mh = p->m + 2;
-  const size_t hm = sizeof(double) * 2 * (size_t)(mh * mh + mh);
-  const size_t nle = sizeof(double) * ((size_t)((p->ov->plan.nlev + 1) & ~1) + PP_CMB_DOUBLES);
-  size_t fib = pamp_lds_bytes(p, A.padN) + hm;
-  size_t pade = nle + hm + sizeof(zd) * (size_t)(5 * mh * mh + 5 * mh) + sizeof(double) * (PP_THREADS / 64) + 64;
-  const size_t lds = fib > pade ? fib : pade;
   void *args[] = {&A, &E};
   rc = PP_EUNSUP;
+  if (Q > 1) {
+    A.Q = Q;
+    A.sb = p->sb;
+    A.subw = (unsigned *)(p->bar + 1) + (size_t)p->L * Q;
+    A.srows = p->srows;
+    A.sstride = p->sstride;
+    A.nlevEven = p->sstride;
+    PP_KD_SWITCH(p->ov->kd, {
+      rc = pamp_launch(p->mixed ? (const void *)pp_expokit_split_kernel<KD_, true>
+                                : (const void *)pp_expokit_split_kernel<KD_, false>,
+                       p->L * Q, args, slds, stream); })
+    return rc;
+  }
+  const size_t nle = sizeof(double) * ((size_t)((p->ov->plan.nlev + 1) & ~1) + PP_CMB_DOUBLES);
+  size_t fib = pamp_lds_bytes(p, A.padN) + hm;
+  size_t pade = nle + hm + zgp;
+  const size_t lds = fib > pade ? fib : pade;
   PP_KD_SWITCH(p->ov->kd, {
     rc = pamp_launch(p->mixed ? (const void *)pp_expokit_kernel<KD_, true> : (const void *)pp_expokit_kernel<KD_, false>,
                      p->L, args, lds, stream); })
