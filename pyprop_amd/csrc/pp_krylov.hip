@@ -981,6 +981,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
 
     double beta0 = 0.0;
     int mEff = m;
+    pp_step_coeffs(A, M, step, zmk(1.0, 0.0));   // the apply's coefficients at t_step (read after a barrier below)
     for (int j = 0; j < m; j++) {
       // ---- Arnoldi iteration j with the normalisation deferred by one
       // reduction (pamp.h:293-350: v_j = w / |w|, w = H^ v_j, h = V^H w,
@@ -1001,7 +1002,6 @@ __global__ void __launch_bounds__(PP_THREADS) pp_pamp_split_kernel(pp_pamp_args 
         if (col >= 0)
           for (int jj = ylo + tid; jj < phi; jj += PP_THREADS) Ys[sl * ystr + jj - ybase] = zld(A.y + (size_t)col * N + jj);
       }
-      pp_step_coeffs(A, M, step, zmk(1.0, 0.0));
       __syncthreads();
       pp_band_rows<KD, MIXED>(A, M, Ys, ystr, ybase, r0, nloc, X);
       __syncthreads();
@@ -1555,7 +1555,6 @@ __global__ void __launch_bounds__(PP_THREADS) pp_expokit_split_kernel(pp_pamp_ar
       if (col >= 0)
         for (int jj = ylo + tid; jj < phi; jj += PP_THREADS) Ys[sl * ystr + jj - ybase] = zld(A.y + (size_t)col * N + jj);
     }
-    pp_step_coeffs(A, M, step, fac);
     __syncthreads();
     pp_band_rows<KD, MIXED>(A, M, Ys, ystr, ybase, r0, nloc, X);
     __syncthreads();
@@ -1565,6 +1564,7 @@ __global__ void __launch_bounds__(PP_THREADS) pp_expokit_split_kernel(pp_pamp_ar
     // ---- psi~ = U psi on the own rows (psi rows up to r0+nloc+KD-1 were
     // written by the neighbouring part at the end of the previous step)
     if (step > 0) group_barrier(fw, &A.bar->err, sep, Q, q);
+    pp_step_coeffs(A, M, step, fac);   // the matvec's coefficients at t_step (read after barriers below)
     const int phi = r0 + nloc + KD < N ? r0 + nloc + KD : N;
     for (int j = r0 + tid; j < phi; j += PP_THREADS) Ys[j - r0] = zld(A.psi + (size_t)a * N + j);
     __syncthreads();
