@@ -7,6 +7,11 @@
 #ifndef PP_THREADS
 #define PP_THREADS 256
 #endif
+// Gauss-Jordan elements a thread of wavefronts 1.. holds in registers per
+// elimination step (the rest, when a step has more, four at a time)
+#ifndef PP_PADE_NU
+#define PP_PADE_NU 6
+#endif
 
 // ------------------------------------------------------------------ exchanged data
 __device__ inline void xst(double *p, double v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
@@ -25,8 +30,8 @@ __device__ inline zd zld(const zd *p) { return zmk(xld(&p->x), xld(&p->y)); }
 // products instead of Horner's seven, the same polynomials).  The linear
 // solve is Gauss-Jordan with zgesv's partial pivoting (largest |re|+|im|,
 // first index on ties) on ping-pong buffers, one barrier per column; the
-// elimination pass also leaves the next column's magnitudes in LDS, so every
-// lane finds the pivot from one array instead of a cross-lane reduction.
+// wavefront that forms the next column also picks its pivot, so a step
+// starts with the pivot's value and row already in LDS (m <= 64).
 // The last squaring forms only the first column.  Matrix products run on
 // v_mfma_f64_16x16x4f64.
 // normscale: 0 scales by the largest |A_ij| as pAMP does; 1 scales until the
@@ -42,13 +47,13 @@ typedef double pp_d4 __attribute__((ext_vector_type(4)));
 
 // Doubles of dynamic LDS the exponential of an m x m Krylov matrix needs at
 // its offset: five m x m complex buffers (Pade) or the eigenvector form's two
-// padded matrices and rotation table, then the pivot magnitudes and the
-// wavefront maxima.
+// padded matrices and rotation table, then the wavefront maxima, two
+// Gauss-Jordan pivot slots and its row permutation.
 __host__ __device__ inline int pp_pade_lds_doubles(int m)
 {
   const int n2 = m + (m & 1), np = n2 / 2;
   const int pade = 2 * 5 * m * m, eig = 2 * 2 * n2 * n2 + 2 * np + 2 * np + np + 2;
-  return (pade > eig ? pade : eig) + 2 * ((m + 3) & ~1) + PP_THREADS / 64 + 2;
+  return (pade > eig ? pade : eig) + PP_THREADS / 64 + 8 + 32 + 2;
 }
 
 // Z = X Y (+ c on the diagonal when diag), m x m complex, column-major, LDS:
@@ -86,6 +91,65 @@ __device__ __attribute__((always_inline)) inline void pade_mm(int m, const zd *X
   __syncthreads();
 }
 
+// Lane l's value of d (l uniform).
+__device__ __attribute__((always_inline)) inline double pade_readlane(double d, int l)
+{
+  const long long b = __double_as_longlong(d);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Wavefront maximum (MAX) or minimum of x, reduced into lane 63 by DPP moves
+// (quad swaps, half-row and row mirrors, then the row broadcasts) and read
+// from there.  Lanes a move leaves out (row mask) combine with themselves.
+template <bool MAX>
+__device__ __attribute__((always_inline)) inline unsigned int pade_wave_ext(unsigned int x)
+{
+#define PADE_EXT_LEVEL(CTRL, RMASK)                                                          \
+  {                                                                                        \
+    const unsigned int o = (unsigned int)__builtin_amdgcn_update_dpp((int)x, (int)x, CTRL, RMASK, 0xF, false); \
+    x = MAX ? (o > x ? o : x) : (o < x ? o : x);                                           \
+  }
+  PADE_EXT_LEVEL(0xB1, 0xF)    // quad_perm [1,0,3,2]
+  PADE_EXT_LEVEL(0x4E, 0xF)    // quad_perm [2,3,0,1]
+  PADE_EXT_LEVEL(0x141, 0xF)   // row_half_mirror
+  PADE_EXT_LEVEL(0x140, 0xF)   // row_mirror
+  PADE_EXT_LEVEL(0x142, 0xA)   // row_bcast:15 into rows 1, 3
+  PADE_EXT_LEVEL(0x143, 0xC)   // row_bcast:31 into rows 2, 3
+#undef PADE_EXT_LEVEL
+  return (unsigned int)__builtin_amdgcn_readlane((int)x, 63);
+}
+
+// Partial pivoting of one Gauss-Jordan column, one wavefront, lane r holding
+// row r of m.  Rows stay where they are; pos is the row's place in zgesv's
+// row-swapped order.  The pivot of step k is zgesv's izamax over places
+// >= k: the largest |re| + |im|, the first place on ties.  |re| + |im| >= 0,
+// so its bits order like its value: the largest high word, then the largest
+// low word among those, then the smallest place among exact ties.  Lane 0
+// stores the pivot's value and row in slot[0..2]; every lane then applies
+// the swap of places k and the pivot's, and at the last step stores
+// row_of[pos] = r.
+__device__ __attribute__((always_inline)) inline void pade_pivot(zd v, bool valid, int k, int m, int &pos, double *slot,
+                                                                 int *row_of)
+{
+  const int lane = threadIdx.x & 63;
+  const bool cand = valid && pos >= k;
+  const unsigned long long b = (unsigned long long)__double_as_longlong(cand ? fabs(v.x) + fabs(v.y) : 0.0);
+  const unsigned int hi = (unsigned int)(b >> 32), lo = (unsigned int)b;
+  const unsigned int mh = pade_wave_ext<true>(hi);
+  const unsigned int ml = pade_wave_ext<true>(hi == mh ? lo : 0u);
+  const int pk = (int)pade_wave_ext<false>(cand && hi == mh && lo == ml ? (unsigned int)pos : 64u);
+  const unsigned long long at = __ballot(cand && pos == pk);   // bits of valid lanes only: r < m
+  const int r = at ? (int)__builtin_ctzll(at) : 0;
+  const double vx = pade_readlane(v.x, r), vy = pade_readlane(v.y, r);
+  if (lane == 0) { slot[0] = vx; slot[1] = vy; slot[2] = (double)r; }
+  if (at) {
+    if (pos == k) pos = pk;
+    if (lane == r) pos = k;
+  }
+  if (k == m - 1 && valid) row_of[pos] = lane;
+}
+
 static __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg, double dt_re, double dt_im, int off,
                                                      zd *cvec_out, double *diag, int normscale)
 {
@@ -102,9 +166,9 @@ static __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg,
   // and A, so the exponential needs 5 m^2 complex values in all
   zd *A = lds, *A2 = lds + mm, *A4 = lds + 2 * mm, *A6 = lds + 3 * mm, *Wt = lds + 4 * mm;
   zd *G0 = A, *G1 = A4;                               // Gauss-Jordan buffers (m x 2m)
-  double *mag = (double *)(lds + 5 * mm);              // pivot magnitudes, two buffers
-  const int MS = (m + 3) & ~1;
-  double *red = mag + 2 * MS;
+  double *red = (double *)(lds + 5 * mm);              // wavefront maxima
+  double *pvs = red + PP_THREADS / 64;                 // pivot slots: value, row (x2)
+  int *row_of = (int *)(pvs + 8);                      // solution row -> Gauss-Jordan row (64)
   const zd fac = zmk(dt_im, -dt_re);  // -i * dt
   for (int e = tid; e < mm; e += PP_THREADS) A[e] = zmul(fac, Hg[e]);
   __syncthreads();
@@ -147,69 +211,88 @@ static __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg,
   __syncthreads();
   pade_mm(m, A, Wt, A6, 0.0, false);   // P = A (c1 + c3 A^2 + c5 A^4)
   // Gauss-Jordan on [Q - P | P] (m x 2m, column-major) in [A | A2] (A and A2
-  // are no longer needed); mag: |re| + |im| of the next pivot column
+  // are no longer needed) with zgesv's partial pivoting, rows left in place:
+  // row r ends as row pos(r) of the solution, row_of inverts that.  Step k
+  // updates columns k+1 .. 2m-1 only (column k is not read again).
+  // Wavefront 0 forms the new column k+1 (lane r: row r, m <= 64) and picks
+  // the next pivot from its registers, leaving (value, row) in one of two LDS
+  // slots before the step's barrier; the other wavefronts update columns
+  // k+2 .. 2m-1.
   for (int e = tid; e < mm; e += PP_THREADS) {
     const zd pv = A6[e];
-    const zd g = zsub(A4[e], pv);
-    G0[e] = g;
+    G0[e] = zsub(A4[e], pv);
     G0[mm + e] = pv;
-    if (e < m) mag[e] = fabs(g.x) + fabs(g.y);
   }
+  int pos = tid & 63;   // wavefront 0: the place of this lane's row
+  if (tid < 64) pade_pivot(tid < m ? zsub(A4[tid], A6[tid]) : zmk(0.0, 0.0), tid < m, 0, m, pos, pvs, row_of);
   __syncthreads();
   zd *gc = G0, *gn = G1;
-  const int w2 = 2 * m * m;
-  const int ti = tid % m, tj = tid / m, di = PP_THREADS % m, dj = PP_THREADS / m;
+  constexpr int T1 = PP_THREADS - 64;
+  const int t1 = tid - 64, ti = (t1 + m) % m, tj = (t1 + m) / m, di = T1 % m, dj = T1 / m, jl = 2 * m - 1;
+  constexpr int NU = PP_PADE_NU;
   for (int k = 0; k < m; k++) {
-    // pivot: the first largest magnitude of rows k .. m-1 (izamax), every
-    // lane comparing the values in row order
-    const double *mk = mag + (k & 1) * MS;
-    int p = k;
-    double best = -1.0;
-    for (int i0 = k & ~1; i0 < m; i0 += 8) {
-      double2 v[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++) v[u] = i0 + 2 * u < m ? *(const double2 *)(mk + i0 + 2 * u) : make_double2(-1.0, -1.0);
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int r = i0 + 2 * u;
-        if (r >= k && r < m && v[u].x > best) { best = v[u].x; p = r; }
-        if (r + 1 >= k && r + 1 < m && v[u].y > best) { best = v[u].y; p = r + 1; }
+    const int nel = (2 * m - k - 2) * m;   // elements of columns k+2 .. 2m-1
+    const double *sl = pvs + (k & 1) * 4;
+    const zd piv = zmk(sl[0], sl[1]);
+    const int p = __builtin_amdgcn_readfirstlane((int)sl[2]);
+    if (tid < 64) {
+      const int i = min(tid, m - 1);
+      const zd gb0 = gc[i + (k + 1) * m], gk0 = gc[i + k * m];
+      if (piv.x == 0.0 && piv.y == 0.0) {   // singular denominator (pamp.h:544-550)
+        if (tid == 0) diag[0] = -1.0;
+        return;
       }
-    }
-    if (best == 0.0) {   // singular denominator (pamp.h:544-550)
-      if (tid == 0) diag[0] = -1.0;
-      return;
-    }
-    const zd piv = gc[p + k * m];
-    const double dn = zabs2(piv);
-    const zd pinv = zmk(piv.x / dn, -piv.y / dn);
-    // columns >= k (eliminated columns are not read again), the operands of
-    // four elements loaded before any is written
-    int i = ti, j = k + tj;
-    for (int e0 = k * m + tid; e0 < w2; e0 += 4 * PP_THREADS) {
-      zd ga[4], gb[4], gk[4];
-      int iu[4];
+      const double dn = zabs2(piv);
+      const zd pinv = zmk(piv.x / dn, -piv.y / dn);
+      const zd rk = zmul(gc[p + (k + 1) * m], pinv);   // pivot row / pivot
+      const zd v = i == p ? rk : zsub(gb0, zmul(gk0, rk));
+      if (tid < m) gn[i + (k + 1) * m] = v;
+      if (k + 1 < m) pade_pivot(v, tid < m, k + 1, m, pos, pvs + ((k + 1) & 1) * 4, row_of);
+    } else {
+      // this thread's first NU elements: the operands that do not depend on
+      // the pivot, loaded with it (columns past the block's end clamped to
+      // its last)
+      int iu[NU], ju[NU];
+      zd ga[NU], gb[NU], gk[NU];
+      int i = ti, j = k + 1 + tj;
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        iu[u] = i;
-        if (e0 + u * PP_THREADS < w2) {
-          const int si = (i == p) ? k : i;       // row swap k <-> p
-          ga[u] = gc[p + j * m];
-          gb[u] = gc[si + j * m];
-          gk[u] = gc[si + k * m];
-        }
+      for (int u = 0; u < NU; u++) {
+        iu[u] = i; ju[u] = min(j, jl);
+        gb[u] = gc[i + ju[u] * m];
+        gk[u] = gc[i + k * m];
         i += di; j += dj;
         if (i >= m) { i -= m; j++; }
       }
+      if (piv.x == 0.0 && piv.y == 0.0) return;   // singular: wavefront 0 reports it
+      const double dn = zabs2(piv);
+      const zd pinv = zmk(piv.x / dn, -piv.y / dn);
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int e = e0 + u * PP_THREADS;
-        if (e < w2) {
-          const zd rk = zmul(ga[u], pinv);     // new pivot row
-          const zd v = iu[u] == k ? rk : zsub(gb[u], zmul(gk[u], rk));
-          gn[e] = v;
-          if (e - iu[u] == (k + 1) * m) mag[((k + 1) & 1) * MS + iu[u]] = fabs(v.x) + fabs(v.y);
+      for (int u = 0; u < NU; u++) ga[u] = gc[p + ju[u] * m];
+#pragma unroll
+      for (int u = 0; u < NU; u++) {
+        const zd rk = zmul(ga[u], pinv);
+        const zd v = iu[u] == p ? rk : zsub(gb[u], zmul(gk[u], rk));
+        if (t1 + u * T1 < nel) gn[iu[u] + ju[u] * m] = v;
+      }
+      // the rest (more than NU * T1 elements): four elements at a time
+      for (int r0 = t1 + NU * T1; r0 < nel; r0 += 4 * T1) {
+        zd ga2[4], gb2[4], gk2[4];
+        int iv[4], jv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          iv[u] = i; jv[u] = min(j, jl);
+          ga2[u] = gc[p + jv[u] * m];
+          gb2[u] = gc[i + jv[u] * m];
+          gk2[u] = gc[i + k * m];
+          i += di; j += dj;
+          if (i >= m) { i -= m; j++; }
         }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (r0 + u * T1 < nel) {
+            const zd rk = zmul(ga2[u], pinv);
+            gn[iv[u] + jv[u] * m] = iv[u] == p ? rk : zsub(gb2[u], zmul(gk2[u], rk));
+          }
       }
     }
     __syncthreads();
@@ -218,7 +301,7 @@ static __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg,
   // X = right half of gc;  E = I + 2X
   if (sc == 0) {   // only the first column of E is needed
     for (int i = tid; i < m; i += PP_THREADS) {
-      zd v = zscale(gc[mm + i], 2.0);
+      zd v = zscale(gc[mm + row_of[i]], 2.0);
       if (i == 0) v.x += 1.0;
       zst(cvec_out + i, v);
       if (i == m - 1) diag[0] = sqrt(zabs2(v));
@@ -229,7 +312,7 @@ static __device__ __attribute__((noinline)) void pade_block(int m, const zd *Hg,
   zd *E = Wt, *F = gc == G0 ? G1 : G0;   // free: Wt and the left half of the other Gauss-Jordan buffer
   for (int e = tid; e < mm; e += PP_THREADS) {
     const int i = e % m, j = e / m;
-    zd v = zscale(gc[mm + e], 2.0);
+    zd v = zscale(gc[mm + row_of[i] + j * m], 2.0);
     if (i == j) v.x += 1.0;
     E[e] = v;
   }

@@ -128,8 +128,8 @@ def test_expokit_chunk_fits_lds():
     """The Expokit engine's solver chunk: the default plan where zgexpv's
     Hessenberg workspace still fits 160 KB of LDS (small shapes, and the
     hydrogen_large shape L = 64, N = 800, kd = 6 at m = 24), a longer chunk
-    where it does not (the same shape at m = 30: the default B = 32 plan and
-    the Pade's 5 m^2 buffers leave less than the 16,896 B of zgexpv
+    where it does not (the same shape at m = 32: the default B = 32 plan and
+    the Pade's 5 m^2 buffers leave less than the 19,040 B of zgexpv
     workspace)."""
     from pyprop_amd import _native as nat
     from pyprop_amd.propagator import expokit_chunk
@@ -138,10 +138,10 @@ def test_expokit_chunk_fits_lds():
     assert expokit_chunk(11, 103, 20, 6, 0) is None
     assert expokit_chunk(64, 800, 24, 6, 0) is None
     assert lib.pp_pamp_lds_estimate(64, 800, 24, 3, 6, 32) + 16 * (26 * 26 + 26) <= 160 * 1024
-    B = expokit_chunk(64, 800, 30, 6, 0)
+    B = expokit_chunk(64, 800, 32, 6, 0)
     assert B == 64
-    assert lib.pp_pamp_lds_estimate(64, 800, 30, 3, 6, B) + 16 * (32 * 32 + 32) <= 160 * 1024
-    assert lib.pp_pamp_lds_estimate(64, 800, 30, 3, 6, 32) + 16 * (32 * 32 + 32) > 160 * 1024
+    assert lib.pp_pamp_lds_estimate(64, 800, 32, 3, 6, B) + 16 * (34 * 34 + 34) <= 160 * 1024
+    assert lib.pp_pamp_lds_estimate(64, 800, 32, 3, 6, 32) + 16 * (34 * 34 + 34) > 160 * 1024
 
 
 def test_krylov_status_mapping_raises_like_the_reference():
