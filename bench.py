@@ -47,7 +47,7 @@ T_START = 50.0            # start time of the hydrogen runs (the pulse lasts 125
 # WRITE_SIZE counter passes recorded in
 # profiles/pp_pamp_kernel.hydrogen_large.pmc.md (FETCH_SIZE scaled x2 and
 # WRITE_SIZE x1 by a 512 MiB device copy of known size).
-PMC_TRAFFIC_PER_STEP = {"hydrogen_large": {"fetch": 53.1e6, "write": 77.9e6}}
+PMC_TRAFFIC_PER_STEP = {"hydrogen_large": {"fetch": 53.3e6, "write": 77.9e6}}
 
 # HBM-side bytes of one multipole stage (the three kernels' FETCH_SIZE +
 # WRITE_SIZE per matvec, FETCH scaled x2 and WRITE x1 by a 512 MiB copy),
