@@ -1049,43 +1049,41 @@ __global__ void __launch_bounds__(256) p3_dots(const zd *V, const zd *w, long n,
   }
 }
 
-// Sum the partials in one fixed order: one wavefront per component, lane l
-// summing blocks l, l+64, ... and a fixed shuffle tree.  mode 0: h_k =
-// <V_k, w> for k <= j, written to hcoef and column j of Hm.  mode 1:
-// beta_{j+1} = |w| with the breakdown test (pamp.h:305-312); mode 2: beta_0
-// = |psi~| (start of a step, resets the breakdown state).
-__global__ void __launch_bounds__(1024) p3_reduce(const double *part, int pstr, int nblk, int mode, int j, int m,
-                                                   zd *hcoef, zd *Hm, double *beta, int *state)
+// part[l*pstr + c] + part[(l+64)*pstr + c] + ... (blocks below nblk), added
+// in that order, the loads of eight terms issued together
+__device__ __attribute__((always_inline)) inline double p3_lane_sum(const double *part, int pstr, int nblk, int c,
+                                                                    int lane)
+{
+  double a = 0.0;
+  for (int b0 = lane; b0 < nblk; b0 += 64 * 8) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) x[u] = part[(long)min(b0 + 64 * u, nblk - 1) * pstr + c];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (b0 + 64 * u < nblk) a += x[u];
+  }
+  return a;
+}
+
+// h_k = <V_k, w> for k <= j from the partials, written to hcoef and column j
+// of Hm: summed in one fixed order, one wavefront per component, lane l
+// summing blocks l, l+64, ... and a fixed shuffle tree.  (|w| and |psi~| are
+// summed the same way by p3_scale, which uses them.)
+__global__ void __launch_bounds__(1024) p3_reduce(const double *part, int pstr, int nblk, int j, int m, zd *hcoef,
+                                                   zd *Hm, const int *state)
 {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  if (mode != 2 && state[0]) return;
-  __syncthreads();   // every thread has read state[0] before mode 2 rewrites it
-  const int ncomp = mode == 0 ? 2 * (j + 1) : 1;
+  if (state[0]) return;
+  const int ncomp = 2 * (j + 1);
   for (int c = wv; c < ncomp; c += 16) {
-    double a = 0.0;
-    for (int b = lane; b < nblk; b += 64) a += part[(long)b * pstr + c];
+    double a = p3_lane_sum(part, pstr, nblk, c, lane);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
-    if (lane != 0) continue;
-    if (mode == 0) {
-      ((double *)hcoef)[c] = a;   // h_k = (v[2k], v[2k+1])
-    } else {
-      const double bt = sqrt(a);
-      if (mode == 2) {
-        beta[0] = bt;
-        state[0] = 0;
-        state[1] = m;
-      } else {
-        const double tol = 1.4901161193847656e-08;   // sqrt(DBL_EPSILON), pamp.h:75
-        if (fabs(bt) < tol) { state[0] = 1; state[1] = j + 1; }
-        else { beta[j + 1] = bt; Hm[(j + 1) + j * m] = zmk(bt, 0.0); }
-      }
-    }
+    if (lane == 0) ((double *)hcoef)[c] = a;   // h_k = (v[2k], v[2k+1])
   }
-  if (mode == 0) {
-    __syncthreads();
-    for (int k = tid; k <= j; k += 1024) Hm[k + j * m] = hcoef[k];
-  }
+  __syncthreads();
+  for (int k = tid; k <= j; k += 1024) Hm[k + j * m] = hcoef[k];
 }
 
 // w -= sum_{k<nk} V_k h_k;  partial |w|^2 per block
@@ -1115,12 +1113,40 @@ __global__ void __launch_bounds__(256) p3_project(const zd *V, zd *w, long n, in
   if (tid == 0) part[(long)blockIdx.x * pstr] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// dst1 = dst2 = src / beta[bi]
-__global__ void __launch_bounds__(256) p3_scale(const zd *src, zd *dst1, zd *dst2, long n, const double *beta, int bi,
-                                                 const int *state)
+// dst1 = dst2 = src / b, b = sqrt of the sum of the nblk partials
+// part[blk*pstr], summed by every block in p3_reduce's fixed order (lane l of
+// wavefront 0 sums blocks l, l+64, ..., then the same shuffle tree), so that
+// no reducing launch sits between the partials and their use.  mode 1: b =
+// beta_bi with the breakdown test (pamp.h:305-312), recorded by block 0;
+// mode 2: b = beta_0 = |psi~| at the start of a step (block 0 resets the
+// breakdown state).
+__global__ void __launch_bounds__(256) p3_scale(const zd *src, zd *dst1, zd *dst2, long n, const double *part, int pstr,
+                                                 int nblk, int mode, int bi, int m, double *beta, zd *Hm, int *state)
 {
-  if (state[0]) return;
-  const double inv = 1.0 / beta[bi];
+  if (mode == 1 && state[0]) return;
+  __shared__ double sb;
+  if (threadIdx.x < 64) {
+    double a = p3_lane_sum(part, pstr, nblk, 0, threadIdx.x);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    if (threadIdx.x == 0) sb = sqrt(a);
+  }
+  __syncthreads();
+  const double bt = sb;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (mode == 1) {
+    const double tol = 1.4901161193847656e-08;   // sqrt(DBL_EPSILON), pamp.h:75
+    if (fabs(bt) < tol) {
+      if (lead) { state[0] = 1; state[1] = bi; }
+      return;
+    }
+    if (lead) { beta[bi] = bt; Hm[bi + (bi - 1) * m] = zmk(bt, 0.0); }
+  } else if (lead) {
+    beta[0] = bt;
+    state[0] = 0;
+    state[1] = m;
+  }
+  const double inv = 1.0 / bt;
   for (long i = blockIdx.x * 256l + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
     const zd v = zscale(src[i], inv);
     dst1[i] = v;
@@ -1710,23 +1736,20 @@ extern "C" int pp_pamp3_propagate(pp_pamp3 *p, void *psi, int nsteps, double dt_
     if (hipMemcpyAsync(p->P, ps, vb, hipMemcpyDeviceToDevice, stream) != hipSuccess) return PP_EHIP;
     if ((rc = p3_solve2(p, 2 /* U x */, p->P, stream))) return rc;
     hipLaunchKernelGGL(p3_dots, dim3(G), dim3(256), 0, stream, p->V, p->P, n, 0, p->part, pstr, p->state, 0);
-    hipLaunchKernelGGL(p3_reduce, dim3(1), dim3(1024), 0, stream, p->part, pstr, G, 2, -1, m, p->hcoef, p->Hm, p->beta,
-                       p->state);
-    hipLaunchKernelGGL(p3_scale, dim3(gridv), dim3(256), 0, stream, p->P, p->V, p->Y, n, p->beta, 0, p->state);
+    hipLaunchKernelGGL(p3_scale, dim3(gridv), dim3(256), 0, stream, p->P, p->V, p->Y, n, p->part, pstr, G, 2, 0, m,
+                       p->beta, p->Hm, p->state);
     for (int j = 0; j < m; j++) {
       if ((rc = p3_solve2(p, 0, p->Y, stream))) return rc;
       if ((rc = p3_apply_raw(p, p->Y, p->Z, sc, stream))) return rc;
       if ((rc = p3_solve2(p, 1, p->Z, stream))) return rc;
       hipLaunchKernelGGL(p3_dots, dim3(G), dim3(256), 0, stream, p->V, p->Z, n, j + 1, p->part, pstr, p->state, 1);
-      hipLaunchKernelGGL(p3_reduce, dim3(1), dim3(1024), 0, stream, p->part, pstr, G, 0, j, m, p->hcoef, p->Hm,
-                         p->beta, p->state);
+      hipLaunchKernelGGL(p3_reduce, dim3(1), dim3(1024), 0, stream, p->part, pstr, G, j, m, p->hcoef, p->Hm,
+                         p->state);
       if (j == m - 1) break;   // the last residual is not used (pamp.h:419-426)
       hipLaunchKernelGGL(p3_project, dim3(G), dim3(256), 0, stream, p->V, p->Z, n, j + 1, p->hcoef, p->part, pstr,
                          p->state);
-      hipLaunchKernelGGL(p3_reduce, dim3(1), dim3(1024), 0, stream, p->part, pstr, G, 1, j, m, p->hcoef, p->Hm,
-                         p->beta, p->state);
       hipLaunchKernelGGL(p3_scale, dim3(gridv), dim3(256), 0, stream, p->Z, p->V + (size_t)(j + 1) * n, p->Y, n,
-                         p->beta, j + 1, p->state);
+                         p->part, pstr, G, 1, j + 1, m, p->beta, p->Hm, p->state);
     }
     hipLaunchKernelGGL(p3_pade, dim3(1), dim3(PP_THREADS), lds, stream, m, p->Hm, dt_re, dt_im, p->cvec,
                        p->diag + (size_t)step * 4, p->state, p->beta, p->expmethod ? 2 : 0);
