@@ -123,7 +123,7 @@ struct pp_pamp3 {
   zd *Y, *X, *Z, *P, *V, *Ym;
   double *part, *scal, *diag, *scal_dt;
   double *urd;             // reciprocals of U's diagonal (the fiber solves)
-  double *tinv;            // U^-1 and U^-T as MFMA blocks (p3_tgemm; null: substitution tiles)
+  double *tinv;            // U^-1, U^-T and U as MFMA blocks (p3_tgemm; null: substitution tiles)
   int tnp;                 // their padded size (0: substitution tiles)
   zd *Hm, *cvec, *hcoef;
   double *beta;
@@ -1233,12 +1233,23 @@ static int p3_tprod_build(pp_pamp3 *p)
   }
   (void)hipFree(dT);
   if (!ok) return PP_EHIP;
-  for (int op = 0; op < 2; op++)
+  // U itself, dense and upper (U(i, j) at Uband[j*(KD+1) + KD + i - j]), for x := U x
+  const int KD = p->ov->kd;
+  std::vector<double> Ub((size_t)N * (KD + 1));
+  if (hipMemcpy(Ub.data(), p->ov->Uband, sizeof(double) * Ub.size(), hipMemcpyDeviceToHost) != hipSuccess)
+    return PP_EHIP;
+  T.resize(3 * dn, 0.0);
+  for (int j = 0; j < N; j++)
+    for (int i = j - KD > 0 ? j - KD : 0; i <= j; i++) T[2 * dn + (size_t)i * NP + j] = Ub[(size_t)j * (KD + 1) + KD + i - j];
+  B.assign(3 * (size_t)nblk * 64, 0.0);
+  for (int set = 0; set < 3; set++) {   // U^-1, U^-T, U: sets 0 and 2 upper, 1 lower
+    const int op = set == 1 ? 1 : 0;
     for (int rb = 0; rb < NB; rb++)
       for (int ks = p3_tblk_first(op, rb); ks < p3_tblk_end(op, rb, KS); ks++) {
-        double *blk = B.data() + ((size_t)op * nblk + p3_tblk_base(op, rb, KS) + ks - p3_tblk_first(op, rb)) * 64;
-        for (int l = 0; l < 64; l++) blk[l] = T[(size_t)op * dn + (size_t)(16 * rb + (l & 15)) * NP + 4 * ks + (l >> 4)];
+        double *blk = B.data() + ((size_t)set * nblk + p3_tblk_base(op, rb, KS) + ks - p3_tblk_first(op, rb)) * 64;
+        for (int l = 0; l < 64; l++) blk[l] = T[(size_t)set * dn + (size_t)(16 * rb + (l & 15)) * NP + 4 * ks + (l >> 4)];
       }
+  }
   if (hipMalloc(&p->tinv, sizeof(double) * B.size()) != hipSuccess) return PP_EHIP;
   return hipMemcpy(p->tinv, B.data(), sizeof(double) * B.size(), hipMemcpyHostToDevice) == hipSuccess ? PP_OK : PP_EHIP;
 }
@@ -1273,7 +1284,7 @@ extern "C" int pp_pamp3_create(int C, int N, int m, int reorth, const pp_overlap
   if (!ok) { pp_pamp3_destroy(p); return PP_EHIP; }
   hipLaunchKernelGGL(p3_recip_diag, dim3(1), dim3(256), 0, 0, ov->Uband, p->urd, N, ov->kd);
   if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) { pp_pamp3_destroy(p); return PP_EHIP; }
-  // U^-1 and U^-T as MFMA blocks for the product form of the overlap solves
+  // U^-1, U^-T and U as MFMA blocks for the product form of the overlap solves and U x
   p->tnp = P3_TPROD ? (N <= 64 ? 64 : N <= 128 ? 128 : 0) : 0;
   if (p->tnp && p3_tprod_build(p) != PP_OK) { pp_pamp3_destroy(p); return PP_EHIP; }
   *out = p;
@@ -1642,10 +1653,10 @@ template <int NB>
 static int p3_tsolve2(pp_pamp3 *p, int op, zd *x, hipStream_t st)
 {
   const int C = p->C, N = p->N, nblk = 2 * NB * (NB + 1);
-  const double *T = p->tinv + (op == 0 ? 0 : (size_t)nblk * 64);
+  const double *T = p->tinv + (size_t)op * nblk * 64;   // op 0: U^-1, 1: U^-T, 2: U
   const dim3 g2((unsigned)(((long)C * N + 7) / 8)), g1((unsigned)(C * ((2 * N + 15) / 16)));
   double *xd = (double *)x;
-  if (op == 0) {
+  if (op != 1) {
     hipLaunchKernelGGL((p3_tgemm<NB, 0, 2>), g2, dim3(64), 0, st, T, xd, C, N);
     hipLaunchKernelGGL((p3_tgemm<NB, 0, 1>), g1, dim3(64), 0, st, T, xd, C, N);
   } else {
@@ -1657,8 +1668,8 @@ static int p3_tsolve2(pp_pamp3 *p, int op, zd *x, hipStream_t st)
 
 static int p3_solve2(pp_pamp3 *p, int op, zd *x, hipStream_t st)
 {
-  if (op != 2 && p->tnp == 64) return p3_tsolve2<4>(p, op, x, st);
-  if (op != 2 && p->tnp == 128) return p3_tsolve2<8>(p, op, x, st);
+  if (p->tnp == 64) return p3_tsolve2<4>(p, op, x, st);
+  if (p->tnp == 128) return p3_tsolve2<8>(p, op, x, st);
   const int C = p->C, N = p->N;
   const size_t lds = sizeof(double) * ((size_t)N * 7 + N + ((N * 7 + N) & 1)) + sizeof(zd) * (size_t)P3_FT * (N + 1);
   const dim3 g2((unsigned)(((long)C * N + P3_FT - 1) / P3_FT)), g1((unsigned)(C * ((N + P3_FT - 1) / P3_FT)));
